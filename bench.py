@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Benchmark: MLUPS of the per-timestep lattice update on one MI355X.
+
+Workload: the 256x256x512 periodic Rayleigh-Plateau box (liquid cylinder of
+R=40 along z with 8 seeded cosine modes, tau_red=1.0, tau_blue=0.7,
+sigma=A=0.01, beta=0.7), synthetic initial state generated on the device.
+A step is one full lattice update (moments kernel + fused collide/stream
+kernel) of all 33,554,432 nodes.  The headline precision is FP32 shifted
+storage; --precision selects f64 / f16.
+
+Prints ONE JSON line (rank 0).  Fields beyond the driver contract:
+  roofline      the dominant kernel (fused collide) against the 8 TB/s HBM
+                peak, with its algorithmic bytes per node
+  cpu_baseline  the reference's own FP64 step on the host cores
+  step_roofline MLUPS x counted bytes per node (312 B FP32 / 624 B FP64 /
+                160 B FP16) / 8 TB/s
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X
+# Counted bytes per node per step (every red/blue population read and written
+# once, phi written and read once).
+B_ALG_STEP = {"f64": 624, "f32": 312, "f16": 160}
+# Algorithmic bytes per node of the fused collide kernel alone: 38 populations
+# read + 38 written + phi read once.
+B_ALG_COLLIDE = {"f64": 2 * 19 * 8 * 2 + 8, "f32": 2 * 19 * 4 * 2 + 4, "f16": 2 * 19 * 2 * 2 + 4}
+DTYPE = {"f64": "f64", "f32": "f32", "f16": "f16-storage/f32-compute"}
+
+
+def cpu_baseline(seconds):
+    threads = int(os.environ.get("CGL_CPU_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    try:
+        r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--threads", str(threads), "--seconds",
+                            str(seconds)], cwd=ROOT, capture_output=True, text=True, timeout=600)
+        line = [l for l in r.stdout.splitlines() if l.startswith("{")]
+        if r.returncode != 0 or not line:
+            return {"value": None, "error": (r.stderr or r.stdout)[-300:]}
+        return json.loads(line[-1])
+    except Exception as e:  # the baseline is a report, never the measurement
+        return {"value": None, "error": str(e)}
+
+
+def traffic_from_profile(precision):
+    """HBM bytes per launch of the fused kernel from committed rocprofv3 PMC
+    summaries (profiles/), scaled by the copy-kernel calibration; None if
+    absent."""
+    path = os.path.join(ROOT, "profiles", f"k_collide_{precision}_256x256x512_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="f32", choices=["f64", "f32", "f16"])
+    ap.add_argument("--xchunk", type=int, default=0)
+    ap.add_argument("--size", default="256x256x512")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design (DESIGN.md), "
+              "not built. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import symmetry_amd
+    from symmetry_amd import ic
+
+    nx, ny, nz = (int(v) for v in args.size.split("x"))
+    cfg = ic.rp_config3()
+    lat = symmetry_amd.Lattice(nx, ny, nz, precision=args.precision, sigma=cfg["sigma"], rho0=cfg["rho0"],
+                               nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], xchunk=args.xchunk)
+    print(f"# device {lat.device_name()} box {nx}x{ny}x{nz} precision {args.precision}", file=sys.stderr, flush=True)
+    lat.init_cylinder(nx / 2 + 0.5, ny / 2 + 0.5, cfg["r0"], 6.0 * cfg["beta"] * 0.134, cfg["amps"])
+    lat.step(args.warmup)
+    lat.sync()
+    nodes = nx * ny * nz
+    # timed region: exactly `steps` steps, synchronised on both sides
+    t0 = time.perf_counter()
+    lat.step(args.steps)
+    lat.sync()
+    wall = time.perf_counter() - t0
+    mlups = nodes * args.steps / wall / 1e6
+    # per-kernel durations with HIP events on the library's stream (separate pass)
+    ksteps = max(10, min(args.steps, 50))
+    ev_total, ms_mom, ms_col = lat.time_steps(ksteps)
+    avg_col = ms_col / ksteps * 1e-3
+    avg_mom = ms_mom / ksteps * 1e-3
+    achieved = B_ALG_COLLIDE[args.precision] * nodes / avg_col
+    traffic = traffic_from_profile(args.precision)
+    lat.close()
+
+    out = {
+        "metric": "MLUPS at 256x256x512 D3Q19 CG-MRT, FP32 vs FP64, % of HBM roofline",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.precision],
+        "data": "synthetic (seeded liquid-cylinder initial state generated on the device)",
+        "config": {"workload": f"Rayleigh-Plateau {nx}x{ny}x{nz} periodic D3Q19 colour-gradient MRT, R=40, "
+                               "8 modes seed 1234, tau_r=1.0 tau_b=0.7, sigma=0.01, beta=0.7",
+                   "precision": args.precision, "nodes": nodes, "xchunk": args.xchunk or 8},
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                     "kernel": "k_collide (fused derivatives+collision+recolour+streaming)",
+                     "alg_bytes_per_node": B_ALG_COLLIDE[args.precision],
+                     "avg_launch_ms": round(avg_col * 1e3, 4), "moments_avg_launch_ms": round(avg_mom * 1e3, 4)},
+        "step_roofline": {"bytes_per_node": B_ALG_STEP[args.precision],
+                          "frac": round(mlups * 1e6 * B_ALG_STEP[args.precision] / HBM_PEAK, 4),
+                          "event_mlups": round(nodes * ksteps / (ev_total * 1e-3) / 1e6, 1)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,112 @@
+/*
+ * cgl.h -- C ABI of libcgl, the MI355X (gfx950) per-timestep lattice update of
+ * the D3Q19 two-component colour-gradient MRT solver (X-Sheen/Symmetry,
+ * Source1.f90, cited as S:N).
+ *
+ * The reference's time loop (S:215-252) calls external Fortran subroutines
+ * with implicit interfaces and whole arrays passed by address.  libcgl takes
+ * the place of that loop body for one GPU: a Fortran driver (or any C caller)
+ * holds host arrays in the reference's column-major layout,
+ *   populations (0:18, zl, yl, xl)  -> double[xl][yl][zl][19] in C terms,
+ *   scalars     (zl, yl, xl)        -> double[xl][yl][zl],
+ * uploads them once, advances the state on the device and downloads fields
+ * only at output cadence.  All pointers are host pointers borrowed for the
+ * duration of the call; device memory is owned by the handle.
+ *
+ * Conventions: every function returns CGL_OK (0) or a negative CGL_ERR_*
+ * code; cgl_last_error() describes the last failure of the calling thread.
+ * One host thread per handle; a handle is not thread-safe.  LOGICAL arrays
+ * cross the ABI as int32 (0 / 1).
+ */
+#ifndef CGL_H
+#define CGL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGL_OK 0
+#define CGL_ERR_ARG (-1)         /* invalid argument or size */
+#define CGL_ERR_HIP (-2)         /* HIP runtime error (no GPU, out of memory, launch failure) */
+#define CGL_ERR_UNSUPPORTED (-3) /* valid request this build does not implement */
+
+#define CGL_BC_PERIODIC 0 /* periodic face pair (the reference's next_* tables, S:295-324) */
+#define CGL_BC_WALL 1     /* half-way bounce-back wall with a ghost layer (S:1557-1590) */
+#define CGL_BC_MIRROR 2   /* specular mirror ghost layer (S:1443-1542) */
+
+#define CGL_PREC_F64 0 /* FP64 storage and collision ("plain") */
+#define CGL_PREC_F32 1 /* FP32 storage shifted by w_i*rho0*(1 +- phi)/2, FP32 collision */
+#define CGL_PREC_F16 2 /* FP16 shifted storage, FP32 phi and collision */
+
+typedef struct cgl_handle cgl_t;
+
+/* Physical and numerical parameters.  Names follow the reference's module
+ * const / main (S:7-26, S:57-84).  Layout is fixed for Fortran bind(C). */
+typedef struct cgl_params {
+    int32_t nx, ny, nz;  /* xl, yl, zl (S:9-11) */
+    int32_t bc[6];       /* faces x-lo, x-hi, y-lo, y-hi, z-lo, z-hi: CGL_BC_* */
+    int32_t precision;   /* CGL_PREC_* */
+    int32_t npar;        /* number of particles (par, S:24); 0 = fluid only */
+    int32_t xchunk;      /* x planes per workgroup of the fused kernel; 0 = auto */
+    int32_t device;      /* HIP device ordinal */
+    double sigma;        /* CSF surface-tension coefficient (sigma, S:66) */
+    double rho0;         /* reference density (rho0, S:67) */
+    double nu_r, nu_b;   /* kinematic viscosities of red and blue (S:69-70) */
+    double beta;         /* recolouring parameter (beta, S:17) */
+} cgl_params;
+
+/* Allocate device state for one box.  Replaces the allocation block of the
+ * reference's main (S:144-179).  Fluid-only, fully periodic boxes are
+ * supported; other face types and particles return CGL_ERR_UNSUPPORTED. */
+int cgl_create(cgl_t **h, const cgl_params *p);
+int cgl_destroy(cgl_t *h);
+
+/* Upload the post-streaming populations ri, bi (S:169-170 layout) as the
+ * state at the top of a time step (what calmacros, S:216, reads). */
+int cgl_upload_state(cgl_t *h, const double *ri, const double *bi);
+
+/* Download the current state in the same layout: equal to the reference's
+ * ri, bi after its streaming and curve calls (S:249-251). */
+int cgl_download_state(cgl_t *h, double *ri, double *bi);
+
+/* Initial condition on the device: liquid cylinder along z through (xc, yc)
+ * in 1-based lattice coordinates, R(z) = r0 + sum_m amp[m] cos(2 pi (m+1)
+ * (z-1)/nz), phi = tanh(kprof (R(z) - r)), Laplace density jump 3 sigma/R(z)
+ * inside, SRT equilibria at rest (the form of initialization, S:850-896). */
+int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, int32_t nmodes, const double *amp);
+
+/* Advance nsteps time steps: the loop body S:216-251 (calmacros,
+ * derivatives, collision, Recolor, boundary, streaming) without output. */
+int cgl_step(cgl_t *h, int32_t nsteps);
+
+/* Moments of the current state (calmacros, S:770-831): rho, rho_R, rho_B,
+ * bare velocity ux, uy, uz and phi (rho_N) in (zl, yl, xl) layout.  Any
+ * pointer may be NULL. */
+int cgl_calmacros(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *ux, double *uy, double *uz,
+                  double *phi);
+
+/* Phase-field gradient, unit normal and interface flag of the current state
+ * (derivatives, S:498-533).  Any pointer may be NULL. */
+int cgl_derivatives(cgl_t *h, double *gradx, double *grady, double *gradz, double *n_x, double *n_y, double *n_z,
+                    int32_t *flag);
+
+/* Block until all queued device work of the handle has finished. */
+int cgl_sync(cgl_t *h);
+
+/* Time nsteps steps on the device with HIP events recorded on the handle's
+ * stream: total elapsed ms, and ms summed over the moments and fused-collide
+ * kernels.  No counterpart in the reference (it has no timers). */
+int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moments, double *ms_collide);
+
+/* Device name of the handle (for logs). */
+int cgl_device_name(cgl_t *h, char *buf, int32_t len);
+
+const char *cgl_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CGL_H */

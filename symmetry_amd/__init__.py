@@ -1,0 +1,160 @@
+"""symmetry_amd -- MI355X-native per-timestep lattice update of the D3Q19
+two-component colour-gradient MRT solver of X-Sheen/Symmetry (Source1.f90).
+
+The product is ``libcgl.so`` (HIP kernels for gfx950 behind the C ABI of
+``include/cgl.h``).  This module is a thin ctypes binding of that ABI for
+Python callers (tests, benchmarks); the Fortran driver binds the same ABI
+through ``fortran/cgl_mod.f90``.  There is no CPU fallback: every call goes to
+the GPU library, and importing fails loudly when it has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcgl.so")
+
+BC_PERIODIC, BC_WALL, BC_MIRROR = 0, 1, 2
+PREC = {"f64": 0, "f32": 1, "f16": 2}
+Q = 19
+
+
+class CglError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    """Mirror of ``cgl_params`` (include/cgl.h)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("bc", ctypes.c_int32 * 6),
+        ("precision", ctypes.c_int32), ("npar", ctypes.c_int32),
+        ("xchunk", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("sigma", ctypes.c_double), ("rho0", ctypes.c_double),
+        ("nu_r", ctypes.c_double), ("nu_b", ctypes.c_double), ("beta", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+# Every symbol include/cgl.h declares.
+EXPORTS = (
+    "cgl_create", "cgl_destroy", "cgl_upload_state", "cgl_download_state", "cgl_init_cylinder",
+    "cgl_step", "cgl_calmacros", "cgl_derivatives", "cgl_sync", "cgl_time_steps",
+    "cgl_device_name", "cgl_last_error",
+)
+
+
+def lib():
+    """Load libcgl.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` "
+                              "or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.cgl_last_error.restype = ctypes.c_char_p
+        vp = ctypes.c_void_p
+        L.cgl_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Params)]
+        L.cgl_destroy.argtypes = [vp]
+        L.cgl_upload_state.argtypes = [vp, vp, vp]
+        L.cgl_download_state.argtypes = [vp, vp, vp]
+        L.cgl_init_cylinder.argtypes = [vp] + [ctypes.c_double] * 4 + [ctypes.c_int32, vp]
+        L.cgl_step.argtypes = [vp, ctypes.c_int32]
+        L.cgl_calmacros.argtypes = [vp] + [vp] * 7
+        L.cgl_derivatives.argtypes = [vp] + [vp] * 7
+        L.cgl_sync.argtypes = [vp]
+        L.cgl_time_steps.argtypes = [vp, ctypes.c_int32] + [ctypes.POINTER(ctypes.c_double)] * 3
+        L.cgl_device_name.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise CglError(f"libcgl error {rc}: {lib().cgl_last_error().decode()}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Lattice:
+    """One box on one GPU.  Host arrays use the reference's layout with
+    0-based indices: populations ``[x, y, z, i]``, scalars ``[x, y, z]``."""
+
+    def __init__(self, nx, ny, nz, precision="f64", sigma=0.01, rho0=1.0, nu_r=1.0 / 6.0, nu_b=1.0 / 6.0,
+                 beta=0.7, bc=(BC_PERIODIC,) * 6, xchunk=0, device=0):
+        p = Params()
+        p.nx, p.ny, p.nz = nx, ny, nz
+        for k, v in enumerate(bc):
+            p.bc[k] = v
+        p.precision = PREC[precision] if isinstance(precision, str) else int(precision)
+        p.npar = 0
+        p.xchunk = xchunk
+        p.device = device
+        p.sigma, p.rho0, p.nu_r, p.nu_b, p.beta = sigma, rho0, nu_r, nu_b, beta
+        self.params = p
+        self.shape = (nx, ny, nz)
+        self.h = ctypes.c_void_p()
+        _check(lib().cgl_create(ctypes.byref(self.h), ctypes.byref(p)))
+
+    def close(self):
+        if self.h:
+            lib().cgl_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, ri, bi):
+        ri = np.ascontiguousarray(ri, dtype=np.float64)
+        bi = np.ascontiguousarray(bi, dtype=np.float64)
+        assert ri.shape == self.shape + (Q,) and bi.shape == ri.shape
+        _check(lib().cgl_upload_state(self.h, _ptr(ri), _ptr(bi)))
+
+    def download(self):
+        ri = np.empty(self.shape + (Q,))
+        bi = np.empty(self.shape + (Q,))
+        _check(lib().cgl_download_state(self.h, _ptr(ri), _ptr(bi)))
+        return ri, bi
+
+    def init_cylinder(self, xc, yc, r0, kprof, amps=()):
+        a = np.ascontiguousarray(amps, dtype=np.float64)
+        _check(lib().cgl_init_cylinder(self.h, xc, yc, r0, kprof, len(a), _ptr(a) if len(a) else None))
+
+    def step(self, n=1):
+        _check(lib().cgl_step(self.h, int(n)))
+
+    def sync(self):
+        _check(lib().cgl_sync(self.h))
+
+    def calmacros(self):
+        names = ("rho", "rho_r", "rho_b", "ux", "uy", "uz", "phi")
+        out = {k: np.empty(self.shape) for k in names}
+        _check(lib().cgl_calmacros(self.h, *(_ptr(out[k]) for k in names)))
+        return out
+
+    def derivatives(self):
+        names = ("gx", "gy", "gz", "nxv", "nyv", "nzv")
+        out = {k: np.empty(self.shape) for k in names}
+        out["flag"] = np.empty(self.shape, dtype=np.int32)
+        _check(lib().cgl_derivatives(self.h, *(_ptr(out[k]) for k in names), _ptr(out["flag"])))
+        return out
+
+    def time_steps(self, n):
+        t = [ctypes.c_double() for _ in range(3)]
+        _check(lib().cgl_time_steps(self.h, int(n), *(ctypes.byref(v) for v in t)))
+        return tuple(v.value for v in t)
+
+    def device_name(self):
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().cgl_device_name(self.h, buf, 256))
+        return buf.value.decode()

@@ -1,0 +1,313 @@
+// cgl_api.hip -- the C ABI of libcgl (declared in include/cgl.h).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "cgl_impl.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                             \
+    do {                                                                                                         \
+        hipError_t e_ = (expr);                                                                                  \
+        if (e_ != hipSuccess) return fail(CGL_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));          \
+    } while (0)
+
+// Dispatch on the storage precision.
+template <class Fn> void dispatch(cgl_handle *h, Fn fn)
+{
+    switch (h->p.precision) {
+    case CGL_PREC_F64: fn(cgl::ImplF64{}); break;
+    case CGL_PREC_F32: fn(cgl::ImplF32{}); break;
+    default: fn(cgl::ImplF16{}); break;
+    }
+}
+
+int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(CGL_ERR_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
+    return CGL_OK;
+}
+
+int ensure_scratch(cgl_handle *h, bool mac, bool aos, bool iflag)
+{
+    if (mac && !h->mac) HIPCHK(hipMalloc((void **)&h->mac, 7 * h->N * sizeof(double)));
+    if (aos && !h->aos) HIPCHK(hipMalloc((void **)&h->aos, (size_t)cgl::Q * h->N * sizeof(double)));
+    if (iflag && !h->iflag) HIPCHK(hipMalloc((void **)&h->iflag, h->N * sizeof(int)));
+    return CGL_OK;
+}
+
+int set_device(cgl_handle *h)
+{
+    HIPCHK(hipSetDevice(h->p.device));
+    return CGL_OK;
+}
+
+// One time step: K1 then K2, then swap buffers.
+void enqueue_step(cgl_handle *h)
+{
+    dispatch(h, [&](auto impl) {
+        decltype(impl)::moments(h, false);
+        decltype(impl)::collide(h);
+    });
+    h->cur ^= 1;
+    h->phicur ^= 1;
+    h->steps_done++;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *cgl_last_error(void) { return g_err; }
+
+int cgl_create(cgl_t **out, const cgl_params *p)
+{
+    if (!out || !p) return fail(CGL_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(CGL_ERR_ARG, "box must be at least 1x1x1");
+    double nn = (double)p->nx * p->ny * p->nz;
+    if (nn > 2.0e9) return fail(CGL_ERR_ARG, "box of %.3g nodes exceeds the 32-bit node index", nn);
+    if (p->precision < CGL_PREC_F64 || p->precision > CGL_PREC_F16)
+        return fail(CGL_ERR_ARG, "unknown precision %d", p->precision);
+    for (int f = 0; f < 6; ++f)
+        if (p->bc[f] != CGL_BC_PERIODIC)
+            return fail(CGL_ERR_UNSUPPORTED, "face %d: only periodic faces are implemented on the GPU", f);
+    if (p->npar != 0) return fail(CGL_ERR_UNSUPPORTED, "particles are not implemented on the GPU");
+    if (!(p->rho0 > 0) || !(p->nu_r > 0) || !(p->nu_b > 0)) return fail(CGL_ERR_ARG, "rho0, nu_r, nu_b must be > 0");
+
+    cgl_handle *h = new (std::nothrow) cgl_handle();
+    if (!h) return fail(CGL_ERR_ARG, "out of host memory");
+    memset(h, 0, sizeof *h);
+    h->p = *p;
+    h->L.nx = p->nx;
+    h->L.ny = p->ny;
+    h->L.nz = p->nz;
+    h->N = (size_t)p->nx * p->ny * p->nz;
+    h->L.N = (long long)h->N;
+    h->L.rho0 = p->rho0;
+    h->L.sigma = p->sigma;
+    h->L.beta = p->beta;
+    h->L.nu_r = p->nu_r;
+    h->L.nu_b = p->nu_b;
+    h->esize = p->precision == CGL_PREC_F64 ? 8 : (p->precision == CGL_PREC_F32 ? 4 : 2);
+    h->phisize = p->precision == CGL_PREC_F64 ? 8 : 4;
+    h->xc = p->xchunk > 0 ? p->xchunk : 8;
+    int rc = set_device(h);
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&h->ev[k]);
+    for (int b = 0; b < 2 && e == hipSuccess; ++b)
+        for (int c = 0; c < 2 && e == hipSuccess; ++c) e = hipMalloc(&h->pops[b][c], (size_t)cgl::Q * h->N * h->esize);
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc(&h->phi[b], h->N * h->phisize);
+    if (e == hipSuccess) e = hipMemset(h->phi[0], 0, h->N * h->phisize);
+    if (e == hipSuccess) e = hipMemset(h->phi[1], 0, h->N * h->phisize);
+    if (e != hipSuccess) {
+        rc = fail(CGL_ERR_HIP, "device allocation failed: %s", hipGetErrorString(e));
+        cgl_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return CGL_OK;
+}
+
+int cgl_destroy(cgl_t *h)
+{
+    if (!h) return CGL_OK;
+    (void)hipSetDevice(h->p.device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (int b = 0; b < 2; ++b) {
+        for (int c = 0; c < 2; ++c)
+            if (h->pops[b][c]) (void)hipFree(h->pops[b][c]);
+        if (h->phi[b]) (void)hipFree(h->phi[b]);
+    }
+    if (h->mac) (void)hipFree(h->mac);
+    if (h->aos) (void)hipFree(h->aos);
+    if (h->iflag) (void)hipFree(h->iflag);
+    for (int k = 0; k < 3; ++k)
+        if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return CGL_OK;
+}
+
+int cgl_upload_state(cgl_t *h, const double *ri, const double *bi)
+{
+    if (!h || !ri || !bi) return fail(CGL_ERR_ARG, "null argument");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if ((rc = ensure_scratch(h, false, true, false))) return rc;
+    const size_t bytes = (size_t)cgl::Q * h->N * sizeof(double);
+    // the shift reference of the uploaded state: phi of each node
+    double *tmp = nullptr;
+    HIPCHK(hipMalloc((void **)&tmp, bytes));
+    hipError_t e = hipMemcpyAsync(h->aos, ri, bytes, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp, bi, bytes, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        dispatch(h, [&](auto impl) {
+            decltype(impl)::phi_aos(h, h->aos, tmp);
+            decltype(impl)::pack(h, h->aos, 1);
+            decltype(impl)::pack(h, tmp, 0);
+        });
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(CGL_ERR_HIP, "upload failed: %s", hipGetErrorString(e));
+    return CGL_OK;
+}
+
+int cgl_download_state(cgl_t *h, double *ri, double *bi)
+{
+    if (!h || !ri || !bi) return fail(CGL_ERR_ARG, "null argument");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if ((rc = ensure_scratch(h, false, true, false))) return rc;
+    const size_t bytes = (size_t)cgl::Q * h->N * sizeof(double);
+    for (int red = 1; red >= 0; --red) {
+        dispatch(h, [&](auto impl) { decltype(impl)::unpack(h, h->aos, red); });
+        if ((rc = check_launch("unpack"))) return rc;
+        HIPCHK(hipMemcpyAsync(red ? ri : bi, h->aos, bytes, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    return CGL_OK;
+}
+
+int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, int32_t nmodes, const double *amp)
+{
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    if (nmodes < 0 || nmodes > 16 || (nmodes > 0 && !amp)) return fail(CGL_ERR_ARG, "nmodes must be 0..16");
+    int rc = set_device(h);
+    if (rc) return rc;
+    cgl::CylIC ic{};
+    ic.xc = xc;
+    ic.yc = yc;
+    ic.r0 = r0;
+    ic.kprof = kprof;
+    ic.nmodes = nmodes;
+    for (int m = 0; m < nmodes; ++m) ic.amp[m] = amp[m];
+    dispatch(h, [&](auto impl) { decltype(impl)::init_cylinder(h, ic); });
+    if ((rc = check_launch("init_cylinder"))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CGL_OK;
+}
+
+int cgl_step(cgl_t *h, int32_t nsteps)
+{
+    if (!h || nsteps < 0) return fail(CGL_ERR_ARG, "bad arguments");
+    int rc = set_device(h);
+    if (rc) return rc;
+    for (int t = 0; t < nsteps; ++t) {
+        enqueue_step(h);
+        if ((rc = check_launch("step"))) return rc;
+    }
+    return CGL_OK;
+}
+
+int cgl_calmacros(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *ux, double *uy, double *uz, double *phi)
+{
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if ((rc = ensure_scratch(h, true, false, false))) return rc;
+    dispatch(h, [&](auto impl) { decltype(impl)::moments(h, true); });
+    if ((rc = check_launch("moments"))) return rc;
+    double *dst[7] = {rho, rho_r, rho_b, ux, uy, uz, phi};
+    for (int k = 0; k < 7; ++k)
+        if (dst[k]) HIPCHK(hipMemcpyAsync(dst[k], h->mac + k * h->N, h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CGL_OK;
+}
+
+int cgl_derivatives(cgl_t *h, double *gradx, double *grady, double *gradz, double *n_x, double *n_y, double *n_z,
+                    int32_t *flag)
+{
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if ((rc = ensure_scratch(h, true, false, true))) return rc;
+    dispatch(h, [&](auto impl) {
+        decltype(impl)::moments(h, false);
+        decltype(impl)::derivatives(h, h->mac, h->iflag);
+    });
+    if ((rc = check_launch("derivatives"))) return rc;
+    double *dst[6] = {gradx, grady, gradz, n_x, n_y, n_z};
+    for (int k = 0; k < 6; ++k)
+        if (dst[k]) HIPCHK(hipMemcpyAsync(dst[k], h->mac + k * h->N, h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (flag) HIPCHK(hipMemcpyAsync(flag, h->iflag, h->N * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CGL_OK;
+}
+
+int cgl_sync(cgl_t *h)
+{
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CGL_OK;
+}
+
+int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moments, double *ms_collide)
+{
+    if (!h || nsteps < 1) return fail(CGL_ERR_ARG, "bad arguments");
+    int rc = set_device(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // pass 1: the whole loop between two events (no per-kernel events inside)
+    HIPCHK(hipEventRecord(h->ev[0], h->stream));
+    for (int t = 0; t < nsteps; ++t) enqueue_step(h);
+    HIPCHK(hipEventRecord(h->ev[1], h->stream));
+    if ((rc = check_launch("step"))) return rc;
+    HIPCHK(hipEventSynchronize(h->ev[1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+    if (ms_total) *ms_total = ms;
+    // pass 2: per-kernel durations, one event pair around each launch
+    double km = 0.0, kc = 0.0;
+    for (int t = 0; t < nsteps; ++t) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventRecord(h->ev[0], h->stream));
+        dispatch(h, [&](auto impl) { decltype(impl)::moments(h, false); });
+        HIPCHK(hipEventRecord(h->ev[1], h->stream));
+        dispatch(h, [&](auto impl) { decltype(impl)::collide(h); });
+        HIPCHK(hipEventRecord(h->ev[2], h->stream));
+        h->cur ^= 1;
+        h->phicur ^= 1;
+        h->steps_done++;
+        HIPCHK(hipEventSynchronize(h->ev[2]));
+        HIPCHK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+        HIPCHK(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
+        km += a;
+        kc += b;
+    }
+    if (ms_moments) *ms_moments = km;
+    if (ms_collide) *ms_collide = kc;
+    return CGL_OK;
+}
+
+int cgl_device_name(cgl_t *h, char *buf, int32_t len)
+{
+    if (!h || !buf || len < 1) return fail(CGL_ERR_ARG, "bad arguments");
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, h->p.device));
+    snprintf(buf, (size_t)len, "%s (%s)", prop.name, prop.gcnArchName);
+    return CGL_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,95 @@
+"""Host-side initial conditions and workload definitions.
+
+The liquid-cylinder state follows the reference's ``initialization``
+(Source1.f90 S:850-896): tanh profile phi = tanh(6 beta 0.134 (R - r)), a
+Laplace density jump 3 sigma / R inside the liquid, colour densities
+rho (1 +- phi)/2 and SRT equilibria at rest (``equilf_SRT``, S:833-848).  The
+reference puts the cylinder axis along y through the box corner with one
+cosine mode; the workloads put it along z through the box centre with one or
+several modes.  Arithmetic follows the reference's operation order so that
+this generator reproduces the reference's own initial state exactly when
+given the reference's geometry.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+Q = 19
+W = np.array([1.0 / 3.0] + [1.0 / 18.0] * 6 + [1.0 / 36.0] * 12)
+_tanh = np.frompyfunc(math.tanh, 1, 1)
+
+
+def cylinder_state(nx, ny, nz, *, axis, center, r0, amps, wavelength, phase0, sigma, rho0, beta=0.7,
+                   exact=True):
+    """Populations (ri, bi) [x, y, z, i] and phi [x, y, z] of a perturbed
+    liquid cylinder.
+
+    axis       'y' (reference) or 'z' (workloads): the cylinder axis.
+    center     the two transverse coordinates of the axis, 1-based lattice
+               units, in (x, z) order for axis 'y' and (x, y) for axis 'z'.
+    amps       a_m of R(s) = r0 + sum_m a_m cos(2 pi m (s - phase0) / wavelength).
+    exact      use libm tanh per node (bit-reproducible); False uses numpy's
+               vectorised tanh (faster, may differ in the last ulp).
+    """
+    x = np.arange(1, nx + 1, dtype=np.float64)[:, None, None]
+    y = np.arange(1, ny + 1, dtype=np.float64)[None, :, None]
+    z = np.arange(1, nz + 1, dtype=np.float64)[None, None, :]
+    if axis == "y":
+        dis = (x - center[0]) ** 2 + (z - center[1]) ** 2
+        s = y
+    elif axis == "z":
+        dis = (x - center[0]) ** 2 + (y - center[1]) ** 2
+        s = z
+    else:
+        raise ValueError(axis)
+    cos = (lambda v: np.vectorize(math.cos, otypes=[np.float64])(v)) if exact else np.cos
+    dis_x = np.zeros_like(s)
+    for m, a in enumerate(amps, start=1):
+        dis_x = dis_x + a * cos(2.0 * math.pi * m * (s - phase0) / wavelength)
+    star = r0 + dis_x - np.sqrt(dis)
+    star = np.broadcast_to(star, (nx, ny, nz))
+    arg = star * 6.0 * beta * 0.134
+    phi = _tanh(arg).astype(np.float64) if exact else np.tanh(arg)
+    plus = np.broadcast_to(sigma / (r0 + dis_x) * 3.0, (nx, ny, nz))
+    rho = np.where(phi > 0.0, rho0 + plus, rho0)
+    rho_r = rho * (phi + 1.0) / 2.0
+    rho_b = rho - rho_r
+    ri = (rho_r[..., None] * W) * 1.0
+    bi = (rho_b[..., None] * W) * 1.0
+    return ri, bi, phi
+
+
+# ---------------------------------------------------------------- workloads
+
+def rp_config1():
+    """Plain FP64 Rayleigh-Plateau, 32x32x128 periodic, R=8, 10% cosine
+    perturbation with the box length as wavelength, sigma=A=0.01, beta=0.7,
+    tau=1.0 for both fluids, rho=1, at rest; 2000 steps."""
+    return dict(nx=32, ny=32, nz=128, r0=8.0, amps=(0.8,), sigma=0.01, rho0=1.0,
+                nu_r=(1.0 - 0.5) / 3.0, nu_b=(1.0 - 0.5) / 3.0, beta=0.7, steps=2000)
+
+
+def rp_config3_amps(seed=1234, r0=40.0):
+    """Amplitudes of the 8 modes of the 256x256x512 workload: uniform in
+    [0, 0.02 R] from numpy's PCG64 generator with the given seed."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.0, 0.02 * r0, 8)
+
+
+def rp_config3():
+    """Precision-sweep Rayleigh-Plateau, 256x256x512 periodic, R=40, 8 modes,
+    tau_red=1.0, tau_blue=0.7, A=sigma=0.01, beta=0.7; 1000 steps."""
+    return dict(nx=256, ny=256, nz=512, r0=40.0, amps=tuple(rp_config3_amps()), sigma=0.01, rho0=1.0,
+                nu_r=(1.0 - 0.5) / 3.0, nu_b=(0.7 - 0.5) / 3.0, beta=0.7, steps=1000)
+
+
+def workload_state(cfg, exact=True, nx=None, ny=None, nz=None):
+    """Host initial state of a z-axis cylinder workload (optionally resized)."""
+    nx = nx or cfg["nx"]
+    ny = ny or cfg["ny"]
+    nz = nz or cfg["nz"]
+    return cylinder_state(nx, ny, nz, axis="z", center=(nx / 2 + 0.5, ny / 2 + 0.5), r0=cfg["r0"],
+                          amps=cfg["amps"], wavelength=float(nz), phase0=1.0, sigma=cfg["sigma"],
+                          rho0=cfg["rho0"], beta=cfg["beta"], exact=exact)

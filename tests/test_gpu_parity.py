@@ -1,0 +1,146 @@
+"""GPU parity: libcgl on the MI355X against the FP64 oracle (oracle/), through
+the C ABI, on identical seeded inputs.
+
+* FP64 ("plain") must reproduce the oracle bit for bit: same operations in
+  the same order, no FMA contraction in the FP64 kernels.
+* FP32 / FP16 shifted storage are compared with the FP64 GPU path; tolerances
+  are stated per test.
+"""
+import numpy as np
+import pytest
+
+import symmetry_amd
+from tests.helpers import oracle_from, rel_err, small_rp
+
+pytestmark = pytest.mark.gpu
+
+
+def _lattice(cfg, precision, **kw):
+    return symmetry_amd.Lattice(cfg["nx"], cfg["ny"], cfg["nz"], precision=precision, sigma=cfg["sigma"],
+                                rho0=cfg["rho0"], nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], **kw)
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def rp1():
+    """Config-1 box (32x32x128 periodic, R=8, 10% mode, tau=1)."""
+    return small_rp()
+
+
+def test_upload_download_roundtrip(rp1):
+    cfg, ri, bi = rp1
+    _print("roundtrip f64", cfg["nx"], cfg["ny"], cfg["nz"])
+    lat = _lattice(cfg, "f64")
+    lat.upload(ri, bi)
+    r2, b2 = lat.download()
+    assert np.array_equal(r2, ri) and np.array_equal(b2, bi)
+
+
+def test_calmacros_and_derivatives_bit_exact(rp1):
+    cfg, ri, bi = rp1
+    _print("stage parity f64")
+    lat = _lattice(cfg, "f64")
+    lat.upload(ri, bi)
+    st = oracle_from(cfg, ri, bi)
+    st.call("calmacros")
+    mac = lat.calmacros()
+    for k in ("rho", "rho_r", "rho_b", "ux", "uy", "uz", "phi"):
+        assert np.array_equal(mac[k], st.arrays[k]), k
+    st.call("derivatives")
+    der = lat.derivatives()
+    for k in ("gx", "gy", "gz", "nxv", "nyv", "nzv"):
+        assert np.array_equal(der[k], st.arrays[k]), k
+    assert np.array_equal(der["flag"], st.flag)
+
+
+@pytest.mark.parametrize("nsteps", [1, 20])
+def test_f64_steps_bit_exact(rp1, nsteps):
+    cfg, ri, bi = rp1
+    _print("f64 steps", nsteps)
+    lat = _lattice(cfg, "f64")
+    lat.upload(ri, bi)
+    lat.step(nsteps)
+    r2, b2 = lat.download()
+    st = oracle_from(cfg, ri, bi)
+    st.step(nsteps)
+    dr = np.max(np.abs(r2 - st.ri))
+    db = np.max(np.abs(b2 - st.bi))
+    _print("max |dr|, |db|:", dr, db)
+    assert dr == 0.0 and db == 0.0
+
+
+@pytest.mark.parametrize("shape,xchunk", [((20, 12, 40), 3), ((9, 5, 70), 4), ((4, 4, 4), 1), ((33, 7, 65), 16)])
+def test_f64_uneven_boxes_bit_exact(shape, xchunk):
+    """Boxes that do not fill the 64x4 workgroup tile or the x chunk."""
+    nx, ny, nz = shape
+    _print("uneven f64", shape, xchunk)
+    cfg, ri, bi = small_rp(nx=nx, ny=ny, nz=nz, r0=min(nx, ny) / 4.0, amps=(0.2,), nu_b=0.05)
+    lat = _lattice(cfg, "f64", xchunk=xchunk)
+    lat.upload(ri, bi)
+    lat.step(3)
+    r2, b2 = lat.download()
+    st = oracle_from(cfg, ri, bi)
+    st.step(3)
+    assert np.array_equal(r2, st.ri) and np.array_equal(b2, st.bi)
+
+
+def test_xchunk_invariance(rp1):
+    cfg, ri, bi = rp1
+    _print("xchunk invariance f32")
+    outs = []
+    for xc in (1, 8, 32):
+        lat = _lattice(cfg, "f32", xchunk=xc)
+        lat.upload(ri, bi)
+        lat.step(5)
+        outs.append(lat.download())
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
+
+
+@pytest.mark.parametrize("prec,tol_phi,tol_u", [("f32", 1e-5, 1e-5), ("f16", 5e-3, 5e-2)])
+def test_reduced_precision_vs_f64(rp1, prec, tol_phi, tol_u):
+    """phi and velocity of FP32 shifted storage within 1e-5 (L2-relative) of
+    the FP64 GPU path after 200 steps; FP16 storage within the looser stated
+    bounds."""
+    cfg, ri, bi = rp1
+    _print("precision", prec)
+    ref = _lattice(cfg, "f64")
+    ref.upload(ri, bi)
+    ref.step(200)
+    m64 = ref.calmacros()
+    lat = _lattice(cfg, prec)
+    lat.upload(ri, bi)
+    lat.step(200)
+    m = lat.calmacros()
+    e_phi = rel_err(m["phi"], m64["phi"])
+    u = np.stack([m["ux"], m["uy"], m["uz"]])
+    u64 = np.stack([m64["ux"], m64["uy"], m64["uz"]])
+    e_u = rel_err(u, u64)
+    _print(prec, "phi L2/Linf", e_phi, "u L2/Linf", e_u)
+    assert e_phi[0] < tol_phi and e_u[0] < tol_u
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
+def test_component_mass_conserved(rp1, prec):
+    cfg, ri, bi = rp1
+    lat = _lattice(cfg, prec)
+    lat.upload(ri, bi)
+    m0 = lat.calmacros()
+    lat.step(100)
+    m1 = lat.calmacros()
+    tol = {"f64": 1e-13, "f32": 1e-6, "f16": 1e-3}[prec]
+    for k in ("rho_r", "rho_b"):
+        a, b = m0[k].sum(), m1[k].sum()
+        _print(prec, k, a, b, abs(b - a) / a)
+        assert abs(b - a) / a < tol
+
+
+def test_device_ic_matches_host_ic():
+    cfg, ri, bi = small_rp(nx=32, ny=32, nz=64, amps=(0.8, 0.3))
+    lat = _lattice(cfg, "f64")
+    lat.init_cylinder(cfg["nx"] / 2 + 0.5, cfg["ny"] / 2 + 0.5, cfg["r0"], 6.0 * 0.7 * 0.134, cfg["amps"])
+    r2, b2 = lat.download()
+    assert np.max(np.abs(r2 - ri)) < 1e-14 and np.max(np.abs(b2 - bi)) < 1e-14

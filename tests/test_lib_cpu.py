@@ -1,0 +1,110 @@
+"""CPU-side checks of the product library and host code (no GPU needed):
+the C ABI loads and exports every symbol include/cgl.h declares, the
+parameter struct layout agrees between C and the bindings, argument
+validation happens before any device call, and the host initial condition
+reproduces the reference's initialization."""
+import ctypes
+import json
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import symmetry_amd
+from symmetry_amd import ic
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "include", "cgl.h")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(symmetry_amd.LIB_PATH):
+        subprocess.run(["make", "-s", "-C", os.path.dirname(symmetry_amd.LIB_PATH)], check=True)
+    return symmetry_amd.lib()
+
+
+def declared_functions():
+    text = open(HDR).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cgl_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_symbols_exported(lib):
+    names = declared_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(symmetry_amd.EXPORTS)
+
+
+def test_params_layout_matches_c(tmp_path):
+    src = tmp_path / "off.c"
+    src.write_text(
+        '#include <stddef.h>\n#include <stdio.h>\n#include "cgl.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu\\n\", sizeof(cgl_params), offsetof(cgl_params, precision),"
+        " offsetof(cgl_params, device), offsetof(cgl_params, sigma), offsetof(cgl_params, beta),"
+        " offsetof(cgl_params, bc));return 0;}\n")
+    exe = tmp_path / "off"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    P = symmetry_amd.Params
+    assert out == [ctypes.sizeof(P), P.precision.offset, P.device.offset, P.sigma.offset, P.beta.offset,
+                   P.bc.offset]
+
+
+def _params(**kw):
+    p = symmetry_amd.Params()
+    p.nx, p.ny, p.nz = 8, 8, 8
+    p.precision = 0
+    p.sigma, p.rho0, p.nu_r, p.nu_b, p.beta = 0.01, 1.0, 1 / 6, 1 / 6, 0.7
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+@pytest.mark.parametrize("kw,code", [({"nx": 0}, -1), ({"precision": 7}, -1), ({"npar": 3}, -3),
+                                     ({"rho0": 0.0}, -1)])
+def test_create_validates_before_touching_the_device(lib, kw, code):
+    h = ctypes.c_void_p()
+    rc = lib.cgl_create(ctypes.byref(h), ctypes.byref(_params(**kw)))
+    assert rc == code
+    assert lib.cgl_last_error()
+
+
+def test_wall_faces_rejected_until_implemented(lib):
+    p = _params()
+    p.bc[4] = symmetry_amd.BC_WALL
+    h = ctypes.c_void_p()
+    assert lib.cgl_create(ctypes.byref(h), ctypes.byref(p)) == -3
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference build absent")
+def test_host_ic_reproduces_reference_initialization():
+    with open(os.path.join(ROOT, "tests", "golden", "d3q19_tables.json")) as f:
+        M = np.array(json.load(f)["M"], dtype=float)
+    rad_c, sigma, rho0 = 30.0, 0.05, 0.05
+    nu_r = 0.1 / math.sqrt(rho0 / (rad_c * sigma))
+    ref = oracle.RefHarness(M, sigma, rho0, nu_r, nu_r / 10.0, 7.5, rad_c)
+    ref.scal()[3] = (ref.yl - 2) * 2.0
+    ref.call("initialization")
+    xl = ref.xl
+    ri, bi, phi = ic.cylinder_state(xl, ref.yl, ref.zl, axis="y", center=(xl - 0.5, ref.zl - 0.5), r0=rad_c,
+                                    amps=(0.1 * rad_c,), wavelength=(ref.yl - 2) * 2.0, phase0=1.5,
+                                    sigma=sigma, rho0=rho0)
+    assert np.array_equal(phi, ref.field("phi"))
+    assert np.array_equal(ri, ref.field("ri"))
+    assert np.array_equal(bi, ref.field("bi"))
+
+
+def test_config3_amplitudes_fixture():
+    """The 8 mode amplitudes of the 256x256x512 workload are committed data."""
+    with open(os.path.join(ROOT, "tests", "golden", "rp_config3_amps.json")) as f:
+        gold = json.load(f)
+    a = ic.rp_config3_amps()
+    assert np.array_equal(a, np.array(gold["amps"]))
+    assert np.all((a >= 0) & (a <= 0.02 * 40.0))
