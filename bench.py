@@ -83,6 +83,7 @@ def main():
     lat = symmetry_amd.Lattice(nx, ny, nz, precision=args.precision, sigma=cfg["sigma"], rho0=cfg["rho0"],
                                nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], xchunk=args.xchunk)
     print(f"# device {lat.device_name()} box {nx}x{ny}x{nz} precision {args.precision}", file=sys.stderr, flush=True)
+    copy_gbps = symmetry_amd.copy_bandwidth(2 << 30, 10)
     lat.init_cylinder(nx / 2 + 0.5, ny / 2 + 0.5, cfg["r0"], 6.0 * cfg["beta"] * 0.134, cfg["amps"])
     lat.step(args.warmup)
     lat.sync()
@@ -126,6 +127,7 @@ def main():
         "step_roofline": {"bytes_per_node": B_ALG_STEP[args.precision],
                           "frac": round(mlups * 1e6 * B_ALG_STEP[args.precision] / HBM_PEAK, 4),
                           "event_mlups": round(nodes * ksteps / (ev_total * 1e-3) / 1e6, 1)},
+        "hbm_copy_gbps": round(copy_gbps, 1),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

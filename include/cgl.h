@@ -37,7 +37,7 @@ extern "C" {
 #define CGL_BC_MIRROR 2   /* specular mirror ghost layer (S:1443-1542) */
 
 #define CGL_PREC_F64 0 /* FP64 storage and collision ("plain") */
-#define CGL_PREC_F32 1 /* FP32 storage shifted by w_i*rho0*(1 +- phi)/2, FP32 collision */
+#define CGL_PREC_F32 1 /* FP32 storage shifted by w_i*rho0*(1 +- phi)/2 of the reading node, FP32 collision */
 #define CGL_PREC_F16 2 /* FP16 shifted storage, FP32 phi and collision */
 
 typedef struct cgl_handle cgl_t;
@@ -99,6 +99,12 @@ int cgl_sync(cgl_t *h);
  * stream: total elapsed ms, and ms summed over the moments and fused-collide
  * kernels.  No counterpart in the reference (it has no timers). */
 int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moments, double *ms_collide);
+
+/* Achievable HBM bandwidth of a device: a streaming copy of `bytes` bytes,
+ * repeated reps times, read + write bytes per second in GB/s.  Also the
+ * known-size kernel used to calibrate rocprofv3 byte counters.  No
+ * counterpart in the reference. */
+int cgl_probe_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double *gbps);
 
 /* Device name of the handle (for logs). */
 int cgl_device_name(cgl_t *h, char *buf, int32_t len);

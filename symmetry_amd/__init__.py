@@ -45,7 +45,7 @@ _lib = None
 EXPORTS = (
     "cgl_create", "cgl_destroy", "cgl_upload_state", "cgl_download_state", "cgl_init_cylinder",
     "cgl_step", "cgl_calmacros", "cgl_derivatives", "cgl_sync", "cgl_time_steps",
-    "cgl_device_name", "cgl_last_error",
+    "cgl_device_name", "cgl_last_error", "cgl_probe_copy_bandwidth",
 )
 
 
@@ -70,6 +70,8 @@ def lib():
         L.cgl_sync.argtypes = [vp]
         L.cgl_time_steps.argtypes = [vp, ctypes.c_int32] + [ctypes.POINTER(ctypes.c_double)] * 3
         L.cgl_device_name.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32]
+        L.cgl_probe_copy_bandwidth.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                               ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -81,6 +83,13 @@ def _check(rc):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def copy_bandwidth(nbytes=2 << 30, reps=20, device=0):
+    """Achievable read+write HBM bandwidth (GB/s) of a streaming copy."""
+    v = ctypes.c_double()
+    _check(lib().cgl_probe_copy_bandwidth(device, int(nbytes), int(reps), ctypes.byref(v)))
+    return v.value
 
 
 class Lattice:

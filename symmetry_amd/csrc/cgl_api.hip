@@ -1,6 +1,7 @@
 // cgl_api.hip -- the C ABI of libcgl (declared in include/cgl.h).
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <new>
@@ -69,9 +70,53 @@ void enqueue_step(cgl_handle *h)
     h->steps_done++;
 }
 
+// Streaming copy, 16 B per thread per iteration: the reference kernel for the
+// box's achievable HBM bandwidth and for calibrating PMC byte counters.
+__global__ __launch_bounds__(256) void k_copy(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) b[i] = a[i];
+}
+
 } // namespace
 
 extern "C" {
+
+int cgl_probe_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double *gbps)
+{
+    if (bytes < (1 << 20) || reps < 1 || !gbps) return fail(CGL_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(device));
+    const size_t n = (size_t)bytes / sizeof(float4);
+    float4 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void **)&a, n * sizeof(float4)));
+    hipError_t e = hipMalloc((void **)&b, n * sizeof(float4));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemset(a, 0, n * sizeof(float4));
+    float ms = 0.f;
+    if (e == hipSuccess) {
+        int dev_cu = 256;
+        (void)hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, device);
+        unsigned blocks = (unsigned)dev_cu * 16;
+        k_copy<<<blocks, 256>>>(a, b, n); // warm-up
+        e = hipEventRecord(e0, 0);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) k_copy<<<blocks, 256>>>(a, b, n);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(e1, 0);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return fail(CGL_ERR_HIP, "copy probe failed: %s", hipGetErrorString(e));
+    *gbps = 2.0 * (double)(n * sizeof(float4)) * reps / (ms * 1e-3) / 1e9;
+    return CGL_OK;
+}
+
 
 const char *cgl_last_error(void) { return g_err; }
 
@@ -104,6 +149,10 @@ int cgl_create(cgl_t **out, const cgl_params *p)
     h->L.beta = p->beta;
     h->L.nu_r = p->nu_r;
     h->L.nu_b = p->nu_b;
+    {
+        const char *env = getenv("CGL_XCD_REMAP");
+        h->L.xcd_remap = env ? atoi(env) : 1;
+    }
     h->esize = p->precision == CGL_PREC_F64 ? 8 : (p->precision == CGL_PREC_F32 ? 4 : 2);
     h->phisize = p->precision == CGL_PREC_F64 ? 8 : 4;
     h->xc = p->xchunk > 0 ? p->xchunk : 8;

@@ -19,8 +19,10 @@
 //
 // FP64 ("plain") keeps the oracle's exact operation order and is compiled
 // without FMA contraction, so it reproduces the FP64 CPU path bit for bit.
-// FP32/FP16 storage keeps populations shifted by w_i*rho0*(1 +- phi)/2 of the
-// writing node (phi of that step, kept for the readers) and collides the
+// FP32/FP16 storage keeps each population shifted by w_i*rho0*a of the node
+// that will read it, a = (1 + phi)/2 for red and (1 - phi)/2 for blue, with
+// phi of the step that wrote it: the writer has that phi in its LDS ring and
+// the reader needs only its own node's value.  Collision works on the
 // deviations f - w_i*rho0 in FP32 with compensated moment sums.
 #pragma once
 
@@ -37,7 +39,7 @@ constexpr int TY = 4;               // workgroup extent along y
 
 struct Lat {
     int nx, ny, nz;
-    int pad;
+    int xcd_remap; // 1: hand each XCD a contiguous range of tiles
     long long N;
     double rho0, sigma, beta, nu_r, nu_b;
 };
@@ -63,6 +65,22 @@ __device__ __forceinline__ int wrapm(int v, int n)
     return v < 0 ? v + n : v;
 }
 
+// Workgroups go to the 8 XCDs of the MI355X round-robin by linear id, and each
+// XCD has its own L2.  Remap the linear id so that every XCD gets one
+// contiguous range of logical tiles: neighbouring tiles, whose halo cache
+// lines overlap, then share an L2 instead of fetching those lines twice.
+constexpr unsigned NXCD = 8;
+__device__ __forceinline__ void tile_of_block(const Lat &L, unsigned &bx, unsigned &by, unsigned &bz)
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y, nb = gx * gy * gridDim.z;
+    unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned full = nb - nb % NXCD;
+    if (L.xcd_remap && lin < full) lin = (lin % NXCD) * (full / NXCD) + lin / NXCD;
+    bx = lin % gx;
+    by = (lin / gx) % gy;
+    bz = lin / (gx * gy);
+}
+
 // Neumaier compensated sum.
 template <class R> struct CSum {
     R s = R(0), c = R(0);
@@ -83,8 +101,7 @@ template <class R> struct CSum {
 // displacement k-1.
 template <class S, class R, class P, bool SHIFT>
 __device__ __forceinline__ void gather(const Lat &L, const S *__restrict__ rs, const S *__restrict__ bs,
-                                       const int xo[3], const int yo[3], const int zo[3], R r[Q], R b[Q],
-                                       R aold[Q])
+                                       const int xo[3], const int yo[3], const int zo[3], R r[Q], R b[Q])
 {
     const size_t N = (size_t)L.N;
 #pragma unroll
@@ -93,19 +110,18 @@ __device__ __forceinline__ void gather(const Lat &L, const S *__restrict__ rs, c
         r[i] = StoreOf<S>::ld(rs + (size_t)i * N + m);
         b[i] = StoreOf<S>::ld(bs + (size_t)i * N + m);
     }
-    (void)aold;
 }
 
 // Moments of the gathered populations.
 //  FP64 path: calmacros (S:792-799) operation for operation.
-//  Shifted path: r, b hold deviations; aold[i] = (1 + phi_old(src))/2 of the
-//  node that wrote direction i.
+//  Shifted path: r, b hold deviations from w_i*rho0*ax and w_i*rho0*(1-ax),
+//  ax = (1 + phi_old(x))/2 of this (reading) node.
 template <class R, bool SHIFT> struct Moments {
     R sr, sb, rho, drho, ux, uy, uz, phi;
 };
 
 template <class R, bool SHIFT>
-__device__ __forceinline__ Moments<R, SHIFT> moments(const Lat &L, const R r[Q], const R b[Q], const R aold[Q])
+__device__ __forceinline__ Moments<R, SHIFT> moments(const Lat &L, const R r[Q], const R b[Q], R ax)
 {
     Moments<R, SHIFT> m;
     if constexpr (!SHIFT) {
@@ -142,18 +158,16 @@ __device__ __forceinline__ Moments<R, SHIFT> moments(const Lat &L, const R r[Q],
         m.uz = jz / m.rho;
         m.phi = (sr - sb) / m.rho;
     } else {
-        CSum<R> cr, cb, ca, cf;
+        CSum<R> cr, cb, cf;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             cr.add(r[i]);
             cb.add(b[i]);
-            ca.add(R(W[i]) * aold[i]);
             cf.add(r[i] + b[i]);
         }
         const R rho0 = R(L.rho0);
-        const R A = ca.get();
-        m.sr = cr.get() + rho0 * A;
-        m.sb = cb.get() + rho0 * (R(1) - A);
+        m.sr = cr.get() + rho0 * ax;
+        m.sb = cb.get() + rho0 * (R(1) - ax);
         m.drho = cf.get();
         m.rho = rho0 + m.drho;
         R jx = 0, jy = 0, jz = 0;
@@ -195,23 +209,20 @@ __global__ __launch_bounds__(256) void k_moments(Lat L, const S *__restrict__ rs
                                                  const P *__restrict__ phio, P *__restrict__ phi,
                                                  double *__restrict__ mac)
 {
-    const int z = blockIdx.x * TZ + threadIdx.x;
-    const int y = blockIdx.y * TY + threadIdx.y;
-    const int x = blockIdx.z;
+    unsigned bx, by, bz;
+    tile_of_block(L, bx, by, bz);
+    const int z = bx * TZ + threadIdx.x;
+    const int y = by * TY + threadIdx.y;
+    const int x = bz;
     if (z >= L.nz || y >= L.ny) return;
     int xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
-    R r[Q], b[Q], aold[Q];
-    gather<S, R, P, SHIFT>(L, rs, bs, xo, yo, zo, r, b, aold);
-    if constexpr (SHIFT) {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            int m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-            aold[i] = R(0.5) * (R(1) + R(phio[m]));
-        }
-    }
-    Moments<R, SHIFT> mo = moments<R, SHIFT>(L, r, b, aold);
+    R r[Q], b[Q];
+    gather<S, R, P, SHIFT>(L, rs, bs, xo, yo, zo, r, b);
     const int n = xo[1] + yo[1] + zo[1];
+    R ax = 0;
+    if constexpr (SHIFT) ax = R(0.5) * (R(1) + R(phio[n]));
+    Moments<R, SHIFT> mo = moments<R, SHIFT>(L, r, b, ax);
     phi[n] = P(mo.phi);
     if constexpr (FULL) {
         const size_t N = (size_t)L.N;
@@ -253,12 +264,14 @@ __device__ __forceinline__ void grad19(F phi_at, R &gx, R &gy, R &gz)
 // CSF force from the curvature of the normals (S:624-667), velocity shift,
 // viscosity interpolation and MRT relaxation with Guo forcing (S:669-689),
 // recolouring (S:708-726).  In: gathered pops, moments, phi, grad phi, n,
-// flag, the 3x3 normal derivatives.  Out: post-recolour pops (FP64) or their
-// deviations from w_i*rho0*a (shifted path).
+// flag, the 3x3 normal derivatives, and (shifted path) aw[i], the red shift
+// fraction of the node that reads direction i.  Out: post-recolour pops
+// (FP64) or their deviations from w_i*rho0*aw[i] / w_i*rho0*(1-aw[i]).
 template <class R, bool SHIFT>
 __device__ __forceinline__ void collide_recolor(const Lat &L, const R r[Q], const R b[Q], const Moments<R, SHIFT> &mo,
                                                 R phin, R gx, R gy, R gz, R nx, R ny, R nz, bool flag,
-                                                bool force_on, const R d[3][3], R rout[Q], R bout[Q])
+                                                bool force_on, const R d[3][3], const R aw[Q], R rout[Q],
+                                                R bout[Q])
 {
     R fx = 0, fy = 0, fz = 0;
     if (force_on) {
@@ -399,15 +412,15 @@ __device__ __forceinline__ void collide_recolor(const Lat &L, const R r[Q], cons
 #pragma unroll
         for (int i = 0; i < Q; ++i) bout[i] = f[i] - rout[i];
     } else {
-        // f holds deviations f' - w rho0; write deviations r' - w rho0 a, b' - w rho0 (1-a)
+        // f holds deviations f' - w rho0; write r' - w rho0 aw, b' - w rho0 (1 - aw)
         const R rho0 = R(L.rho0);
-        const R a = R(0.5) * (R(1) + phin);
         const R ar = sr / (sr + sb);
         const R gam = flag ? sr * sb / (sr + sb) * R(L.beta) : R(0);
         const bool evr = sr <= R(EVAP_LIM), evb = sb <= R(EVAP_LIM);
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             R dot = R(CX[i]) * nx + R(CY[i]) * ny + R(CZ[i]) * nz;
+            const R a = aw[i];
             R v = ar * f[i] + R(W[i]) * rho0 * (ar - a) + gam * R(W[i]) * dot;
             if (evr) v = -R(W[i]) * rho0 * a;
             if (evb) v = f[i] + R(W[i]) * rho0 * (R(1) - a);
@@ -424,13 +437,10 @@ __device__ __forceinline__ void collide_recolor(const Lat &L, const R r[Q], cons
 //   sphi[5]  phi planes x-2 .. x+2 with a 2-node halo in y and z
 //   sn[3]    unit normals of planes x-1 .. x+1 with a 1-node halo (the
 //            curvature stencil needs n at the 18 neighbours)
-//   sold[3]  phi_old planes x-1 .. x+1 with a 1-node halo: the shift that
-//            the upstream writers applied (shifted storage only)
 // Streaming is the pull gather from x - c_i (periodic wrap).
 template <class R, class P> struct K2Smem {
     P phi[5][TY + 4][TZ + 4];
     R n[3][3][TY + 2][TZ + 2];
-    P old[3][TY + 2][TZ + 2];
 };
 
 __device__ __forceinline__ int slot5(int p) { return (p + 10) % 5; }
@@ -443,7 +453,9 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const S *__restrict__ ri
 {
     __shared__ K2Smem<R, P> sm;
     const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
-    const int z0 = blockIdx.x * TZ, y0 = blockIdx.y * TY, x0 = blockIdx.z * xc;
+    unsigned bx, by, bz;
+    tile_of_block(L, bx, by, bz);
+    const int z0 = bx * TZ, y0 = by * TY, x0 = bz * xc;
     const int x1 = min(x0 + xc, L.nx);
     const int z = z0 + tz, y = y0 + ty;
     const bool active = z < L.nz && y < L.ny;
@@ -456,15 +468,6 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const S *__restrict__ ri
             int jy = e / (TZ + 4), jz = e - jy * (TZ + 4);
             int gy = wrapm(y0 - 2 + jy, L.ny), gz = wrapm(z0 - 2 + jz, L.nz);
             dst[jy][jz] = phi[gx + gy * L.nz + gz];
-        }
-    };
-    auto load_old = [&](int p) {
-        const int gx = wrapm(p, L.nx) * sx;
-        P(*dst)[TZ + 2] = sm.old[slot3(p)];
-        for (int e = tid; e < (TY + 2) * (TZ + 2); e += TZ * TY) {
-            int jy = e / (TZ + 2), jz = e - jy * (TZ + 2);
-            int gy = wrapm(y0 - 1 + jy, L.ny), gz = wrapm(z0 - 1 + jz, L.nz);
-            dst[jy][jz] = phio[gx + gy * L.nz + gz];
         }
     };
     // unit normal of plane p on the (TY+2) x (TZ+2) halo region (S:498-532)
@@ -489,8 +492,6 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const S *__restrict__ ri
     };
 
     for (int p = x0 - 2; p <= x0 + 1; ++p) load_phi(p);
-    if constexpr (SHIFT)
-        for (int p = x0 - 1; p <= x0; ++p) load_old(p);
     __syncthreads();
     normals(x0 - 1);
     normals(x0);
@@ -498,7 +499,6 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const S *__restrict__ ri
     for (int x = x0; x < x1; ++x) {
         __syncthreads(); // readers of the slots refilled below are done
         load_phi(x + 2);
-        if constexpr (SHIFT) load_old(x + 1);
         __syncthreads();
         normals(x + 1);
         __syncthreads();
@@ -540,19 +540,22 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const S *__restrict__ ri
         // pull gather = streaming (S:743-753)
         int xo[3], yo[3], zo[3];
         node_offsets(L, x, y, z, xo, yo, zo);
-        R r[Q], b[Q], aold[Q];
-        gather<S, R, P, SHIFT>(L, rin, bin, xo, yo, zo, r, b, aold);
+        R r[Q], b[Q];
+        gather<S, R, P, SHIFT>(L, rin, bin, xo, yo, zo, r, b);
+        const int n = x * sx + y * L.nz + z;
+        R ax = 0, aw[Q];
         if constexpr (SHIFT) {
+            ax = R(0.5) * (R(1) + R(phio[n]));
+            // shift of the slot read by x + c_i at the next step: that node's phi now
 #pragma unroll
             for (int i = 0; i < Q; ++i)
-                aold[i] = R(0.5) * (R(1) + R(sm.old[slot3(x - CX[i])][ty + 1 - CY[i]][tz + 1 - CZ[i]]));
+                aw[i] = R(0.5) * (R(1) + R(sm.phi[slot5(x + CX[i])][ty + 2 + CY[i]][tz + 2 + CZ[i]]));
         }
-        const Moments<R, SHIFT> mo = moments<R, SHIFT>(L, r, b, aold);
+        const Moments<R, SHIFT> mo = moments<R, SHIFT>(L, r, b, ax);
         const R phin = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
         R ro[Q], bo[Q];
-        collide_recolor<R, SHIFT>(L, r, b, mo, phin, gx, gy, gz, nx, ny, nz, flag, flag, d, ro, bo);
+        collide_recolor<R, SHIFT>(L, r, b, mo, phin, gx, gy, gz, nx, ny, nz, flag, flag, d, aw, ro, bo);
         const size_t N = (size_t)L.N;
-        const int n = x * sx + y * L.nz + z;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             StoreOf<S>::st(rout + (size_t)i * N + n, ro[i]);
@@ -613,8 +616,9 @@ template <class P> __device__ __forceinline__ float shift_a(P ph, int red)
     return red ? a : 1.0f - a;
 }
 
-// Pack one colour: S_i(y) = P_i(y + c_i) - w_i rho0 a(y) (inverse of the
-// periodic streaming permutation, so that the next pull gather returns P).
+// Pack one colour: S_i(y) = P_i(y + c_i) - w_i rho0 a(y + c_i) (inverse of
+// the periodic streaming permutation, so that the next pull gather returns P;
+// the shift is that of the reading node y + c_i).
 template <class S, class P, bool SHIFT>
 __global__ __launch_bounds__(256) void k_pack(Lat L, const double *__restrict__ aos, S *__restrict__ soa,
                                               const P *__restrict__ phio, int red)
@@ -627,17 +631,15 @@ __global__ __launch_bounds__(256) void k_pack(Lat L, const double *__restrict__ 
     node_offsets(L, x, y, z, xo, yo, zo);
     const int n = xo[1] + yo[1] + zo[1];
     const size_t N = (size_t)L.N;
-    double sh = 0.0;
-    if constexpr (SHIFT) sh = L.rho0 * (double)shift_a(phio[n], red);
     for (int i = 0; i < Q; ++i) {
         int src = xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
         double v = aos[(size_t)src * Q + i];
-        if constexpr (SHIFT) v = v - W[i] * sh;
+        if constexpr (SHIFT) v = v - W[i] * (L.rho0 * (double)shift_a(phio[src], red));
         StoreOf<S>::st(soa + (size_t)i * N + n, (decltype(StoreOf<S>::ld(soa)))v);
     }
 }
 
-// Unpack one colour: P_i(x) = S_i(x - c_i) + w_i rho0 a(x - c_i), as FP64 AoS.
+// Unpack one colour: P_i(x) = S_i(x - c_i) + w_i rho0 a(x), as FP64 AoS.
 template <class S, class P, bool SHIFT>
 __global__ __launch_bounds__(256) void k_unpack(Lat L, const S *__restrict__ soa, double *__restrict__ aos,
                                                 const P *__restrict__ phio, int red)
@@ -653,7 +655,7 @@ __global__ __launch_bounds__(256) void k_unpack(Lat L, const S *__restrict__ soa
     for (int i = 0; i < Q; ++i) {
         int src = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
         double v = double(StoreOf<S>::ld(soa + (size_t)i * N + src));
-        if constexpr (SHIFT) v = v + W[i] * (L.rho0 * (double)shift_a(phio[src], red));
+        if constexpr (SHIFT) v = v + W[i] * (L.rho0 * (double)shift_a(phio[n], red));
         aos[(size_t)n * Q + i] = v;
     }
 }
@@ -680,7 +682,7 @@ __device__ __forceinline__ double cyl_phi(const Lat &L, const CylIC &ic, int x, 
     return tanh((Rz - sqrt(dx * dx + dy * dy)) * ic.kprof);
 }
 
-// Pass 1: phi of the initial state, the shift reference of each slot.
+// Pass 1: phi of the initial state, the shift reference of each reader.
 template <class P>
 __global__ __launch_bounds__(256) void k_init_phi(Lat L, CylIC ic, P *__restrict__ phio)
 {
@@ -692,7 +694,8 @@ __global__ __launch_bounds__(256) void k_init_phi(Lat L, CylIC ic, P *__restrict
     phio[((size_t)x * L.ny + y) * L.nz + z] = P(cyl_phi(L, ic, x, y, z, &Rz));
 }
 
-// Pass 2: node x's equilibria P_i(x) = w_i rho_c go to slot S_i(x - c_i).
+// Pass 2: node x's equilibria P_i(x) = w_i rho_c go to slot S_i(x - c_i),
+// shifted by node x's own fraction (x reads that slot).
 template <class S, class P, bool SHIFT>
 __global__ __launch_bounds__(256) void k_init_pops(Lat L, CylIC ic, S *__restrict__ rs, S *__restrict__ bs,
                                                    const P *__restrict__ phio)
@@ -708,12 +711,13 @@ __global__ __launch_bounds__(256) void k_init_pops(Lat L, CylIC ic, S *__restric
     double ph = cyl_phi(L, ic, x, y, z, &Rz);
     double rho = ph > 0.0 ? L.rho0 + L.sigma / Rz * 3.0 : L.rho0;
     double rr = rho * (ph + 1.0) / 2.0, rb = rho - rr;
+    const P phs = P(ph); // the value pass 1 stored for this node
     for (int i = 0; i < Q; ++i) {
         int dst = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
         double vr = rr * W[i], vb = rb * W[i];
         if constexpr (SHIFT) {
-            vr -= W[i] * (L.rho0 * (double)shift_a(phio[dst], 1));
-            vb -= W[i] * (L.rho0 * (double)shift_a(phio[dst], 0));
+            vr -= W[i] * (L.rho0 * (double)shift_a(phs, 1));
+            vb -= W[i] * (L.rho0 * (double)shift_a(phs, 0));
         }
         StoreOf<S>::st(rs + (size_t)i * N + dst, (decltype(StoreOf<S>::ld(rs)))vr);
         StoreOf<S>::st(bs + (size_t)i * N + dst, (decltype(StoreOf<S>::ld(bs)))vb);

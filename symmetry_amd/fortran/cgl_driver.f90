@@ -1,0 +1,155 @@
+! cgl_driver.f90 -- host driver of the GPU lattice update.
+!
+! Rebuilds the reference's `program main` (Source1.f90 S:28-253) around
+! libcgl: the physical parameters keep the reference's names (sigma, rho0,
+! tau_R, tau_B, beta, Rad_c; S:57-84), host fields keep its layout
+! ri(0:18,zl,yl,xl), rho_N(zl,yl,xl), and the field dump keeps the format of
+! output_results (OUT_<t>.dat, S:898-924).  The time loop (S:215-252) becomes
+! cgl_step calls between output steps; fields cross to the host only at the
+! output cadence.
+!
+! Input: an optional namelist file (first command-line argument) with group
+! &cgl (see the declarations below).  The initial state is the reference's
+! initialization (S:850-896) for a liquid cylinder along z through the box
+! centre with nmodes cosine modes of wavelength zl.
+program cgl_driver
+  use iso_c_binding
+  use cgl
+  implicit none
+  integer :: xl = 32, yl = 32, zl = 128, nsteps = 2000, out_every = 100, prec = 0, xchunk = 0, nmodes = 1
+  double precision :: sigma = 0.01d0, rho0 = 1.0d0, tau_R = 1.0d0, tau_B = 1.0d0, beta = 0.7d0, Rad_c = 8.0d0
+  double precision :: amp(16) = 0.0d0
+  logical :: ic_on_device = .false., write_tecplot = .true., dump_binary = .false.
+  character(len=256) :: outdir = '.'
+  namelist /cgl/ xl, yl, zl, nsteps, out_every, prec, xchunk, sigma, rho0, tau_R, tau_B, beta, Rad_c, &
+                 nmodes, amp, ic_on_device, write_tecplot, dump_binary, outdir
+
+  double precision, allocatable :: ri(:,:,:,:), bi(:,:,:,:)
+  double precision, allocatable :: rho(:,:,:), rho_R(:,:,:), rho_B(:,:,:), rho_N(:,:,:), ux(:,:,:), uy(:,:,:), uz(:,:,:)
+  double precision :: wi(0:18), nu_R, nu_B, ms(3)
+  double precision, parameter :: PI = 4.0d0*datan(1.0d0)
+  type(cgl_params) :: p
+  type(c_ptr) :: h
+  character(len=512) :: arg
+  integer :: t, nchunk, u, ios
+
+  amp(1) = 0.8d0
+  if (command_argument_count() >= 1) then
+    call get_command_argument(1, arg)
+    open(newunit=u, file=trim(arg), status='old', action='read', iostat=ios)
+    if (ios /= 0) then
+      write(*,*) 'cannot open ', trim(arg)
+      error stop 2
+    end if
+    read(u, nml=cgl)
+    close(u)
+  end if
+
+  ! relaxation times to viscosities (inverse of S:71-72)
+  nu_R = (tau_R - 0.5d0)/3.0d0
+  nu_B = (tau_B - 0.5d0)/3.0d0
+  wi(0) = 1.0d0/3.0d0
+  wi(1:6) = 1.0d0/18.0d0
+  wi(7:18) = 1.0d0/36.0d0
+  write(*,*) 'box', xl, yl, zl, ' nu_R=', nu_R, ' nu_B=', nu_B, ' precision', prec
+
+  p%nx = xl; p%ny = yl; p%nz = zl
+  p%bc = CGL_BC_PERIODIC
+  p%precision = prec
+  p%npar = 0
+  p%xchunk = xchunk
+  p%device = 0
+  p%sigma = sigma; p%rho0 = rho0; p%nu_r = nu_R; p%nu_b = nu_B; p%beta = beta
+  call cgl_check(cgl_create(h, p), 'cgl_create')
+
+  allocate(rho(zl,yl,xl), rho_R(zl,yl,xl), rho_B(zl,yl,xl), rho_N(zl,yl,xl), ux(zl,yl,xl), uy(zl,yl,xl), uz(zl,yl,xl))
+  if (ic_on_device) then
+    call cgl_check(cgl_init_cylinder(h, dble(xl)/2.0d0 + 0.5d0, dble(yl)/2.0d0 + 0.5d0, Rad_c, &
+                   6.0d0*beta*0.134d0, int(nmodes, c_int32_t), amp), 'cgl_init_cylinder')
+  else
+    allocate(ri(0:18,zl,yl,xl), bi(0:18,zl,yl,xl))
+    call initial_state()
+    call cgl_check(cgl_upload_state(h, ri, bi), 'cgl_upload_state')
+    deallocate(ri, bi)
+  end if
+
+  t = 0
+  do
+    if (mod(t, out_every) == 0 .or. t == nsteps) then
+      call cgl_check(cgl_calmacros(h, rho, rho_R, rho_B, ux, uy, uz, rho_N), 'cgl_calmacros')
+      write(*,'(i8, 3(1x, es22.15))') t, sum(rho_R), sum(rho_B), maxval(sqrt(ux**2 + uy**2 + uz**2))
+      if (write_tecplot) call output_results(t)
+      if (dump_binary) call dump_fields(t)
+    end if
+    if (t >= nsteps) exit
+    nchunk = min(out_every - mod(t, out_every), nsteps - t)
+    call cgl_check(cgl_step(h, int(nchunk, c_int32_t)), 'cgl_step')
+    t = t + nchunk
+  end do
+  call cgl_check(cgl_sync(h), 'cgl_sync')
+  call cgl_check(cgl_destroy(h), 'cgl_destroy')
+
+contains
+
+  ! initialization (S:850-896) for a z-axis cylinder through the box centre
+  subroutine initial_state()
+    integer :: x, y, z, m
+    double precision :: x0, y0, dis, dis_x, star, plus, ph, rh, rr
+    x0 = dble(xl)/2.0d0 + 0.5d0
+    y0 = dble(yl)/2.0d0 + 0.5d0
+    do x = 1, xl
+      do y = 1, yl
+        do z = 1, zl
+          dis = (dble(x) - x0)**2 + (dble(y) - y0)**2
+          dis_x = 0.0d0
+          do m = 1, nmodes
+            dis_x = dis_x + amp(m)*cos(2.0d0*PI*dble(m)*(dble(z) - 1.0d0)/dble(zl))
+          end do
+          star = Rad_c + dis_x - sqrt(dis)
+          ph = tanh(star*6.0d0*beta*0.134d0)
+          plus = sigma/(Rad_c + dis_x)*3.0d0
+          if (ph > 0.0d0) then
+            rh = rho0 + plus
+          else
+            rh = rho0
+          end if
+          rr = rh*(ph + 1.0d0)/2.0d0
+          ri(:,z,y,x) = rr*wi*1.0d0
+          bi(:,z,y,x) = (rh - rr)*wi*1.0d0
+        end do
+      end do
+    end do
+  end subroutine initial_state
+
+  ! output_results (S:898-924): Tecplot POINT zone, x fastest
+  subroutine output_results(tt)
+    integer, intent(in) :: tt
+    integer :: x, y, z, uo
+    character(len=50) :: filename
+    write(filename, *) tt
+    open(newunit=uo, file=trim(outdir)//'/OUT_'//trim(adjustl(filename))//'.dat', status='unknown')
+    write(uo,*) 'TITLE="Contact_Angles"'
+    write(uo,*) 'VARIABLES=X,Y,Z,WALL,RHO_N,U,V,W,RHO'
+    write(uo,*) 'ZONE F=POINT I=', xl, 'J=', yl, 'K=', zl
+    do z = 1, zl
+      do y = 1, yl
+        do x = 1, xl
+          write(uo, "(4(1xI6),5(1xF12.8))") x, y, z, 0, rho_N(z,y,x), ux(z,y,x), uy(z,y,x), uz(z,y,x), rho(z,y,x)
+        end do
+      end do
+    end do
+    close(uo)
+  end subroutine output_results
+
+  ! raw FP64 fields (rho, rho_R, rho_B, ux, uy, uz, rho_N), stream access
+  subroutine dump_fields(tt)
+    integer, intent(in) :: tt
+    integer :: uo
+    character(len=50) :: filename
+    write(filename, *) tt
+    open(newunit=uo, file=trim(outdir)//'/FIELDS_'//trim(adjustl(filename))//'.bin', access='stream', &
+         form='unformatted', status='replace')
+    write(uo) rho, rho_R, rho_B, ux, uy, uz, rho_N
+    close(uo)
+  end subroutine dump_fields
+end program cgl_driver

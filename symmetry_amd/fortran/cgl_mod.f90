@@ -1,0 +1,93 @@
+! cgl_mod.f90 -- Fortran binding of libcgl's C ABI (include/cgl.h).
+!
+! The reference calls its lattice-update subroutines with implicit
+! interfaces and whole arrays by address (Source1.f90 S:216-251).  This
+! module gives the same call site explicit iso_c_binding interfaces to the
+! GPU library: host arrays keep the reference's layout, ri(0:18,zl,yl,xl)
+! and rho(zl,yl,xl), and are passed by address.
+module cgl
+  use iso_c_binding
+  implicit none
+
+  integer(c_int32_t), parameter :: CGL_BC_PERIODIC = 0, CGL_BC_WALL = 1, CGL_BC_MIRROR = 2
+  integer(c_int32_t), parameter :: CGL_PREC_F64 = 0, CGL_PREC_F32 = 1, CGL_PREC_F16 = 2
+
+  type, bind(C) :: cgl_params
+    integer(c_int32_t) :: nx, ny, nz
+    integer(c_int32_t) :: bc(6)
+    integer(c_int32_t) :: precision, npar, xchunk, device
+    real(c_double) :: sigma, rho0, nu_r, nu_b, beta
+  end type cgl_params
+
+  interface
+    integer(c_int) function cgl_create(h, p) bind(C, name="cgl_create")
+      import :: c_int, c_ptr, cgl_params
+      type(c_ptr), intent(out) :: h
+      type(cgl_params), intent(in) :: p
+    end function
+    integer(c_int) function cgl_destroy(h) bind(C, name="cgl_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cgl_upload_state(h, ri, bi) bind(C, name="cgl_upload_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: ri(*), bi(*)
+    end function
+    integer(c_int) function cgl_download_state(h, ri, bi) bind(C, name="cgl_download_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: ri(*), bi(*)
+    end function
+    integer(c_int) function cgl_init_cylinder(h, xc, yc, r0, kprof, nmodes, amp) bind(C, name="cgl_init_cylinder")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), value :: xc, yc, r0, kprof
+      integer(c_int32_t), value :: nmodes
+      real(c_double), intent(in) :: amp(*)
+    end function
+    integer(c_int) function cgl_step(h, nsteps) bind(C, name="cgl_step")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nsteps
+    end function
+    integer(c_int) function cgl_calmacros(h, rho, rho_r, rho_b, ux, uy, uz, phi) bind(C, name="cgl_calmacros")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: rho(*), rho_r(*), rho_b(*), ux(*), uy(*), uz(*), phi(*)
+    end function
+    integer(c_int) function cgl_sync(h) bind(C, name="cgl_sync")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cgl_time_steps(h, nsteps, ms_total, ms_moments, ms_collide) bind(C, name="cgl_time_steps")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nsteps
+      real(c_double), intent(out) :: ms_total, ms_moments, ms_collide
+    end function
+    type(c_ptr) function cgl_last_error_c() bind(C, name="cgl_last_error")
+      import :: c_ptr
+    end function
+  end interface
+
+contains
+
+  ! error stop with the library's message when rc /= 0 (the reference has no
+  ! error handling; the drop-in stops instead of continuing on bad state)
+  subroutine cgl_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc == 0) return
+    call c_f_pointer(cgl_last_error_c(), msg, [512])
+    n = 0
+    do while (n < 512)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    write(*, '(a, a, i0, a, 512a)') what, ': libcgl error ', rc, ': ', msg(1:n)
+    error stop 1
+  end subroutine cgl_check
+end module cgl

@@ -124,7 +124,11 @@ def test_reduced_precision_vs_f64(rp1, prec, tol_phi, tol_u):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
-def test_component_mass_conserved(rp1, prec):
+def test_mass_conserved(rp1, prec):
+    """Total mass is conserved to round-off.  Component masses are not
+    conserved by the reference itself: its recolouring hands colour mass below
+    EVAP_LIM=1e-8 to the other component (S:720-725), so their drift must equal
+    the FP64 path's."""
     cfg, ri, bi = rp1
     lat = _lattice(cfg, prec)
     lat.upload(ri, bi)
@@ -132,10 +136,17 @@ def test_component_mass_conserved(rp1, prec):
     lat.step(100)
     m1 = lat.calmacros()
     tol = {"f64": 1e-13, "f32": 1e-6, "f16": 1e-3}[prec]
+    a, b = m0["rho"].sum(), m1["rho"].sum()
+    _print(prec, "total mass", a, b, abs(b - a) / a)
+    assert abs(b - a) / a < tol
+    ref = _lattice(cfg, "f64")
+    ref.upload(ri, bi)
+    ref.step(100)
+    m64 = ref.calmacros()
     for k in ("rho_r", "rho_b"):
-        a, b = m0[k].sum(), m1[k].sum()
-        _print(prec, k, a, b, abs(b - a) / a)
-        assert abs(b - a) / a < tol
+        drift, drift64 = m1[k].sum() - m0[k].sum(), m64[k].sum() - m0[k].sum()
+        _print(prec, k, "drift", drift, "f64 drift", drift64)
+        assert abs(drift - drift64) <= tol * a
 
 
 def test_device_ic_matches_host_ic():
