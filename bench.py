@@ -29,9 +29,14 @@ HBM_PEAK = 8.0e12  # B/s, MI355X
 # Counted bytes per node per step (every red/blue population read and written
 # once, phi written and read once).
 B_ALG_STEP = {"f64": 624, "f32": 312, "f16": 160}
-# Algorithmic bytes per node of the fused collide kernel alone: 38 populations
-# read + 38 written + phi read once.
-B_ALG_COLLIDE = {"f64": 2 * 19 * 8 * 2 + 8, "f32": 2 * 19 * 4 * 2 + 4, "f16": 2 * 19 * 2 * 2 + 4}
+# Algorithmic bytes per node of the fused collide kernel alone.
+#  rb: 38 populations read + 38 written + phi read once.
+#  fs: 19 total populations and the per-node recolour scalars (4 x FP32, or
+#      6 x FP64) read and written, phi read once.
+B_ALG_COLLIDE = {
+    ("f64", "rb"): 2 * 19 * 8 * 2 + 8, ("f32", "rb"): 2 * 19 * 4 * 2 + 4, ("f16", "rb"): 2 * 19 * 2 * 2 + 4,
+    ("f64", "fs"): 2 * (19 * 8 + 48) + 8, ("f32", "fs"): 2 * (19 * 4 + 16) + 4, ("f16", "fs"): 2 * (19 * 2 + 16) + 4,
+}
 DTYPE = {"f64": "f64", "f32": "f32", "f16": "f16-storage/f32-compute"}
 
 
@@ -48,11 +53,11 @@ def cpu_baseline(seconds):
         return {"value": None, "error": str(e)}
 
 
-def traffic_from_profile(precision):
+def traffic_from_profile(precision, layout):
     """HBM bytes per launch of the fused kernel from committed rocprofv3 PMC
     summaries (profiles/), scaled by the copy-kernel calibration; None if
     absent."""
-    path = os.path.join(ROOT, "profiles", f"k_collide_{precision}_256x256x512_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"k_collide_{precision}_{layout}_256x256x512_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -65,7 +70,10 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="f32", choices=["f64", "f32", "f16"])
+    ap.add_argument("--variant-steps", type=int, default=50,
+                    help="timed steps for each other precision/layout variant (0 = skip)")
     ap.add_argument("--xchunk", type=int, default=0)
+    ap.add_argument("--layout", default="fs", choices=["rb", "fs"])
     ap.add_argument("--size", default="256x256x512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -81,12 +89,15 @@ def main():
     nx, ny, nz = (int(v) for v in args.size.split("x"))
     cfg = ic.rp_config3()
     lat = symmetry_amd.Lattice(nx, ny, nz, precision=args.precision, sigma=cfg["sigma"], rho0=cfg["rho0"],
-                               nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], xchunk=args.xchunk)
-    print(f"# device {lat.device_name()} box {nx}x{ny}x{nz} precision {args.precision}", file=sys.stderr, flush=True)
+                               nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], xchunk=args.xchunk,
+                               layout=args.layout)
+    print(f"# device {lat.device_name()} box {nx}x{ny}x{nz} precision {args.precision} layout {args.layout}",
+          file=sys.stderr, flush=True)
     copy_gbps = symmetry_amd.copy_bandwidth(2 << 30, 10)
     lat.init_cylinder(nx / 2 + 0.5, ny / 2 + 0.5, cfg["r0"], 6.0 * cfg["beta"] * 0.134, cfg["amps"])
-    lat.step(args.warmup)
+    lat.step(args.warmup)  # with --xchunk 0 the first warm-up steps also autotune the x chunk
     lat.sync()
+    xchunk, still_tuning = lat.tuned_xchunk()
     nodes = nx * ny * nz
     # timed region: exactly `steps` steps, synchronised on both sides
     t0 = time.perf_counter()
@@ -99,9 +110,30 @@ def main():
     ev_total, ms_mom, ms_col = lat.time_steps(ksteps)
     avg_col = ms_col / ksteps * 1e-3
     avg_mom = ms_mom / ksteps * 1e-3
-    achieved = B_ALG_COLLIDE[args.precision] * nodes / avg_col
-    traffic = traffic_from_profile(args.precision)
+    b_col = B_ALG_COLLIDE[(args.precision, args.layout)]
+    achieved = b_col * nodes / avg_col
+    traffic = traffic_from_profile(args.precision, args.layout)
     lat.close()
+
+    # the same workload in every precision and layout (MLUPS only)
+    variants = {}
+    if args.variant_steps > 0:
+        for prec in ("f64", "f32", "f16"):
+            for lay in ("rb", "fs"):
+                key = f"{prec}-{lay}"
+                if prec == args.precision and lay == args.layout:
+                    variants[key] = round(mlups, 1)
+                    continue
+                v = symmetry_amd.Lattice(nx, ny, nz, precision=prec, sigma=cfg["sigma"], rho0=cfg["rho0"],
+                                         nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], layout=lay)
+                v.init_cylinder(nx / 2 + 0.5, ny / 2 + 0.5, cfg["r0"], 6.0 * cfg["beta"] * 0.134, cfg["amps"])
+                v.step(20)
+                v.sync()
+                t1 = time.perf_counter()
+                v.step(args.variant_steps)
+                v.sync()
+                variants[key] = round(nodes * args.variant_steps / (time.perf_counter() - t1) / 1e6, 1)
+                v.close()
 
     out = {
         "metric": "MLUPS at 256x256x512 D3Q19 CG-MRT, FP32 vs FP64, % of HBM roofline",
@@ -118,16 +150,18 @@ def main():
         "data": "synthetic (seeded liquid-cylinder initial state generated on the device)",
         "config": {"workload": f"Rayleigh-Plateau {nx}x{ny}x{nz} periodic D3Q19 colour-gradient MRT, R=40, "
                                "8 modes seed 1234, tau_r=1.0 tau_b=0.7, sigma=0.01, beta=0.7",
-                   "precision": args.precision, "nodes": nodes, "xchunk": args.xchunk or 8},
+                   "precision": args.precision, "layout": args.layout, "nodes": nodes,
+                   "xchunk": xchunk, "xchunk_autotuned": args.xchunk == 0 and not still_tuning},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "kernel": "k_collide (fused derivatives+collision+recolour+streaming)",
-                     "alg_bytes_per_node": B_ALG_COLLIDE[args.precision],
+                     "alg_bytes_per_node": b_col,
                      "avg_launch_ms": round(avg_col * 1e3, 4), "moments_avg_launch_ms": round(avg_mom * 1e3, 4)},
         "step_roofline": {"bytes_per_node": B_ALG_STEP[args.precision],
                           "frac": round(mlups * 1e6 * B_ALG_STEP[args.precision] / HBM_PEAK, 4),
                           "event_mlups": round(nodes * ksteps / (ev_total * 1e-3) / 1e6, 1)},
         "hbm_copy_gbps": round(copy_gbps, 1),
+        "variants_mlups": variants,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

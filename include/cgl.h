@@ -40,6 +40,10 @@ extern "C" {
 #define CGL_PREC_F32 1 /* FP32 storage shifted by w_i*rho0*(1 +- phi)/2 of the reading node, FP32 collision */
 #define CGL_PREC_F16 2 /* FP16 shifted storage, FP32 phi and collision */
 
+#define CGL_LAYOUT_RB 0 /* red and blue populations, 2 x 19 structure-of-arrays */
+#define CGL_LAYOUT_FS 1 /* factorised: total populations (19 SoA) + per-node recolour scalars,
+                           r_i = A f_i + w_i G.c_i (the form Recolor produces, S:713-726) */
+
 typedef struct cgl_handle cgl_t;
 
 /* Physical and numerical parameters.  Names follow the reference's module
@@ -49,8 +53,10 @@ typedef struct cgl_params {
     int32_t bc[6];       /* faces x-lo, x-hi, y-lo, y-hi, z-lo, z-hi: CGL_BC_* */
     int32_t precision;   /* CGL_PREC_* */
     int32_t npar;        /* number of particles (par, S:24); 0 = fluid only */
-    int32_t xchunk;      /* x planes per workgroup of the fused kernel; 0 = auto */
+    int32_t xchunk;      /* x planes per workgroup of the fused kernel; 0 = autotune on the first steps */
     int32_t device;      /* HIP device ordinal */
+    int32_t layout;      /* CGL_LAYOUT_*: device storage of the populations */
+    int32_t reserved;    /* must be 0 */
     double sigma;        /* CSF surface-tension coefficient (sigma, S:66) */
     double rho0;         /* reference density (rho0, S:67) */
     double nu_r, nu_b;   /* kinematic viscosities of red and blue (S:69-70) */
@@ -91,6 +97,11 @@ int cgl_calmacros(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *u
  * (derivatives, S:498-533).  Any pointer may be NULL. */
 int cgl_derivatives(cgl_t *h, double *gradx, double *grady, double *gradz, double *n_x, double *n_y, double *n_z,
                     int32_t *flag);
+
+/* The fused kernel's x chunk in use and whether the autotuner (xchunk = 0)
+ * is still timing candidates on the first steps.  No counterpart in the
+ * reference. */
+int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning);
 
 /* Block until all queued device work of the handle has finished. */
 int cgl_sync(cgl_t *h);

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libcgl.so")
 
 BC_PERIODIC, BC_WALL, BC_MIRROR = 0, 1, 2
 PREC = {"f64": 0, "f32": 1, "f16": 2}
+LAYOUT = {"rb": 0, "fs": 1}
 Q = 19
 
 
@@ -34,6 +35,7 @@ class Params(ctypes.Structure):
         ("bc", ctypes.c_int32 * 6),
         ("precision", ctypes.c_int32), ("npar", ctypes.c_int32),
         ("xchunk", ctypes.c_int32), ("device", ctypes.c_int32),
+        ("layout", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("sigma", ctypes.c_double), ("rho0", ctypes.c_double),
         ("nu_r", ctypes.c_double), ("nu_b", ctypes.c_double), ("beta", ctypes.c_double),
     ]
@@ -45,7 +47,7 @@ _lib = None
 EXPORTS = (
     "cgl_create", "cgl_destroy", "cgl_upload_state", "cgl_download_state", "cgl_init_cylinder",
     "cgl_step", "cgl_calmacros", "cgl_derivatives", "cgl_sync", "cgl_time_steps",
-    "cgl_device_name", "cgl_last_error", "cgl_probe_copy_bandwidth",
+    "cgl_device_name", "cgl_last_error", "cgl_probe_copy_bandwidth", "cgl_tuned_xchunk",
 )
 
 
@@ -70,6 +72,7 @@ def lib():
         L.cgl_sync.argtypes = [vp]
         L.cgl_time_steps.argtypes = [vp, ctypes.c_int32] + [ctypes.POINTER(ctypes.c_double)] * 3
         L.cgl_device_name.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32]
+        L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.cgl_probe_copy_bandwidth.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                                ctypes.POINTER(ctypes.c_double)]
         _lib = L
@@ -97,7 +100,7 @@ class Lattice:
     0-based indices: populations ``[x, y, z, i]``, scalars ``[x, y, z]``."""
 
     def __init__(self, nx, ny, nz, precision="f64", sigma=0.01, rho0=1.0, nu_r=1.0 / 6.0, nu_b=1.0 / 6.0,
-                 beta=0.7, bc=(BC_PERIODIC,) * 6, xchunk=0, device=0):
+                 beta=0.7, bc=(BC_PERIODIC,) * 6, xchunk=0, device=0, layout="rb"):
         p = Params()
         p.nx, p.ny, p.nz = nx, ny, nz
         for k, v in enumerate(bc):
@@ -106,6 +109,7 @@ class Lattice:
         p.npar = 0
         p.xchunk = xchunk
         p.device = device
+        p.layout = LAYOUT[layout] if isinstance(layout, str) else int(layout)
         p.sigma, p.rho0, p.nu_r, p.nu_b, p.beta = sigma, rho0, nu_r, nu_b, beta
         self.params = p
         self.shape = (nx, ny, nz)
@@ -162,6 +166,12 @@ class Lattice:
         t = [ctypes.c_double() for _ in range(3)]
         _check(lib().cgl_time_steps(self.h, int(n), *(ctypes.byref(v) for v in t)))
         return tuple(v.value for v in t)
+
+    def tuned_xchunk(self):
+        """(x chunk in use, True while the autotuner is still running)."""
+        xc, tu = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().cgl_tuned_xchunk(self.h, ctypes.byref(xc), ctypes.byref(tu)))
+        return xc.value, bool(tu.value)
 
     def device_name(self):
         buf = ctypes.create_string_buffer(256)

@@ -127,6 +127,11 @@ int cgl_create(cgl_t **out, const cgl_params *p)
     if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(CGL_ERR_ARG, "box must be at least 1x1x1");
     double nn = (double)p->nx * p->ny * p->nz;
     if (nn > 2.0e9) return fail(CGL_ERR_ARG, "box of %.3g nodes exceeds the 32-bit node index", nn);
+    // kernels address each per-direction array (and the 16-48 B per-node
+    // recolour scalars) through 32-bit byte offsets
+    const double eb = p->precision == CGL_PREC_F64 ? 48.0 : 16.0;
+    if (nn * eb >= 4294967296.0)
+        return fail(CGL_ERR_UNSUPPORTED, "box of %.3g nodes exceeds the 4 GiB per-array addressing", nn);
     if (p->precision < CGL_PREC_F64 || p->precision > CGL_PREC_F16)
         return fail(CGL_ERR_ARG, "unknown precision %d", p->precision);
     for (int f = 0; f < 6; ++f)
@@ -134,6 +139,8 @@ int cgl_create(cgl_t **out, const cgl_params *p)
             return fail(CGL_ERR_UNSUPPORTED, "face %d: only periodic faces are implemented on the GPU", f);
     if (p->npar != 0) return fail(CGL_ERR_UNSUPPORTED, "particles are not implemented on the GPU");
     if (!(p->rho0 > 0) || !(p->nu_r > 0) || !(p->nu_b > 0)) return fail(CGL_ERR_ARG, "rho0, nu_r, nu_b must be > 0");
+    if (p->layout != CGL_LAYOUT_RB && p->layout != CGL_LAYOUT_FS) return fail(CGL_ERR_ARG, "unknown layout %d", p->layout);
+    if (p->reserved != 0) return fail(CGL_ERR_ARG, "reserved field must be 0");
 
     cgl_handle *h = new (std::nothrow) cgl_handle();
     if (!h) return fail(CGL_ERR_ARG, "out of host memory");
@@ -156,6 +163,9 @@ int cgl_create(cgl_t **out, const cgl_params *p)
     h->esize = p->precision == CGL_PREC_F64 ? 8 : (p->precision == CGL_PREC_F32 ? 4 : 2);
     h->phisize = p->precision == CGL_PREC_F64 ? 8 : 4;
     h->xc = p->xchunk > 0 ? p->xchunk : 8;
+    h->tuning = p->xchunk > 0 ? 0 : 1;
+    h->layout = p->layout;
+    h->state_layout = cgl::RB;
     int rc = set_device(h);
     if (rc) {
         delete h;
@@ -163,8 +173,16 @@ int cgl_create(cgl_t **out, const cgl_params *p)
     }
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&h->ev[k]);
-    for (int b = 0; b < 2 && e == hipSuccess; ++b)
+    // RB: both population buffers.  FS: one RB buffer (staging of uploaded or
+    // initial states) plus the factorised double buffer.
+    const int nrb = p->layout == CGL_LAYOUT_RB ? 2 : 1;
+    for (int b = 0; b < nrb && e == hipSuccess; ++b)
         for (int c = 0; c < 2 && e == hipSuccess; ++c) e = hipMalloc(&h->pops[b][c], (size_t)cgl::Q * h->N * h->esize);
+    if (p->layout == CGL_LAYOUT_FS) {
+        const size_t ns = p->precision == CGL_PREC_F64 ? 6 * sizeof(double) : 4 * sizeof(float);
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc(&h->fsf[b], (size_t)cgl::Q * h->N * h->esize);
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc(&h->fss[b], h->N * ns);
+    }
     for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc(&h->phi[b], h->N * h->phisize);
     if (e == hipSuccess) e = hipMemset(h->phi[0], 0, h->N * h->phisize);
     if (e == hipSuccess) e = hipMemset(h->phi[1], 0, h->N * h->phisize);
@@ -186,6 +204,8 @@ int cgl_destroy(cgl_t *h)
         for (int c = 0; c < 2; ++c)
             if (h->pops[b][c]) (void)hipFree(h->pops[b][c]);
         if (h->phi[b]) (void)hipFree(h->phi[b]);
+        if (h->fsf[b]) (void)hipFree(h->fsf[b]);
+        if (h->fss[b]) (void)hipFree(h->fss[b]);
     }
     if (h->mac) (void)hipFree(h->mac);
     if (h->aos) (void)hipFree(h->aos);
@@ -210,6 +230,8 @@ int cgl_upload_state(cgl_t *h, const double *ri, const double *bi)
     hipError_t e = hipMemcpyAsync(h->aos, ri, bytes, hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(tmp, bi, bytes, hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) {
+        h->cur = 0; // uploaded states are held in RB form in buffer 0
+        h->state_layout = cgl::RB;
         dispatch(h, [&](auto impl) {
             decltype(impl)::phi_aos(h, h->aos, tmp);
             decltype(impl)::pack(h, h->aos, 1);
@@ -252,9 +274,43 @@ int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, i
     ic.kprof = kprof;
     ic.nmodes = nmodes;
     for (int m = 0; m < nmodes; ++m) ic.amp[m] = amp[m];
+    h->cur = 0; // initial states are held in RB form in buffer 0
+    h->state_layout = cgl::RB;
     dispatch(h, [&](auto impl) { decltype(impl)::init_cylinder(h, ic); });
     if ((rc = check_launch("init_cylinder"))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
+    return CGL_OK;
+}
+
+// Autotuner of the fused kernel's x chunk (planes per workgroup).  Results do
+// not depend on the chunk, so the first steps of a handle serve as the timing
+// runs: each candidate advances TUNE_STEPS real steps between two events.
+static const int TUNE_XC[] = {2, 4, 8, 16, 32, 64};
+static const int TUNE_N = sizeof(TUNE_XC) / sizeof(TUNE_XC[0]);
+static const int TUNE_STEPS = 2;
+
+static int tune_some(cgl_handle *h, int32_t *left)
+{
+    while (h->tuning && *left >= TUNE_STEPS) {
+        h->xc = TUNE_XC[h->tune_idx];
+        HIPCHK(hipEventRecord(h->ev[0], h->stream));
+        for (int t = 0; t < TUNE_STEPS; ++t) enqueue_step(h);
+        HIPCHK(hipEventRecord(h->ev[1], h->stream));
+        int rc = check_launch("step");
+        if (rc) return rc;
+        HIPCHK(hipEventSynchronize(h->ev[1]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+        if (h->tune_idx == 0 || ms < h->tune_best_ms) {
+            h->tune_best_ms = ms;
+            h->tune_best_xc = h->xc;
+        }
+        *left -= TUNE_STEPS;
+        if (++h->tune_idx == TUNE_N || TUNE_XC[h->tune_idx] >= 2 * h->p.nx) {
+            h->tuning = 0;
+            h->xc = h->tune_best_xc;
+        }
+    }
     return CGL_OK;
 }
 
@@ -263,10 +319,19 @@ int cgl_step(cgl_t *h, int32_t nsteps)
     if (!h || nsteps < 0) return fail(CGL_ERR_ARG, "bad arguments");
     int rc = set_device(h);
     if (rc) return rc;
+    if ((rc = tune_some(h, &nsteps))) return rc;
     for (int t = 0; t < nsteps; ++t) {
         enqueue_step(h);
         if ((rc = check_launch("step"))) return rc;
     }
+    return CGL_OK;
+}
+
+int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning)
+{
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    if (xchunk) *xchunk = h->xc;
+    if (tuning) *tuning = h->tuning;
     return CGL_OK;
 }
 

@@ -44,19 +44,53 @@ struct Lat {
     double rho0, sigma, beta, nu_r, nu_b;
 };
 
+// Element access through a 32-bit byte offset from a wave-uniform base: every
+// per-direction array is below 4 GiB (checked at creation), so loads and
+// stores use the scalar-base + 32-bit vector-offset addressing of gfx950
+// instead of 64-bit vector address arithmetic.
+template <class T> __device__ __forceinline__ const T &at(const T *base, unsigned idx)
+{
+    return *(const T *)((const char *)base + idx * (unsigned)sizeof(T));
+}
+template <class T> __device__ __forceinline__ T &at(T *base, unsigned idx)
+{
+    return *(T *)((char *)base + idx * (unsigned)sizeof(T));
+}
+
 template <class S> struct StoreOf;
 template <> struct StoreOf<double> {
     __device__ static double ld(const double *p) { return *p; }
     __device__ static void st(double *p, double v) { *p = v; }
+    __device__ static double ld(const double *p, unsigned i) { return at(p, i); }
+    __device__ static void st(double *p, unsigned i, double v) { at(p, i) = v; }
 };
 template <> struct StoreOf<float> {
     __device__ static float ld(const float *p) { return *p; }
     __device__ static void st(float *p, float v) { *p = v; }
+    __device__ static float ld(const float *p, unsigned i) { return at(p, i); }
+    __device__ static void st(float *p, unsigned i, float v) { at(p, i) = v; }
 };
 template <> struct StoreOf<_Float16> {
     __device__ static float ld(const _Float16 *p) { return (float)*p; }
     __device__ static void st(_Float16 *p, float v) { *p = (_Float16)v; }
+    __device__ static float ld(const _Float16 *p, unsigned i) { return (float)at(p, i); }
+    __device__ static void st(_Float16 *p, unsigned i, float v) { at(p, i) = (_Float16)v; }
 };
+
+// Arithmetic helpers: FP64 keeps IEEE division and square root (the FP64 path
+// reproduces the CPU path bit for bit); FP32 uses the hardware reciprocal
+// (1 ulp), multiplies by compile-time reciprocals of constants, and rsqrt.
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double fsqrt(double v) { return __builtin_sqrt(v); }
+__device__ __forceinline__ float fsqrt(float v) { return __builtin_sqrtf(v); }
+template <class R> __device__ __forceinline__ R div_c(R a, double c)
+{
+    if constexpr (sizeof(R) == 8)
+        return a / R(c);
+    else
+        return a * R(1.0 / c);
+}
 
 __device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 __device__ __forceinline__ int wrapm(int v, int n)
@@ -81,58 +115,144 @@ __device__ __forceinline__ void tile_of_block(const Lat &L, unsigned &bx, unsign
     bz = lin / (gx * gy);
 }
 
-// Neumaier compensated sum.
-template <class R> struct CSum {
-    R s = R(0), c = R(0);
-    __device__ __forceinline__ void add(R x)
-    {
-        R t = s + x;
-        if (__builtin_fabs(s) >= __builtin_fabs(x))
-            c += (s - t) + x;
-        else
-            c += (x - t) + s;
-        s = t;
-    }
-    __device__ __forceinline__ R get() const { return s + c; }
+// ------------------------------------------------------------ state layouts
+// RB  red and blue populations, 2 x 19 SoA arrays (the reference's ri, bi
+//     right before streaming, S:249-250).
+// FS  factorised red/blue: the post-collision total f_i (19 SoA arrays) and
+//     per-node recolour scalars such that the node's red populations are
+//       r_i = A f_i + w_i (G . c_i),   b_i = f_i - r_i,
+//     which is exactly the form Recolor produces (S:713-726), so the factored
+//     state holds the same red/blue populations in 19 + (4 or 6) values.
+//     FP64 keeps A, gamma and n separately, G = gamma n, so a reader rebuilds
+//     r_i with the reference's own rounding: r = A*f + (gamma*w_i)*(n . c_i).
+enum { RB = 0, FS = 1 };
+template <bool SHIFT> struct FsScal;
+template <> struct FsScal<false> { static constexpr int NS = 6; }; // A, gamma, nx, ny, nz, unused (FP64)
+template <> struct FsScal<true> { static constexpr int NS = 4; };  // A, Gx, Gy, Gz (FP32)
+
+// n . c_i with zero terms skipped, in the reference's order (S:717).
+template <class R> __device__ __forceinline__ R dot_c(int i, R nx, R ny, R nz)
+{
+    R dot = 0;
+    bool first = true;
+    if (CX[i] != 0) { dot = CX[i] == 1 ? nx : -nx; first = false; }
+    if (CY[i] != 0) { R t = CY[i] == 1 ? ny : -ny; dot = first ? t : dot + t; first = false; }
+    if (CZ[i] != 0) { R t = CZ[i] == 1 ? nz : -nz; dot = first ? t : dot + t; }
+    return dot;
+}
+
+// Gathered node values.
+//  FP64: r, b = the reference's post-stream red and blue populations.
+//  shifted: ft_i = f_i - w_i rho0 (deviation of the total) and sr = rho_R.
+template <class R, bool SHIFT> struct Node;
+template <class R> struct Node<R, false> {
+    R r[Q], b[Q];
+};
+template <class R> struct Node<R, true> {
+    R ft[Q];
+    R sr;
 };
 
-// Pull gather of the 19 red and blue populations at node (x, y, z), with
-// unshifting.  off[d][k] is the wrapped index offset of coordinate d at
-// displacement k-1.
-template <class S, class R, class P, bool SHIFT>
-__device__ __forceinline__ void gather(const Lat &L, const S *__restrict__ rs, const S *__restrict__ bs,
-                                       const int xo[3], const int yo[3], const int zo[3], R r[Q], R b[Q])
+// Pull gather at node (x, y, z) from x - c_i (streaming, S:743-753).
+// ax: red shift fraction of this node (RB shifted storage only).
+template <int LIN, class S, class R, bool SHIFT>
+__device__ __forceinline__ void gather(const Lat &L, const void *p0, const void *p1, const unsigned xo[3],
+                                       const unsigned yo[3], const unsigned zo[3], R ax, Node<R, SHIFT> &nd)
 {
     const size_t N = (size_t)L.N;
+    const R rho0 = R(L.rho0);
+    if constexpr (LIN == RB) {
+        const S *rs = (const S *)p0, *bs = (const S *)p1;
+        if constexpr (!SHIFT) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        int m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-        r[i] = StoreOf<S>::ld(rs + (size_t)i * N + m);
-        b[i] = StoreOf<S>::ld(bs + (size_t)i * N + m);
+            for (int i = 0; i < Q; ++i) {
+                const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+                const S *ri = rs + (size_t)i * N, *bi = bs + (size_t)i * N;
+                nd.r[i] = StoreOf<S>::ld(ri, m);
+                nd.b[i] = StoreOf<S>::ld(bi, m);
+            }
+        } else {
+            // deviations from w_i rho0 ax and w_i rho0 (1 - ax): small, summed plainly
+            R sr = 0;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+                const S *ri = rs + (size_t)i * N, *bi = bs + (size_t)i * N;
+                R rv = StoreOf<S>::ld(ri, m);
+                R bv = StoreOf<S>::ld(bi, m);
+                sr += rv;
+                nd.ft[i] = rv + bv;
+            }
+            nd.sr = sr + rho0 * ax;
+        }
+    } else {
+        const S *fs = (const S *)p0;
+        if constexpr (!SHIFT) {
+            const double *sc = (const double *)p1;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+                const S *fi = fs + (size_t)i * N;
+                R f = StoreOf<S>::ld(fi, m);
+                const double2 s01 = at((const double2 *)sc, 3 * m);
+                R r = s01.x * f;
+                if (i > 0) {
+                    const double2 s23 = at((const double2 *)sc, 3 * m + 1);
+                    const double s4 = at(sc, 6 * m + 4);
+                    r = r + (s01.y * R(W[i])) * dot_c<R>(i, s23.x, s23.y, s4);
+                }
+                nd.r[i] = r;
+                nd.b[i] = f - r;
+            }
+        } else {
+            // rho_R = rho0 sum_i w_i A_i + sum_i A_i f~_i + sum_i w_i G_i.c_i: the
+            // O(1) part is summed per weight class, the rest are small deviations.
+            const float4 *sc = (const float4 *)p1;
+            R a0 = 0, a1 = 0, a2 = 0, dev = 0, g1 = 0, g2 = 0;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+                const S *fi = fs + (size_t)i * N;
+                R f = StoreOf<S>::ld(fi, m);
+                const float4 g = at(sc, m);
+                dev += R(g.x) * f;
+                R gc = 0;
+                if (CX[i] == 1) gc += R(g.y);
+                if (CX[i] == -1) gc -= R(g.y);
+                if (CY[i] == 1) gc += R(g.z);
+                if (CY[i] == -1) gc -= R(g.z);
+                if (CZ[i] == 1) gc += R(g.w);
+                if (CZ[i] == -1) gc -= R(g.w);
+                if (i == 0) a0 = R(g.x);
+                else if (i <= 6) { a1 += R(g.x); g1 += gc; }
+                else { a2 += R(g.x); g2 += gc; }
+                nd.ft[i] = f;
+            }
+            nd.sr = rho0 * (a0 * R(W[0]) + a1 * R(W[1]) + a2 * R(W[7])) + dev + (g1 * R(W[1]) + g2 * R(W[7]));
+        }
     }
 }
 
 // Moments of the gathered populations.
-//  FP64 path: calmacros (S:792-799) operation for operation.
-//  Shifted path: r, b hold deviations from w_i*rho0*ax and w_i*rho0*(1-ax),
-//  ax = (1 + phi_old(x))/2 of this (reading) node.
+//  FP64: calmacros (S:792-799) operation for operation.
+//  shifted: compensated sums of the deviations.
 template <class R, bool SHIFT> struct Moments {
-    R sr, sb, rho, drho, ux, uy, uz, phi;
+    R sr, sb, rho, ux, uy, uz, phi;
 };
 
 template <class R, bool SHIFT>
-__device__ __forceinline__ Moments<R, SHIFT> moments(const Lat &L, const R r[Q], const R b[Q], R ax)
+__device__ __forceinline__ Moments<R, SHIFT> moments(const Lat &L, const Node<R, SHIFT> &nd)
 {
     Moments<R, SHIFT> m;
     if constexpr (!SHIFT) {
         R f[Q];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = r[i] + b[i];
+        for (int i = 0; i < Q; ++i) f[i] = nd.r[i] + nd.b[i];
         R sr = 0, sb = 0;
 #pragma unroll
-        for (int i = 0; i < Q; ++i) sr += r[i];
+        for (int i = 0; i < Q; ++i) sr += nd.r[i];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) sb += b[i];
+        for (int i = 0; i < Q; ++i) sb += nd.b[i];
         R jx = 0, jy = 0, jz = 0;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
@@ -152,28 +272,18 @@ __device__ __forceinline__ Moments<R, SHIFT> moments(const Lat &L, const R r[Q],
         m.sr = sr;
         m.sb = sb;
         m.rho = sr + sb;
-        m.drho = m.rho - R(L.rho0);
         m.ux = jx / m.rho;
         m.uy = jy / m.rho;
         m.uz = jz / m.rho;
-        m.phi = (sr - sb) / m.rho;
+        m.phi = (sr - sb) / m.rho;  // FP64 path: IEEE division as the reference
     } else {
-        CSum<R> cr, cb, cf;
+        // the density deviation is the sum of the stored deviations: the O(1)
+        // part rho0 is never summed, so no compensation is needed
+        R df = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            cr.add(r[i]);
-            cb.add(b[i]);
-            cf.add(r[i] + b[i]);
-        }
-        const R rho0 = R(L.rho0);
-        m.sr = cr.get() + rho0 * ax;
-        m.sb = cb.get() + rho0 * (R(1) - ax);
-        m.drho = cf.get();
-        m.rho = rho0 + m.drho;
-        R jx = 0, jy = 0, jz = 0;
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            R f = r[i] + b[i];
+            const R f = nd.ft[i];
+            df += f;
             if (CX[i] == 1) jx += f;
             if (CX[i] == -1) jx -= f;
             if (CY[i] == 1) jy += f;
@@ -181,31 +291,36 @@ __device__ __forceinline__ Moments<R, SHIFT> moments(const Lat &L, const R r[Q],
             if (CZ[i] == 1) jz += f;
             if (CZ[i] == -1) jz -= f;
         }
-        m.ux = jx / m.rho;
-        m.uy = jy / m.rho;
-        m.uz = jz / m.rho;
-        m.phi = (m.sr - m.sb) / (m.sr + m.sb);
+        m.rho = R(L.rho0) + df;
+        m.sr = nd.sr;
+        m.sb = m.rho - m.sr;
+        const R irho = fdiv(R(1), m.rho);
+        m.ux = jx * irho;
+        m.uy = jy * irho;
+        m.uz = jz * irho;
+        m.phi = (m.sr - m.sb) * irho;
     }
     return m;
 }
 
 // Node offsets (already multiplied by strides) for coordinates c-1, c, c+1.
-__device__ __forceinline__ void node_offsets(const Lat &L, int x, int y, int z, int xo[3], int yo[3], int zo[3])
+__device__ __forceinline__ void node_offsets(const Lat &L, int x, int y, int z, unsigned xo[3], unsigned yo[3],
+                                             unsigned zo[3])
 {
-    const int sx = L.ny * L.nz;
+    const unsigned sx = (unsigned)L.ny * (unsigned)L.nz;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        xo[k] = wrap1(x + k - 1, L.nx) * sx;
-        yo[k] = wrap1(y + k - 1, L.ny) * L.nz;
-        zo[k] = wrap1(z + k - 1, L.nz);
+        xo[k] = (unsigned)wrap1(x + k - 1, L.nx) * sx;
+        yo[k] = (unsigned)wrap1(y + k - 1, L.ny) * (unsigned)L.nz;
+        zo[k] = (unsigned)wrap1(z + k - 1, L.nz);
     }
 }
 
 // ---------------------------------------------------------------- K1 moments
 // Writes phi of the step.  With FULL it also writes the macroscopic fields
 // rho, rho_R, rho_B, ux, uy, uz, phi (bare velocity, as calmacros) as FP64.
-template <class S, class R, class P, bool SHIFT, bool FULL>
-__global__ __launch_bounds__(256) void k_moments(Lat L, const S *__restrict__ rs, const S *__restrict__ bs,
+template <int LIN, class S, class R, class P, bool SHIFT, bool FULL>
+__global__ __launch_bounds__(256) void k_moments(Lat L, const void *__restrict__ p0, const void *__restrict__ p1,
                                                  const P *__restrict__ phio, P *__restrict__ phi,
                                                  double *__restrict__ mac)
 {
@@ -215,15 +330,15 @@ __global__ __launch_bounds__(256) void k_moments(Lat L, const S *__restrict__ rs
     const int y = by * TY + threadIdx.y;
     const int x = bz;
     if (z >= L.nz || y >= L.ny) return;
-    int xo[3], yo[3], zo[3];
+    unsigned xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
-    R r[Q], b[Q];
-    gather<S, R, P, SHIFT>(L, rs, bs, xo, yo, zo, r, b);
-    const int n = xo[1] + yo[1] + zo[1];
+    const unsigned n = xo[1] + yo[1] + zo[1];
     R ax = 0;
-    if constexpr (SHIFT) ax = R(0.5) * (R(1) + R(phio[n]));
-    Moments<R, SHIFT> mo = moments<R, SHIFT>(L, r, b, ax);
-    phi[n] = P(mo.phi);
+    if constexpr (SHIFT && LIN == RB) ax = R(0.5) * (R(1) + R(at(phio, n)));
+    Node<R, SHIFT> nd;
+    gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd);
+    Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
+    at(phi, n) = P(mo.phi);
     if constexpr (FULL) {
         const size_t N = (size_t)L.N;
         mac[n] = mo.rho;
@@ -262,16 +377,13 @@ __device__ __forceinline__ void grad19(F phi_at, R &gx, R &gy, R &gz)
 
 // ------------------------------------------------------------- collision (a)
 // CSF force from the curvature of the normals (S:624-667), velocity shift,
-// viscosity interpolation and MRT relaxation with Guo forcing (S:669-689),
-// recolouring (S:708-726).  In: gathered pops, moments, phi, grad phi, n,
-// flag, the 3x3 normal derivatives, and (shifted path) aw[i], the red shift
-// fraction of the node that reads direction i.  Out: post-recolour pops
-// (FP64) or their deviations from w_i*rho0*aw[i] / w_i*rho0*(1-aw[i]).
+// viscosity interpolation and MRT relaxation with Guo forcing (S:669-689).
+// In: the node's total populations f (FP64) or deviations f - w rho0
+// (shifted), moments, phi, grad phi, n, the 3x3 normal derivatives.  Out: f
+// replaced by the post-collision populations (or their deviations).
 template <class R, bool SHIFT>
-__device__ __forceinline__ void collide_recolor(const Lat &L, const R r[Q], const R b[Q], const Moments<R, SHIFT> &mo,
-                                                R phin, R gx, R gy, R gz, R nx, R ny, R nz, bool flag,
-                                                bool force_on, const R d[3][3], const R aw[Q], R rout[Q],
-                                                R bout[Q])
+__device__ __forceinline__ void collide(const Lat &L, R f[Q], const Moments<R, SHIFT> &mo, R phin, R gx, R gy,
+                                        R gz, R nx, R ny, R nz, bool force_on, const R d[3][3])
 {
     R fx = 0, fy = 0, fz = 0;
     if (force_on) {
@@ -283,37 +395,63 @@ __device__ __forceinline__ void collide_recolor(const Lat &L, const R r[Q], cons
         fz = val * gz;
     }
     const R rho = mo.rho;
-    const R ux = mo.ux + R(0.5) * fx / rho;
-    const R uy = mo.uy + R(0.5) * fy / rho;
-    const R uz = mo.uz + R(0.5) * fz / rho;
+    const R ux = mo.ux + fdiv(R(0.5) * fx, rho);
+    const R uy = mo.uy + fdiv(R(0.5) * fy, rho);
+    const R uz = mo.uz + fdiv(R(0.5) * fz, rho);
     R nu;
     if (phin <= R(-1))
         nu = R(L.nu_b);
     else if (phin >= R(1))
         nu = R(L.nu_r);
     else
-        nu = R(2) / ((R(1) + phin) / R(L.nu_r) + (R(1) - phin) / R(L.nu_b));
-    const R su = R(1) / (R(3) * nu + R(0.5));
-    const R sq = R(8) * (R(2) - su) / (R(8) - su);
-    R f[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = r[i] + b[i];
+        nu = fdiv(R(2), fdiv(R(1) + phin, R(L.nu_r)) + fdiv(R(1) - phin, R(L.nu_b)));
+    const R su = fdiv(R(1), R(3) * nu + R(0.5));
+    const R sq = fdiv(R(8) * (R(2) - su), R(8) - su);
     R m[Q];
+    if constexpr (!SHIFT) {
+        // dense order of the reference's MATMUL(M, f) (S:685), zero terms skipped
 #pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        R acc = 0;
+        for (int k = 0; k < Q; ++k) {
+            R acc = 0;
 #pragma unroll
-        for (int j = 0; j < Q; ++j) {
-            const int c = MT.m[k][j];
-            if (c == 0) continue;
-            if (c == 1)
-                acc = acc + f[j];
-            else if (c == -1)
-                acc = acc - f[j];
-            else
-                acc = acc + R(c) * f[j];
+            for (int j = 0; j < Q; ++j) {
+                const int c = MT.m[k][j];
+                if (c == 0) continue;
+                if (c == 1)
+                    acc = acc + f[j];
+                else if (c == -1)
+                    acc = acc - f[j];
+                else
+                    acc = acc + R(c) * f[j];
+            }
+            m[k] = acc;
         }
-        m[k] = acc;
+    } else {
+        // even moments from pair sums, odd moments from pair differences
+        R ps[NPAIR], pd[NPAIR];
+#pragma unroll
+        for (int q = 0; q < NPAIR; ++q) {
+            ps[q] = f[PAIR[q]] + f[OPC[PAIR[q]]];
+            pd[q] = f[PAIR[q]] - f[OPC[PAIR[q]]];
+        }
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const bool ev = MT.even[k];
+            R acc = ev ? R(MT.m[k][0]) * f[0] : R(0);
+#pragma unroll
+            for (int q = 0; q < NPAIR; ++q) {
+                const int c = MT.m[k][PAIR[q]];
+                const R v = ev ? ps[q] : pd[q];
+                if (c == 0) continue;
+                if (c == 1)
+                    acc += v;
+                else if (c == -1)
+                    acc -= v;
+                else
+                    acc += R(c) * v;
+            }
+            m[k] = acc;
+        }
     }
     // equilibrium and forcing moments (equilf, S:405-447)
     R meq[Q], fk[Q];
@@ -323,11 +461,11 @@ __device__ __forceinline__ void collide_recolor(const Lat &L, const R r[Q], cons
         meq[1] = R(19) * usqr - R(11);
         meq[2] = R(3) - R(5.5) * usqr;
         meq[3] = ux;
-        meq[4] = -ux / R(1.5);
+        meq[4] = div_c<R>(-ux, 1.5);
         meq[5] = uy;
-        meq[6] = -uy / R(1.5);
+        meq[6] = div_c<R>(-uy, 1.5);
         meq[7] = uz;
-        meq[8] = -uz / R(1.5);
+        meq[8] = div_c<R>(-uz, 1.5);
         meq[9] = R(3) * ux * ux - usqr;
         meq[10] = R(-0.5) * (R(3) * ux * ux - usqr);
         meq[11] = uy * uy - uz * uz;
@@ -355,11 +493,11 @@ __device__ __forceinline__ void collide_recolor(const Lat &L, const R r[Q], cons
         fk[1] = R(38) * uf;
         fk[2] = R(-11) * uf;
         fk[3] = fx;
-        fk[4] = -fx / R(1.5);
+        fk[4] = div_c<R>(-fx, 1.5);
         fk[5] = fy;
-        fk[6] = -fy / R(1.5);
+        fk[6] = div_c<R>(-fy, 1.5);
         fk[7] = fz;
-        fk[8] = -fz / R(1.5);
+        fk[8] = div_c<R>(-fz, 1.5);
         fk[9] = R(6) * ux * fx - R(2) * uf;
         fk[10] = -(R(3) * ux * fx - uf);
         fk[11] = R(2) * (uy * fy - uz * fz);
@@ -375,61 +513,125 @@ __device__ __forceinline__ void collide_recolor(const Lat &L, const R r[Q], cons
 #pragma unroll
     for (int k = 0; k < Q; ++k) m[k] = m[k] - si[k] * (m[k] - meq[k]) + (R(1) - R(0.5) * si[k]) * fk[k];
     // back to populations: f_i = sum_k Minv[i][k] m_k
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        R acc = 0;
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            if (MT.m[k][i] == 0) continue;
-            acc = acc + R(MT.inv[i][k]) * m[k];
-        }
-        f[i] = acc;
-    }
-    // recolouring (S:708-726)
-    const R sr = mo.sr, sb = mo.sb;
     if constexpr (!SHIFT) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) rout[i] = sr / rho * f[i];
-        if (flag) {
-#pragma unroll
-            for (int i = 1; i < Q; ++i) {
-                R dot = 0;
-                bool first = true;
-                if (CX[i] != 0) { dot = CX[i] == 1 ? nx : -nx; first = false; }
-                if (CY[i] != 0) { R t = CY[i] == 1 ? ny : -ny; dot = first ? t : dot + t; first = false; }
-                if (CZ[i] != 0) { R t = CZ[i] == 1 ? nz : -nz; dot = first ? t : dot + t; }
-                rout[i] = rout[i] + sr * sb / rho * R(L.beta) * R(W[i]) * dot;
-            }
-        }
-        if (sr <= R(EVAP_LIM)) {
-#pragma unroll
-            for (int i = 0; i < Q; ++i) rout[i] = 0;
-        }
-        if (sb <= R(EVAP_LIM)) {
-#pragma unroll
-            for (int i = 0; i < Q; ++i) rout[i] = f[i];
-        }
-#pragma unroll
-        for (int i = 0; i < Q; ++i) bout[i] = f[i] - rout[i];
-    } else {
-        // f holds deviations f' - w rho0; write r' - w rho0 aw, b' - w rho0 (1 - aw)
-        const R rho0 = R(L.rho0);
-        const R ar = sr / (sr + sb);
-        const R gam = flag ? sr * sb / (sr + sb) * R(L.beta) : R(0);
-        const bool evr = sr <= R(EVAP_LIM), evb = sb <= R(EVAP_LIM);
-#pragma unroll
         for (int i = 0; i < Q; ++i) {
-            R dot = R(CX[i]) * nx + R(CY[i]) * ny + R(CZ[i]) * nz;
-            const R a = aw[i];
-            R v = ar * f[i] + R(W[i]) * rho0 * (ar - a) + gam * R(W[i]) * dot;
-            if (evr) v = -R(W[i]) * rho0 * a;
-            if (evb) v = f[i] + R(W[i]) * rho0 * (R(1) - a);
-            rout[i] = v;
-            bout[i] = f[i] - v;
+            R acc = 0;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                if (MT.m[k][i] == 0) continue;
+                acc = acc + R(MT.inv[i][k]) * m[k];
+            }
+            f[i] = acc;
+        }
+    } else {
+        // scaled moments, then even and odd parts per direction pair
+        R mh[Q];
+#pragma unroll
+        for (int k = 0; k < Q; ++k) mh[k] = m[k] * R(1.0 / MT.norm2[k]);
+        {
+            R acc = 0;
+#pragma unroll
+            for (int k = 0; k < Q; ++k)
+                if (MT.m[k][0] != 0) acc += R(MT.m[k][0]) * mh[k];
+            f[0] = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < NPAIR; ++q) {
+            const int p = PAIR[q];
+            R e = 0, o = 0;
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const int c = MT.m[k][p];
+                if (c == 0) continue;
+                R &acc = MT.even[k] ? e : o;
+                if (c == 1)
+                    acc += mh[k];
+                else if (c == -1)
+                    acc -= mh[k];
+                else
+                    acc += R(c) * mh[k];
+            }
+            f[p] = e + o;
+            f[OPC[p]] = e - o;
         }
     }
 }
 
+// Recolouring parameters of a node (S:713-725): colour fraction A, the
+// perturbation amplitude gamma (0 where the node is not flagged) and the
+// EVAP_LIM overrides (r = 0, then r = f).
+template <class R> struct Recol {
+    R A, gam;
+    bool evr, evb;
+};
+template <class R, bool SHIFT>
+__device__ __forceinline__ Recol<R> recol(const Lat &L, const Moments<R, SHIFT> &mo, bool flag)
+{
+    Recol<R> c;
+    c.A = fdiv(mo.sr, mo.rho);
+    c.gam = flag ? fdiv(mo.sr * mo.sb, mo.rho) * R(L.beta) : R(0);
+    c.evr = mo.sr <= R(EVAP_LIM);
+    c.evb = mo.sb <= R(EVAP_LIM);
+    return c;
+}
+
+// Output writers.  RB: the recoloured red and blue populations (S:713-726);
+// shifted RB stores deviations from w_i rho0 aw_i and w_i rho0 (1 - aw_i),
+// aw_i = red fraction of the node that reads direction i next.  FS: the
+// post-collision f and the node's recolour scalars.
+template <int LOUT, class S, class R, bool SHIFT>
+__device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, unsigned n, const R f[Q],
+                                           const Recol<R> &c, R nx, R ny, R nz, const R aw[Q])
+{
+    const size_t N = (size_t)L.N;
+    if constexpr (LOUT == RB) {
+        S *ro = (S *)o0, *bo = (S *)o1;
+        if constexpr (!SHIFT) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                R r = c.A * f[i];
+                if (i > 0 && c.gam != R(0)) r = r + c.gam * R(W[i]) * dot_c<R>(i, nx, ny, nz);
+                if (c.evr) r = 0;
+                if (c.evb) r = f[i];
+                S *rp = ro + (size_t)i * N, *bp = bo + (size_t)i * N;
+                StoreOf<S>::st(rp, n, r);
+                StoreOf<S>::st(bp, n, f[i] - r);
+            }
+        } else {
+            const R rho0 = R(L.rho0);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const R dot = R(CX[i]) * nx + R(CY[i]) * ny + R(CZ[i]) * nz;
+                const R a = aw[i];
+                R v = c.A * f[i] + R(W[i]) * rho0 * (c.A - a) + c.gam * R(W[i]) * dot;
+                if (c.evr) v = -R(W[i]) * rho0 * a;
+                if (c.evb) v = f[i] + R(W[i]) * rho0 * (R(1) - a);
+                S *rp = ro + (size_t)i * N, *bp = bo + (size_t)i * N;
+                StoreOf<S>::st(rp, n, v);
+                StoreOf<S>::st(bp, n, f[i] - v);
+            }
+        }
+    } else {
+        S *fo = (S *)o0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            S *fp = fo + (size_t)i * N;
+            StoreOf<S>::st(fp, n, f[i]);
+        }
+        R A = c.A, g = c.gam;
+        if (c.evr) { A = 0; g = 0; }
+        if (c.evb) { A = 1; g = 0; }
+        if constexpr (!SHIFT) {
+            double2 *sc = (double2 *)o1;
+            at(sc, 3 * n) = make_double2(A, g);
+            at(sc, 3 * n + 1) = make_double2(nx, ny);
+            at(sc, 3 * n + 2) = make_double2(nz, 0.0);
+        } else {
+            at((float4 *)o1, n) = make_float4(float(A), float(g * nx), float(g * ny), float(g * nz));
+        }
+    }
+}
 
 // ----------------------------------------------------------- K2 fused step
 // One workgroup owns a TZ x TY column of (z, y) and marches x over
@@ -446,9 +648,9 @@ template <class R, class P> struct K2Smem {
 __device__ __forceinline__ int slot5(int p) { return (p + 10) % 5; }
 __device__ __forceinline__ int slot3(int p) { return (p + 9) % 3; }
 
-template <class S, class R, class P, bool SHIFT>
-__global__ __launch_bounds__(256) void k_collide(Lat L, const S *__restrict__ rin, const S *__restrict__ bin,
-                                                 S *__restrict__ rout, S *__restrict__ bout,
+template <int LIN, int LOUT, class S, class R, class P, bool SHIFT>
+__global__ __launch_bounds__(256) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
+                                                 void *__restrict__ o0, void *__restrict__ o1,
                                                  const P *__restrict__ phi, const P *__restrict__ phio, int xc)
 {
     __shared__ K2Smem<R, P> sm;
@@ -467,7 +669,7 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const S *__restrict__ ri
         for (int e = tid; e < (TY + 4) * (TZ + 4); e += TZ * TY) {
             int jy = e / (TZ + 4), jz = e - jy * (TZ + 4);
             int gy = wrapm(y0 - 2 + jy, L.ny), gz = wrapm(z0 - 2 + jz, L.nz);
-            dst[jy][jz] = phi[gx + gy * L.nz + gz];
+            dst[jy][jz] = at(phi, (unsigned)(gx + gy * L.nz + gz));
         }
     };
     // unit normal of plane p on the (TY+2) x (TZ+2) halo region (S:498-532)
@@ -478,12 +680,20 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const S *__restrict__ ri
             R gx, gy, gz;
             grad19<R>([&](int dx, int dy, int dz) { return sm.phi[slot5(p + dx)][jy + 1 + dy][jz + 1 + dz]; }, gx,
                       gy, gz);
-            R fnorm = __builtin_sqrt(gx * gx + gy * gy + gz * gz);
             R nx = 0, ny = 0, nz = 0;
-            if (fnorm > R(EVAP_LIM)) {
-                nx = gx / fnorm;
-                ny = gy / fnorm;
-                nz = gz / fnorm;
+            const R g2 = gx * gx + gy * gy + gz * gz;
+            if constexpr (sizeof(R) == 8) {
+                const R fnorm = fsqrt(g2);
+                if (fnorm > R(EVAP_LIM)) {
+                    nx = gx / fnorm;
+                    ny = gy / fnorm;
+                    nz = gz / fnorm;
+                }
+            } else if (g2 > R(EVAP_LIM * EVAP_LIM)) {
+                const R inv = __builtin_amdgcn_rsqf(g2);
+                nx = gx * inv;
+                ny = gy * inv;
+                nz = gz * inv;
             }
             sm.n[0][s][jy][jz] = nx;
             sm.n[1][s][jy][jz] = ny;
@@ -508,8 +718,8 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const S *__restrict__ ri
         R gx, gy, gz;
         grad19<R>([&](int dx, int dy, int dz) { return sm.phi[slot5(x + dx)][ty + 2 + dy][tz + 2 + dz]; }, gx, gy,
                   gz);
-        const R fnorm = __builtin_sqrt(gx * gx + gy * gy + gz * gz);
-        const bool flag = fnorm > R(EVAP_LIM);
+        const R g2 = gx * gx + gy * gy + gz * gz;
+        const bool flag = sizeof(R) == 8 ? fsqrt(g2) > R(EVAP_LIM) : g2 > R(EVAP_LIM * EVAP_LIM);
         const R nx = sm.n[0][slot3(x)][ty + 1][tz + 1];
         const R ny = sm.n[1][slot3(x)][ty + 1][tz + 1];
         const R nz = sm.n[2][slot3(x)][ty + 1][tz + 1];
@@ -537,30 +747,32 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const S *__restrict__ ri
                 for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
         }
 
-        // pull gather = streaming (S:743-753)
-        int xo[3], yo[3], zo[3];
+        unsigned xo[3], yo[3], zo[3];
         node_offsets(L, x, y, z, xo, yo, zo);
-        R r[Q], b[Q];
-        gather<S, R, P, SHIFT>(L, rin, bin, xo, yo, zo, r, b);
-        const int n = x * sx + y * L.nz + z;
+        const unsigned n = xo[1] + yo[1] + zo[1];
         R ax = 0, aw[Q];
-        if constexpr (SHIFT) {
-            ax = R(0.5) * (R(1) + R(phio[n]));
+        if constexpr (SHIFT && LIN == RB) ax = R(0.5) * (R(1) + R(at(phio, n)));
+        if constexpr (SHIFT && LOUT == RB) {
             // shift of the slot read by x + c_i at the next step: that node's phi now
 #pragma unroll
             for (int i = 0; i < Q; ++i)
                 aw[i] = R(0.5) * (R(1) + R(sm.phi[slot5(x + CX[i])][ty + 2 + CY[i]][tz + 2 + CZ[i]]));
         }
-        const Moments<R, SHIFT> mo = moments<R, SHIFT>(L, r, b, ax);
+        Node<R, SHIFT> nd;
+        gather<LIN, S, R, SHIFT>(L, i0, i1, xo, yo, zo, ax, nd);
+        const Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
         const R phin = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
-        R ro[Q], bo[Q];
-        collide_recolor<R, SHIFT>(L, r, b, mo, phin, gx, gy, gz, nx, ny, nz, flag, flag, d, aw, ro, bo);
-        const size_t N = (size_t)L.N;
+        R f[Q];
+        if constexpr (!SHIFT) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            StoreOf<S>::st(rout + (size_t)i * N + n, ro[i]);
-            StoreOf<S>::st(bout + (size_t)i * N + n, bo[i]);
+            for (int i = 0; i < Q; ++i) f[i] = nd.r[i] + nd.b[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) f[i] = nd.ft[i];
         }
+        collide<R, SHIFT>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, flag, d);
+        const Recol<R> rc = recol<R, SHIFT>(L, mo, flag);
+        write_node<LOUT, S, R, SHIFT>(L, o0, o1, n, f, rc, nx, ny, nz, aw);
     }
 }
 
@@ -575,7 +787,7 @@ __global__ __launch_bounds__(256) void k_derivatives(Lat L, const P *__restrict_
     const int y = blockIdx.y * TY + threadIdx.y;
     const int x = blockIdx.z;
     if (z >= L.nz || y >= L.ny) return;
-    int xo[3], yo[3], zo[3];
+    unsigned xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
     using R = double;
     R gx, gy, gz;
@@ -627,7 +839,7 @@ __global__ __launch_bounds__(256) void k_pack(Lat L, const double *__restrict__ 
     const int y = blockIdx.y * TY + threadIdx.y;
     const int x = blockIdx.z;
     if (z >= L.nz || y >= L.ny) return;
-    int xo[3], yo[3], zo[3];
+    unsigned xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
     const int n = xo[1] + yo[1] + zo[1];
     const size_t N = (size_t)L.N;
@@ -648,7 +860,7 @@ __global__ __launch_bounds__(256) void k_unpack(Lat L, const S *__restrict__ soa
     const int y = blockIdx.y * TY + threadIdx.y;
     const int x = blockIdx.z;
     if (z >= L.nz || y >= L.ny) return;
-    int xo[3], yo[3], zo[3];
+    unsigned xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
     const int n = xo[1] + yo[1] + zo[1];
     const size_t N = (size_t)L.N;
@@ -657,6 +869,36 @@ __global__ __launch_bounds__(256) void k_unpack(Lat L, const S *__restrict__ soa
         double v = double(StoreOf<S>::ld(soa + (size_t)i * N + src));
         if constexpr (SHIFT) v = v + W[i] * (L.rho0 * (double)shift_a(phio[n], red));
         aos[(size_t)n * Q + i] = v;
+    }
+}
+
+// Unpack the factorised state: P_i(x) = r_i or b_i rebuilt at x - c_i, as
+// FP64 AoS (the same reconstruction the pull gather does).
+template <class S, bool SHIFT>
+__global__ __launch_bounds__(256) void k_unpack_fs(Lat L, const S *__restrict__ fs, const void *__restrict__ scv,
+                                                   double *__restrict__ aos, int red)
+{
+    const int z = blockIdx.x * TZ + threadIdx.x;
+    const int y = blockIdx.y * TY + threadIdx.y;
+    const int x = blockIdx.z;
+    if (z >= L.nz || y >= L.ny) return;
+    unsigned xo[3], yo[3], zo[3];
+    node_offsets(L, x, y, z, xo, yo, zo);
+    const int n = xo[1] + yo[1] + zo[1];
+    const size_t N = (size_t)L.N;
+    for (int i = 0; i < Q; ++i) {
+        const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+        double f = double(StoreOf<S>::ld(fs + (size_t)i * N + m)), r;
+        if constexpr (!SHIFT) {
+            const double *sc = (const double *)scv + (size_t)m * 6;
+            r = sc[0] * f;
+            if (i > 0) r = r + (sc[1] * W[i]) * dot_c<double>(i, sc[2], sc[3], sc[4]);
+        } else {
+            const float4 g = ((const float4 *)scv)[m];
+            f = f + W[i] * L.rho0;
+            r = double(g.x) * f + W[i] * (CX[i] * double(g.y) + CY[i] * double(g.z) + CZ[i] * double(g.w));
+        }
+        aos[(size_t)n * Q + i] = red ? r : f - r;
     }
 }
 
@@ -704,7 +946,7 @@ __global__ __launch_bounds__(256) void k_init_pops(Lat L, CylIC ic, S *__restric
     const int y = blockIdx.y * TY + threadIdx.y;
     const int x = blockIdx.z;
     if (z >= L.nz || y >= L.ny) return;
-    int xo[3], yo[3], zo[3];
+    unsigned xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
     const size_t N = (size_t)L.N;
     double Rz;

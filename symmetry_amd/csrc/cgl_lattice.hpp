@@ -51,6 +51,7 @@ struct MTab {
     int m[Q][Q];
     double norm2[Q];
     double inv[Q][Q]; // inv[i][k] = m[k][i] / norm2[k]
+    bool even[Q];     // moment k takes equal values on opposite directions
 };
 
 constexpr MTab make_mtab()
@@ -66,10 +67,20 @@ constexpr MTab make_mtab()
     }
     for (int i = 0; i < Q; ++i)
         for (int k = 0; k < Q; ++k) t.inv[i][k] = double(t.m[k][i]) / t.norm2[k];
+    for (int k = 0; k < Q; ++k) {
+        t.even[k] = true;
+        for (int i = 0; i < Q; ++i)
+            if (t.m[k][OPC[i]] != t.m[k][i]) t.even[k] = false;
+    }
     return t;
 }
 
 constexpr MTab MT = make_mtab();
+
+// Direction pairs (p, opc(p)) with p the first of each pair.  With the moment
+// parities MT.even they halve the work of the MRT transforms.
+constexpr int NPAIR = 9;
+constexpr int PAIR[NPAIR] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
 
 // Moments of the rest-state weights: M w = (1, -11, 3, 0, ..., 0).
 constexpr double MW0 = 1.0, MW1 = -11.0, MW2 = 3.0;

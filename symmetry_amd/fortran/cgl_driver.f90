@@ -16,13 +16,13 @@ program cgl_driver
   use iso_c_binding
   use cgl
   implicit none
-  integer :: xl = 32, yl = 32, zl = 128, nsteps = 2000, out_every = 100, prec = 0, xchunk = 0, nmodes = 1
+  integer :: xl = 32, yl = 32, zl = 128, nsteps = 2000, out_every = 100, prec = 0, xchunk = 0, nmodes = 1, layout = 0
   double precision :: sigma = 0.01d0, rho0 = 1.0d0, tau_R = 1.0d0, tau_B = 1.0d0, beta = 0.7d0, Rad_c = 8.0d0
   double precision :: amp(16) = 0.0d0
   logical :: ic_on_device = .false., write_tecplot = .true., dump_binary = .false.
   character(len=256) :: outdir = '.'
   namelist /cgl/ xl, yl, zl, nsteps, out_every, prec, xchunk, sigma, rho0, tau_R, tau_B, beta, Rad_c, &
-                 nmodes, amp, ic_on_device, write_tecplot, dump_binary, outdir
+                 nmodes, amp, ic_on_device, write_tecplot, dump_binary, outdir, layout
 
   double precision, allocatable :: ri(:,:,:,:), bi(:,:,:,:)
   double precision, allocatable :: rho(:,:,:), rho_R(:,:,:), rho_B(:,:,:), rho_N(:,:,:), ux(:,:,:), uy(:,:,:), uz(:,:,:)
@@ -59,6 +59,8 @@ program cgl_driver
   p%npar = 0
   p%xchunk = xchunk
   p%device = 0
+  p%layout = layout
+  p%reserved = 0
   p%sigma = sigma; p%rho0 = rho0; p%nu_r = nu_R; p%nu_b = nu_B; p%beta = beta
   call cgl_check(cgl_create(h, p), 'cgl_create')
 

@@ -11,11 +11,12 @@ module cgl
 
   integer(c_int32_t), parameter :: CGL_BC_PERIODIC = 0, CGL_BC_WALL = 1, CGL_BC_MIRROR = 2
   integer(c_int32_t), parameter :: CGL_PREC_F64 = 0, CGL_PREC_F32 = 1, CGL_PREC_F16 = 2
+  integer(c_int32_t), parameter :: CGL_LAYOUT_RB = 0, CGL_LAYOUT_FS = 1
 
   type, bind(C) :: cgl_params
     integer(c_int32_t) :: nx, ny, nz
     integer(c_int32_t) :: bc(6)
-    integer(c_int32_t) :: precision, npar, xchunk, device
+    integer(c_int32_t) :: precision, npar, xchunk, device, layout, reserved
     real(c_double) :: sigma, rho0, nu_r, nu_b, beta
   end type cgl_params
 

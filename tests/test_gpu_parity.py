@@ -56,11 +56,15 @@ def test_calmacros_and_derivatives_bit_exact(rp1):
     assert np.array_equal(der["flag"], st.flag)
 
 
+@pytest.mark.parametrize("layout", ["rb", "fs"])
 @pytest.mark.parametrize("nsteps", [1, 20])
-def test_f64_steps_bit_exact(rp1, nsteps):
+def test_f64_steps_bit_exact(rp1, nsteps, layout):
+    """Both device layouts hold the same red/blue populations: the factorised
+    one rebuilds r_i = A f_i + (gamma w_i)(n.c_i) with the reference's
+    rounding, so FP64 stays bit-exact."""
     cfg, ri, bi = rp1
-    _print("f64 steps", nsteps)
-    lat = _lattice(cfg, "f64")
+    _print("f64 steps", nsteps, layout)
+    lat = _lattice(cfg, "f64", layout=layout)
     lat.upload(ri, bi)
     lat.step(nsteps)
     r2, b2 = lat.download()
@@ -70,15 +74,20 @@ def test_f64_steps_bit_exact(rp1, nsteps):
     db = np.max(np.abs(b2 - st.bi))
     _print("max |dr|, |db|:", dr, db)
     assert dr == 0.0 and db == 0.0
+    st.call("calmacros")
+    mac = lat.calmacros()
+    for k in ("rho", "rho_r", "rho_b", "ux", "uy", "uz", "phi"):
+        assert np.array_equal(mac[k], st.arrays[k]), k
 
 
+@pytest.mark.parametrize("layout", ["rb", "fs"])
 @pytest.mark.parametrize("shape,xchunk", [((20, 12, 40), 3), ((9, 5, 70), 4), ((4, 4, 4), 1), ((33, 7, 65), 16)])
-def test_f64_uneven_boxes_bit_exact(shape, xchunk):
+def test_f64_uneven_boxes_bit_exact(shape, xchunk, layout):
     """Boxes that do not fill the 64x4 workgroup tile or the x chunk."""
     nx, ny, nz = shape
-    _print("uneven f64", shape, xchunk)
+    _print("uneven f64", shape, xchunk, layout)
     cfg, ri, bi = small_rp(nx=nx, ny=ny, nz=nz, r0=min(nx, ny) / 4.0, amps=(0.2,), nu_b=0.05)
-    lat = _lattice(cfg, "f64", xchunk=xchunk)
+    lat = _lattice(cfg, "f64", xchunk=xchunk, layout=layout)
     lat.upload(ri, bi)
     lat.step(3)
     r2, b2 = lat.download()
@@ -87,12 +96,13 @@ def test_f64_uneven_boxes_bit_exact(shape, xchunk):
     assert np.array_equal(r2, st.ri) and np.array_equal(b2, st.bi)
 
 
-def test_xchunk_invariance(rp1):
+@pytest.mark.parametrize("layout", ["rb", "fs"])
+def test_xchunk_invariance(rp1, layout):
     cfg, ri, bi = rp1
-    _print("xchunk invariance f32")
+    _print("xchunk invariance f32", layout)
     outs = []
     for xc in (1, 8, 32):
-        lat = _lattice(cfg, "f32", xchunk=xc)
+        lat = _lattice(cfg, "f32", xchunk=xc, layout=layout)
         lat.upload(ri, bi)
         lat.step(5)
         outs.append(lat.download())
@@ -100,18 +110,19 @@ def test_xchunk_invariance(rp1):
         assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
 
 
+@pytest.mark.parametrize("layout", ["rb", "fs"])
 @pytest.mark.parametrize("prec,tol_phi,tol_u", [("f32", 1e-5, 1e-5), ("f16", 5e-3, 5e-2)])
-def test_reduced_precision_vs_f64(rp1, prec, tol_phi, tol_u):
+def test_reduced_precision_vs_f64(rp1, prec, tol_phi, tol_u, layout):
     """phi and velocity of FP32 shifted storage within 1e-5 (L2-relative) of
     the FP64 GPU path after 200 steps; FP16 storage within the looser stated
     bounds."""
     cfg, ri, bi = rp1
-    _print("precision", prec)
+    _print("precision", prec, layout)
     ref = _lattice(cfg, "f64")
     ref.upload(ri, bi)
     ref.step(200)
     m64 = ref.calmacros()
-    lat = _lattice(cfg, prec)
+    lat = _lattice(cfg, prec, layout=layout)
     lat.upload(ri, bi)
     lat.step(200)
     m = lat.calmacros()
@@ -123,21 +134,22 @@ def test_reduced_precision_vs_f64(rp1, prec, tol_phi, tol_u):
     assert e_phi[0] < tol_phi and e_u[0] < tol_u
 
 
+@pytest.mark.parametrize("layout", ["rb", "fs"])
 @pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
-def test_mass_conserved(rp1, prec):
+def test_mass_conserved(rp1, prec, layout):
     """Total mass is conserved to round-off.  Component masses are not
     conserved by the reference itself: its recolouring hands colour mass below
     EVAP_LIM=1e-8 to the other component (S:720-725), so their drift must equal
     the FP64 path's."""
     cfg, ri, bi = rp1
-    lat = _lattice(cfg, prec)
+    lat = _lattice(cfg, prec, layout=layout)
     lat.upload(ri, bi)
     m0 = lat.calmacros()
     lat.step(100)
     m1 = lat.calmacros()
     tol = {"f64": 1e-13, "f32": 1e-6, "f16": 1e-3}[prec]
     a, b = m0["rho"].sum(), m1["rho"].sum()
-    _print(prec, "total mass", a, b, abs(b - a) / a)
+    _print(prec, layout, "total mass", a, b, abs(b - a) / a)
     assert abs(b - a) / a < tol
     ref = _lattice(cfg, "f64")
     ref.upload(ri, bi)
@@ -155,3 +167,34 @@ def test_device_ic_matches_host_ic():
     lat.init_cylinder(cfg["nx"] / 2 + 0.5, cfg["ny"] / 2 + 0.5, cfg["r0"], 6.0 * 0.7 * 0.134, cfg["amps"])
     r2, b2 = lat.download()
     assert np.max(np.abs(r2 - ri)) < 1e-14 and np.max(np.abs(b2 - bi)) < 1e-14
+
+
+def test_fs_device_ic_steps_equal_rb():
+    """The factorised layout started from the device initial state follows the
+    red/blue layout bit for bit in FP64."""
+    cfg, _, _ = small_rp(nx=24, ny=16, nz=64, amps=(0.8, 0.3))
+    outs = []
+    for layout in ("rb", "fs"):
+        lat = _lattice(cfg, "f64", layout=layout)
+        lat.init_cylinder(cfg["nx"] / 2 + 0.5, cfg["ny"] / 2 + 0.5, cfg["r0"], 6.0 * 0.7 * 0.134, cfg["amps"])
+        lat.step(7)
+        outs.append(lat.download())
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("layout", ["rb", "fs"])
+def test_autotuner_picks_a_candidate_and_keeps_results(rp1, layout):
+    """xchunk=0 times candidate x chunks on the first real steps; the state it
+    produces equals a run with a fixed chunk."""
+    cfg, ri, bi = rp1
+    auto = _lattice(cfg, "f32", layout=layout, xchunk=0)
+    auto.upload(ri, bi)
+    auto.step(15)
+    xc, tuning = auto.tuned_xchunk()
+    _print(layout, "autotuned xchunk", xc, tuning)
+    assert not tuning and xc in (2, 4, 8, 16, 32, 64)
+    fixed = _lattice(cfg, "f32", layout=layout, xchunk=8)
+    fixed.upload(ri, bi)
+    fixed.step(15)
+    a, b = auto.download(), fixed.download()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
