@@ -64,8 +64,9 @@ typedef struct cgl_params {
 } cgl_params;
 
 /* Allocate device state for one box.  Replaces the allocation block of the
- * reference's main (S:144-179).  Fluid-only, fully periodic boxes are
- * supported; other face types and particles return CGL_ERR_UNSUPPORTED. */
+ * reference's main (S:144-179).  Faces may be periodic (in pairs) or
+ * half-way bounce-back walls; npar > 0 adds moving particles (red/blue
+ * layout only).  Mirror faces return CGL_ERR_UNSUPPORTED. */
 int cgl_create(cgl_t **h, const cgl_params *p);
 int cgl_destroy(cgl_t *h);
 
@@ -105,6 +106,26 @@ int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning);
 
 /* Block until all queued device work of the handle has finished. */
 int cgl_sync(cgl_t *h);
+
+/* Particles (npar of the handle): centres in 1-based lattice coordinates
+ * [npar][3], velocities and angular velocities [npar][3] (NULL = at rest),
+ * radius, mass, moment of inertia and contact angle (radians) per particle,
+ * and the repulsion scale Fg (S:84).  Computes the masks of set_walls
+ * (S:212, S:326-395).  The reference holds these in center(par,3), upx..rpz,
+ * Rad, mass, inertia_moment, cita (S:21, S:45-50, S:76-84). */
+int cgl_set_particles(cgl_t *h, const double *center, const double *up, const double *rp, const double *rad,
+                      const double *mass, const double *inertia, const double *cita, double fg);
+
+/* Particle state after the last step: centre, velocity, angular velocity,
+ * momentum-exchange force and torque (Fpx.., Tpx.., S:1247-1252), capillary
+ * force and torque (tension, tension_t, S:2020-2022), repulsion (Frex..,
+ * S:1633-1665), each [npar][3].  Any pointer may be NULL. */
+int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *force, double *torque,
+                      double *fcap, double *tcap, double *frep);
+
+/* Masks of set_walls in force (walls, walls_all, pin, pout; S:326-395),
+ * (zl, yl, xl) layout.  Any pointer may be NULL. */
+int cgl_download_masks(cgl_t *h, int32_t *walls, int32_t *walls_all, int32_t *pin, int32_t *pout);
 
 /* Time nsteps steps on the device with HIP events recorded on the handle's
  * stream: total elapsed ms, and ms summed over the moments and fused-collide

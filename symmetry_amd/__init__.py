@@ -48,6 +48,7 @@ EXPORTS = (
     "cgl_create", "cgl_destroy", "cgl_upload_state", "cgl_download_state", "cgl_init_cylinder",
     "cgl_step", "cgl_calmacros", "cgl_derivatives", "cgl_sync", "cgl_time_steps",
     "cgl_device_name", "cgl_last_error", "cgl_probe_copy_bandwidth", "cgl_tuned_xchunk",
+    "cgl_set_particles", "cgl_get_particles", "cgl_download_masks",
 )
 
 
@@ -72,6 +73,9 @@ def lib():
         L.cgl_sync.argtypes = [vp]
         L.cgl_time_steps.argtypes = [vp, ctypes.c_int32] + [ctypes.POINTER(ctypes.c_double)] * 3
         L.cgl_device_name.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32]
+        L.cgl_set_particles.argtypes = [vp] + [vp] * 7 + [ctypes.c_double]
+        L.cgl_get_particles.argtypes = [vp] + [vp] * 8
+        L.cgl_download_masks.argtypes = [vp] + [vp] * 4
         L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.cgl_probe_copy_bandwidth.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                                ctypes.POINTER(ctypes.c_double)]
@@ -100,13 +104,13 @@ class Lattice:
     0-based indices: populations ``[x, y, z, i]``, scalars ``[x, y, z]``."""
 
     def __init__(self, nx, ny, nz, precision="f64", sigma=0.01, rho0=1.0, nu_r=1.0 / 6.0, nu_b=1.0 / 6.0,
-                 beta=0.7, bc=(BC_PERIODIC,) * 6, xchunk=0, device=0, layout="rb"):
+                 beta=0.7, bc=(BC_PERIODIC,) * 6, xchunk=0, device=0, layout="rb", npar=0):
         p = Params()
         p.nx, p.ny, p.nz = nx, ny, nz
         for k, v in enumerate(bc):
             p.bc[k] = v
         p.precision = PREC[precision] if isinstance(precision, str) else int(precision)
-        p.npar = 0
+        p.npar = npar
         p.xchunk = xchunk
         p.device = device
         p.layout = LAYOUT[layout] if isinstance(layout, str) else int(layout)
@@ -166,6 +170,25 @@ class Lattice:
         t = [ctypes.c_double() for _ in range(3)]
         _check(lib().cgl_time_steps(self.h, int(n), *(ctypes.byref(v) for v in t)))
         return tuple(v.value for v in t)
+
+    def set_particles(self, center, rad, mass, inertia, cita, fg, up=None, rp=None):
+        f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        args = [f(center), f(up), f(rp), f(rad), f(mass), f(inertia), f(cita)]
+        self._keep = args
+        _check(lib().cgl_set_particles(self.h, *(_ptr(a) for a in args), float(fg)))
+
+    def get_particles(self):
+        n = self.params.npar
+        names = ("center", "up", "rp", "force", "torque", "fcap", "tcap", "frep")
+        out = {k: np.empty((n, 3)) for k in names}
+        _check(lib().cgl_get_particles(self.h, *(_ptr(out[k]) for k in names)))
+        return out
+
+    def masks(self):
+        names = ("walls", "walls_all", "pin", "pout")
+        out = {k: np.empty(self.shape, dtype=np.int32) for k in names}
+        _check(lib().cgl_download_masks(self.h, *(_ptr(out[k]) for k in names)))
+        return out
 
     def tuned_xchunk(self):
         """(x chunk in use, True while the autotuner is still running)."""

@@ -62,8 +62,12 @@ int set_device(cgl_handle *h)
 void enqueue_step(cgl_handle *h)
 {
     dispatch(h, [&](auto impl) {
-        decltype(impl)::moments(h, false);
-        decltype(impl)::collide(h);
+        if (h->ext) {
+            decltype(impl)::step_ext(h);
+        } else {
+            decltype(impl)::moments(h, false);
+            decltype(impl)::collide(h);
+        }
     });
     h->cur ^= 1;
     h->phicur ^= 1;
@@ -134,10 +138,21 @@ int cgl_create(cgl_t **out, const cgl_params *p)
         return fail(CGL_ERR_UNSUPPORTED, "box of %.3g nodes exceeds the 4 GiB per-array addressing", nn);
     if (p->precision < CGL_PREC_F64 || p->precision > CGL_PREC_F16)
         return fail(CGL_ERR_ARG, "unknown precision %d", p->precision);
-    for (int f = 0; f < 6; ++f)
-        if (p->bc[f] != CGL_BC_PERIODIC)
-            return fail(CGL_ERR_UNSUPPORTED, "face %d: only periodic faces are implemented on the GPU", f);
-    if (p->npar != 0) return fail(CGL_ERR_UNSUPPORTED, "particles are not implemented on the GPU");
+    int walls = 0;
+    for (int a = 0; a < 3; ++a) {
+        for (int s = 0; s < 2; ++s) {
+            const int f = p->bc[2 * a + s];
+            if (f == CGL_BC_MIRROR)
+                return fail(CGL_ERR_UNSUPPORTED, "face %d: mirror faces are not implemented on the GPU", 2 * a + s);
+            if (f != CGL_BC_PERIODIC && f != CGL_BC_WALL) return fail(CGL_ERR_ARG, "face %d: unknown type %d", 2 * a + s, f);
+            walls |= f == CGL_BC_WALL;
+        }
+        if ((p->bc[2 * a] == CGL_BC_PERIODIC) != (p->bc[2 * a + 1] == CGL_BC_PERIODIC))
+            return fail(CGL_ERR_ARG, "axis %d: periodic faces come in pairs", a);
+    }
+    if (p->npar < 0) return fail(CGL_ERR_ARG, "npar must be >= 0");
+    if ((walls || p->npar > 0) && p->layout != CGL_LAYOUT_RB)
+        return fail(CGL_ERR_UNSUPPORTED, "walls and particles are implemented for the red/blue layout");
     if (!(p->rho0 > 0) || !(p->nu_r > 0) || !(p->nu_b > 0)) return fail(CGL_ERR_ARG, "rho0, nu_r, nu_b must be > 0");
     if (p->layout != CGL_LAYOUT_RB && p->layout != CGL_LAYOUT_FS) return fail(CGL_ERR_ARG, "unknown layout %d", p->layout);
     if (p->reserved != 0) return fail(CGL_ERR_ARG, "reserved field must be 0");
@@ -166,6 +181,12 @@ int cgl_create(cgl_t **out, const cgl_params *p)
     h->tuning = p->xchunk > 0 ? 0 : 1;
     h->layout = p->layout;
     h->state_layout = cgl::RB;
+    h->npar = p->npar;
+    h->ext = walls || p->npar > 0;
+    for (int a = 0; a < 3; ++a)
+        for (int s = 0; s < 2; ++s) h->E.wall[a][s] = p->bc[2 * a + s] == CGL_BC_WALL;
+    h->E.npar = p->npar;
+    h->P.npar = p->npar;
     int rc = set_device(h);
     if (rc) {
         delete h;
@@ -184,6 +205,31 @@ int cgl_create(cgl_t **out, const cgl_params *p)
         for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc(&h->fss[b], h->N * ns);
     }
     for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc(&h->phi[b], h->N * h->phisize);
+    if (h->npar > 0) {
+        // masks, NEAR-band macroscopic fields and gradients, particle state
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc((void **)&h->ntype[b], h->N * sizeof(unsigned));
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMemset(h->ntype[b], 0, h->N * sizeof(unsigned));
+        int **ints[3] = {&h->walls_i, &h->pin_i, &h->pout_i};
+        for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMalloc((void **)ints[k], h->N * sizeof(int));
+        for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMemset(*ints[k], 0, h->N * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void **)&h->E.mac, 6 * h->N * sizeof(double));
+        if (e == hipSuccess) e = hipMemset(h->E.mac, 0, 6 * h->N * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&h->E.der, 6 * h->N * sizeof(double));
+        if (e == hipSuccess) e = hipMemset(h->E.der, 0, 6 * h->N * sizeof(double));
+        const size_t n3 = 3 * (size_t)h->npar * sizeof(double), n1 = (size_t)h->npar * sizeof(double);
+        double **v3[10] = {&h->P.center, &h->P.center_old, &h->P.center_prev, &h->P.up, &h->P.rp,
+                           &h->P.fp, &h->P.tp, &h->P.frep, &h->P.ten, &h->P.tent};
+        double **v1[5] = {&h->P.rad, &h->P.mass, &h->P.inertia, &h->P.cita, &h->P.shb};
+        for (int k = 0; k < 10 && e == hipSuccess; ++k) e = hipMalloc((void **)v3[k], n3);
+        for (int k = 0; k < 10 && e == hipSuccess; ++k) e = hipMemset(*v3[k], 0, n3);
+        for (int k = 0; k < 5 && e == hipSuccess; ++k) e = hipMalloc((void **)v1[k], n1);
+        for (int k = 0; k < 5 && e == hipSuccess; ++k) e = hipMemset(*v1[k], 0, n1);
+        if (e == hipSuccess) e = hipMalloc((void **)&h->P.part_lag, (size_t)h->npar * cgl::LAG_BLOCKS * 7 * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&h->P.part_mom, (size_t)h->npar * cgl::PSLICES * 6 * sizeof(double));
+        h->E.pout = h->pout_i;
+        h->E.center = h->P.center;
+        h->E.cita = h->P.cita;
+    }
     if (e == hipSuccess) e = hipMemset(h->phi[0], 0, h->N * h->phisize);
     if (e == hipSuccess) e = hipMemset(h->phi[1], 0, h->N * h->phisize);
     if (e != hipSuccess) {
@@ -208,6 +254,14 @@ int cgl_destroy(cgl_t *h)
         if (h->fss[b]) (void)hipFree(h->fss[b]);
     }
     if (h->mac) (void)hipFree(h->mac);
+    if (h->npar > 0) {
+        void *ptrs[] = {h->ntype[0], h->ntype[1], h->walls_i, h->pin_i, h->pout_i, h->E.mac, h->E.der,
+                        h->P.center, h->P.center_old, h->P.center_prev, h->P.up, h->P.rp, h->P.fp, h->P.tp,
+                        h->P.frep, h->P.ten, h->P.tent, h->P.rad, h->P.mass, h->P.inertia, h->P.cita, h->P.shb,
+                        h->P.part_lag, h->P.part_mom};
+        for (void *q : ptrs)
+            if (q) (void)hipFree(q);
+    }
     if (h->aos) (void)hipFree(h->aos);
     if (h->iflag) (void)hipFree(h->iflag);
     for (int k = 0; k < 3; ++k)
@@ -232,6 +286,7 @@ int cgl_upload_state(cgl_t *h, const double *ri, const double *bi)
     if (e == hipSuccess) {
         h->cur = 0; // uploaded states are held in RB form in buffer 0
         h->state_layout = cgl::RB;
+        h->primed = 0;
         dispatch(h, [&](auto impl) {
             decltype(impl)::phi_aos(h, h->aos, tmp);
             decltype(impl)::pack(h, h->aos, 1);
@@ -276,6 +331,7 @@ int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, i
     for (int m = 0; m < nmodes; ++m) ic.amp[m] = amp[m];
     h->cur = 0; // initial states are held in RB form in buffer 0
     h->state_layout = cgl::RB;
+    h->primed = 0;
     dispatch(h, [&](auto impl) { decltype(impl)::init_cylinder(h, ic); });
     if ((rc = check_launch("init_cylinder"))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -341,7 +397,12 @@ int cgl_calmacros(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *u
     int rc = set_device(h);
     if (rc) return rc;
     if ((rc = ensure_scratch(h, true, false, false))) return rc;
-    dispatch(h, [&](auto impl) { decltype(impl)::moments(h, true); });
+    dispatch(h, [&](auto impl) {
+        if (h->ext)
+            decltype(impl)::moments_ext(h, true);
+        else
+            decltype(impl)::moments(h, true);
+    });
     if ((rc = check_launch("moments"))) return rc;
     double *dst[7] = {rho, rho_r, rho_b, ux, uy, uz, phi};
     for (int k = 0; k < 7; ++k)
@@ -358,8 +419,12 @@ int cgl_derivatives(cgl_t *h, double *gradx, double *grady, double *gradz, doubl
     if (rc) return rc;
     if ((rc = ensure_scratch(h, true, false, true))) return rc;
     dispatch(h, [&](auto impl) {
-        decltype(impl)::moments(h, false);
-        decltype(impl)::derivatives(h, h->mac, h->iflag);
+        if (h->ext) {
+            decltype(impl)::derivatives_ext(h, h->mac, h->iflag);
+        } else {
+            decltype(impl)::moments(h, false);
+            decltype(impl)::derivatives(h, h->mac, h->iflag);
+        }
     });
     if ((rc = check_launch("derivatives"))) return rc;
     double *dst[6] = {gradx, grady, gradz, n_x, n_y, n_z};
@@ -394,7 +459,17 @@ int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moment
     if (ms_total) *ms_total = ms;
     // pass 2: per-kernel durations, one event pair around each launch
     double km = 0.0, kc = 0.0;
-    for (int t = 0; t < nsteps; ++t) {
+    for (int t = 0; t < nsteps && h->ext; ++t) {
+        // with walls/particles: one event pair around the whole step
+        float a = 0.f;
+        HIPCHK(hipEventRecord(h->ev[0], h->stream));
+        enqueue_step(h);
+        HIPCHK(hipEventRecord(h->ev[2], h->stream));
+        HIPCHK(hipEventSynchronize(h->ev[2]));
+        HIPCHK(hipEventElapsedTime(&a, h->ev[0], h->ev[2]));
+        kc += a;
+    }
+    for (int t = 0; t < nsteps && !h->ext; ++t) {
         float a = 0.f, b = 0.f;
         HIPCHK(hipEventRecord(h->ev[0], h->stream));
         dispatch(h, [&](auto impl) { decltype(impl)::moments(h, false); });
@@ -412,6 +487,96 @@ int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moment
     }
     if (ms_moments) *ms_moments = km;
     if (ms_collide) *ms_collide = kc;
+    return CGL_OK;
+}
+
+int cgl_set_particles(cgl_t *h, const double *center, const double *up, const double *rp, const double *rad,
+                      const double *mass, const double *inertia, const double *cita, double fg)
+{
+    if (!h || !center || !rad || !mass || !inertia || !cita) return fail(CGL_ERR_ARG, "null argument");
+    if (h->npar == 0) return fail(CGL_ERR_ARG, "the handle was created with npar = 0");
+    int rc = set_device(h);
+    if (rc) return rc;
+    const int n3[3] = {h->p.nx, h->p.ny, h->p.nz};
+    for (int k = 0; k < h->npar; ++k) {
+        if (!(rad[k] > 0) || !(mass[k] > 0) || !(inertia[k] > 0)) return fail(CGL_ERR_ARG, "particle %d: bad radius or mass", k);
+        for (int a = 0; a < 3; ++a)
+            if (!h->E.wall[a][0] && 2.0 * (rad[k] + 4.0) + 3.0 > n3[a])
+                return fail(CGL_ERR_ARG, "particle %d: box along periodic axis %d is smaller than the particle's reach", k, a);
+    }
+    const size_t b3 = 3 * (size_t)h->npar * sizeof(double), b1 = (size_t)h->npar * sizeof(double);
+    HIPCHK(hipMemcpyAsync(h->P.center, center, b3, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.center_old, center, b3, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.center_prev, center, b3, hipMemcpyHostToDevice, h->stream));
+    if (up) HIPCHK(hipMemcpyAsync(h->P.up, up, b3, hipMemcpyHostToDevice, h->stream));
+    else HIPCHK(hipMemsetAsync(h->P.up, 0, b3, h->stream));
+    if (rp) HIPCHK(hipMemcpyAsync(h->P.rp, rp, b3, hipMemcpyHostToDevice, h->stream));
+    else HIPCHK(hipMemsetAsync(h->P.rp, 0, b3, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.rad, rad, b1, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.mass, mass, b1, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.inertia, inertia, b1, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.cita, cita, b1, hipMemcpyHostToDevice, h->stream));
+    h->P.fg = fg;
+    // repulsion planes: wall faces sit half-way between ghost and first layer
+    for (int a = 0; a < 3; ++a) {
+        h->P.rep_on[2 * a] = h->E.wall[a][0];
+        h->P.rep_on[2 * a + 1] = h->E.wall[a][1];
+        h->P.rep_plane[2 * a] = 1.5;
+        h->P.rep_plane[2 * a + 1] = n3[a] - 0.5;
+    }
+    // set_walls of the start-up (S:212): fresh masks
+    for (int b = 0; b < 2; ++b) HIPCHK(hipMemsetAsync(h->ntype[b], 0, h->N * sizeof(unsigned), h->stream));
+    HIPCHK(hipMemsetAsync(h->walls_i, 0, h->N * sizeof(int), h->stream));
+    HIPCHK(hipMemsetAsync(h->pin_i, 0, h->N * sizeof(int), h->stream));
+    HIPCHK(hipMemsetAsync(h->pout_i, 0, h->N * sizeof(int), h->stream));
+    h->ntcur = 0;
+    dispatch(h, [&](auto impl) { decltype(impl)::build_masks(h, 0); });
+    if ((rc = check_launch("set_walls"))) return rc;
+    h->primed = 0;
+    h->cold = 1;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CGL_OK;
+}
+
+int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *force, double *torque,
+                      double *fcap, double *tcap, double *frep)
+{
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    if (h->npar == 0) return fail(CGL_ERR_ARG, "the handle was created with npar = 0");
+    int rc = set_device(h);
+    if (rc) return rc;
+    const size_t b3 = 3 * (size_t)h->npar * sizeof(double);
+    double *dst[8] = {center, up, rp, force, torque, fcap, tcap, frep};
+    double *src[8] = {h->P.center, h->P.up, h->P.rp, h->P.fp, h->P.tp, h->P.ten, h->P.tent, h->P.frep};
+    for (int k = 0; k < 8; ++k)
+        if (dst[k]) HIPCHK(hipMemcpyAsync(dst[k], src[k], b3, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CGL_OK;
+}
+
+__global__ void k_masks_out(const unsigned *nt, size_t N, int *walls_all)
+{
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) walls_all[n] = (nt[n] & cgl::NT_SOLID) ? 10 : 0;
+}
+
+int cgl_download_masks(cgl_t *h, int32_t *walls, int32_t *walls_all, int32_t *pin, int32_t *pout)
+{
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    if (h->npar == 0) return fail(CGL_ERR_ARG, "the handle was created with npar = 0");
+    int rc = set_device(h);
+    if (rc) return rc;
+    const size_t bytes = h->N * sizeof(int);
+    if (walls) HIPCHK(hipMemcpyAsync(walls, h->walls_i, bytes, hipMemcpyDeviceToHost, h->stream));
+    if (pin) HIPCHK(hipMemcpyAsync(pin, h->pin_i, bytes, hipMemcpyDeviceToHost, h->stream));
+    if (pout) HIPCHK(hipMemcpyAsync(pout, h->pout_i, bytes, hipMemcpyDeviceToHost, h->stream));
+    if (walls_all) {
+        if ((rc = ensure_scratch(h, false, false, true))) return rc;
+        k_masks_out<<<(unsigned)((h->N + 255) / 256), 256, 0, h->stream>>>(h->ntype[h->ntcur], h->N, h->iflag);
+        if ((rc = check_launch("masks"))) return rc;
+        HIPCHK(hipMemcpyAsync(walls_all, h->iflag, bytes, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
     return CGL_OK;
 }
 

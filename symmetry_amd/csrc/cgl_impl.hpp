@@ -7,6 +7,7 @@
 
 #include "../../include/cgl.h"
 #include "cgl_kernels.hpp"
+#include "cgl_particles.hpp"
 
 struct cgl_handle {
     cgl_params p;
@@ -34,6 +35,16 @@ struct cgl_handle {
     hipStream_t stream;
     hipEvent_t ev[3];
     int steps_done;
+    // walls and particles
+    int ext;             // 1: wall faces or particles present (EXT kernels)
+    int npar;
+    int primed;          // start-up derivatives done for the current state
+    int cold;            // first step after set_particles: ffi = fi (S:894)
+    cgl::Ext E;          // wall faces and device pointers (ntype set per launch)
+    unsigned *ntype[2];  // node-type masks, double buffered (the previous step's are walls_old)
+    int ntcur;           // buffer holding the masks in force
+    int *walls_i, *pin_i, *pout_i;
+    cgl::Part P;         // particle state (device pointers)
 };
 
 namespace cgl {
@@ -120,17 +131,126 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     static void pack(cgl_handle *h, const double *aos, int red)
     {
         k_pack<S, P, SHIFT><<<node_grid(h->L), node_block(), 0, h->stream>>>(
-            h->L, aos, (S *)h->pops[h->cur][red ? 0 : 1], (const P *)h->phi[h->phicur], red);
+            h->L, aos, (S *)h->pops[h->cur][red ? 0 : 1], (const P *)h->phi[h->phicur], red, h->E);
     }
     static void unpack(cgl_handle *h, double *aos, int red)
     {
         if (h->state_layout == RB)
             k_unpack<S, P, SHIFT><<<node_grid(h->L), node_block(), 0, h->stream>>>(
-                h->L, (const S *)h->pops[h->cur][red ? 0 : 1], aos, (const P *)h->phi[h->phicur], red);
+                h->L, (const S *)h->pops[h->cur][red ? 0 : 1], aos, (const P *)h->phi[h->phicur], red, h->E);
         else
             k_unpack_fs<S, SHIFT><<<node_grid(h->L), node_block(), 0, h->stream>>>(
                 h->L, (const S *)h->fsf[h->cur], h->fss[h->cur], aos, red);
     }
+    // ---- walls and particles
+    static Ext ext_with(const cgl_handle *h, const unsigned *nt)
+    {
+        Ext E = h->E;
+        E.ntype = h->npar ? nt : nullptr;
+        return E;
+    }
+    static dim3 pgrid(const cgl_handle *h, int slices) { return dim3(h->npar, slices, 1); }
+    static void moments_ext(cgl_handle *h, bool full)
+    {
+        const void *i0, *i1;
+        state_in(h, i0, i1);
+        const P *phio = (const P *)h->phi[h->phicur];
+        P *phi = (P *)h->phi[h->phicur ^ 1];
+        const Ext E = ext_with(h, h->ntype[h->ntcur]);
+        if (full)
+            k_moments<RB, S, R, P, SHIFT, true, true>
+                <<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, h->mac, E);
+        else
+            k_moments<RB, S, R, P, SHIFT, false, true>
+                <<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, nullptr, E);
+        wall_phi(h, phi);
+        if (full) wall_phi(h, h->mac + 6 * h->N); // the FP64 phi output
+    }
+    template <class T> static void wall_phi(cgl_handle *h, T *phi)
+    {
+        int m = h->L.nx * h->L.ny;
+        if (h->L.ny * h->L.nz > m) m = h->L.ny * h->L.nz;
+        if (h->L.nx * h->L.nz > m) m = h->L.nx * h->L.nz;
+        k_wall_phi<T><<<dim3((m + 255) / 256, 6, 1), 256, 0, h->stream>>>(h->L, h->E, phi);
+    }
+    // start-up derivatives (S:214) for the capillary force of the first step
+    static void prime(cgl_handle *h)
+    {
+        moments_ext(h, false);
+        P *phi = (P *)h->phi[h->phicur ^ 1];
+        const Ext E = ext_with(h, h->ntype[h->ntcur]);
+        k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[h->ntcur], h->pin_i, phi);
+        k_derivs_ext<P><<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, E, phi, nullptr, nullptr);
+    }
+    static void derivatives_ext(cgl_handle *h, double *out, int *flag)
+    {
+        moments_ext(h, false);
+        P *phi = (P *)h->phi[h->phicur ^ 1];
+        const Ext E = ext_with(h, h->ntype[h->ntcur]);
+        if (h->npar)
+            k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[h->ntcur], h->pin_i, phi);
+        Ext E2 = E;
+        E2.der = nullptr;
+        k_derivs_ext<P><<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, E2, phi, out, flag);
+    }
+    // masks of the current particle centres into buffer `buf` (set_walls, S:326-395)
+    static void build_masks(cgl_handle *h, int buf)
+    {
+        const Ext E = ext_with(h, h->ntype[buf]);
+        k_masks_solid<<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[buf], h->walls_i);
+        k_masks_links<<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[buf], h->walls_i, h->pin_i,
+                                                               h->pout_i);
+    }
+    // one time step with walls and/or particles (S:216-251)
+    static void step_ext(cgl_handle *h)
+    {
+        if (h->npar && !h->primed) {
+            prime(h);
+            h->primed = 1;
+        }
+        moments_ext(h, false); // calmacros with the masks in force
+        P *phi = (P *)h->phi[h->phicur ^ 1];
+        const int nb = h->ntcur ^ 1;
+        if (h->npar) {
+            const Ext Eo = ext_with(h, h->ntype[h->ntcur]);
+            const S *rs = (const S *)h->pops[h->cur][0], *bs = (const S *)h->pops[h->cur][1];
+            k_lagrange<P><<<pgrid(h, LAG_BLOCKS), PB, 0, h->stream>>>(h->L, Eo, h->P, phi, h->walls_i);
+            k_momentum<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->ntype[h->ntcur],
+                                                                          h->walls_i, h->pout_i, rs, bs, h->cold);
+            h->cold = 0;
+            k_move<<<1, PB, 0, h->stream>>>(h->L, Eo, h->P, 1);
+            // set_walls: clear the spare buffer (built two steps ago) and the
+            // int masks (built last step), then the new masks
+            k_masks_clear<<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->P.center_prev, 4.0,
+                                                                   h->ntype[nb], nullptr, nullptr, nullptr);
+            k_masks_clear<<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->P.center_old, 4.0,
+                                                                   h->ntype[nb] /*already clear*/, h->walls_i,
+                                                                   h->pin_i, h->pout_i);
+            build_masks(h, nb);
+            const Ext En = ext_with(h, h->ntype[nb]);
+            k_refill<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[h->ntcur], h->ntype[nb],
+                                                                 h->pout_i, phi);
+            k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[nb], h->pin_i, phi);
+        }
+        {
+            const void *i0, *i1;
+            void *o0, *o1;
+            state_in(h, i0, i1);
+            state_out(h, o0, o1);
+            const P *phio = (const P *)h->phi[h->phicur];
+            const Ext En = ext_with(h, h->ntype[nb]);
+            dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, (h->L.nx + h->xc - 1) / h->xc);
+            k_collide<RB, RB, S, R, P, SHIFT, true>
+                <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
+            if (h->npar) {
+                S *ro = (S *)o0, *bo = (S *)o1;
+                k_curve<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(
+                    h->L, En, h->P, h->ntype[nb], h->walls_i, h->pout_i, ro, bo, SHIFT ? (const float *)phi : nullptr);
+            }
+        }
+        if (h->npar) h->ntcur = nb;
+    }
+
     static void init_cylinder(cgl_handle *h, const CylIC &ic)
     {
         k_init_phi<P><<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, ic, (P *)h->phi[h->phicur]);

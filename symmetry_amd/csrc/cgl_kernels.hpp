@@ -48,6 +48,21 @@ struct Lat {
 // per-direction array is below 4 GiB (checked at creation), so loads and
 // stores use the scalar-base + 32-bit vector-offset addressing of gfx950
 // instead of 64-bit vector address arithmetic.
+// Walls and particles (kernels instantiated with EXT = true).  Node types are
+// bit flags of the masks of the reference's set_walls (S:326-395).
+enum : unsigned { NT_SOLID = 1u, NT_POUT = 2u, NT_NEAR = 4u, NT_REFILL = 8u, NT_PIN = 16u };
+struct Ext {
+    int wall[3][2];          // 1: half-way bounce-back wall face with a ghost layer (S:1557-1590)
+    int npar;
+    int pad;
+    const unsigned *ntype;   // node types of the masks in force
+    const int *pout;         // live-fluid particle id, 1-based (S:365)
+    const double *center;    // particle centres [npar][3], 1-based lattice coordinates
+    const double *cita;      // contact angle per particle (S:21)
+    double *mac;             // 6 x N: rho, rho_R, rho_B, ux, uy, uz at NEAR nodes
+    double *der;             // 6 x N: gradx, grady, gradz, n_x, n_y, n_z at NEAR nodes
+};
+
 template <class T> __device__ __forceinline__ const T &at(const T *base, unsigned idx)
 {
     return *(const T *)((const char *)base + idx * (unsigned)sizeof(T));
@@ -155,19 +170,49 @@ template <class R> struct Node<R, true> {
 
 // Pull gather at node (x, y, z) from x - c_i (streaming, S:743-753).
 // ax: red shift fraction of this node (RB shifted storage only).
+// Half-way wall bounce-back in pull form: a population whose upstream node is
+// a wall ghost layer is the node's own opposite population (boundary then
+// streaming, S:1557-1590 and S:743-753).  bbmask bit i set = direction i.
+struct WallBB {
+    unsigned mask = 0;
+};
+__device__ __forceinline__ bool is_ghost(const Ext &E, const Lat &L, int x, int y, int z)
+{
+    return (E.wall[0][0] && x == 0) || (E.wall[0][1] && x == L.nx - 1) || (E.wall[1][0] && y == 0) ||
+           (E.wall[1][1] && y == L.ny - 1) || (E.wall[2][0] && z == 0) || (E.wall[2][1] && z == L.nz - 1);
+}
+__device__ __forceinline__ unsigned wall_bb_mask(const Ext &E, const Lat &L, int x, int y, int z)
+{
+    const bool lo[3] = {E.wall[0][0] && x == 1, E.wall[1][0] && y == 1, E.wall[2][0] && z == 1};
+    const bool hi[3] = {E.wall[0][1] && x == L.nx - 2, E.wall[1][1] && y == L.ny - 2, E.wall[2][1] && z == L.nz - 2};
+    unsigned m = 0;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+        // the upstream node x - c_i lies in a ghost layer
+        const bool g = (CX[i] == 1 && lo[0]) || (CX[i] == -1 && hi[0]) || (CY[i] == 1 && lo[1]) ||
+                       (CY[i] == -1 && hi[1]) || (CZ[i] == 1 && lo[2]) || (CZ[i] == -1 && hi[2]);
+        if (g) m |= 1u << i;
+    }
+    return m;
+}
+
 template <int LIN, class S, class R, bool SHIFT>
 __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void *p1, const unsigned xo[3],
-                                       const unsigned yo[3], const unsigned zo[3], R ax, Node<R, SHIFT> &nd)
+                                       const unsigned yo[3], const unsigned zo[3], R ax, Node<R, SHIFT> &nd,
+                                       unsigned bb = 0)
 {
     const size_t N = (size_t)L.N;
     const R rho0 = R(L.rho0);
+    const unsigned self = xo[1] + yo[1] + zo[1];
     if constexpr (LIN == RB) {
         const S *rs = (const S *)p0, *bs = (const S *)p1;
         if constexpr (!SHIFT) {
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-                const S *ri = rs + (size_t)i * N, *bi = bs + (size_t)i * N;
+                const bool w = (bb >> i) & 1u;
+                const unsigned m = w ? self : xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+                const int d = w ? OPC[i] : i;
+                const S *ri = rs + (size_t)d * N, *bi = bs + (size_t)d * N;
                 nd.r[i] = StoreOf<S>::ld(ri, m);
                 nd.b[i] = StoreOf<S>::ld(bi, m);
             }
@@ -176,8 +221,10 @@ __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void 
             R sr = 0;
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-                const S *ri = rs + (size_t)i * N, *bi = bs + (size_t)i * N;
+                const bool w = (bb >> i) & 1u;
+                const unsigned m = w ? self : xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+                const int d = w ? OPC[i] : i;
+                const S *ri = rs + (size_t)d * N, *bi = bs + (size_t)d * N;
                 R rv = StoreOf<S>::ld(ri, m);
                 R bv = StoreOf<S>::ld(bi, m);
                 sr += rv;
@@ -319,10 +366,10 @@ __device__ __forceinline__ void node_offsets(const Lat &L, int x, int y, int z, 
 // ---------------------------------------------------------------- K1 moments
 // Writes phi of the step.  With FULL it also writes the macroscopic fields
 // rho, rho_R, rho_B, ux, uy, uz, phi (bare velocity, as calmacros) as FP64.
-template <int LIN, class S, class R, class P, bool SHIFT, bool FULL>
+template <int LIN, class S, class R, class P, bool SHIFT, bool FULL, bool EXT = false>
 __global__ __launch_bounds__(256) void k_moments(Lat L, const void *__restrict__ p0, const void *__restrict__ p1,
                                                  const P *__restrict__ phio, P *__restrict__ phi,
-                                                 double *__restrict__ mac)
+                                                 double *__restrict__ mac, Ext E = Ext{})
 {
     unsigned bx, by, bz;
     tile_of_block(L, bx, by, bz);
@@ -330,14 +377,40 @@ __global__ __launch_bounds__(256) void k_moments(Lat L, const void *__restrict__
     const int y = by * TY + threadIdx.y;
     const int x = bz;
     if (z >= L.nz || y >= L.ny) return;
+    if constexpr (EXT)
+        if (is_ghost(E, L, x, y, z)) return; // phi copied from the adjacent layer afterwards (S:806)
     unsigned xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
     const unsigned n = xo[1] + yo[1] + zo[1];
     R ax = 0;
     if constexpr (SHIFT && LIN == RB) ax = R(0.5) * (R(1) + R(at(phio, n)));
     Node<R, SHIFT> nd;
-    gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd);
+    unsigned nt = 0;
+    if constexpr (EXT) {
+        gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd, wall_bb_mask(E, L, x, y, z));
+        if (E.ntype) nt = at(E.ntype, n);
+    } else {
+        gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd);
+    }
     Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
+    if constexpr (EXT) {
+        if (nt & NT_SOLID) { // calmacros at solid nodes (S:784-789); rho_R, rho_B untouched
+            mo.phi = 0;
+            mo.rho = R(L.rho0);
+            mo.ux = mo.uy = mo.uz = 0;
+        }
+        if (nt & NT_NEAR) {
+            const size_t N = (size_t)L.N;
+            E.mac[n] = mo.rho;
+            if (!(nt & NT_SOLID)) {
+                E.mac[N + n] = mo.sr;
+                E.mac[2 * N + n] = mo.sb;
+            }
+            E.mac[3 * N + n] = mo.ux;
+            E.mac[4 * N + n] = mo.uy;
+            E.mac[5 * N + n] = mo.uz;
+        }
+    }
     at(phi, n) = P(mo.phi);
     if constexpr (FULL) {
         const size_t N = (size_t)L.N;
@@ -648,10 +721,46 @@ template <class R, class P> struct K2Smem {
 __device__ __forceinline__ int slot5(int p) { return (p + 10) % 5; }
 __device__ __forceinline__ int slot3(int p) { return (p + 9) % 3; }
 
-template <int LIN, int LOUT, class S, class R, class P, bool SHIFT>
+// Akai wetting correction of a unit normal n at a live fluid node whose
+// particle centre is c (S:541-582): the candidate closer to n of the two
+// normals at contact angle cita to the wall normal nw.  FP64.
+__device__ __forceinline__ void wetting(double xn, double yn, double zn, const double *c, double cita, double n[3])
+{
+    double nw[3] = {xn - c[0], yn - c[1], zn - c[2]};
+    double st = sqrt(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
+    nw[0] /= st;
+    nw[1] /= st;
+    nw[2] /= st;
+    double dmax = fabs(nw[0] - n[0]);
+    if (fabs(nw[1] - n[1]) > dmax) dmax = fabs(nw[1] - n[1]);
+    if (fabs(nw[2] - n[2]) > dmax) dmax = fabs(nw[2] - n[2]);
+    const double theta = dmax < 1.0e-5 ? 1.0e-5 : acos(nw[0] * n[0] + nw[1] * n[1] + nw[2] * n[2]);
+    const double ca = cos(cita), sa = sin(cita);
+    const double a1 = ca - sa * cos(theta) / sin(theta), a2 = ca + sa * cos(theta) / sin(theta);
+    const double b1 = sa / sin(theta), b2 = -sa / sin(theta);
+    double v1[3], v2[3];
+    for (int k = 0; k < 3; ++k) {
+        v1[k] = a1 * nw[k] + b1 * n[k];
+        v2[k] = a2 * nw[k] + b2 * n[k];
+    }
+    const double db1 = (v1[0] - n[0]) * (v1[0] - n[0]) + (v1[1] - n[1]) * (v1[1] - n[1]) + (v1[2] - n[2]) * (v1[2] - n[2]);
+    const double db2 = (v2[0] - n[0]) * (v2[0] - n[0]) + (v2[1] - n[1]) * (v2[1] - n[1]) + (v2[2] - n[2]) * (v2[2] - n[2]);
+    for (int k = 0; k < 3; ++k) n[k] = db1 < db2 ? v1[k] : v2[k];
+}
+
+// No curvature force next to a wall ghost layer (the guard z >= 3, x >= 3 of
+// S:624, whose normals the gradient pass never writes).
+__device__ __forceinline__ bool away_from_walls(const Ext &E, const Lat &L, int x, int y, int z)
+{
+    return !((E.wall[0][0] && x < 2) || (E.wall[0][1] && x > L.nx - 3) || (E.wall[1][0] && y < 2) ||
+             (E.wall[1][1] && y > L.ny - 3) || (E.wall[2][0] && z < 2) || (E.wall[2][1] && z > L.nz - 3));
+}
+
+template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT = false>
 __global__ __launch_bounds__(256) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
                                                  void *__restrict__ o0, void *__restrict__ o1,
-                                                 const P *__restrict__ phi, const P *__restrict__ phio, int xc)
+                                                 const P *__restrict__ phi, const P *__restrict__ phio, int xc,
+                                                 Ext E = Ext{})
 {
     __shared__ K2Smem<R, P> sm;
     const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
@@ -672,28 +781,57 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const void *__restrict__
             dst[jy][jz] = at(phi, (unsigned)(gx + gy * L.nz + gz));
         }
     };
-    // unit normal of plane p on the (TY+2) x (TZ+2) halo region (S:498-532)
+    // unit normal of plane p on the (TY+2) x (TZ+2) halo region (S:498-532);
+    // with EXT: zero at solid and ghost nodes, wetting at live fluid nodes
     auto normals = [&](int p) {
         const int s = slot3(p);
         for (int e = tid; e < (TY + 2) * (TZ + 2); e += TZ * TY) {
             int jy = e / (TZ + 2), jz = e - jy * (TZ + 2);
-            R gx, gy, gz;
-            grad19<R>([&](int dx, int dy, int dz) { return sm.phi[slot5(p + dx)][jy + 1 + dy][jz + 1 + dz]; }, gx,
-                      gy, gz);
             R nx = 0, ny = 0, nz = 0;
-            const R g2 = gx * gx + gy * gy + gz * gz;
-            if constexpr (sizeof(R) == 8) {
-                const R fnorm = fsqrt(g2);
-                if (fnorm > R(EVAP_LIM)) {
-                    nx = gx / fnorm;
-                    ny = gy / fnorm;
-                    nz = gz / fnorm;
+            bool skip = false;
+            int gxw = 0, gyw = 0, gzw = 0;
+            unsigned nt = 0;
+            if constexpr (EXT) {
+                gxw = wrapm(p, L.nx);
+                gyw = wrapm(y0 - 1 + jy, L.ny);
+                gzw = wrapm(z0 - 1 + jz, L.nz);
+                skip = is_ghost(E, L, gxw, gyw, gzw);
+                if (!skip && E.ntype) {
+                    nt = at(E.ntype, (unsigned)(gxw * sx + gyw * L.nz + gzw));
+                    skip = (nt & NT_SOLID) != 0;
                 }
-            } else if (g2 > R(EVAP_LIM * EVAP_LIM)) {
-                const R inv = __builtin_amdgcn_rsqf(g2);
-                nx = gx * inv;
-                ny = gy * inv;
-                nz = gz * inv;
+            }
+            if (!skip) {
+                R gx, gy, gz;
+                grad19<R>([&](int dx, int dy, int dz) { return sm.phi[slot5(p + dx)][jy + 1 + dy][jz + 1 + dz]; },
+                          gx, gy, gz);
+                const R g2 = gx * gx + gy * gy + gz * gz;
+                bool fl = false;
+                if constexpr (sizeof(R) == 8) {
+                    const R fnorm = fsqrt(g2);
+                    if (fnorm > R(EVAP_LIM)) {
+                        nx = gx / fnorm;
+                        ny = gy / fnorm;
+                        nz = gz / fnorm;
+                        fl = true;
+                    }
+                } else if (g2 > R(EVAP_LIM * EVAP_LIM)) {
+                    const R inv = __builtin_amdgcn_rsqf(g2);
+                    nx = gx * inv;
+                    ny = gy * inv;
+                    nz = gz * inv;
+                    fl = true;
+                }
+                if constexpr (EXT) {
+                    if (fl && (nt & NT_POUT)) {
+                        const int pp = at(E.pout, (unsigned)(gxw * sx + gyw * L.nz + gzw)) - 1;
+                        double nv[3] = {double(nx), double(ny), double(nz)};
+                        wetting(gxw + 1.0, gyw + 1.0, gzw + 1.0, E.center + 3 * pp, E.cita[pp], nv);
+                        nx = R(nv[0]);
+                        ny = R(nv[1]);
+                        nz = R(nv[2]);
+                    }
+                }
             }
             sm.n[0][s][jy][jz] = nx;
             sm.n[1][s][jy][jz] = ny;
@@ -713,19 +851,55 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const void *__restrict__
         normals(x + 1);
         __syncthreads();
         if (!active) continue;
+        unsigned nt = 0, bb = 0;
+        if constexpr (EXT) {
+            if (is_ghost(E, L, x, y, z)) continue; // ghost layers are not collided
+            bb = wall_bb_mask(E, L, x, y, z);
+        }
 
         // gradient, normal and flag at the node (S:498-532)
         R gx, gy, gz;
         grad19<R>([&](int dx, int dy, int dz) { return sm.phi[slot5(x + dx)][ty + 2 + dy][tz + 2 + dz]; }, gx, gy,
                   gz);
         const R g2 = gx * gx + gy * gy + gz * gz;
-        const bool flag = sizeof(R) == 8 ? fsqrt(g2) > R(EVAP_LIM) : g2 > R(EVAP_LIM * EVAP_LIM);
+        bool flag = sizeof(R) == 8 ? fsqrt(g2) > R(EVAP_LIM) : g2 > R(EVAP_LIM * EVAP_LIM);
         const R nx = sm.n[0][slot3(x)][ty + 1][tz + 1];
         const R ny = sm.n[1][slot3(x)][ty + 1][tz + 1];
         const R nz = sm.n[2][slot3(x)][ty + 1][tz + 1];
+
+        unsigned xo[3], yo[3], zo[3];
+        node_offsets(L, x, y, z, xo, yo, zo);
+        const unsigned n = xo[1] + yo[1] + zo[1];
+        bool force_on = flag;
+        if constexpr (EXT) {
+            if (E.ntype) nt = at(E.ntype, n);
+            if (nt & NT_SOLID) flag = false;
+            if (flag && (nt & NT_POUT)) { // wetting rescales the gradient too (S:569-581)
+                const R fnorm = fsqrt(g2);
+                gx = fnorm * nx;
+                gy = fnorm * ny;
+                gz = fnorm * nz;
+            }
+            force_on = flag && !(nt & NT_POUT) && away_from_walls(E, L, x, y, z);
+            if (nt & NT_NEAR) {
+                const size_t N = (size_t)L.N;
+                const bool fl = flag;
+                E.der[n] = fl ? double(gx) : 0.0;
+                E.der[N + n] = fl ? double(gy) : 0.0;
+                E.der[2 * N + n] = fl ? double(gz) : 0.0;
+                E.der[3 * N + n] = double(nx);
+                E.der[4 * N + n] = double(ny);
+                E.der[5 * N + n] = double(nz);
+                if (!(nt & NT_SOLID) && !fl) { // non-interface fluid: gradient kept, normal zero (S:519-530)
+                    E.der[n] = double(gx);
+                    E.der[N + n] = double(gy);
+                    E.der[2 * N + n] = double(gz);
+                }
+            }
+        }
         // normal derivatives for the curvature (S:635-660)
         R d[3][3];
-        if (flag) {
+        if (force_on) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
 #pragma unroll
@@ -747,20 +921,77 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const void *__restrict__
                 for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
         }
 
-        unsigned xo[3], yo[3], zo[3];
-        node_offsets(L, x, y, z, xo, yo, zo);
-        const unsigned n = xo[1] + yo[1] + zo[1];
         R ax = 0, aw[Q];
         if constexpr (SHIFT && LIN == RB) ax = R(0.5) * (R(1) + R(at(phio, n)));
         if constexpr (SHIFT && LOUT == RB) {
-            // shift of the slot read by x + c_i at the next step: that node's phi now
+            // shift of the slot read by x + c_i at the next step: that node's phi
+            // now; a slot that bounces back off a wall is read by x itself
+            const R aself = R(0.5) * (R(1) + R(sm.phi[slot5(x)][ty + 2][tz + 2]));
 #pragma unroll
-            for (int i = 0; i < Q; ++i)
+            for (int i = 0; i < Q; ++i) {
                 aw[i] = R(0.5) * (R(1) + R(sm.phi[slot5(x + CX[i])][ty + 2 + CY[i]][tz + 2 + CZ[i]]));
+                if constexpr (EXT)
+                    if ((bb >> OPC[i]) & 1u) aw[i] = aself;
+            }
+        }
+        if constexpr (EXT && LIN == RB && LOUT == RB) {
+            if (nt & NT_SOLID) {
+                // solid nodes are not collided: their streamed populations pass
+                // through to the next streaming (S:622, S:712, S:743)
+                const size_t N = (size_t)L.N;
+                const S *rs = (const S *)i0, *bs = (const S *)i1;
+                S *ro = (S *)o0, *bo = (S *)o1;
+#pragma unroll
+                for (int i = 0; i < Q; ++i) {
+                    const bool w = (bb >> i) & 1u;
+                    const unsigned m = w ? n : xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+                    const int dsrc = w ? OPC[i] : i;
+                    R rv = StoreOf<S>::ld(rs + (size_t)dsrc * N, m), bv = StoreOf<S>::ld(bs + (size_t)dsrc * N, m);
+                    if constexpr (SHIFT) {
+                        const R dsh = R(W[i]) * R(L.rho0) * (ax - aw[i]);
+                        rv += dsh;
+                        bv -= dsh;
+                    }
+                    StoreOf<S>::st(ro + (size_t)i * N, n, rv);
+                    StoreOf<S>::st(bo + (size_t)i * N, n, bv);
+                }
+                continue;
+            }
         }
         Node<R, SHIFT> nd;
-        gather<LIN, S, R, SHIFT>(L, i0, i1, xo, yo, zo, ax, nd);
-        const Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
+        Moments<R, SHIFT> mo;
+        bool refilled = false;
+        if constexpr (EXT) refilled = (nt & NT_REFILL) != 0;
+        if (refilled) {
+            // refilled node (S:1364-1391): SRT equilibria of the refill densities
+            // and the rigid-body velocity; macros as refilling set them
+            const size_t N = (size_t)L.N;
+            const double rr = E.mac[N + n], rbv = E.mac[2 * N + n];
+            const double ux = E.mac[3 * N + n], uy = E.mac[4 * N + n], uz = E.mac[5 * N + n];
+            const double usqr = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const double uc = ux * CX[i] + uy * CY[i] + uz * CZ[i];
+                const double poly = (1.0 + 3.0 * uc + 4.5 * uc * uc - 1.5 * usqr);
+                const double er = rr * W[i] * poly, eb = rbv * W[i] * poly;
+                if constexpr (!SHIFT) {
+                    nd.r[i] = er;
+                    nd.b[i] = eb;
+                } else {
+                    nd.ft[i] = R((er + eb) - W[i] * L.rho0);
+                }
+            }
+            mo.sr = R(rr);
+            mo.sb = R(rbv);
+            mo.rho = R(E.mac[n]);
+            mo.ux = R(ux);
+            mo.uy = R(uy);
+            mo.uz = R(uz);
+            mo.phi = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
+        } else {
+            gather<LIN, S, R, SHIFT>(L, i0, i1, xo, yo, zo, ax, nd, bb);
+            mo = moments<R, SHIFT>(L, nd);
+        }
         const R phin = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
         R f[Q];
         if constexpr (!SHIFT) {
@@ -770,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const void *__restrict__
 #pragma unroll
             for (int i = 0; i < Q; ++i) f[i] = nd.ft[i];
         }
-        collide<R, SHIFT>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, flag, d);
+        collide<R, SHIFT>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, force_on, d);
         const Recol<R> rc = recol<R, SHIFT>(L, mo, flag);
         write_node<LOUT, S, R, SHIFT>(L, o0, o1, n, f, rc, nx, ny, nz, aw);
     }
@@ -831,9 +1062,11 @@ template <class P> __device__ __forceinline__ float shift_a(P ph, int red)
 // Pack one colour: S_i(y) = P_i(y + c_i) - w_i rho0 a(y + c_i) (inverse of
 // the periodic streaming permutation, so that the next pull gather returns P;
 // the shift is that of the reading node y + c_i).
+// With wall faces, a slot whose reader x + c_i is a ghost node holds the
+// population that bounces back into x: P_opc(i)(x).
 template <class S, class P, bool SHIFT>
 __global__ __launch_bounds__(256) void k_pack(Lat L, const double *__restrict__ aos, S *__restrict__ soa,
-                                              const P *__restrict__ phio, int red)
+                                              const P *__restrict__ phio, int red, Ext E = Ext{})
 {
     const int z = blockIdx.x * TZ + threadIdx.x;
     const int y = blockIdx.y * TY + threadIdx.y;
@@ -843,18 +1076,26 @@ __global__ __launch_bounds__(256) void k_pack(Lat L, const double *__restrict__ 
     node_offsets(L, x, y, z, xo, yo, zo);
     const int n = xo[1] + yo[1] + zo[1];
     const size_t N = (size_t)L.N;
+    const bool live = !is_ghost(E, L, x, y, z);
     for (int i = 0; i < Q; ++i) {
         int src = xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
-        double v = aos[(size_t)src * Q + i];
+        int qi = i;
+        if (live && is_ghost(E, L, x + CX[i], y + CY[i], z + CZ[i])) {
+            src = n;
+            qi = OPC[i];
+        }
+        double v = aos[(size_t)src * Q + qi];
         if constexpr (SHIFT) v = v - W[i] * (L.rho0 * (double)shift_a(phio[src], red));
         StoreOf<S>::st(soa + (size_t)i * N + n, (decltype(StoreOf<S>::ld(soa)))v);
     }
 }
 
 // Unpack one colour: P_i(x) = S_i(x - c_i) + w_i rho0 a(x), as FP64 AoS.
+// With wall faces, a population that would come from a ghost node is the
+// node's own opposite slot (half-way bounce-back).
 template <class S, class P, bool SHIFT>
 __global__ __launch_bounds__(256) void k_unpack(Lat L, const S *__restrict__ soa, double *__restrict__ aos,
-                                                const P *__restrict__ phio, int red)
+                                                const P *__restrict__ phio, int red, Ext E = Ext{})
 {
     const int z = blockIdx.x * TZ + threadIdx.x;
     const int y = blockIdx.y * TY + threadIdx.y;
@@ -864,9 +1105,15 @@ __global__ __launch_bounds__(256) void k_unpack(Lat L, const S *__restrict__ soa
     node_offsets(L, x, y, z, xo, yo, zo);
     const int n = xo[1] + yo[1] + zo[1];
     const size_t N = (size_t)L.N;
+    const bool live = !is_ghost(E, L, x, y, z);
     for (int i = 0; i < Q; ++i) {
         int src = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-        double v = double(StoreOf<S>::ld(soa + (size_t)i * N + src));
+        int qi = i;
+        if (live && is_ghost(E, L, x - CX[i], y - CY[i], z - CZ[i])) {
+            src = n;
+            qi = OPC[i];
+        }
+        double v = double(StoreOf<S>::ld(soa + (size_t)qi * N + src));
         if constexpr (SHIFT) v = v + W[i] * (L.rho0 * (double)shift_a(phio[n], red));
         aos[(size_t)n * Q + i] = v;
     }

@@ -1,0 +1,632 @@
+// cgl_particles.hpp -- moving-particle kernels of the lattice update for
+// gfx950: masks (set_walls, S:326-395), refill of uncovered nodes
+// (refilling, S:1351-1418), ghost phi at live solid nodes (S:468-492),
+// moving-wall bounce-back (curve, S:990-1034), momentum exchange
+// (momentum, S:1178-1261), Lagrange-point capillary force (Lagrange_points
+// + capillary_force2, S:1799-2028), repulsion (S:1594-1671) and the explicit
+// Newton-Euler update (movement, S:1337-1346).  All particle quantities are
+// FP64.
+//
+// Every kernel visits the bounding box of each particle (grid.x = particle,
+// grid.y = slices of the box).  Per-particle sums are reduced in FP64 first
+// within a workgroup and then, in a fixed order, over workgroups, so results
+// are reproducible from run to run.
+//
+// Positions are the reference's 1-based lattice coordinates.  On periodic
+// axes a box is unwrapped around the particle, so displacements are the
+// minimum-image ones.
+#pragma once
+
+#include "cgl_kernels.hpp"
+
+namespace cgl {
+
+constexpr int PB = 256;        // threads per particle-kernel workgroup
+constexpr int PSLICES = 8;     // workgroups per particle for box kernels
+constexpr int LAG_BLOCKS = 8;  // workgroups per particle for the Lagrange points
+constexpr int LA_M = 180, LA_N = 90, LA_NUM = LA_M * (LA_N + 1);
+
+struct Part {
+    int npar;
+    int pad;
+    double fg;                   // repulsion scale Fg (S:84)
+    double rep_plane[6];         // repulsion planes per face (S:1607-1612)
+    int rep_on[6];
+    double *center, *center_old, *center_prev; // [npar][3]: now, last step, the step before
+    double *up, *rp;             // velocity, angular velocity
+    double *fp, *tp, *frep, *ten, *tent; // momentum-exchange force/torque, repulsion, capillary force/torque
+    double *rad, *mass, *inertia, *cita, *shb;
+    double *part_lag;            // [npar][LAG_BLOCKS][7]
+    double *part_mom;            // [npar][PSLICES][6]
+};
+
+// The box of particle p around centre c with margin m: unwrapped 0-based
+// lower corner and extent per axis.
+struct Box {
+    int lo[3], w[3];
+};
+__device__ __forceinline__ Box box_of(const double *c, double R, double m)
+{
+    Box b;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = (int)floor(c[a] - 1.0 - R - m);
+        int hi = (int)ceil(c[a] - 1.0 + R + m);
+        b.w[a] = hi - b.lo[a] + 1;
+    }
+    return b;
+}
+
+// Node k of the box: unwrapped coordinates X[3] and the node index; false
+// when the node falls outside a wall-bounded axis.
+__device__ __forceinline__ bool box_node(const Lat &L, const Box &b, unsigned k, int X[3], unsigned &n)
+{
+    const int n3[3] = {L.nx, L.ny, L.nz};
+    int c[3];
+    c[2] = (int)(k % b.w[2]);
+    k /= b.w[2];
+    c[1] = (int)(k % b.w[1]);
+    c[0] = (int)(k / b.w[1]);
+    int g[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        X[a] = b.lo[a] + c[a];
+        g[a] = wrapm(X[a], n3[a]);
+    }
+    n = ((unsigned)g[0] * L.ny + g[1]) * L.nz + g[2];
+    return true;
+}
+__device__ __forceinline__ bool box_in_domain(const Ext &E, const Lat &L, const int X[3])
+{
+    const int n3[3] = {L.nx, L.ny, L.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if ((E.wall[a][0] || E.wall[a][1]) && (X[a] < 0 || X[a] >= n3[a])) return false;
+    return true;
+}
+__device__ __forceinline__ bool interior(const Ext &E, const Lat &L, const int X[3])
+{
+    const int n3[3] = {L.nx, L.ny, L.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (E.wall[a][0] && X[a] < 1) return false;
+        if (E.wall[a][1] && X[a] > n3[a] - 2) return false;
+    }
+    return true;
+}
+__device__ __forceinline__ unsigned nb_index(const Lat &L, const int X[3], int dx, int dy, int dz)
+{
+    const int gx = wrapm(X[0] + dx, L.nx), gy = wrapm(X[1] + dy, L.ny), gz = wrapm(X[2] + dz, L.nz);
+    return ((unsigned)gx * L.ny + gy) * L.nz + gz;
+}
+__device__ __forceinline__ unsigned box_size(const Box &b) { return (unsigned)b.w[0] * b.w[1] * b.w[2]; }
+
+// Rigid-body velocity at lattice node X (1-based position X+1) of particle
+// p, from centre c (S:1009-1014).
+__device__ __forceinline__ void wall_vel(const Part &P, int p, const double *c, const int X[3], double xr[3],
+                                         double uw[3])
+{
+    xr[0] = (double)(X[0] + 1) - c[0];
+    xr[1] = (double)(X[1] + 1) - c[1];
+    xr[2] = (double)(X[2] + 1) - c[2];
+    const double *u = P.up + 3 * p, *w = P.rp + 3 * p;
+    uw[0] = u[0] + xr[2] * w[1] - xr[1] * w[2];
+    uw[1] = u[1] + xr[0] * w[2] - xr[2] * w[0];
+    uw[2] = u[2] + xr[1] * w[0] - xr[0] * w[1];
+}
+
+// ------------------------------------------------------------------ masks
+// Zero the masks around centres c with margin m (int fields optional).
+static __global__ __launch_bounds__(PB) void k_masks_clear(Lat L, Ext E, Part P, const double *ctr, double margin,
+                                                    unsigned *nt, int *walls, int *pin, int *pout)
+{
+    const int p = blockIdx.x;
+    const Box b = box_of(ctr + 3 * p, P.rad[p], margin);
+    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+        int X[3];
+        unsigned n;
+        box_node(L, b, k, X, n);
+        if (!box_in_domain(E, L, X)) continue;
+        nt[n] = 0;
+        if (walls) {
+            walls[n] = 0;
+            pin[n] = 0;
+            pout[n] = 0;
+        }
+    }
+}
+
+// Solid nodes by the FP64 distance test of S:344-345 (same operation order);
+// the highest particle index wins where spheres overlap, as in the serial
+// reference.  NEAR marks the band within R+3 whose macroscopic fields and
+// gradients the particle kernels read.
+static __global__ __launch_bounds__(PB) void k_masks_solid(Lat L, Ext E, Part P, unsigned *nt, int *walls)
+{
+    const int p = blockIdx.x;
+    const double *c = P.center + 3 * p;
+    const double R = P.rad[p];
+    const Box b = box_of(c, R, 3.0);
+    const double r2 = R * R, rn = (R + 3.0) * (R + 3.0);
+    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+        int X[3];
+        unsigned n;
+        box_node(L, b, k, X, n);
+        if (!box_in_domain(E, L, X)) continue;
+        const double dx = (double)(X[0] + 1) - c[0], dy = (double)(X[1] + 1) - c[1], dz = (double)(X[2] + 1) - c[2];
+        const double dist = dx * dx + dy * dy + dz * dz;
+        if (dist <= r2) {
+            atomicMax(walls + n, p + 1);
+            atomicOr(nt + n, NT_SOLID | NT_NEAR);
+        } else if (dist <= rn) {
+            atomicOr(nt + n, NT_NEAR);
+        }
+    }
+}
+
+// Live fluid (pout) and live solid (pin) marks (S:354-373).
+static __global__ __launch_bounds__(PB) void k_masks_links(Lat L, Ext E, Part P, unsigned *nt, const int *walls, int *pin,
+                                                    int *pout)
+{
+    const int p = blockIdx.x;
+    const Box b = box_of(P.center + 3 * p, P.rad[p], 1.0);
+    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+        int X[3];
+        unsigned n;
+        box_node(L, b, k, X, n);
+        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
+        if (nt[n] & NT_SOLID) continue;
+        for (int i = 1; i < Q; ++i) {
+            const unsigned m = nb_index(L, X, CX[i], CY[i], CZ[i]);
+            if (walls[m] == p + 1) {
+                atomicMax(pout + n, p + 1);
+                atomicMax(pin + m, p + 1);
+                atomicOr(nt + n, NT_POUT);
+                atomicOr(nt + m, NT_PIN);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ refill
+// Nodes uncovered by the move (live fluid now, solid before): rigid-body
+// velocity about the old centre, unweighted neighbour means of the colour
+// densities over previously fluid neighbours, equilibria in K2 (S:1364-1391).
+template <class P>
+__global__ __launch_bounds__(PB) void k_refill(Lat L, Ext E, Part Pt, const unsigned *nt_old, unsigned *nt_new,
+                                               const int *pout, P *phi)
+{
+    const int p = blockIdx.x;
+    const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
+    const size_t N = (size_t)L.N;
+    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+        int X[3];
+        unsigned n;
+        box_node(L, b, k, X, n);
+        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
+        if (pout[n] != p + 1 || !(nt_old[n] & NT_SOLID)) continue;
+        double xr[3], uw[3];
+        wall_vel(Pt, p, Pt.center_old + 3 * p, X, xr, uw);
+        double sr = 0.0, sb = 0.0, num = 0.0;
+        for (int i = 1; i < Q; ++i) {
+            const unsigned m = nb_index(L, X, CX[i], CY[i], CZ[i]);
+            if (!(nt_old[m] & NT_SOLID)) {
+                sr = sr + E.mac[N + m];
+                sb = sb + E.mac[2 * N + m];
+                num = num + 1.0;
+            }
+        }
+        const double rr = sr / num, rb = sb / num, rho = rr + rb;
+        E.mac[n] = rho;
+        E.mac[N + n] = rr;
+        E.mac[2 * N + n] = rb;
+        E.mac[3 * N + n] = uw[0];
+        E.mac[4 * N + n] = uw[1];
+        E.mac[5 * N + n] = uw[2];
+        phi[n] = P((rr - rb) / rho);
+        atomicOr(nt_new + n, NT_REFILL);
+    }
+}
+
+// Ghost phi at live solid nodes: w-weighted mean over fluid neighbours
+// (S:468-491).
+template <class P>
+__global__ __launch_bounds__(PB) void k_ghost_phi(Lat L, Ext E, Part Pt, const unsigned *nt, const int *pin, P *phi)
+{
+    const int p = blockIdx.x;
+    const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
+    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+        int X[3];
+        unsigned n;
+        box_node(L, b, k, X, n);
+        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
+        if (pin[n] != p + 1) continue;
+        double wsum = 0.0, acc = 0.0;
+        for (int i = 1; i < Q; ++i) {
+            const unsigned m = nb_index(L, X, CX[i], CY[i], CZ[i]);
+            if (!(nt[m] & NT_SOLID)) {
+                wsum = wsum + W[i];
+                acc = acc + W[i] * double(phi[m]);
+            }
+        }
+        phi[n] = P(acc / wsum);
+    }
+}
+
+// --------------------------------------------------------------- bounce-back
+// Moving-wall half-way bounce-back on links from live fluid x into its
+// particle (S:1004-1018):  r_i(x) = r*_opp(x) + 6 w_i rho_R(x) (u_w . c_i).
+// The pull gather of the next step reads direction i of x from the solid
+// node x - c_i, so the value is written into that solid node's slot.
+template <class S, bool SHIFT>
+__global__ __launch_bounds__(PB) void k_curve(Lat L, Ext E, Part Pt, const unsigned *nt, const int *walls,
+                                              const int *pout, S *ro, S *bo, const float *phis)
+{
+    const int p = blockIdx.x;
+    const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
+    const size_t N = (size_t)L.N;
+    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+        int X[3];
+        unsigned n;
+        box_node(L, b, k, X, n);
+        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
+        if (pout[n] != p + 1 || (nt[n] & NT_SOLID)) continue;
+        double xr[3], uw[3];
+        wall_vel(Pt, p, Pt.center + 3 * p, X, xr, uw);
+        const double rr = E.mac[N + n], rbv = E.mac[2 * N + n];
+        for (int i = 1; i < Q; ++i) {
+            const unsigned m = nb_index(L, X, -CX[i], -CY[i], -CZ[i]);
+            if (walls[m] != p + 1) continue;
+            const double star = uw[0] * CX[i] + uw[1] * CY[i] + uw[2] * CZ[i];
+            const int o = OPC[i];
+            double vr = double(StoreOf<S>::ld(ro + (size_t)o * N, n));
+            double vb = double(StoreOf<S>::ld(bo + (size_t)o * N, n));
+            if constexpr (SHIFT) {
+                // slot o of x is read by x + c_o = m; slot i of m is read by x
+                const double am = 0.5 * (1.0 + double(phis[m])), ax = 0.5 * (1.0 + double(phis[n]));
+                vr += W[o] * L.rho0 * am;
+                vb += W[o] * L.rho0 * (1.0 - am);
+                vr = vr + 6.0 * W[i] * rr * star - W[i] * L.rho0 * ax;
+                vb = vb + 6.0 * W[i] * rbv * star - W[i] * L.rho0 * (1.0 - ax);
+            } else {
+                vr = vr + 6.0 * W[i] * rr * star;
+                vb = vb + 6.0 * W[i] * rbv * star;
+            }
+            StoreOf<S>::st(ro + (size_t)i * N, m, (decltype(StoreOf<S>::ld(ro)))vr);
+            StoreOf<S>::st(bo + (size_t)i * N, m, (decltype(StoreOf<S>::ld(bo)))vb);
+        }
+    }
+}
+
+// Fixed-order FP64 reduction of K values per thread across the workgroup.
+template <int K> __device__ __forceinline__ void block_sum(double v[K], double *out)
+{
+    __shared__ double sh[PB][K];
+    for (int k = 0; k < K; ++k) sh[threadIdx.x][k] = v[k];
+    __syncthreads();
+    for (int s = PB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < K; ++k) sh[threadIdx.x][k] += sh[threadIdx.x + s][k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < K; ++k) out[k] = sh[0][k];
+}
+
+// ---------------------------------------------------------- momentum exchange
+// Galilean-invariant momentum exchange on the links of particle p
+// (S:1198-1252): f_i(x) is the bounced-back population (held in the solid
+// slot), ffi_opp(x) the node's post-collision one.  On the first step after
+// the start-up (cold != 0) there is no previous collision: the reference's
+// initialization sets ffi = fi (S:894), the node's own populations, which the
+// pull layout holds at x + c_i.
+template <class S, bool SHIFT>
+__global__ __launch_bounds__(PB) void k_momentum(Lat L, Ext E, Part Pt, const unsigned *nt, const int *walls,
+                                                 const int *pout, const S *rs, const S *bs, int cold)
+{
+    const int p = blockIdx.x;
+    const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
+    const size_t N = (size_t)L.N;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+        int X[3];
+        unsigned n;
+        box_node(L, b, k, X, n);
+        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
+        if (pout[n] != p + 1 || (nt[n] & NT_SOLID)) continue;
+        double xr[3], uw[3];
+        wall_vel(Pt, p, Pt.center + 3 * p, X, xr, uw);
+        for (int i = 1; i < Q; ++i) {
+            const unsigned m = nb_index(L, X, -CX[i], -CY[i], -CZ[i]);
+            if (walls[m] != p + 1) continue;
+            const int o = OPC[i];
+            double fi = double(StoreOf<S>::ld(rs + (size_t)i * N, m)) + double(StoreOf<S>::ld(bs + (size_t)i * N, m));
+            const unsigned no = cold ? nb_index(L, X, CX[i], CY[i], CZ[i]) : n;
+            double fo = double(StoreOf<S>::ld(rs + (size_t)o * N, no)) + double(StoreOf<S>::ld(bs + (size_t)o * N, no));
+            if constexpr (SHIFT) {
+                fi += W[i] * L.rho0;
+                fo += W[o] * L.rho0;
+            }
+            const double fmx = ((double)CX[i] - uw[0]) * fi - ((double)CX[o] - uw[0]) * fo;
+            const double fmy = ((double)CY[i] - uw[1]) * fi - ((double)CY[o] - uw[1]) * fo;
+            const double fmz = ((double)CZ[i] - uw[2]) * fi - ((double)CZ[o] - uw[2]) * fo;
+            acc[0] -= fmx;
+            acc[1] -= fmy;
+            acc[2] -= fmz;
+            acc[3] -= xr[1] * fmz - xr[2] * fmy;
+            acc[4] -= xr[2] * fmx - xr[0] * fmz;
+            acc[5] -= xr[0] * fmy - xr[1] * fmx;
+        }
+    }
+    block_sum<6>(acc, Pt.part_mom + ((size_t)p * PSLICES + blockIdx.y) * 6);
+}
+
+// --------------------------------------------------- capillary force (FP64)
+__device__ __forceinline__ void unitize3(double a[3])
+{
+    double st = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    a[0] /= st;
+    a[1] /= st;
+    a[2] /= st;
+}
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3])
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = -(a[0] * b[2] - a[2] * b[0]);
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// 180 x 91 surface points of sphere p (S:1821-1962) and their capillary
+// force and torque (S:1993-2024): IDW-interpolated phase field over the
+// fluid corners of the enclosing cell, the nearest interface corner's normal
+// with the wetting correction, force 4.5 k beta (1-phi^2)^2 dS |n x n_w| sigma.
+template <class P>
+__global__ __launch_bounds__(PB) void k_lagrange(Lat L, Ext E, Part Pt, const P *phi, const int *walls)
+{
+    const int p = blockIdx.x;
+    const double PI = 3.14159265358979323846;
+    const double alpha = 2.0 / 180.0 * PI;
+    const double kcap = 0.134 * 1.43;
+    const double *c = Pt.center + 3 * p;
+    const double R = Pt.rad[p], cita = Pt.cita[p];
+    const size_t N = (size_t)L.N;
+    static const int ex[8] = {0, 1, 1, 0, 0, 1, 1, 0};
+    static const int ey[8] = {0, 0, 1, 1, 0, 0, 1, 1};
+    static const int ez[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int num = blockIdx.y * PB + threadIdx.x; num < LA_NUM; num += gridDim.y * PB) {
+        const int na = num / (LA_N + 1), nb = num - na * (LA_N + 1);
+        const double aa = (double)na * alpha, bb = (double)nb * alpha;
+        double Lp[3];
+        Lp[0] = c[0] + R * sin(bb) * cos(aa);
+        Lp[1] = c[1] + R * sin(bb) * sin(aa);
+        Lp[2] = c[2] + R * cos(bb);
+        const double ds = R * R * sin(bb) * (alpha * alpha);
+        int c0[3];
+        for (int a = 0; a < 3; ++a) c0[a] = (int)Lp[a];
+        double wsum = 0.0, lrho = 0.0;
+        for (int k = 0; k < 8; ++k) {
+            const int f1[3] = {c0[0] + ex[k], c0[1] + ey[k], c0[2] + ez[k]};
+            const int Xc[3] = {f1[0] - 1, f1[1] - 1, f1[2] - 1};
+            const unsigned m = nb_index(L, Xc, 0, 0, 0);
+            if (walls[m] <= 0) {
+                const double star = double(phi[m]) + E.der[m] * (Lp[0] - (double)f1[0]) +
+                                    E.der[N + m] * (Lp[1] - (double)f1[1]) + E.der[2 * N + m] * (Lp[2] - (double)f1[2]);
+                const double dd = ((double)f1[0] - Lp[0]) * ((double)f1[0] - Lp[0]) +
+                                  ((double)f1[1] - Lp[1]) * ((double)f1[1] - Lp[1]) +
+                                  ((double)f1[2] - Lp[2]) * ((double)f1[2] - Lp[2]);
+                const double wgt = 1.0 / dd;
+                lrho = lrho + wgt * star;
+                wsum = wsum + wgt;
+            }
+        }
+        lrho = lrho / wsum;
+        if (lrho < -1.0) lrho = -1.0;
+        if (lrho > 1.0) lrho = 1.0;
+        if (lrho > 0.0) acc[6] += ds;
+        if (!(fabs(lrho) < 0.99)) continue;
+        double nw[3] = {Lp[0] - c[0], Lp[1] - c[1], Lp[2] - c[2]};
+        unitize3(nw);
+        double dmin = 100.0;
+        int best[3] = {c0[0], c0[1], c0[2]};
+        for (int k = 0; k < 8; ++k) {
+            const int f1[3] = {c0[0] + ex[k], c0[1] + ey[k], c0[2] + ez[k]};
+            const int Xc[3] = {f1[0] - 1, f1[1] - 1, f1[2] - 1};
+            const unsigned m = nb_index(L, Xc, 0, 0, 0);
+            if (walls[m] > 0) continue;
+            const double nx = E.der[3 * N + m], ny = E.der[4 * N + m], nz = E.der[5 * N + m];
+            if (!(fabs(nx * nx + ny * ny + nz * nz) > 0.0)) continue;
+            const double dd = ((double)f1[0] - Lp[0]) * ((double)f1[0] - Lp[0]) +
+                              ((double)f1[1] - Lp[1]) * ((double)f1[1] - Lp[1]) +
+                              ((double)f1[2] - Lp[2]) * ((double)f1[2] - Lp[2]);
+            if (sqrt(dd) < dmin) {
+                dmin = sqrt(dd);
+                best[0] = f1[0];
+                best[1] = f1[1];
+                best[2] = f1[2];
+            }
+        }
+        const int Xb[3] = {best[0] - 1, best[1] - 1, best[2] - 1};
+        const unsigned mb = nb_index(L, Xb, 0, 0, 0);
+        double nt[3] = {E.der[3 * N + mb], E.der[4 * N + mb], E.der[5 * N + mb]};
+        unitize3(nt);
+        const double theta = acos(nw[0] * nt[0] + nw[1] * nt[1] + nw[2] * nt[2]);
+        const double ca = cos(cita), sa = sin(cita);
+        const double a1 = ca - sa * cos(theta) / sin(theta), a2 = ca + sa * cos(theta) / sin(theta);
+        const double b1 = sa / sin(theta), b2 = -sa / sin(theta);
+        double v1[3], v2[3];
+        for (int k = 0; k < 3; ++k) {
+            v1[k] = a1 * nw[k] + b1 * nt[k];
+            v2[k] = a2 * nw[k] + b2 * nt[k];
+        }
+        const double db1 = (v1[0] - nt[0]) * (v1[0] - nt[0]) + (v1[1] - nt[1]) * (v1[1] - nt[1]) +
+                           (v1[2] - nt[2]) * (v1[2] - nt[2]);
+        const double db2 = (v2[0] - nt[0]) * (v2[0] - nt[0]) + (v2[1] - nt[1]) * (v2[1] - nt[1]) +
+                           (v2[2] - nt[2]) * (v2[2] - nt[2]);
+        double ln[3];
+        for (int k = 0; k < 3; ++k) ln[k] = db1 < db2 ? v1[k] : v2[k];
+        // capillary_force2 (S:1996-2022)
+        const double xr[3] = {Lp[0] - c[0], Lp[1] - c[1], Lp[2] - c[2]};
+        double nw2[3] = {xr[0], xr[1], xr[2]};
+        unitize3(nw2);
+        const double ie[3] = {-ln[0], -ln[1], -ln[2]};
+        double tmp[3], ntan[3];
+        cross3(ie, nw2, tmp);
+        const double proj = sqrt(tmp[0] * tmp[0] + tmp[1] * tmp[1] + tmp[2] * tmp[2]);
+        cross3(tmp, ie, ntan);
+        unitize3(ntan);
+        const double om = 1.0 - lrho * lrho;
+        const double star = 4.5 * kcap * L.beta * (om * om) * ds * proj;
+        const bool right = cita == PI / 2.0;
+        double fv[3];
+        for (int k = 0; k < 3; ++k) fv[k] = star * (right ? nw2[k] : ntan[k]) * L.sigma;
+        double tt[3];
+        cross3(xr, fv, tt);
+        for (int k = 0; k < 3; ++k) {
+            acc[k] += fv[k];
+            acc[3 + k] += tt[k];
+        }
+    }
+    block_sum<7>(acc, Pt.part_lag + ((size_t)p * LAG_BLOCKS + blockIdx.y) * 7);
+}
+
+// ------------------------------------------------- repulsion + Newton-Euler
+// One workgroup: per-particle sums of the partial results in a fixed order,
+// soft wall and pair repulsion (S:1602-1668), explicit update of velocity,
+// angular velocity and centre (S:1337-1346).
+static __global__ __launch_bounds__(PB) void k_move(Lat L, Ext E, Part P, int with_capillary)
+{
+    const int np = P.npar;
+    const double range = 3.0, order = 0.01;
+    const int n3[3] = {L.nx, L.ny, L.nz};
+    for (int p = threadIdx.x; p < np; p += PB) {
+        double m[6] = {0, 0, 0, 0, 0, 0}, c[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int s = 0; s < PSLICES; ++s)
+            for (int k = 0; k < 6; ++k) m[k] += P.part_mom[((size_t)p * PSLICES + s) * 6 + k];
+        if (with_capillary)
+            for (int s = 0; s < LAG_BLOCKS; ++s)
+                for (int k = 0; k < 7; ++k) c[k] += P.part_lag[((size_t)p * LAG_BLOCKS + s) * 7 + k];
+        for (int k = 0; k < 3; ++k) {
+            P.fp[3 * p + k] = m[k];
+            P.tp[3 * p + k] = m[3 + k];
+            P.ten[3 * p + k] = c[k];
+            P.tent[3 * p + k] = c[3 + k];
+        }
+        P.shb[p] = c[6];
+        // walls (S:1606-1650): high faces push with +, low faces with -
+        const double R = P.rad[p];
+        double fr[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a) {
+            if (P.rep_on[2 * a]) {
+                const double d = P.center[3 * p + a] - P.rep_plane[2 * a] - R;
+                if (d < range) fr[a] -= 2.0 * (P.fg / order) * ((d - range) * (d - range)) / (range * range);
+            }
+            if (P.rep_on[2 * a + 1]) {
+                const double d = P.rep_plane[2 * a + 1] - P.center[3 * p + a] - R;
+                if (d < range) fr[a] += 2.0 * (P.fg / order) * ((d - range) * (d - range)) / (range * range);
+            }
+        }
+        // pairs (S:1655-1668), minimum image on periodic axes
+        for (int j = 0; j < np; ++j) {
+            if (j == p) continue;
+            double dc[3];
+            for (int a = 0; a < 3; ++a) {
+                dc[a] = P.center[3 * p + a] - P.center[3 * j + a];
+                if (!(E.wall[a][0] || E.wall[a][1])) {
+                    if (dc[a] > 0.5 * n3[a]) dc[a] -= n3[a];
+                    if (dc[a] < -0.5 * n3[a]) dc[a] += n3[a];
+                }
+            }
+            const double d = sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]) - (R + P.rad[j]);
+            if (d < range)
+                for (int a = 0; a < 3; ++a) fr[a] = fr[a] - (P.fg / order) * ((d - range) * (d - range)) / (range * range) * dc[a] / d;
+        }
+        for (int a = 0; a < 3; ++a) P.frep[3 * p + a] = fr[a];
+    }
+    __syncthreads(); // every repulsion read the old centres
+    for (int p = threadIdx.x; p < np; p += PB) {
+        for (int k = 0; k < 3; ++k) {
+            const double ftot = P.fp[3 * p + k] + P.frep[3 * p + k] + P.ten[3 * p + k];
+            P.up[3 * p + k] = P.up[3 * p + k] + ftot / P.mass[p];
+            P.rp[3 * p + k] = P.rp[3 * p + k] + (P.tp[3 * p + k] + P.tent[3 * p + k]) / P.inertia[p];
+            P.center_prev[3 * p + k] = P.center_old[3 * p + k];
+            P.center_old[3 * p + k] = P.center[3 * p + k];
+            P.center[3 * p + k] = P.center[3 * p + k] + P.up[3 * p + k] + 0.5 * ftot / P.mass[p];
+        }
+    }
+}
+
+// --------------------------------------------- wall-layer phi and derivatives
+// phi of the wall ghost layers = phi of the adjacent layer (S:806-807).  The
+// reference copies whole planes face by face (z, then x, then y), so a ghost
+// node on an edge or corner ends up with phi of the nearest live node; the
+// source index clamps every wall coordinate into the live range, which gives
+// that result without ordering the six faces.
+template <class P> __global__ __launch_bounds__(256) void k_wall_phi(Lat L, Ext E, P *phi)
+{
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int axis = blockIdx.y / 2, side = blockIdx.y % 2;
+    if (!E.wall[axis][side]) return;
+    const int n3[3] = {L.nx, L.ny, L.nz};
+    const int u = (axis + 1) % 3, v = (axis + 2) % 3;
+    if (t >= (unsigned)(n3[u] * n3[v])) return;
+    int cd[3], cs[3];
+    cd[u] = (int)(t / n3[v]);
+    cd[v] = (int)(t % n3[v]);
+    cd[axis] = side ? n3[axis] - 1 : 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        cs[a] = cd[a];
+        if (E.wall[a][0] && cs[a] < 1) cs[a] = 1;
+        if (E.wall[a][1] && cs[a] > n3[a] - 2) cs[a] = n3[a] - 2;
+    }
+    phi[((size_t)cd[0] * L.ny + cd[1]) * L.nz + cd[2]] = phi[((size_t)cs[0] * L.ny + cs[1]) * L.nz + cs[2]];
+}
+
+// Derivatives of the current phi with the masks in force (S:498-587): zero at
+// solid and ghost nodes, wetting at live fluid interface nodes.  Stores the
+// NEAR band for the capillary kernel and, with out != null, every node as
+// FP64 (gradx, grady, gradz, n_x, n_y, n_z) plus the flag.
+template <class P>
+__global__ __launch_bounds__(256) void k_derivs_ext(Lat L, Ext E, const P *__restrict__ phi, double *__restrict__ out,
+                                                    int *__restrict__ flag_out)
+{
+    const int z = blockIdx.x * TZ + threadIdx.x;
+    const int y = blockIdx.y * TY + threadIdx.y;
+    const int x = blockIdx.z;
+    if (z >= L.nz || y >= L.ny) return;
+    unsigned xo[3], yo[3], zo[3];
+    node_offsets(L, x, y, z, xo, yo, zo);
+    const unsigned n = xo[1] + yo[1] + zo[1];
+    const size_t N = (size_t)L.N;
+    double g[3] = {0, 0, 0}, nv[3] = {0, 0, 0};
+    bool fl = false;
+    const unsigned nt = E.ntype ? E.ntype[n] : 0u;
+    if (!is_ghost(E, L, x, y, z) && !(nt & NT_SOLID)) {
+        grad19<double>([&](int dx, int dy, int dz) { return double(phi[xo[1 + dx] + yo[1 + dy] + zo[1 + dz]]); },
+                       g[0], g[1], g[2]);
+        const double fnorm = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        if (fnorm > EVAP_LIM) {
+            fl = true;
+            for (int k = 0; k < 3; ++k) nv[k] = g[k] / fnorm;
+            if (nt & NT_POUT) {
+                const int pp = E.pout[n] - 1;
+                wetting(x + 1.0, y + 1.0, z + 1.0, E.center + 3 * pp, E.cita[pp], nv);
+                for (int k = 0; k < 3; ++k) g[k] = fnorm * nv[k];
+            }
+        }
+    }
+    if ((nt & NT_NEAR) && E.der)
+        for (int k = 0; k < 3; ++k) {
+            E.der[k * N + n] = g[k];
+            E.der[(3 + k) * N + n] = nv[k];
+        }
+    if (out) {
+        for (int k = 0; k < 3; ++k) {
+            out[k * N + n] = g[k];
+            out[(3 + k) * N + n] = nv[k];
+        }
+        flag_out[n] = fl ? 1 : 0;
+    }
+}
+
+} // namespace cgl

@@ -93,3 +93,52 @@ def workload_state(cfg, exact=True, nx=None, ny=None, nz=None):
     return cylinder_state(nx, ny, nz, axis="z", center=(nx / 2 + 0.5, ny / 2 + 0.5), r0=cfg["r0"],
                           amps=cfg["amps"], wavelength=float(nz), phase0=1.0, sigma=cfg["sigma"],
                           rho0=cfg["rho0"], beta=cfg["beta"], exact=exact)
+
+
+def flat_interface_state(nx, ny, nz, *, z0, rho0, beta=0.7, drop=None, sigma=0.0, amp=0.0):
+    """Red fluid below the surface z = z0 + amp cos(2 pi (x - 1) / nx)
+    (1-based lattice units), blue above, phi = tanh(6 beta 0.134 (z0 - z)) at
+    rest, optionally with a red drop ``drop = (cx, cy, cz, r)`` (union of the
+    two red regions, with the Laplace jump 3 sigma / r inside the drop).  Same
+    profile and equilibria as ``cylinder_state``."""
+    x = np.arange(1, nx + 1, dtype=np.float64)[:, None, None]
+    y = np.arange(1, ny + 1, dtype=np.float64)[None, :, None]
+    z = np.arange(1, nz + 1, dtype=np.float64)[None, None, :]
+    zs = z0 + amp * np.cos(2.0 * math.pi * (x - 1.0) / nx)
+    star = np.broadcast_to(zs - z, (nx, ny, nz)).copy()
+    rho = np.full((nx, ny, nz), rho0)
+    if drop is not None:
+        cx, cy, cz, r = drop
+        sd = r - np.sqrt((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2)
+        rho = np.where(sd > 0.0, rho0 + 3.0 * sigma / r, rho0)
+        star = np.maximum(star, sd)
+    phi = np.tanh(star * 6.0 * beta * 0.134)
+    rho_r = rho * (phi + 1.0) / 2.0
+    rho_b = rho - rho_r
+    return (rho_r[..., None] * W) * 1.0, (rho_b[..., None] * W) * 1.0, phi
+
+
+def sphere_particles(centers, rad, rho_p, cita_deg, sigma):
+    """Rigid spheres of the reference's kind (S:76-84): mass 4/3 pi rho_p R^3,
+    moment of inertia 2/5 m R^2, contact angle in radians and the repulsion
+    scale Fg = -2 pi R sigma."""
+    centers = np.asarray(centers, dtype=np.float64).reshape(-1, 3)
+    n = len(centers)
+    rad_a = np.broadcast_to(np.asarray(rad, dtype=np.float64), (n,)).copy()
+    rho_a = np.broadcast_to(np.asarray(rho_p, dtype=np.float64), (n,))
+    mass = 4.0 / 3.0 * rho_a * math.pi * rad_a ** 3
+    inertia = 2.0 / 5.0 * mass * rad_a ** 2
+    cita = np.broadcast_to(np.asarray(cita_deg, dtype=np.float64), (n,)) / 180.0 * math.pi
+    fg = -math.pi * 2.0 * float(rad_a[0]) * sigma
+    return dict(center=centers, rad=rad_a, mass=mass, inertia=inertia, cita=cita.copy(), fg=fg)
+
+
+def particle_config2(n=96, rad=10.0):
+    """Single moving sphere at a flat interface: n^3 box, walls at z = 1 and
+    z = n (ghost layers), periodic in x and y, red below z = n/2, sphere of
+    radius ``rad`` at the box centre with density 1.5 and a 60 degree contact
+    angle, tau = 0.8, A = 0.005, beta = 0.7; 5000 steps."""
+    tau = 0.8
+    return dict(nx=n, ny=n, nz=n, bc=(0, 0, 0, 0, 1, 1), z0=n / 2.0, center=(n / 2.0, n / 2.0, n / 2.0),
+                rad=rad, rho_p=1.5, cita_deg=60.0, sigma=0.005, rho0=1.0, nu_r=(tau - 0.5) / 3.0,
+                nu_b=(tau - 0.5) / 3.0, beta=0.7, steps=5000)

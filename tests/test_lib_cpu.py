@@ -67,8 +67,8 @@ def _params(**kw):
     return p
 
 
-@pytest.mark.parametrize("kw,code", [({"nx": 0}, -1), ({"precision": 7}, -1), ({"npar": 3}, -3),
-                                     ({"rho0": 0.0}, -1)])
+@pytest.mark.parametrize("kw,code", [({"nx": 0}, -1), ({"precision": 7}, -1), ({"npar": -1}, -1),
+                                     ({"npar": 3, "layout": 1}, -3), ({"rho0": 0.0}, -1)])
 def test_create_validates_before_touching_the_device(lib, kw, code):
     h = ctypes.c_void_p()
     rc = lib.cgl_create(ctypes.byref(h), ctypes.byref(_params(**kw)))
@@ -76,10 +76,16 @@ def test_create_validates_before_touching_the_device(lib, kw, code):
     assert lib.cgl_last_error()
 
 
-def test_wall_faces_rejected_until_implemented(lib):
-    p = _params()
-    p.bc[4] = symmetry_amd.BC_WALL
+def test_face_validation(lib):
     h = ctypes.c_void_p()
+    p = _params()
+    p.bc[4] = p.bc[5] = 2  # mirror faces are not implemented on the GPU
+    assert lib.cgl_create(ctypes.byref(h), ctypes.byref(p)) == -3
+    p = _params()
+    p.bc[4] = symmetry_amd.BC_WALL  # a periodic face needs a periodic partner
+    assert lib.cgl_create(ctypes.byref(h), ctypes.byref(p)) == -1
+    p = _params(layout=1)
+    p.bc[4] = p.bc[5] = symmetry_amd.BC_WALL  # walls run on the red/blue layout only
     assert lib.cgl_create(ctypes.byref(h), ctypes.byref(p)) == -3
 
 
