@@ -107,19 +107,22 @@ int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning);
 /* Block until all queued device work of the handle has finished. */
 int cgl_sync(cgl_t *h);
 
-/* Particles (npar of the handle): centres in 1-based lattice coordinates
- * [npar][3], velocities and angular velocities [npar][3] (NULL = at rest),
- * radius, mass, moment of inertia and contact angle (radians) per particle,
- * and the repulsion scale Fg (S:84).  Computes the masks of set_walls
- * (S:212, S:326-395).  The reference holds these in center(par,3), upx..rpz,
- * Rad, mass, inertia_moment, cita (S:21, S:45-50, S:76-84). */
+/* Particles (npar of the handle), in the reference's layout: per-particle
+ * vectors are (npar, 3) Fortran arrays, component-major, like center(par,3)
+ * (S:45); velocities and angular velocities pack upx/upy/upz and rpx/rpy/rpz
+ * (S:49) the same way (NULL = at rest).  Centres are 1-based lattice
+ * coordinates.  Then radius, mass, moment of inertia and contact angle
+ * (radians) per particle, and the repulsion scale Fg (S:84).  Computes the
+ * masks as the start-up set_walls does (S:212, S:326-395).  Replaces the
+ * particle set-up of S:76-84 and initia_center (S:926-976). */
 int cgl_set_particles(cgl_t *h, const double *center, const double *up, const double *rp, const double *rad,
                       const double *mass, const double *inertia, const double *cita, double fg);
 
 /* Particle state after the last step: centre, velocity, angular velocity,
  * momentum-exchange force and torque (Fpx.., Tpx.., S:1247-1252), capillary
  * force and torque (tension, tension_t, S:2020-2022), repulsion (Frex..,
- * S:1633-1665), each [npar][3].  Any pointer may be NULL. */
+ * S:1633-1665), each an (npar, 3) component-major array as above.  Any
+ * pointer may be NULL. */
 int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *force, double *torque,
                       double *fcap, double *tcap, double *frep);
 

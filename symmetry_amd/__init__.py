@@ -172,17 +172,20 @@ class Lattice:
         return tuple(v.value for v in t)
 
     def set_particles(self, center, rad, mass, inertia, cita, fg, up=None, rp=None):
+        """center, up, rp: [npar, 3] arrays (passed to the library in the
+        reference's component-major (npar, 3) Fortran layout)."""
         f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
-        args = [f(center), f(up), f(rp), f(rad), f(mass), f(inertia), f(cita)]
+        t = lambda a: None if a is None else np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+        args = [t(center), t(up), t(rp), f(rad), f(mass), f(inertia), f(cita)]
         self._keep = args
         _check(lib().cgl_set_particles(self.h, *(_ptr(a) for a in args), float(fg)))
 
     def get_particles(self):
         n = self.params.npar
         names = ("center", "up", "rp", "force", "torque", "fcap", "tcap", "frep")
-        out = {k: np.empty((n, 3)) for k in names}
+        out = {k: np.empty((3, n)) for k in names}
         _check(lib().cgl_get_particles(self.h, *(_ptr(out[k]) for k in names)))
-        return out
+        return {k: np.ascontiguousarray(v.T) for k, v in out.items()}
 
     def masks(self):
         names = ("walls", "walls_all", "pin", "pout")

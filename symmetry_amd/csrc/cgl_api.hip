@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <new>
+#include <vector>
 
 #include "cgl_impl.hpp"
 
@@ -504,14 +505,20 @@ int cgl_set_particles(cgl_t *h, const double *center, const double *up, const do
             if (!h->E.wall[a][0] && 2.0 * (rad[k] + 4.0) + 3.0 > n3[a])
                 return fail(CGL_ERR_ARG, "particle %d: box along periodic axis %d is smaller than the particle's reach", k, a);
     }
-    const size_t b3 = 3 * (size_t)h->npar * sizeof(double), b1 = (size_t)h->npar * sizeof(double);
-    HIPCHK(hipMemcpyAsync(h->P.center, center, b3, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(h->P.center_old, center, b3, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(h->P.center_prev, center, b3, hipMemcpyHostToDevice, h->stream));
-    if (up) HIPCHK(hipMemcpyAsync(h->P.up, up, b3, hipMemcpyHostToDevice, h->stream));
-    else HIPCHK(hipMemsetAsync(h->P.up, 0, b3, h->stream));
-    if (rp) HIPCHK(hipMemcpyAsync(h->P.rp, rp, b3, hipMemcpyHostToDevice, h->stream));
-    else HIPCHK(hipMemsetAsync(h->P.rp, 0, b3, h->stream));
+    const size_t np = (size_t)h->npar, b3 = 3 * np * sizeof(double), b1 = np * sizeof(double);
+    // (npar, 3) component-major host arrays -> [npar][3] on the device
+    std::vector<double> c3(3 * np), u3(3 * np, 0.0), w3(3 * np, 0.0);
+    for (size_t p = 0; p < np; ++p)
+        for (int k = 0; k < 3; ++k) {
+            c3[3 * p + k] = center[k * np + p];
+            if (up) u3[3 * p + k] = up[k * np + p];
+            if (rp) w3[3 * p + k] = rp[k * np + p];
+        }
+    HIPCHK(hipMemcpy(h->P.center, c3.data(), b3, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->P.center_old, c3.data(), b3, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->P.center_prev, c3.data(), b3, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->P.up, u3.data(), b3, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->P.rp, w3.data(), b3, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpyAsync(h->P.rad, rad, b1, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->P.mass, mass, b1, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->P.inertia, inertia, b1, hipMemcpyHostToDevice, h->stream));
@@ -545,12 +552,17 @@ int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *
     if (h->npar == 0) return fail(CGL_ERR_ARG, "the handle was created with npar = 0");
     int rc = set_device(h);
     if (rc) return rc;
-    const size_t b3 = 3 * (size_t)h->npar * sizeof(double);
+    const size_t np = (size_t)h->npar, b3 = 3 * np * sizeof(double);
     double *dst[8] = {center, up, rp, force, torque, fcap, tcap, frep};
     double *src[8] = {h->P.center, h->P.up, h->P.rp, h->P.fp, h->P.tp, h->P.ten, h->P.tent, h->P.frep};
-    for (int k = 0; k < 8; ++k)
-        if (dst[k]) HIPCHK(hipMemcpyAsync(dst[k], src[k], b3, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<double> tmp(3 * np);
+    for (int v = 0; v < 8; ++v) {
+        if (!dst[v]) continue;
+        HIPCHK(hipMemcpy(tmp.data(), src[v], b3, hipMemcpyDeviceToHost));
+        for (size_t p = 0; p < np; ++p) // [npar][3] -> (npar, 3) component-major
+            for (int k = 0; k < 3; ++k) dst[v][k * np + p] = tmp[3 * p + k];
+    }
     return CGL_OK;
 }
 

@@ -11,7 +11,13 @@
 ! Input: an optional namelist file (first command-line argument) with group
 ! &cgl (see the declarations below).  The initial state is the reference's
 ! initialization (S:850-896) for a liquid cylinder along z through the box
-! centre with nmodes cosine modes of wavelength zl.
+! centre with nmodes cosine modes of wavelength zl (ic = 'cylinder'), or a
+! flat interface at z = z0 with red fluid below (ic = 'interface').  With
+! zwalls the z faces are half-way bounce-back walls with ghost layers (the
+! reference's z=1 kind, S:1557-1590).  npar > 0 adds spheres at
+! (px, py, pz) of radius rad_p, density rho_p and contact angle cita_deg,
+! with mass and moment of inertia as S:81-82 and Fg as S:84; their state is
+! appended to PARTICLES.dat at every output step.
 program cgl_driver
   use iso_c_binding
   use cgl
@@ -19,13 +25,24 @@ program cgl_driver
   integer :: xl = 32, yl = 32, zl = 128, nsteps = 2000, out_every = 100, prec = 0, xchunk = 0, nmodes = 1, layout = 0
   double precision :: sigma = 0.01d0, rho0 = 1.0d0, tau_R = 1.0d0, tau_B = 1.0d0, beta = 0.7d0, Rad_c = 8.0d0
   double precision :: amp(16) = 0.0d0
-  logical :: ic_on_device = .false., write_tecplot = .true., dump_binary = .false.
+  logical :: ic_on_device = .false., write_tecplot = .true., dump_binary = .false., zwalls = .false.
   character(len=256) :: outdir = '.'
+  character(len=16) :: ic = 'cylinder'
+  integer, parameter :: MAXPAR = 64
+  integer :: npar = 0
+  double precision :: z0 = 0.0d0, rad_p = 10.0d0, rho_p = 1.5d0, cita_deg = 90.0d0
+  double precision :: px(MAXPAR) = 0.0d0, py(MAXPAR) = 0.0d0, pz(MAXPAR) = 0.0d0
   namelist /cgl/ xl, yl, zl, nsteps, out_every, prec, xchunk, sigma, rho0, tau_R, tau_B, beta, Rad_c, &
-                 nmodes, amp, ic_on_device, write_tecplot, dump_binary, outdir, layout
+                 nmodes, amp, ic_on_device, write_tecplot, dump_binary, outdir, layout, ic, z0, zwalls, &
+                 npar, px, py, pz, rad_p, rho_p, cita_deg
 
   double precision, allocatable :: ri(:,:,:,:), bi(:,:,:,:)
   double precision, allocatable :: rho(:,:,:), rho_R(:,:,:), rho_B(:,:,:), rho_N(:,:,:), ux(:,:,:), uy(:,:,:), uz(:,:,:)
+  integer(c_int32_t), allocatable, target :: walls_all(:,:,:)
+  ! particle state in the reference's layout: center(par,3) etc. (S:45-50)
+  double precision, allocatable :: center(:,:), up(:,:), rp(:,:), fp(:,:), tp(:,:), fcap(:,:), tcap(:,:), frep(:,:)
+  double precision, allocatable :: prad(:), pmass(:), pinert(:), pcita(:)
+  double precision :: fg
   double precision :: wi(0:18), nu_R, nu_B, ms(3)
   double precision, parameter :: PI = 4.0d0*datan(1.0d0)
   type(cgl_params) :: p
@@ -53,10 +70,15 @@ program cgl_driver
   wi(7:18) = 1.0d0/36.0d0
   write(*,*) 'box', xl, yl, zl, ' nu_R=', nu_R, ' nu_B=', nu_B, ' precision', prec
 
+  if (npar < 0 .or. npar > MAXPAR) then
+    write(*,*) 'npar must be in 0 ..', MAXPAR
+    error stop 2
+  end if
   p%nx = xl; p%ny = yl; p%nz = zl
   p%bc = CGL_BC_PERIODIC
+  if (zwalls) p%bc(5:6) = CGL_BC_WALL
   p%precision = prec
-  p%npar = 0
+  p%npar = npar
   p%xchunk = xchunk
   p%device = 0
   p%layout = layout
@@ -65,14 +87,35 @@ program cgl_driver
   call cgl_check(cgl_create(h, p), 'cgl_create')
 
   allocate(rho(zl,yl,xl), rho_R(zl,yl,xl), rho_B(zl,yl,xl), rho_N(zl,yl,xl), ux(zl,yl,xl), uy(zl,yl,xl), uz(zl,yl,xl))
-  if (ic_on_device) then
+  allocate(walls_all(zl,yl,xl))
+  walls_all = 0
+  if (ic_on_device .and. trim(ic) == 'cylinder') then
     call cgl_check(cgl_init_cylinder(h, dble(xl)/2.0d0 + 0.5d0, dble(yl)/2.0d0 + 0.5d0, Rad_c, &
                    6.0d0*beta*0.134d0, int(nmodes, c_int32_t), amp), 'cgl_init_cylinder')
   else
     allocate(ri(0:18,zl,yl,xl), bi(0:18,zl,yl,xl))
-    call initial_state()
+    if (trim(ic) == 'interface') then
+      call interface_state()
+    else
+      call initial_state()
+    end if
     call cgl_check(cgl_upload_state(h, ri, bi), 'cgl_upload_state')
     deallocate(ri, bi)
+  end if
+  if (npar > 0) then
+    allocate(center(npar,3), up(npar,3), rp(npar,3), fp(npar,3), tp(npar,3), fcap(npar,3), tcap(npar,3), &
+             frep(npar,3), prad(npar), pmass(npar), pinert(npar), pcita(npar))
+    center(:,1) = px(1:npar); center(:,2) = py(1:npar); center(:,3) = pz(1:npar)
+    up = 0.0d0; rp = 0.0d0
+    prad = rad_p
+    pmass = 4.0d0/3.0d0*rho_p*PI*rad_p**3            ! S:81
+    pinert = 2.0d0/5.0d0*pmass*rad_p**2              ! S:82
+    pcita = cita_deg/180.0d0*PI
+    fg = -PI*2.0d0*rad_p*sigma                       ! S:84
+    call cgl_check(cgl_set_particles(h, center, up, rp, prad, pmass, pinert, pcita, fg), 'cgl_set_particles')
+    open(newunit=u, file=trim(outdir)//'/PARTICLES.dat', status='replace')
+    write(u, '(a)') 'VARIABLES=T,P,X,Y,Z,U,V,W,FX,FY,FZ,FCX,FCY,FCZ'
+    close(u)
   end if
 
   t = 0
@@ -80,6 +123,7 @@ program cgl_driver
     if (mod(t, out_every) == 0 .or. t == nsteps) then
       call cgl_check(cgl_calmacros(h, rho, rho_R, rho_B, ux, uy, uz, rho_N), 'cgl_calmacros')
       write(*,'(i8, 3(1x, es22.15))') t, sum(rho_R), sum(rho_B), maxval(sqrt(ux**2 + uy**2 + uz**2))
+      if (npar > 0) call output_particles(t)
       if (write_tecplot) call output_results(t)
       if (dump_binary) call dump_fields(t)
     end if
@@ -123,6 +167,36 @@ contains
     end do
   end subroutine initial_state
 
+  ! flat interface at z = z0, red below, at rest; same profile and
+  ! equilibria as initialization (S:874, S:883-889)
+  subroutine interface_state()
+    integer :: x, y, z
+    double precision :: ph, rr
+    do x = 1, xl
+      do y = 1, yl
+        do z = 1, zl
+          ph = tanh((z0 - dble(z))*6.0d0*beta*0.134d0)
+          rr = rho0*(ph + 1.0d0)/2.0d0
+          ri(:,z,y,x) = rr*wi*1.0d0
+          bi(:,z,y,x) = (rho0 - rr)*wi*1.0d0
+        end do
+      end do
+    end do
+  end subroutine interface_state
+
+  ! particle state and masks at output steps
+  subroutine output_particles(tt)
+    integer, intent(in) :: tt
+    integer :: k, uo
+    call cgl_check(cgl_get_particles(h, center, up, rp, fp, tp, fcap, tcap, frep), 'cgl_get_particles')
+    call cgl_check(cgl_download_masks(h, c_null_ptr, c_loc(walls_all), c_null_ptr, c_null_ptr), 'cgl_download_masks')
+    open(newunit=uo, file=trim(outdir)//'/PARTICLES.dat', status='old', position='append')
+    do k = 1, npar
+      write(uo, '(i8, 1x, i4, 12(1x, es24.16e3))') tt, k, center(k,:), up(k,:), fp(k,:), fcap(k,:)
+    end do
+    close(uo)
+  end subroutine output_particles
+
   ! output_results (S:898-924): Tecplot POINT zone, x fastest
   subroutine output_results(tt)
     integer, intent(in) :: tt
@@ -136,7 +210,8 @@ contains
     do z = 1, zl
       do y = 1, yl
         do x = 1, xl
-          write(uo, "(4(1xI6),5(1xF12.8))") x, y, z, 0, rho_N(z,y,x), ux(z,y,x), uy(z,y,x), uz(z,y,x), rho(z,y,x)
+          write(uo, "(4(1xI6),5(1xF12.8))") x, y, z, walls_all(z,y,x), rho_N(z,y,x), ux(z,y,x), uy(z,y,x), &
+                uz(z,y,x), rho(z,y,x)
         end do
       end do
     end do

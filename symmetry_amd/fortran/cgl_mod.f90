@@ -67,6 +67,24 @@ module cgl
       integer(c_int32_t), value :: nsteps
       real(c_double), intent(out) :: ms_total, ms_moments, ms_collide
     end function
+    integer(c_int) function cgl_set_particles(h, center, up, rp, rad, mass, inertia, cita, fg) &
+        bind(C, name="cgl_set_particles")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: center(*), up(*), rp(*), rad(*), mass(*), inertia(*), cita(*)
+      real(c_double), value :: fg
+    end function
+    integer(c_int) function cgl_get_particles(h, center, up, rp, force, torque, fcap, tcap, frep) &
+        bind(C, name="cgl_get_particles")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: center(*), up(*), rp(*), force(*), torque(*), fcap(*), tcap(*), frep(*)
+    end function
+    ! each mask by c_loc of an int32 (zl, yl, xl) array, or c_null_ptr to skip it
+    integer(c_int) function cgl_download_masks(h, walls, walls_all, pin, pout) bind(C, name="cgl_download_masks")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, walls, walls_all, pin, pout
+    end function
     type(c_ptr) function cgl_last_error_c() bind(C, name="cgl_last_error")
       import :: c_ptr
     end function

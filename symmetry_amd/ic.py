@@ -95,12 +95,13 @@ def workload_state(cfg, exact=True, nx=None, ny=None, nz=None):
                           rho0=cfg["rho0"], beta=cfg["beta"], exact=exact)
 
 
-def flat_interface_state(nx, ny, nz, *, z0, rho0, beta=0.7, drop=None, sigma=0.0, amp=0.0):
+def flat_interface_state(nx, ny, nz, *, z0, rho0, beta=0.7, drop=None, sigma=0.0, amp=0.0, exact=False):
     """Red fluid below the surface z = z0 + amp cos(2 pi (x - 1) / nx)
     (1-based lattice units), blue above, phi = tanh(6 beta 0.134 (z0 - z)) at
     rest, optionally with a red drop ``drop = (cx, cy, cz, r)`` (union of the
     two red regions, with the Laplace jump 3 sigma / r inside the drop).  Same
-    profile and equilibria as ``cylinder_state``."""
+    profile and equilibria as ``cylinder_state``; ``exact`` uses libm tanh
+    per node (bit-reproducible)."""
     x = np.arange(1, nx + 1, dtype=np.float64)[:, None, None]
     y = np.arange(1, ny + 1, dtype=np.float64)[None, :, None]
     z = np.arange(1, nz + 1, dtype=np.float64)[None, None, :]
@@ -112,7 +113,8 @@ def flat_interface_state(nx, ny, nz, *, z0, rho0, beta=0.7, drop=None, sigma=0.0
         sd = r - np.sqrt((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2)
         rho = np.where(sd > 0.0, rho0 + 3.0 * sigma / r, rho0)
         star = np.maximum(star, sd)
-    phi = np.tanh(star * 6.0 * beta * 0.134)
+    arg = star * 6.0 * beta * 0.134
+    phi = _tanh(arg).astype(np.float64) if exact else np.tanh(arg)
     rho_r = rho * (phi + 1.0) / 2.0
     rho_b = rho - rho_r
     return (rho_r[..., None] * W) * 1.0, (rho_b[..., None] * W) * 1.0, phi

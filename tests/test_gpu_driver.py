@@ -43,3 +43,45 @@ def test_fortran_driver_matches_oracle(tmp_path):
     assert [int(v) for v in first[:4]] == [1, 1, 1, 0]
     assert abs(float(first[4]) - st.phi[0, 0, 0]) < 1e-8
     assert len(lines) == 3 + nx * ny * nz
+
+
+def test_fortran_driver_particle_at_interface(tmp_path):
+    """The particle workload's set-up (z walls, flat interface, one sphere of
+    density 1.5 and a 60 degree contact angle) through the Fortran driver
+    equals the same run through the ctypes binding, bit for bit: the driver's
+    initial state, particle parameters (S:81-84) and calls are the ones the
+    library tests check against the oracle."""
+    import symmetry_amd
+    n, steps, rad = 32, 8, 5.0
+    c = (16.3, 15.8, 16.4)
+    nml = tmp_path / "in.nml"
+    nml.write_text(f"&cgl\n xl={n}, yl={n}, zl={n}, nsteps={steps}, out_every={steps}, prec=0,\n"
+                   f" sigma=0.005d0, rho0=1.0d0, tau_R=0.8d0, tau_B=0.8d0, beta=0.7d0,\n"
+                   f" ic='interface', z0=16.0d0, zwalls=.true., npar=1, px(1)={c[0]}d0, py(1)={c[1]}d0,\n"
+                   f" pz(1)={c[2]}d0, rad_p={rad}d0, rho_p=1.5d0, cita_deg=60.0d0,\n"
+                   f" dump_binary=.true., write_tecplot=.true., outdir='{tmp_path}'\n/\n")
+    print("driver particle", n, steps, flush=True)
+    r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    raw = np.fromfile(tmp_path / f"FIELDS_{steps}.bin", dtype=np.float64).reshape(7, n, n, n)
+    rows = [l.split() for l in (tmp_path / "PARTICLES.dat").read_text().splitlines()[1:]]
+    last = [r_ for r_ in rows if int(r_[0]) == steps][0]
+    ri, bi, _ = ic.flat_interface_state(n, n, n, z0=16.0, rho0=1.0, beta=0.7, exact=True)
+    parts = ic.sphere_particles([c], rad, 1.5, 60.0, 0.005)
+    lat = symmetry_amd.Lattice(n, n, n, precision="f64", sigma=0.005, rho0=1.0, nu_r=(0.8 - 0.5) / 3.0,
+                               nu_b=(0.8 - 0.5) / 3.0, beta=0.7, bc=(0, 0, 0, 0, 1, 1), npar=1)
+    lat.upload(ri, bi)
+    lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"], parts["fg"])
+    lat.step(steps)
+    mac = lat.calmacros()
+    for k, name in enumerate(("rho", "rho_r", "rho_b", "ux", "uy", "uz", "phi")):
+        assert np.array_equal(raw[k], mac[name]), name
+    got = lat.get_particles()
+    vals = [float(v) for v in last[2:]]
+    assert vals[0:3] == list(got["center"][0]) and vals[3:6] == list(got["up"][0])
+    assert vals[6:9] == list(got["force"][0]) and vals[9:12] == list(got["fcap"][0])
+    masks = lat.masks()
+    lines = (tmp_path / f"OUT_{steps}.dat").read_text().splitlines()
+    wall_col = np.array([int(l.split()[3]) for l in lines[3:]]).reshape(n, n, n)  # (z, y, x), x fastest
+    assert np.array_equal(wall_col.transpose(2, 1, 0), masks["walls_all"])
