@@ -64,6 +64,52 @@ def traffic_from_profile(precision, layout):
         return json.load(f).get("bytes_per_launch")
 
 
+def run_ensemble(steps, seq_steps):
+    """The batched ensemble: 64 uneven periodic Rayleigh-Plateau boxes (seed 7)
+    in FP32 factorised storage, one launch per kernel over all boxes, steps
+    replayed from a HIP graph; against the same boxes run one after another as
+    plain FP64 launches (one handle per box, every step's kernels launched
+    directly).  Aggregate MLUPS = total nodes x steps / wall seconds."""
+    import symmetry_amd
+    from symmetry_amd import ic
+
+    boxes = ic.ensemble_config4()
+    kprof = 6.0 * 0.7 * 0.134
+    bt = symmetry_amd.Batch([(b["nx"], b["ny"], b["nz"], b["nu_r"], b["nu_b"]) for b in boxes], precision="f32",
+                            layout="fs", sigma=0.01, rho0=1.0, beta=0.7, xchunk=8)
+    for k, b in enumerate(boxes):
+        bt.init_cylinder(k, b["nx"] / 2 + 0.5, b["ny"] / 2 + 0.5, b["r0"], kprof, b["amps"])
+    bt.step(10)
+    bt.sync()
+    t0 = time.perf_counter()
+    bt.step(steps)
+    bt.sync()
+    t_batch = time.perf_counter() - t0
+    nodes = bt.nodes
+    bt.close()
+    # sequential plain FP64 baseline
+    os.environ["CGL_GRAPHS"] = "0"
+    t_seq = 0.0
+    for b in boxes:
+        lat = symmetry_amd.Lattice(b["nx"], b["ny"], b["nz"], precision="f64", sigma=0.01, rho0=1.0, nu_r=b["nu_r"],
+                                   nu_b=b["nu_b"], beta=0.7, layout="rb")
+        lat.init_cylinder(b["nx"] / 2 + 0.5, b["ny"] / 2 + 0.5, b["r0"], kprof, b["amps"])
+        lat.step(14)  # x-chunk autotuning + warm-up
+        lat.sync()
+        t1 = time.perf_counter()
+        lat.step(seq_steps)
+        lat.sync()
+        t_seq += time.perf_counter() - t1
+        lat.close()
+    del os.environ["CGL_GRAPHS"]
+    batch_mlups = nodes * steps / t_batch / 1e6
+    seq_mlups = nodes * seq_steps / t_seq / 1e6
+    return {"workload": "64 periodic Rayleigh-Plateau boxes, nx=ny in {32,48,64,96}, nz in 64..512, seed 7",
+            "boxes": len(boxes), "nodes": nodes, "batched": "f32 factorised, one launch per kernel, graph replay",
+            "steps": steps, "mlups": round(batch_mlups, 1), "sequential_f64_steps": seq_steps,
+            "sequential_f64_mlups": round(seq_mlups, 1), "ratio": round(batch_mlups / seq_mlups, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +123,10 @@ def main():
     ap.add_argument("--size", default="256x256x512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--ensemble-steps", type=int, default=500,
+                    help="timed steps of the 64-box batched ensemble (0 = skip)")
+    ap.add_argument("--ensemble-seq-steps", type=int, default=100,
+                    help="timed steps per box of the sequential FP64 baseline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design (DESIGN.md), "
@@ -135,6 +185,8 @@ def main():
                 variants[key] = round(nodes * args.variant_steps / (time.perf_counter() - t1) / 1e6, 1)
                 v.close()
 
+    ensemble = run_ensemble(args.ensemble_steps, args.ensemble_seq_steps) if args.ensemble_steps > 0 else None
+
     out = {
         "metric": "MLUPS at 256x256x512 D3Q19 CG-MRT, FP32 vs FP64, % of HBM roofline",
         "value": round(mlups, 1),
@@ -163,6 +215,8 @@ def main():
         "hbm_copy_gbps": round(copy_gbps, 1),
         "variants_mlups": variants,
     }
+    if ensemble:
+        out["ensemble"] = ensemble
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

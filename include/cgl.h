@@ -130,6 +130,32 @@ int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *
  * (zl, yl, xl) layout.  Any pointer may be NULL. */
 int cgl_download_masks(cgl_t *h, int32_t *walls, int32_t *walls_all, int32_t *pin, int32_t *pout);
 
+/* ---------------------------------------------------------------- batches
+ * Many independent periodic boxes of uneven sizes advanced together: one
+ * launch per kernel covers the tiles of every box (nodes packed box after
+ * box), and cgl_step replays the steps from a captured HIP graph.  The
+ * reference runs one box per process; there is no counterpart call. */
+typedef struct cgl_box {
+    int32_t nx, ny, nz; /* box size, as xl, yl, zl */
+    int32_t reserved;   /* must be 0 */
+    double nu_r, nu_b;  /* kinematic viscosities of the two fluids (tau per box) */
+} cgl_box;
+
+/* precision, layout, sigma, rho0, beta, xchunk and device come from `p`
+ * (its nx, ny, nz, nu_r, nu_b are ignored); faces must be periodic and npar
+ * zero.  cgl_step, cgl_sync, cgl_time_steps and cgl_destroy take the batch
+ * handle; per-box data go through the cgl_box_* calls. */
+int cgl_batch_create(cgl_t **h, const cgl_params *p, int32_t nbox, const cgl_box *boxes);
+/* As cgl_upload_state / cgl_init_cylinder for box `box`; every box is set
+ * before the batch's first step. */
+int cgl_box_upload_state(cgl_t *h, int32_t box, const double *ri, const double *bi);
+int cgl_box_init_cylinder(cgl_t *h, int32_t box, double xc, double yc, double r0, double kprof, int32_t nmodes,
+                          const double *amp);
+/* As cgl_download_state / cgl_calmacros for box `box`. */
+int cgl_box_download_state(cgl_t *h, int32_t box, double *ri, double *bi);
+int cgl_box_calmacros(cgl_t *h, int32_t box, double *rho, double *rho_r, double *rho_b, double *ux, double *uy,
+                      double *uz, double *phi);
+
 /* Time nsteps steps on the device with HIP events recorded on the handle's
  * stream: total elapsed ms, and ms summed over the moments and fused-collide
  * kernels.  No counterpart in the reference (it has no timers). */

@@ -41,6 +41,12 @@ class Params(ctypes.Structure):
     ]
 
 
+class Box(ctypes.Structure):
+    """cgl_box of include/cgl.h."""
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("nu_r", ctypes.c_double), ("nu_b", ctypes.c_double)]
+
+
 _lib = None
 
 # Every symbol include/cgl.h declares.
@@ -49,6 +55,8 @@ EXPORTS = (
     "cgl_step", "cgl_calmacros", "cgl_derivatives", "cgl_sync", "cgl_time_steps",
     "cgl_device_name", "cgl_last_error", "cgl_probe_copy_bandwidth", "cgl_tuned_xchunk",
     "cgl_set_particles", "cgl_get_particles", "cgl_download_masks",
+    "cgl_batch_create", "cgl_box_upload_state", "cgl_box_init_cylinder", "cgl_box_download_state",
+    "cgl_box_calmacros",
 )
 
 
@@ -76,6 +84,11 @@ def lib():
         L.cgl_set_particles.argtypes = [vp] + [vp] * 7 + [ctypes.c_double]
         L.cgl_get_particles.argtypes = [vp] + [vp] * 8
         L.cgl_download_masks.argtypes = [vp] + [vp] * 4
+        L.cgl_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Params), ctypes.c_int32, ctypes.POINTER(Box)]
+        L.cgl_box_upload_state.argtypes = [vp, ctypes.c_int32, vp, vp]
+        L.cgl_box_download_state.argtypes = [vp, ctypes.c_int32, vp, vp]
+        L.cgl_box_init_cylinder.argtypes = [vp, ctypes.c_int32] + [ctypes.c_double] * 4 + [ctypes.c_int32, vp]
+        L.cgl_box_calmacros.argtypes = [vp, ctypes.c_int32] + [vp] * 7
         L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.cgl_probe_copy_bandwidth.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                                ctypes.POINTER(ctypes.c_double)]
@@ -203,3 +216,68 @@ class Lattice:
         buf = ctypes.create_string_buffer(256)
         _check(lib().cgl_device_name(self.h, buf, 256))
         return buf.value.decode()
+
+
+class Batch:
+    """Independent periodic boxes of uneven sizes advanced together (one
+    launch per kernel over all boxes, steps replayed from a HIP graph).
+    ``boxes``: sequence of (nx, ny, nz, nu_r, nu_b)."""
+
+    def __init__(self, boxes, precision="f32", sigma=0.01, rho0=1.0, beta=0.7, xchunk=0, device=0, layout="fs"):
+        p = Params()
+        p.precision = PREC[precision] if isinstance(precision, str) else int(precision)
+        p.layout = LAYOUT[layout] if isinstance(layout, str) else int(layout)
+        p.xchunk = xchunk
+        p.device = device
+        p.sigma, p.rho0, p.beta = sigma, rho0, beta
+        arr = (Box * len(boxes))()
+        for k, (nx, ny, nz, nu_r, nu_b) in enumerate(boxes):
+            arr[k].nx, arr[k].ny, arr[k].nz, arr[k].nu_r, arr[k].nu_b = nx, ny, nz, nu_r, nu_b
+        self.shapes = [tuple(b[:3]) for b in boxes]
+        self.nodes = sum(nx * ny * nz for nx, ny, nz in self.shapes)
+        self.h = ctypes.c_void_p()
+        _check(lib().cgl_batch_create(ctypes.byref(self.h), ctypes.byref(p), len(boxes), arr))
+
+    def close(self):
+        if self.h:
+            lib().cgl_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, b, ri, bi):
+        ri = np.ascontiguousarray(ri, dtype=np.float64)
+        bi = np.ascontiguousarray(bi, dtype=np.float64)
+        assert ri.shape == self.shapes[b] + (Q,) and bi.shape == ri.shape
+        _check(lib().cgl_box_upload_state(self.h, b, _ptr(ri), _ptr(bi)))
+
+    def init_cylinder(self, b, xc, yc, r0, kprof, amps):
+        a = np.ascontiguousarray(amps, dtype=np.float64)
+        _check(lib().cgl_box_init_cylinder(self.h, b, xc, yc, r0, kprof, len(a), _ptr(a)))
+
+    def download(self, b):
+        ri = np.empty(self.shapes[b] + (Q,))
+        bi = np.empty(self.shapes[b] + (Q,))
+        _check(lib().cgl_box_download_state(self.h, b, _ptr(ri), _ptr(bi)))
+        return ri, bi
+
+    def calmacros(self, b):
+        names = ("rho", "rho_r", "rho_b", "ux", "uy", "uz", "phi")
+        out = {k: np.empty(self.shapes[b]) for k in names}
+        _check(lib().cgl_box_calmacros(self.h, b, *(_ptr(out[k]) for k in names)))
+        return out
+
+    def step(self, n=1):
+        _check(lib().cgl_step(self.h, int(n)))
+
+    def sync(self):
+        _check(lib().cgl_sync(self.h))
+
+    def time_steps(self, n):
+        t, m, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().cgl_time_steps(self.h, int(n), ctypes.byref(t), ctypes.byref(m), ctypes.byref(c)))
+        return t.value, m.value, c.value

@@ -45,11 +45,14 @@ int check_launch(const char *what)
     return CGL_OK;
 }
 
+// Scratch for host-layout conversions, sized for the largest box a call can
+// address (scratch_n nodes).
 int ensure_scratch(cgl_handle *h, bool mac, bool aos, bool iflag)
 {
-    if (mac && !h->mac) HIPCHK(hipMalloc((void **)&h->mac, 7 * h->N * sizeof(double)));
-    if (aos && !h->aos) HIPCHK(hipMalloc((void **)&h->aos, (size_t)cgl::Q * h->N * sizeof(double)));
-    if (iflag && !h->iflag) HIPCHK(hipMalloc((void **)&h->iflag, h->N * sizeof(int)));
+    const size_t n = h->scratch_n;
+    if (mac && !h->mac) HIPCHK(hipMalloc((void **)&h->mac, 7 * n * sizeof(double)));
+    if (aos && !h->aos) HIPCHK(hipMalloc((void **)&h->aos, (size_t)cgl::Q * n * sizeof(double)));
+    if (iflag && !h->iflag) HIPCHK(hipMalloc((void **)&h->iflag, n * sizeof(int)));
     return CGL_OK;
 }
 
@@ -63,7 +66,10 @@ int set_device(cgl_handle *h)
 void enqueue_step(cgl_handle *h)
 {
     dispatch(h, [&](auto impl) {
-        if (h->ext) {
+        if (h->nbox) {
+            decltype(impl)::batch_moments(h);
+            decltype(impl)::batch_collide(h);
+        } else if (h->ext) {
             decltype(impl)::step_ext(h);
         } else {
             decltype(impl)::moments(h, false);
@@ -125,6 +131,8 @@ int cgl_probe_copy_bandwidth(int32_t device, int64_t bytes, int32_t reps, double
 
 const char *cgl_last_error(void) { return g_err; }
 
+static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls);
+
 int cgl_create(cgl_t **out, const cgl_params *p)
 {
     if (!out || !p) return fail(CGL_ERR_ARG, "null argument");
@@ -157,7 +165,13 @@ int cgl_create(cgl_t **out, const cgl_params *p)
     if (!(p->rho0 > 0) || !(p->nu_r > 0) || !(p->nu_b > 0)) return fail(CGL_ERR_ARG, "rho0, nu_r, nu_b must be > 0");
     if (p->layout != CGL_LAYOUT_RB && p->layout != CGL_LAYOUT_FS) return fail(CGL_ERR_ARG, "unknown layout %d", p->layout);
     if (p->reserved != 0) return fail(CGL_ERR_ARG, "reserved field must be 0");
+    return alloc_handle(out, p, (size_t)p->nx * p->ny * p->nz, walls);
+}
 
+// Allocation of a handle's device state for N nodes (one box, or a packed
+// batch of boxes).
+static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls)
+{
     cgl_handle *h = new (std::nothrow) cgl_handle();
     if (!h) return fail(CGL_ERR_ARG, "out of host memory");
     memset(h, 0, sizeof *h);
@@ -165,8 +179,14 @@ int cgl_create(cgl_t **out, const cgl_params *p)
     h->L.nx = p->nx;
     h->L.ny = p->ny;
     h->L.nz = p->nz;
-    h->N = (size_t)p->nx * p->ny * p->nz;
-    h->L.N = (long long)h->N;
+    h->N = N;
+    h->scratch_n = N;
+    h->gkey = -1;
+    {
+        const char *env = getenv("CGL_GRAPHS"); // 0: launch every step's kernels directly
+        h->use_graph = env ? atoi(env) != 0 : 1;
+    }
+    h->L.N = (long long)p->nx * p->ny * p->nz;
     h->L.rho0 = p->rho0;
     h->L.sigma = p->sigma;
     h->L.beta = p->beta;
@@ -265,6 +285,9 @@ int cgl_destroy(cgl_t *h)
     }
     if (h->aos) (void)hipFree(h->aos);
     if (h->iflag) (void)hipFree(h->iflag);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->dboxes) (void)hipFree(h->dboxes);
+    delete[] h->hboxes;
     for (int k = 0; k < 3; ++k)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -274,6 +297,7 @@ int cgl_destroy(cgl_t *h)
 
 int cgl_upload_state(cgl_t *h, const double *ri, const double *bi)
 {
+    if (h && h->nbox) return fail(CGL_ERR_ARG, "a batch handle takes the cgl_box_* entry points");
     if (!h || !ri || !bi) return fail(CGL_ERR_ARG, "null argument");
     int rc = set_device(h);
     if (rc) return rc;
@@ -303,6 +327,7 @@ int cgl_upload_state(cgl_t *h, const double *ri, const double *bi)
 
 int cgl_download_state(cgl_t *h, double *ri, double *bi)
 {
+    if (h && h->nbox) return fail(CGL_ERR_ARG, "a batch handle takes the cgl_box_* entry points");
     if (!h || !ri || !bi) return fail(CGL_ERR_ARG, "null argument");
     int rc = set_device(h);
     if (rc) return rc;
@@ -319,6 +344,7 @@ int cgl_download_state(cgl_t *h, double *ri, double *bi)
 
 int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, int32_t nmodes, const double *amp)
 {
+    if (h && h->nbox) return fail(CGL_ERR_ARG, "a batch handle takes the cgl_box_* entry points");
     if (!h) return fail(CGL_ERR_ARG, "null handle");
     if (nmodes < 0 || nmodes > 16 || (nmodes > 0 && !amp)) return fail(CGL_ERR_ARG, "nmodes must be 0..16");
     int rc = set_device(h);
@@ -336,6 +362,39 @@ int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, i
     dispatch(h, [&](auto impl) { decltype(impl)::init_cylinder(h, ic); });
     if ((rc = check_launch("init_cylinder"))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
+    return CGL_OK;
+}
+
+// Capture two steps from the current buffer parity into a graph (kept until
+// the parity or the state layout changes).  Two steps return the buffers to
+// the same parity, so one graph serves every replay.  The steps contain
+// kernel launches only.
+static int capture_two_steps(cgl_handle *h)
+{
+    const int key = h->cur | h->phicur << 1 | h->state_layout << 2;
+    if (h->gexec && h->gkey == key) return CGL_OK;
+    if (h->gexec) {
+        HIPCHK(hipGraphExecDestroy(h->gexec));
+        h->gexec = nullptr;
+        h->gkey = -1;
+    }
+    const int cur = h->cur, phicur = h->phicur, done = h->steps_done;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(h);
+    enqueue_step(h);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    h->cur = cur; // the captured steps have not run yet
+    h->phicur = phicur;
+    h->steps_done = done;
+    if (e != hipSuccess) return fail(CGL_ERR_HIP, "step capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        h->gexec = nullptr;
+        return fail(CGL_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    }
+    h->gkey = key;
     return CGL_OK;
 }
 
@@ -377,6 +436,20 @@ int cgl_step(cgl_t *h, int32_t nsteps)
     int rc = set_device(h);
     if (rc) return rc;
     if ((rc = tune_some(h, &nsteps))) return rc;
+    if (h->use_graph && !h->ext && !h->tuning && nsteps >= 2) {
+        // steady state: replay two captured steps (no host work per step)
+        if (h->state_layout != h->layout) { // the layout conversion step runs eagerly
+            enqueue_step(h);
+            if ((rc = check_launch("step"))) return rc;
+            --nsteps;
+        }
+        if ((rc = capture_two_steps(h))) return rc;
+        for (int k = 0; k + 1 < nsteps; k += 2) {
+            HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+            h->steps_done += 2;
+        }
+        nsteps &= 1;
+    }
     for (int t = 0; t < nsteps; ++t) {
         enqueue_step(h);
         if ((rc = check_launch("step"))) return rc;
@@ -394,6 +467,7 @@ int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning)
 
 int cgl_calmacros(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *ux, double *uy, double *uz, double *phi)
 {
+    if (h && h->nbox) return fail(CGL_ERR_ARG, "a batch handle takes the cgl_box_* entry points");
     if (!h) return fail(CGL_ERR_ARG, "null handle");
     int rc = set_device(h);
     if (rc) return rc;
@@ -415,6 +489,7 @@ int cgl_calmacros(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *u
 int cgl_derivatives(cgl_t *h, double *gradx, double *grady, double *gradz, double *n_x, double *n_y, double *n_z,
                     int32_t *flag)
 {
+    if (h && h->nbox) return fail(CGL_ERR_ARG, "a batch handle takes the cgl_box_* entry points");
     if (!h) return fail(CGL_ERR_ARG, "null handle");
     int rc = set_device(h);
     if (rc) return rc;
@@ -473,9 +548,19 @@ int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moment
     for (int t = 0; t < nsteps && !h->ext; ++t) {
         float a = 0.f, b = 0.f;
         HIPCHK(hipEventRecord(h->ev[0], h->stream));
-        dispatch(h, [&](auto impl) { decltype(impl)::moments(h, false); });
+        dispatch(h, [&](auto impl) {
+            if (h->nbox)
+                decltype(impl)::batch_moments(h);
+            else
+                decltype(impl)::moments(h, false);
+        });
         HIPCHK(hipEventRecord(h->ev[1], h->stream));
-        dispatch(h, [&](auto impl) { decltype(impl)::collide(h); });
+        dispatch(h, [&](auto impl) {
+            if (h->nbox)
+                decltype(impl)::batch_collide(h);
+            else
+                decltype(impl)::collide(h);
+        });
         HIPCHK(hipEventRecord(h->ev[2], h->stream));
         h->cur ^= 1;
         h->phicur ^= 1;
@@ -590,6 +675,172 @@ int cgl_download_masks(cgl_t *h, int32_t *walls, int32_t *walls_all, int32_t *pi
     }
     HIPCHK(hipStreamSynchronize(h->stream));
     return CGL_OK;
+}
+
+// ------------------------------------------------------------------ batches
+
+int cgl_batch_create(cgl_t **out, const cgl_params *p, int32_t nbox, const cgl_box *boxes)
+{
+    if (!out || !p || !boxes || nbox < 1) return fail(CGL_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    if (p->precision < CGL_PREC_F64 || p->precision > CGL_PREC_F16)
+        return fail(CGL_ERR_ARG, "unknown precision %d", p->precision);
+    if (p->layout != CGL_LAYOUT_RB && p->layout != CGL_LAYOUT_FS) return fail(CGL_ERR_ARG, "unknown layout %d", p->layout);
+    for (int f = 0; f < 6; ++f)
+        if (p->bc[f] != CGL_BC_PERIODIC) return fail(CGL_ERR_UNSUPPORTED, "batched boxes are periodic");
+    if (p->npar != 0) return fail(CGL_ERR_UNSUPPORTED, "batched boxes carry no particles");
+    if (!(p->rho0 > 0)) return fail(CGL_ERR_ARG, "rho0 must be > 0");
+    if (p->reserved != 0) return fail(CGL_ERR_ARG, "reserved field must be 0");
+    const double eb = p->precision == CGL_PREC_F64 ? 48.0 : 16.0;
+    size_t total = 0, nmax = 0;
+    for (int b = 0; b < nbox; ++b) {
+        const cgl_box &B = boxes[b];
+        if (B.nx < 1 || B.ny < 1 || B.nz < 1) return fail(CGL_ERR_ARG, "box %d: must be at least 1x1x1", b);
+        if (!(B.nu_r > 0) || !(B.nu_b > 0)) return fail(CGL_ERR_ARG, "box %d: nu_r, nu_b must be > 0", b);
+        if (B.reserved != 0) return fail(CGL_ERR_ARG, "box %d: reserved field must be 0", b);
+        const size_t n = (size_t)B.nx * B.ny * B.nz;
+        if ((double)n * eb >= 4294967296.0) return fail(CGL_ERR_UNSUPPORTED, "box %d exceeds the 4 GiB per-array addressing", b);
+        total += n;
+        if (n > nmax) nmax = n;
+    }
+    cgl_params q = *p;
+    q.nx = boxes[0].nx;
+    q.ny = boxes[0].ny;
+    q.nz = boxes[0].nz;
+    q.nu_r = boxes[0].nu_r;
+    q.nu_b = boxes[0].nu_b;
+    int rc = alloc_handle(out, &q, total, 0);
+    if (rc) return rc;
+    cgl_handle *h = *out;
+    h->tuning = 0;
+    h->xc = p->xchunk > 0 ? p->xchunk : 8;
+    h->scratch_n = nmax;
+    h->hboxes = new (std::nothrow) cgl::BoxDev[nbox];
+    if (!h->hboxes) {
+        cgl_destroy(h);
+        *out = nullptr;
+        return fail(CGL_ERR_ARG, "out of host memory");
+    }
+    h->nbox = nbox;
+    unsigned long long base = 0, t1 = 0, t2 = 0;
+    for (int b = 0; b < nbox; ++b) {
+        cgl::BoxDev &D = h->hboxes[b];
+        memset(&D, 0, sizeof D);
+        D.L = h->L;
+        D.L.nx = boxes[b].nx;
+        D.L.ny = boxes[b].ny;
+        D.L.nz = boxes[b].nz;
+        D.L.N = (long long)boxes[b].nx * boxes[b].ny * boxes[b].nz;
+        D.L.nu_r = boxes[b].nu_r;
+        D.L.nu_b = boxes[b].nu_b;
+        D.base = base;
+        D.gz = (boxes[b].nz + cgl::TZ - 1) / cgl::TZ;
+        D.gy = (boxes[b].ny + cgl::TY - 1) / cgl::TY;
+        D.gx2 = (boxes[b].nx + h->xc - 1) / h->xc;
+        D.t1 = (unsigned)t1;
+        D.t2 = (unsigned)t2;
+        t1 += (unsigned long long)D.gz * D.gy * boxes[b].nx;
+        t2 += (unsigned long long)D.gz * D.gy * D.gx2;
+        base += (unsigned long long)D.L.N;
+    }
+    if (t1 >= (1ull << 31) || t2 >= (1ull << 31)) {
+        cgl_destroy(h);
+        *out = nullptr;
+        return fail(CGL_ERR_UNSUPPORTED, "batch exceeds the launch's workgroup count");
+    }
+    h->tiles1 = (unsigned)t1;
+    h->tiles2 = (unsigned)t2;
+    hipError_t e = hipMalloc((void **)&h->dboxes, nbox * sizeof(cgl::BoxDev));
+    if (e == hipSuccess) e = hipMemcpy(h->dboxes, h->hboxes, nbox * sizeof(cgl::BoxDev), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        rc = fail(CGL_ERR_HIP, "box table: %s", hipGetErrorString(e));
+        cgl_destroy(h);
+        *out = nullptr;
+        return rc;
+    }
+    return CGL_OK;
+}
+
+// A single-box handle that aliases box `b` of a batch: same buffers, offset
+// to the box's nodes.  Scratch is the batch's (sized for its largest box).
+static int box_view(cgl_handle *h, int32_t b, cgl_handle *v)
+{
+    if (!h || !h->nbox) return fail(CGL_ERR_ARG, "not a batch handle");
+    if (b < 0 || b >= h->nbox) return fail(CGL_ERR_ARG, "box %d out of range", b);
+    int rc = set_device(h);
+    if (rc) return rc;
+    if ((rc = ensure_scratch(h, true, true, true))) return rc;
+    *v = *h;
+    const cgl::BoxDev &D = h->hboxes[b];
+    v->nbox = 0;
+    v->hboxes = nullptr;
+    v->dboxes = nullptr;
+    v->gexec = nullptr;
+    v->gkey = -1;
+    v->L = D.L;
+    v->N = (size_t)D.L.N;
+    v->p.nx = D.L.nx;
+    v->p.ny = D.L.ny;
+    v->p.nz = D.L.nz;
+    v->p.nu_r = D.L.nu_r;
+    v->p.nu_b = D.L.nu_b;
+    const size_t base = (size_t)D.base;
+    const size_t ns = h->p.precision == CGL_PREC_F64 ? 6 * sizeof(double) : 4 * sizeof(float);
+    for (int k = 0; k < 2; ++k) {
+        for (int c = 0; c < 2; ++c)
+            if (h->pops[k][c]) v->pops[k][c] = (char *)h->pops[k][c] + (size_t)cgl::Q * base * h->esize;
+        if (h->fsf[k]) v->fsf[k] = (char *)h->fsf[k] + (size_t)cgl::Q * base * h->esize;
+        if (h->fss[k]) v->fss[k] = (char *)h->fss[k] + base * ns;
+        v->phi[k] = (char *)h->phi[k] + base * h->phisize;
+    }
+    return CGL_OK;
+}
+
+static void box_back(cgl_handle *h, const cgl_handle *v)
+{
+    h->cur = v->cur;
+    h->phicur = v->phicur;
+    h->state_layout = v->state_layout;
+}
+
+int cgl_box_upload_state(cgl_t *h, int32_t box, const double *ri, const double *bi)
+{
+    cgl_handle v;
+    int rc = box_view(h, box, &v);
+    if (rc) return rc;
+    if (h->steps_done > 0) return fail(CGL_ERR_ARG, "upload every box of a batch before its first step");
+    if ((rc = cgl_upload_state(&v, ri, bi))) return rc;
+    box_back(h, &v);
+    return CGL_OK;
+}
+
+int cgl_box_init_cylinder(cgl_t *h, int32_t box, double xc, double yc, double r0, double kprof, int32_t nmodes,
+                          const double *amp)
+{
+    cgl_handle v;
+    int rc = box_view(h, box, &v);
+    if (rc) return rc;
+    if (h->steps_done > 0) return fail(CGL_ERR_ARG, "initialise every box of a batch before its first step");
+    if ((rc = cgl_init_cylinder(&v, xc, yc, r0, kprof, nmodes, amp))) return rc;
+    box_back(h, &v);
+    return CGL_OK;
+}
+
+int cgl_box_download_state(cgl_t *h, int32_t box, double *ri, double *bi)
+{
+    cgl_handle v;
+    int rc = box_view(h, box, &v);
+    if (rc) return rc;
+    return cgl_download_state(&v, ri, bi);
+}
+
+int cgl_box_calmacros(cgl_t *h, int32_t box, double *rho, double *rho_r, double *rho_b, double *ux, double *uy,
+                      double *uz, double *phi)
+{
+    cgl_handle v;
+    int rc = box_view(h, box, &v);
+    if (rc) return rc;
+    return cgl_calmacros(&v, rho, rho_r, rho_b, ux, uy, uz, phi);
 }
 
 int cgl_device_name(cgl_t *h, char *buf, int32_t len)

@@ -32,6 +32,7 @@ struct cgl_handle {
     double *mac;      // scratch: 7*N doubles for macroscopic downloads (lazy)
     double *aos;      // scratch: 19*N doubles for host-layout conversion (lazy)
     int *iflag;       // scratch: N ints (lazy)
+    size_t scratch_n; // nodes the scratch buffers are sized for
     hipStream_t stream;
     hipEvent_t ev[3];
     int steps_done;
@@ -45,6 +46,15 @@ struct cgl_handle {
     int ntcur;           // buffer holding the masks in force
     int *walls_i, *pin_i, *pout_i;
     cgl::Part P;         // particle state (device pointers)
+    // batches of independent boxes (nbox > 0): packed node index
+    int nbox;
+    cgl::BoxDev *dboxes; // device table
+    cgl::BoxDev *hboxes; // host copy
+    unsigned tiles1, tiles2;
+    // steady-state steps replayed from a captured graph (two steps per replay)
+    int use_graph;       // replay steady-state steps from a captured graph
+    hipGraphExec_t gexec;
+    int gkey;            // cur | phicur << 1 | state_layout << 2 at capture, -1 = none
 };
 
 namespace cgl {
@@ -115,6 +125,41 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         else
             k_collide<FS, FS, S, R, P, SHIFT>
                 <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc);
+        h->state_layout = h->layout;
+    }
+    // ---- batches: one launch per kernel over the tiles of every box
+    static void batch_moments(cgl_handle *h)
+    {
+        const void *i0, *i1;
+        state_in(h, i0, i1);
+        const P *phio = (const P *)h->phi[h->phicur];
+        P *phi = (P *)h->phi[h->phicur ^ 1];
+        const int remap = h->L.xcd_remap;
+        if (h->state_layout == RB)
+            k_moments_batch<RB, S, R, P, SHIFT>
+                <<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, remap, i0, i1, phio, phi);
+        else
+            k_moments_batch<FS, S, R, P, SHIFT>
+                <<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, remap, i0, i1, phio, phi);
+    }
+    static void batch_collide(cgl_handle *h)
+    {
+        const void *i0, *i1;
+        void *o0, *o1;
+        state_in(h, i0, i1);
+        state_out(h, o0, o1);
+        const P *phio = (const P *)h->phi[h->phicur];
+        const P *phi = (const P *)h->phi[h->phicur ^ 1];
+        const int remap = h->L.xcd_remap;
+        if (h->state_layout == RB && h->layout == RB)
+            k_collide_batch<RB, RB, S, R, P, SHIFT><<<h->tiles2, node_block(), 0, h->stream>>>(
+                h->dboxes, h->nbox, remap, h->xc, i0, i1, o0, o1, phi, phio);
+        else if (h->state_layout == RB)
+            k_collide_batch<RB, FS, S, R, P, SHIFT><<<h->tiles2, node_block(), 0, h->stream>>>(
+                h->dboxes, h->nbox, remap, h->xc, i0, i1, o0, o1, phi, phio);
+        else
+            k_collide_batch<FS, FS, S, R, P, SHIFT><<<h->tiles2, node_block(), 0, h->stream>>>(
+                h->dboxes, h->nbox, remap, h->xc, i0, i1, o0, o1, phi, phio);
         h->state_layout = h->layout;
     }
     // derivatives of the phi written by the last moments() call

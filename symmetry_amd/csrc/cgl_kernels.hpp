@@ -366,13 +366,13 @@ __device__ __forceinline__ void node_offsets(const Lat &L, int x, int y, int z, 
 // ---------------------------------------------------------------- K1 moments
 // Writes phi of the step.  With FULL it also writes the macroscopic fields
 // rho, rho_R, rho_B, ux, uy, uz, phi (bare velocity, as calmacros) as FP64.
-template <int LIN, class S, class R, class P, bool SHIFT, bool FULL, bool EXT = false>
-__global__ __launch_bounds__(256) void k_moments(Lat L, const void *__restrict__ p0, const void *__restrict__ p1,
-                                                 const P *__restrict__ phio, P *__restrict__ phi,
-                                                 double *__restrict__ mac, Ext E = Ext{})
+// One node of tile (bx, by) of plane bz.
+template <int LIN, class S, class R, class P, bool SHIFT, bool FULL, bool EXT>
+__device__ __forceinline__ void moments_node(const Lat &L, const void *__restrict__ p0, const void *__restrict__ p1,
+                                             const P *__restrict__ phio, P *__restrict__ phi,
+                                             double *__restrict__ mac, const Ext &E, unsigned bx, unsigned by,
+                                             unsigned bz)
 {
-    unsigned bx, by, bz;
-    tile_of_block(L, bx, by, bz);
     const int z = bx * TZ + threadIdx.x;
     const int y = by * TY + threadIdx.y;
     const int x = bz;
@@ -422,6 +422,16 @@ __global__ __launch_bounds__(256) void k_moments(Lat L, const void *__restrict__
         mac[5 * N + n] = mo.uz;
         mac[6 * N + n] = mo.phi;
     }
+}
+
+template <int LIN, class S, class R, class P, bool SHIFT, bool FULL, bool EXT = false>
+__global__ __launch_bounds__(256) void k_moments(Lat L, const void *__restrict__ p0, const void *__restrict__ p1,
+                                                 const P *__restrict__ phio, P *__restrict__ phi,
+                                                 double *__restrict__ mac, Ext E = Ext{})
+{
+    unsigned bx, by, bz;
+    tile_of_block(L, bx, by, bz);
+    moments_node<LIN, S, R, P, SHIFT, FULL, EXT>(L, p0, p1, phio, phi, mac, E, bx, by, bz);
 }
 
 // ----------------------------------------------------- gradient of phi (b)
@@ -756,16 +766,15 @@ __device__ __forceinline__ bool away_from_walls(const Ext &E, const Lat &L, int 
              (E.wall[1][1] && y > L.ny - 3) || (E.wall[2][0] && z < 2) || (E.wall[2][1] && z > L.nz - 3));
 }
 
-template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT = false>
-__global__ __launch_bounds__(256) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
-                                                 void *__restrict__ o0, void *__restrict__ o1,
-                                                 const P *__restrict__ phi, const P *__restrict__ phio, int xc,
-                                                 Ext E = Ext{})
+// The (z-tile bx, y-tile by, x-chunk bz) column of one workgroup.
+template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT>
+__device__ __forceinline__ void collide_tile(const Lat &L, const void *__restrict__ i0, const void *__restrict__ i1,
+                                             void *__restrict__ o0, void *__restrict__ o1,
+                                             const P *__restrict__ phi, const P *__restrict__ phio, int xc,
+                                             const Ext &E, unsigned bx, unsigned by, unsigned bz)
 {
     __shared__ K2Smem<R, P> sm;
     const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
-    unsigned bx, by, bz;
-    tile_of_block(L, bx, by, bz);
     const int z0 = bx * TZ, y0 = by * TY, x0 = bz * xc;
     const int x1 = min(x0 + xc, L.nx);
     const int z = z0 + tz, y = y0 + ty;
@@ -1005,6 +1014,98 @@ __global__ __launch_bounds__(256) void k_collide(Lat L, const void *__restrict__
         const Recol<R> rc = recol<R, SHIFT>(L, mo, flag);
         write_node<LOUT, S, R, SHIFT>(L, o0, o1, n, f, rc, nx, ny, nz, aw);
     }
+}
+
+template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT = false>
+__global__ __launch_bounds__(256) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
+                                                 void *__restrict__ o0, void *__restrict__ o1,
+                                                 const P *__restrict__ phi, const P *__restrict__ phio, int xc,
+                                                 Ext E = Ext{})
+{
+    unsigned bx, by, bz;
+    tile_of_block(L, bx, by, bz);
+    collide_tile<LIN, LOUT, S, R, P, SHIFT, EXT>(L, i0, i1, o0, o1, phi, phio, xc, E, bx, by, bz);
+}
+
+// ------------------------------------------------- batches of independent boxes
+// Boxes are packed one after another in the node index: box b owns nodes
+// [base, base + N_b) of every per-node array and its own SoA block of each
+// population array.  One launch covers the tiles of all boxes; a workgroup
+// finds its box by binary search over the first-tile table.
+struct BoxDev {
+    Lat L;
+    unsigned long long base; // first node of the box in the packed index
+    unsigned t1, t2;         // first tile of the box in the moments / collide launch
+    unsigned gz, gy, gx2;    // z tiles, y tiles, x chunks of the collide launch
+    unsigned pad;
+};
+
+__device__ __forceinline__ unsigned xcd_order(unsigned lin, unsigned nb)
+{
+    const unsigned full = nb - nb % NXCD;
+    return lin < full ? (lin % NXCD) * (full / NXCD) + lin / NXCD : lin;
+}
+
+template <bool K2> __device__ __forceinline__ int find_box(const BoxDev *__restrict__ B, int nbox, unsigned t)
+{
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const unsigned first = K2 ? B[mid].t2 : B[mid].t1;
+        if (first <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Offset the two arrays of a state layout to box `base`.
+template <int LAY, class S, bool SHIFT>
+__device__ __forceinline__ void box_state(const void *&a, const void *&b, unsigned long long base)
+{
+    a = (const S *)a + (size_t)Q * base;
+    if constexpr (LAY == RB)
+        b = (const S *)b + (size_t)Q * base;
+    else if constexpr (SHIFT)
+        b = (const float4 *)b + base;
+    else
+        b = (const double *)b + (size_t)FsScal<false>::NS * base;
+}
+
+template <int LIN, class S, class R, class P, bool SHIFT>
+__global__ __launch_bounds__(256) void k_moments_batch(const BoxDev *__restrict__ B, int nbox, int remap,
+                                                       const void *__restrict__ p0, const void *__restrict__ p1,
+                                                       const P *__restrict__ phio, P *__restrict__ phi)
+{
+    const unsigned t = remap ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int b = find_box<false>(B, nbox, t);
+    const Lat L = B[b].L;
+    const unsigned long long base = B[b].base;
+    const unsigned lt = t - B[b].t1, gz = B[b].gz, gy = B[b].gy;
+    const void *a = p0, *c = p1;
+    box_state<LIN, S, SHIFT>(a, c, base);
+    moments_node<LIN, S, R, P, SHIFT, false, false>(L, a, c, phio + base, phi + base, nullptr, Ext{}, lt % gz,
+                                                    (lt / gz) % gy, lt / (gz * gy));
+}
+
+template <int LIN, int LOUT, class S, class R, class P, bool SHIFT>
+__global__ __launch_bounds__(256) void k_collide_batch(const BoxDev *__restrict__ B, int nbox, int remap, int xc,
+                                                       const void *__restrict__ i0, const void *__restrict__ i1,
+                                                       void *__restrict__ o0, void *__restrict__ o1,
+                                                       const P *__restrict__ phi, const P *__restrict__ phio)
+{
+    const unsigned t = remap ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int b = find_box<true>(B, nbox, t);
+    const Lat L = B[b].L;
+    const unsigned long long base = B[b].base;
+    const unsigned lt = t - B[b].t2, gz = B[b].gz, gy = B[b].gy;
+    const void *a0 = i0, *a1 = i1, *c0 = o0, *c1 = o1;
+    box_state<LIN, S, SHIFT>(a0, a1, base);
+    box_state<LOUT, S, SHIFT>(c0, c1, base);
+    collide_tile<LIN, LOUT, S, R, P, SHIFT, false>(L, a0, a1, const_cast<void *>(c0), const_cast<void *>(c1),
+                                                   phi + base, phio + base, xc, Ext{}, lt % gz, (lt / gz) % gy,
+                                                   lt / (gz * gy));
 }
 
 // ------------------------------------------ derivatives output (per stage)

@@ -144,3 +144,25 @@ def particle_config2(n=96, rad=10.0):
     return dict(nx=n, ny=n, nz=n, bc=(0, 0, 0, 0, 1, 1), z0=n / 2.0, center=(n / 2.0, n / 2.0, n / 2.0),
                 rad=rad, rho_p=1.5, cita_deg=60.0, sigma=0.005, rho0=1.0, nu_r=(tau - 0.5) / 3.0,
                 nu_b=(tau - 0.5) / 3.0, beta=0.7, steps=5000)
+
+
+def ensemble_config4(seed=7, nbox=64):
+    """Batched ensemble of uneven periodic Rayleigh-Plateau boxes.  Per box, in
+    this draw order from numpy's PCG64 generator with the given seed:
+    nx = ny from {32, 48, 64, 96}; nz from {64, 96, ..., 512}; cylinder radius
+    R uniform in [0.15 nx, 0.3 nx]; one cosine mode of wavelength nz with
+    amplitude uniform in [0.01, 0.1] R; tau uniform in [0.6, 1.2] for both
+    fluids.  sigma = A = 0.01, beta = 0.7, rho0 = 1 as in the other
+    Rayleigh-Plateau workloads; 2000 steps."""
+    rng = np.random.default_rng(seed)
+    boxes = []
+    for _ in range(nbox):
+        nx = int(rng.choice([32, 48, 64, 96]))
+        nz = int(rng.choice(np.arange(64, 513, 32)))
+        r0 = float(rng.uniform(0.15 * nx, 0.3 * nx))
+        amp = float(rng.uniform(0.01, 0.1)) * r0
+        tau = float(rng.uniform(0.6, 1.2))
+        nu = (tau - 0.5) / 3.0
+        boxes.append(dict(nx=nx, ny=nx, nz=nz, r0=r0, amps=(amp,), tau=tau, nu_r=nu, nu_b=nu, sigma=0.01,
+                          rho0=1.0, beta=0.7))
+    return boxes

@@ -114,3 +114,30 @@ def test_config3_amplitudes_fixture():
     a = ic.rp_config3_amps()
     assert np.array_equal(a, np.array(gold["amps"]))
     assert np.all((a >= 0) & (a <= 0.02 * 40.0))
+
+
+def test_ensemble_draws_match_fixture():
+    with open(os.path.join(ROOT, "tests", "golden", "ensemble_config4.json")) as f:
+        fix = json.load(f)["boxes"]
+    boxes = ic.ensemble_config4()
+    assert len(boxes) == len(fix) == 64
+    for b, g in zip(boxes, fix):
+        for k in ("nx", "ny", "nz", "r0", "tau"):
+            assert b[k] == g[k]
+        assert list(b["amps"]) == g["amps"]
+    for b in boxes:
+        assert b["nx"] in (32, 48, 64, 96) and b["nz"] % 32 == 0 and 64 <= b["nz"] <= 512
+        assert 0.15 * b["nx"] <= b["r0"] <= 0.3 * b["nx"] and 0.6 <= b["tau"] <= 1.2
+        assert 0.01 * b["r0"] <= b["amps"][0] <= 0.1 * b["r0"]
+
+
+def test_batch_create_validates(lib):
+    boxes = (symmetry_amd.Box * 2)()
+    boxes[0].nx, boxes[0].ny, boxes[0].nz, boxes[0].nu_r, boxes[0].nu_b = 8, 8, 8, 0.1, 0.1
+    boxes[1].nx, boxes[1].ny, boxes[1].nz, boxes[1].nu_r, boxes[1].nu_b = 0, 8, 8, 0.1, 0.1
+    h = ctypes.c_void_p()
+    assert lib.cgl_batch_create(ctypes.byref(h), ctypes.byref(_params()), 2, boxes) == -1
+    boxes[1].nx = 8
+    p = _params()
+    p.bc[0] = p.bc[1] = 1
+    assert lib.cgl_batch_create(ctypes.byref(h), ctypes.byref(p), 2, boxes) == -3
