@@ -46,7 +46,11 @@ PINS = {
     "or_set_walls": "compiled reference (bit-exact ints, S:326-395); brute-force sphere count",
     "or_refilling": "compiled reference (S:1351-1418)",
     "or_momentum": "compiled reference (S:1178-1261)",
-    "or_lagrange_capillary": "compiled reference (S:1799-2028)",
+    "or_lagrange_capillary": "compiled reference (S:1799-2028) at the reference's contact angle of 90 degrees; "
+                             "other angles take the S:2015-2019 else-branch, which the reference never runs: "
+                             "parity unpinned for contact angles other than 90 degrees",
+    "periodic faces": "translation invariance of a step (tests/test_oracle_invariance.py); the reference has none",
+    "high-side wall faces": "reflection symmetry of a step between two walls (tests/test_oracle_invariance.py)",
     "or_repulsive": "compiled reference (S:1594-1671)",
     "or_movement": "compiled reference (S:1263-1349)",
     "or_step": "compiled reference, whole steps with and without particles",
@@ -219,6 +223,17 @@ REF_CALL = {
 REF_BC = (BC_WALL, BC_MIRROR, BC_MIRROR, BC_MIRROR, BC_WALL, BC_MIRROR)
 
 
+_SCRATCH = None
+
+
+def _scratch_dir():
+    global _SCRATCH
+    if _SCRATCH is None:
+        import tempfile
+        _SCRATCH = tempfile.mkdtemp(prefix="cgl_ref_")
+    return _SCRATCH
+
+
 class RefHarness:
     """The compiled reference at its own compile-time box (152^3, par=12).
 
@@ -242,6 +257,9 @@ class RefHarness:
 
     @staticmethod
     def _run(fn, *args):
+        """Run a reference call on a large-stack thread, in a scratch working
+        directory: the reference's Lagrange routine writes Laglange.dat into
+        the current directory every call."""
         box = {}
 
         def work():
@@ -251,13 +269,16 @@ class RefHarness:
                 box["e"] = e
 
         old = threading.stack_size()
+        cwd = os.getcwd()
+        os.chdir(_scratch_dir())
         threading.stack_size(1 << 30)
         try:
             th = threading.Thread(target=work)
             th.start()
+            th.join()
         finally:
             threading.stack_size(old)
-        th.join()
+            os.chdir(cwd)
         if "e" in box:
             raise box["e"]
         return box.get("r")

@@ -17,6 +17,13 @@
  * half-way bounce-back wall with a ghost layer (the reference's x=1 and z=1,
  * S:806-807, S:1557-1590) or a specular mirror ghost layer (the reference's
  * x=xl, y=1, y=yl, z=zl, S:1443-1542).
+ *
+ * Pins: every stage is checked bit for bit against the compiled reference on
+ * its own geometry (tests/test_oracle_vs_ref.py); periodic faces and
+ * high-side walls, which the reference lacks, by translation and reflection
+ * symmetry (tests/test_oracle_invariance.py).  Parity unpinned: the capillary
+ * force of or_lagrange_capillary for contact angles other than 90 degrees
+ * (the S:2015-2019 else-branch; the reference's angle is a compile-time 90).
  */
 #ifndef CGL_ORACLE_H
 #define CGL_ORACLE_H
