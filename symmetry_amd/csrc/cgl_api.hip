@@ -737,10 +737,13 @@ int cgl_batch_create(cgl_t **out, const cgl_params *p, int32_t nbox, const cgl_b
         D.gz = (boxes[b].nz + cgl::TZ - 1) / cgl::TZ;
         D.gy = (boxes[b].ny + cgl::TY - 1) / cgl::TY;
         D.gx2 = (boxes[b].nx + h->xc - 1) / h->xc;
+        const int kty = p->precision == CGL_PREC_F64 ? cgl::KTY<double> : cgl::KTY<float>;
+        D.gy2 = (boxes[b].ny + kty - 1) / kty;
+        D.gx1 = (boxes[b].nx + cgl::K1_XC - 1) / cgl::K1_XC;
         D.t1 = (unsigned)t1;
         D.t2 = (unsigned)t2;
-        t1 += (unsigned long long)D.gz * D.gy * boxes[b].nx;
-        t2 += (unsigned long long)D.gz * D.gy * D.gx2;
+        t1 += (unsigned long long)D.gz * D.gy * D.gx1;
+        t2 += (unsigned long long)D.gz * D.gy2 * D.gx2;
         base += (unsigned long long)D.L.N;
     }
     if (t1 >= (1ull << 31) || t2 >= (1ull << 31)) {

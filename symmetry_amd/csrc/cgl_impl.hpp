@@ -61,6 +61,11 @@ namespace cgl {
 
 inline dim3 node_grid(const Lat &L) { return dim3((L.nz + TZ - 1) / TZ, (L.ny + TY - 1) / TY, L.nx); }
 inline dim3 node_block() { return dim3(TZ, TY, 1); }
+template <class R> inline dim3 k2_block() { return dim3(TZ, KTY<R>, 1); }
+template <class R> inline dim3 k2_grid(const Lat &L, int xc)
+{
+    return dim3((L.nz + TZ - 1) / TZ, (L.ny + KTY<R> - 1) / KTY<R>, (L.nx + xc - 1) / xc);
+}
 
 // Input buffers of the current state and output buffers of the next one.
 inline void state_in(const cgl_handle *h, const void *&i0, const void *&i1)
@@ -84,6 +89,9 @@ inline void state_out(const cgl_handle *h, void *&o0, void *&o1)
     }
 }
 
+// x planes per workgroup of the x-marching moments kernel (factorised layout)
+constexpr int K1_XC = 8;
+
 template <class S, class R, class P, bool SHIFT> struct Impl {
     static void moments(cgl_handle *h, bool full)
     {
@@ -101,10 +109,15 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                 k_moments<LIN, S, R, P, SHIFT, false>
                     <<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, mac);
         };
-        if (h->state_layout == RB)
+        if (h->state_layout == RB) {
             launch(std::integral_constant<int, RB>{});
-        else
+        } else if (!full) {
+            const int xc = K1_XC;
+            dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, (h->L.nx + xc - 1) / xc);
+            k_moments_fsx<S, R, P, SHIFT><<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, xc);
+        } else {
             launch(std::integral_constant<int, FS>{});
+        }
     }
     // one fused step; afterwards the state is held in h->layout
     static void collide(cgl_handle *h)
@@ -115,16 +128,16 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         state_out(h, o0, o1);
         const P *phio = (const P *)h->phi[h->phicur];
         const P *phi = (const P *)h->phi[h->phicur ^ 1];
-        dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, (h->L.nx + h->xc - 1) / h->xc);
+        const dim3 grid = k2_grid<R>(h->L, h->xc);
         if (h->state_layout == RB && h->layout == RB)
             k_collide<RB, RB, S, R, P, SHIFT>
-                <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc);
+                <<<grid, k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc);
         else if (h->state_layout == RB)
             k_collide<RB, FS, S, R, P, SHIFT>
-                <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc);
+                <<<grid, k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc);
         else
             k_collide<FS, FS, S, R, P, SHIFT>
-                <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc);
+                <<<grid, k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc);
         h->state_layout = h->layout;
     }
     // ---- batches: one launch per kernel over the tiles of every box
@@ -137,10 +150,10 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         const int remap = h->L.xcd_remap;
         if (h->state_layout == RB)
             k_moments_batch<RB, S, R, P, SHIFT>
-                <<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, remap, i0, i1, phio, phi);
+                <<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, remap, K1_XC, i0, i1, phio, phi);
         else
             k_moments_batch<FS, S, R, P, SHIFT>
-                <<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, remap, i0, i1, phio, phi);
+                <<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, remap, K1_XC, i0, i1, phio, phi);
     }
     static void batch_collide(cgl_handle *h)
     {
@@ -152,13 +165,13 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         const P *phi = (const P *)h->phi[h->phicur ^ 1];
         const int remap = h->L.xcd_remap;
         if (h->state_layout == RB && h->layout == RB)
-            k_collide_batch<RB, RB, S, R, P, SHIFT><<<h->tiles2, node_block(), 0, h->stream>>>(
+            k_collide_batch<RB, RB, S, R, P, SHIFT><<<h->tiles2, k2_block<R>(), 0, h->stream>>>(
                 h->dboxes, h->nbox, remap, h->xc, i0, i1, o0, o1, phi, phio);
         else if (h->state_layout == RB)
-            k_collide_batch<RB, FS, S, R, P, SHIFT><<<h->tiles2, node_block(), 0, h->stream>>>(
+            k_collide_batch<RB, FS, S, R, P, SHIFT><<<h->tiles2, k2_block<R>(), 0, h->stream>>>(
                 h->dboxes, h->nbox, remap, h->xc, i0, i1, o0, o1, phi, phio);
         else
-            k_collide_batch<FS, FS, S, R, P, SHIFT><<<h->tiles2, node_block(), 0, h->stream>>>(
+            k_collide_batch<FS, FS, S, R, P, SHIFT><<<h->tiles2, k2_block<R>(), 0, h->stream>>>(
                 h->dboxes, h->nbox, remap, h->xc, i0, i1, o0, o1, phi, phio);
         h->state_layout = h->layout;
     }
@@ -284,9 +297,8 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             state_out(h, o0, o1);
             const P *phio = (const P *)h->phi[h->phicur];
             const Ext En = ext_with(h, h->ntype[nb]);
-            dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, (h->L.nx + h->xc - 1) / h->xc);
             k_collide<RB, RB, S, R, P, SHIFT, true>
-                <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
+                <<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
             if (h->npar) {
                 S *ro = (S *)o0, *bo = (S *)o1;
                 k_curve<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(

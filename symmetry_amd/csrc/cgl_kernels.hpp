@@ -11,11 +11,13 @@
 //          stored populations were shifted with (FP32/FP16 storage only).
 //
 // Per step (S:216-251 without particles):
-//   K1 k_moments  pull-gather -> rho_R, rho_B -> phi          (calmacros, S:770-831)
+//   K1 k_moments(_fsx) pull-gather -> rho_R, rho_B -> phi     (calmacros, S:770-831)
 //   K2 k_collide  x-marching tile, LDS rings of phi and n    (derivatives S:452-597,
 //                 -> grad phi, n, curvature, CSF force,        collision S:599-695,
 //                 MRT + Guo, recolour, write pops              Recolor S:697-732,
 //                                                              streaming S:735-768)
+// The factorised layout (FS, below) keeps the recolour scalars of planes
+// x-1..x+1 in an LDS ring in both kernels.
 //
 // FP64 ("plain") keeps the oracle's exact operation order and is compiled
 // without FMA contraction, so it reproduces the FP64 CPU path bit for bit.
@@ -25,6 +27,8 @@
 // the reader needs only its own node's value.  Collision works on the
 // deviations f - w_i*rho0 in FP32 with compensated moment sums.
 #pragma once
+
+#include <type_traits>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -36,6 +40,10 @@ namespace cgl {
 constexpr double EVAP_LIM = 1.0e-8; // S:16
 constexpr int TZ = 64;              // workgroup extent along z (one wavefront)
 constexpr int TY = 4;               // workgroup extent along y
+// Workgroup extent along y of the fused collide kernel: 8 rows for FP32/FP16
+// arithmetic (two 512-thread workgroups fit a CU's LDS), 4 for FP64, whose
+// larger LDS rings would leave room for only one 512-thread workgroup.
+template <class R> constexpr int KTY = sizeof(R) == 8 ? 4 : 8;
 
 struct Lat {
     int nx, ny, nz;
@@ -168,6 +176,58 @@ template <class R> struct Node<R, true> {
     R sr;
 };
 
+// Recolour scalars of one node in the FP64 factorised layout.
+struct FsScal64 {
+    double a, g, nx, ny, nz;
+};
+
+// Pull gather of the factorised layout: f_i from x - c_i and the red part
+// rebuilt with the source node's scalars, which `scal(i, m)` returns for
+// direction i and source node m (from global memory or from an LDS copy).
+//  FP64:    r_i = A f_i + (gamma w_i)(n . c_i), the reference's rounding.
+//  shifted: rho_R = rho0 sum_i w_i A_i + sum_i A_i f~_i + sum_i w_i G_i.c_i:
+//           the O(1) part is summed per weight class, the rest are small
+//           deviations.
+template <class S, class R, bool SHIFT, class Scal>
+__device__ __forceinline__ void gather_fs(const S *__restrict__ fs, size_t N, const unsigned xo[3],
+                                          const unsigned yo[3], const unsigned zo[3], Scal scal,
+                                          Node<R, SHIFT> &nd, R rho0)
+{
+    if constexpr (!SHIFT) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+            R f = StoreOf<S>::ld(fs + (size_t)i * N, m);
+            const FsScal64 sc = scal(i, m);
+            R r = sc.a * f;
+            if (i > 0) r = r + (sc.g * R(W[i])) * dot_c<R>(i, sc.nx, sc.ny, sc.nz);
+            nd.r[i] = r;
+            nd.b[i] = f - r;
+        }
+    } else {
+        R a0 = 0, a1 = 0, a2 = 0, dev = 0, g1 = 0, g2 = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+            R f = StoreOf<S>::ld(fs + (size_t)i * N, m);
+            const float4 g = scal(i, m);
+            dev += R(g.x) * f;
+            R gc = 0;
+            if (CX[i] == 1) gc += R(g.y);
+            if (CX[i] == -1) gc -= R(g.y);
+            if (CY[i] == 1) gc += R(g.z);
+            if (CY[i] == -1) gc -= R(g.z);
+            if (CZ[i] == 1) gc += R(g.w);
+            if (CZ[i] == -1) gc -= R(g.w);
+            if (i == 0) a0 = R(g.x);
+            else if (i <= 6) { a1 += R(g.x); g1 += gc; }
+            else { a2 += R(g.x); g2 += gc; }
+            nd.ft[i] = f;
+        }
+        nd.sr = rho0 * (a0 * R(W[0]) + a1 * R(W[1]) + a2 * R(W[7])) + dev + (g1 * R(W[1]) + g2 * R(W[7]));
+    }
+}
+
 // Pull gather at node (x, y, z) from x - c_i (streaming, S:743-753).
 // ax: red shift fraction of this node (RB shifted storage only).
 // Half-way wall bounce-back in pull form: a population whose upstream node is
@@ -236,46 +296,17 @@ __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void 
         const S *fs = (const S *)p0;
         if constexpr (!SHIFT) {
             const double *sc = (const double *)p1;
-#pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-                const S *fi = fs + (size_t)i * N;
-                R f = StoreOf<S>::ld(fi, m);
-                const double2 s01 = at((const double2 *)sc, 3 * m);
-                R r = s01.x * f;
-                if (i > 0) {
+            gather_fs<S, R, SHIFT>(
+                fs, N, xo, yo, zo,
+                [&](int, unsigned m) {
+                    const double2 s01 = at((const double2 *)sc, 3 * m);
                     const double2 s23 = at((const double2 *)sc, 3 * m + 1);
-                    const double s4 = at(sc, 6 * m + 4);
-                    r = r + (s01.y * R(W[i])) * dot_c<R>(i, s23.x, s23.y, s4);
-                }
-                nd.r[i] = r;
-                nd.b[i] = f - r;
-            }
+                    return FsScal64{s01.x, s01.y, s23.x, s23.y, at(sc, 6 * m + 4)};
+                },
+                nd, rho0);
         } else {
-            // rho_R = rho0 sum_i w_i A_i + sum_i A_i f~_i + sum_i w_i G_i.c_i: the
-            // O(1) part is summed per weight class, the rest are small deviations.
             const float4 *sc = (const float4 *)p1;
-            R a0 = 0, a1 = 0, a2 = 0, dev = 0, g1 = 0, g2 = 0;
-#pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-                const S *fi = fs + (size_t)i * N;
-                R f = StoreOf<S>::ld(fi, m);
-                const float4 g = at(sc, m);
-                dev += R(g.x) * f;
-                R gc = 0;
-                if (CX[i] == 1) gc += R(g.y);
-                if (CX[i] == -1) gc -= R(g.y);
-                if (CY[i] == 1) gc += R(g.z);
-                if (CY[i] == -1) gc -= R(g.z);
-                if (CZ[i] == 1) gc += R(g.w);
-                if (CZ[i] == -1) gc -= R(g.w);
-                if (i == 0) a0 = R(g.x);
-                else if (i <= 6) { a1 += R(g.x); g1 += gc; }
-                else { a2 += R(g.x); g2 += gc; }
-                nd.ft[i] = f;
-            }
-            nd.sr = rho0 * (a0 * R(W[0]) + a1 * R(W[1]) + a2 * R(W[7])) + dev + (g1 * R(W[1]) + g2 * R(W[7]));
+            gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, [&](int, unsigned m) { return at(sc, m); }, nd, rho0);
         }
     }
 }
@@ -434,6 +465,87 @@ __global__ __launch_bounds__(256) void k_moments(Lat L, const void *__restrict__
     moments_node<LIN, S, R, P, SHIFT, FULL, EXT>(L, p0, p1, phio, phi, mac, E, bx, by, bz);
 }
 
+// K1 for the factorised layout, marching in x: the recolour scalars of
+// planes x-1, x, x+1 (with a one-node y/z halo) sit in an LDS ring, so each
+// node's scalars are fetched from memory once instead of by all 19 readers.
+template <bool SHIFT, int ROWS = TY> struct FsxRing;
+template <int ROWS> struct FsxRing<true, ROWS> {
+    float4 s[3][ROWS + 2][TZ + 2];
+};
+template <int ROWS> struct FsxRing<false, ROWS> {
+    double s[3][5][ROWS + 2][TZ + 2];
+};
+
+template <class S, class R, class P, bool SHIFT>
+__device__ __forceinline__ void moments_fsx_tile(const Lat &L, const void *__restrict__ p0,
+                                                 const void *__restrict__ p1, const P *__restrict__ phio,
+                                                 P *__restrict__ phi, int xc, unsigned bx, unsigned by, unsigned bz)
+{
+    __shared__ FsxRing<SHIFT> sm;
+    const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
+    const int z0 = bx * TZ, y0 = by * TY, x0 = bz * xc;
+    const int x1 = min(x0 + xc, L.nx);
+    const int z = z0 + tz, y = y0 + ty;
+    const bool active = z < L.nz && y < L.ny;
+    const unsigned sx = (unsigned)L.ny * (unsigned)L.nz;
+    const size_t N = (size_t)L.N;
+    const S *fs = (const S *)p0;
+    (void)phio;
+    auto load = [&](int p) {
+        const int sl = (p + 9) % 3;
+        const unsigned gx = (unsigned)wrapm(p, L.nx) * sx;
+        for (int e = tid; e < (TY + 2) * (TZ + 2); e += TZ * TY) {
+            const int jy = e / (TZ + 2), jz = e - jy * (TZ + 2);
+            const unsigned n = gx + (unsigned)wrapm(y0 - 1 + jy, L.ny) * (unsigned)L.nz + wrapm(z0 - 1 + jz, L.nz);
+            if constexpr (SHIFT) {
+                sm.s[sl][jy][jz] = at((const float4 *)p1, n);
+            } else {
+                const double *sc = (const double *)p1;
+                const double2 s01 = at((const double2 *)sc, 3 * n);
+                const double2 s23 = at((const double2 *)sc, 3 * n + 1);
+                sm.s[sl][0][jy][jz] = s01.x;
+                sm.s[sl][1][jy][jz] = s01.y;
+                sm.s[sl][2][jy][jz] = s23.x;
+                sm.s[sl][3][jy][jz] = s23.y;
+                sm.s[sl][4][jy][jz] = at(sc, 6 * n + 4);
+            }
+        }
+    };
+    load(x0 - 1);
+    load(x0);
+    for (int x = x0; x < x1; ++x) {
+        load(x + 1);
+        __syncthreads();
+        if (active) {
+            unsigned xo[3], yo[3], zo[3];
+            node_offsets(L, x, y, z, xo, yo, zo);
+            Node<R, SHIFT> nd;
+            // scalars of the source x - c_i from the ring
+            auto scal = [&](int i, unsigned) {
+                const int sl = (x - CX[i] + 9) % 3, jy = ty + 1 - CY[i], jz = tz + 1 - CZ[i];
+                if constexpr (SHIFT)
+                    return sm.s[sl][jy][jz];
+                else
+                    return FsScal64{sm.s[sl][0][jy][jz], sm.s[sl][1][jy][jz], sm.s[sl][2][jy][jz],
+                                    sm.s[sl][3][jy][jz], sm.s[sl][4][jy][jz]};
+            };
+            gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, scal, nd, R(L.rho0));
+            const Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
+            at(phi, xo[1] + yo[1] + zo[1]) = P(mo.phi);
+        }
+        __syncthreads(); // the next load overwrites the oldest slot
+    }
+}
+
+template <class S, class R, class P, bool SHIFT>
+__global__ __launch_bounds__(256) void k_moments_fsx(Lat L, const void *__restrict__ p0, const void *__restrict__ p1,
+                                                     const P *__restrict__ phio, P *__restrict__ phi, int xc)
+{
+    unsigned bx, by, bz;
+    tile_of_block(L, bx, by, bz);
+    moments_fsx_tile<S, R, P, SHIFT>(L, p0, p1, phio, phi, xc, bx, by, bz);
+}
+
 // ----------------------------------------------------- gradient of phi (b)
 // Gradient 3*sum_i w_i c_i phi(x + c_i) over i = 1..18 in the reference's
 // order (S:511-520), with zero terms skipped (they add exact zeros).
@@ -490,6 +602,48 @@ __device__ __forceinline__ void collide(const Lat &L, R f[Q], const Moments<R, S
         nu = fdiv(R(2), fdiv(R(1) + phin, R(L.nu_r)) + fdiv(R(1) - phin, R(L.nu_b)));
     const R su = fdiv(R(1), R(3) * nu + R(0.5));
     const R sq = fdiv(R(8) * (R(2) - su), R(8) - su);
+    if constexpr (SHIFT) {
+        // The relaxation rates split by moment parity: every even moment that
+        // is not conserved relaxes with su and every odd one with sq (S:682),
+        // the equilibrium moments are M feq of the SRT equilibrium and the
+        // Guo forcing moments are M G (S:405-447).  Conserved moments (rho,
+        // j) leave the same result at any rate because u carries the half
+        // force.  So the MRT update is the two-relaxation-time update in
+        // population space, per direction pair (i, opp i):
+        //   f_i += -su (f+_i - feq+_i) - sq (f-_i - feq-_i)
+        //          + (1 - su/2) G+_i + (1 - sq/2) G-_i
+        // with f+- the half sum and difference of the pair.  Here on shifted
+        // values f~ = f - w rho0 (the rho0 part is its own equilibrium).
+        R drho = f[0];
+#pragma unroll
+        for (int i = 1; i < Q; ++i) drho += f[i];
+        const R usqr = ux * ux + uy * uy + uz * uz;
+        const R uf = ux * fx + uy * fy + uz * fz;
+        const R ep0 = drho - R(1.5) * rho * usqr; // w-less even equilibrium part
+        const R r45 = R(4.5) * rho, r3 = R(3) * rho;
+        const R ap = R(1) - R(0.5) * su, am = R(1) - R(0.5) * sq;
+        const R gp0 = R(-3) * uf;
+        const R hsu = R(0.5) * su, hsq = R(0.5) * sq;
+        // rest population: f0 += -su (f0 - feq0) + (1 - su/2) G0
+        f[0] = f[0] - su * (f[0] - R(W[0]) * ep0) + ap * (R(W[0]) * gp0);
+#pragma unroll
+        for (int q = 0; q < NPAIR; ++q) {
+            const int i = PAIR[q], o = OPC[i];
+            const R cu = dot_c<R>(i, ux, uy, uz);
+            const R cf = dot_c<R>(i, fx, fy, fz);
+            const R w = R(W[i]);
+            const R feqp = w * (ep0 + r45 * cu * cu);
+            const R feqm = w * (r3 * cu);
+            const R gp = w * (gp0 + R(9) * cu * cf);
+            const R gm = w * (R(3) * cf);
+            const R sp = f[i] + f[o], dm = f[i] - f[o];
+            const R A = su * feqp - hsu * sp + ap * gp;
+            const R B = sq * feqm - hsq * dm + am * gm;
+            f[i] = f[i] + (A + B);
+            f[o] = f[o] + (A - B);
+        }
+        return;
+    }
     R m[Q];
     if constexpr (!SHIFT) {
         // dense order of the reference's MATMUL(M, f) (S:685), zero terms skipped
@@ -724,8 +878,8 @@ __device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, uns
 //            curvature stencil needs n at the 18 neighbours)
 // Streaming is the pull gather from x - c_i (periodic wrap).
 template <class R, class P> struct K2Smem {
-    P phi[5][TY + 4][TZ + 4];
-    R n[3][3][TY + 2][TZ + 2];
+    P phi[5][KTY<R> + 4][TZ + 4];
+    R n[3][3][KTY<R> + 2][TZ + 2];
 };
 
 __device__ __forceinline__ int slot5(int p) { return (p + 10) % 5; }
@@ -775,7 +929,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
 {
     __shared__ K2Smem<R, P> sm;
     const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
-    const int z0 = bx * TZ, y0 = by * TY, x0 = bz * xc;
+    const int z0 = bx * TZ, y0 = by * KTY<R>, x0 = bz * xc;
     const int x1 = min(x0 + xc, L.nx);
     const int z = z0 + tz, y = y0 + ty;
     const bool active = z < L.nz && y < L.ny;
@@ -784,17 +938,17 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
     auto load_phi = [&](int p) {
         const int gx = wrapm(p, L.nx) * sx;
         P(*dst)[TZ + 4] = sm.phi[slot5(p)];
-        for (int e = tid; e < (TY + 4) * (TZ + 4); e += TZ * TY) {
+        for (int e = tid; e < (KTY<R> + 4) * (TZ + 4); e += TZ * KTY<R>) {
             int jy = e / (TZ + 4), jz = e - jy * (TZ + 4);
             int gy = wrapm(y0 - 2 + jy, L.ny), gz = wrapm(z0 - 2 + jz, L.nz);
             dst[jy][jz] = at(phi, (unsigned)(gx + gy * L.nz + gz));
         }
     };
-    // unit normal of plane p on the (TY+2) x (TZ+2) halo region (S:498-532);
+    // unit normal of plane p on the (KTY<R>+2) x (TZ+2) halo region (S:498-532);
     // with EXT: zero at solid and ghost nodes, wetting at live fluid nodes
     auto normals = [&](int p) {
         const int s = slot3(p);
-        for (int e = tid; e < (TY + 2) * (TZ + 2); e += TZ * TY) {
+        for (int e = tid; e < (KTY<R> + 2) * (TZ + 2); e += TZ * KTY<R>) {
             int jy = e / (TZ + 2), jz = e - jy * (TZ + 2);
             R nx = 0, ny = 0, nz = 0;
             bool skip = false;
@@ -848,7 +1002,28 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
         }
     };
 
+    // FP32/FP16 factorised input: the recolour scalars of planes x-1..x+1 in an
+    // LDS ring, as in the x-marching moments kernel (the FP64 ring would not
+    // leave room for a second workgroup per CU)
+    constexpr bool RING = LIN == FS && SHIFT;
+    struct NoRing {};
+    __shared__ std::conditional_t<RING, FsxRing<true, KTY<R>>, NoRing> ring;
+    auto load_scal = [&](int p) {
+        if constexpr (RING) {
+            const int sl = (p + 9) % 3;
+            const unsigned gx = (unsigned)wrapm(p, L.nx) * (unsigned)sx;
+            for (int e = tid; e < (KTY<R> + 2) * (TZ + 2); e += TZ * KTY<R>) {
+                const int jy = e / (TZ + 2), jz = e - jy * (TZ + 2);
+                const unsigned n =
+                    gx + (unsigned)wrapm(y0 - 1 + jy, L.ny) * (unsigned)L.nz + wrapm(z0 - 1 + jz, L.nz);
+                ring.s[sl][jy][jz] = at((const float4 *)i1, n);
+            }
+        }
+    };
+
     for (int p = x0 - 2; p <= x0 + 1; ++p) load_phi(p);
+    load_scal(x0 - 1);
+    load_scal(x0);
     __syncthreads();
     normals(x0 - 1);
     normals(x0);
@@ -856,6 +1031,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
     for (int x = x0; x < x1; ++x) {
         __syncthreads(); // readers of the slots refilled below are done
         load_phi(x + 2);
+        load_scal(x + 1);
         __syncthreads();
         normals(x + 1);
         __syncthreads();
@@ -998,7 +1174,13 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
             mo.uz = R(uz);
             mo.phi = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
         } else {
-            gather<LIN, S, R, SHIFT>(L, i0, i1, xo, yo, zo, ax, nd, bb);
+            if constexpr (RING)
+                gather_fs<S, R, SHIFT>(
+                    (const S *)i0, (size_t)L.N, xo, yo, zo,
+                    [&](int i, unsigned) { return ring.s[(x - CX[i] + 9) % 3][ty + 1 - CY[i]][tz + 1 - CZ[i]]; },
+                    nd, R(L.rho0));
+            else
+                gather<LIN, S, R, SHIFT>(L, i0, i1, xo, yo, zo, ax, nd, bb);
             mo = moments<R, SHIFT>(L, nd);
         }
         const R phin = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
@@ -1017,7 +1199,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
 }
 
 template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT = false>
-__global__ __launch_bounds__(256) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
+__global__ __launch_bounds__(TZ * KTY<R>) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
                                                  void *__restrict__ o0, void *__restrict__ o1,
                                                  const P *__restrict__ phi, const P *__restrict__ phio, int xc,
                                                  Ext E = Ext{})
@@ -1036,7 +1218,8 @@ struct BoxDev {
     Lat L;
     unsigned long long base; // first node of the box in the packed index
     unsigned t1, t2;         // first tile of the box in the moments / collide launch
-    unsigned gz, gy, gx2;    // z tiles, y tiles, x chunks of the collide launch
+    unsigned gz, gy, gx1;    // z tiles, y tiles, x chunks of the moments launch
+    unsigned gy2, gx2;       // y tiles, x chunks of the collide launch
     unsigned pad;
 };
 
@@ -1073,8 +1256,10 @@ __device__ __forceinline__ void box_state(const void *&a, const void *&b, unsign
         b = (const double *)b + (size_t)FsScal<false>::NS * base;
 }
 
+// Moments over x chunks of xc planes: the x-marching kernel for the
+// factorised layout, plane by plane for red/blue.
 template <int LIN, class S, class R, class P, bool SHIFT>
-__global__ __launch_bounds__(256) void k_moments_batch(const BoxDev *__restrict__ B, int nbox, int remap,
+__global__ __launch_bounds__(256) void k_moments_batch(const BoxDev *__restrict__ B, int nbox, int remap, int xc,
                                                        const void *__restrict__ p0, const void *__restrict__ p1,
                                                        const P *__restrict__ phio, P *__restrict__ phi)
 {
@@ -1083,14 +1268,21 @@ __global__ __launch_bounds__(256) void k_moments_batch(const BoxDev *__restrict_
     const Lat L = B[b].L;
     const unsigned long long base = B[b].base;
     const unsigned lt = t - B[b].t1, gz = B[b].gz, gy = B[b].gy;
+    const unsigned bx = lt % gz, by = (lt / gz) % gy, cx = lt / (gz * gy);
     const void *a = p0, *c = p1;
     box_state<LIN, S, SHIFT>(a, c, base);
-    moments_node<LIN, S, R, P, SHIFT, false, false>(L, a, c, phio + base, phi + base, nullptr, Ext{}, lt % gz,
-                                                    (lt / gz) % gy, lt / (gz * gy));
+    if constexpr (LIN == FS) {
+        moments_fsx_tile<S, R, P, SHIFT>(L, a, c, phio + base, phi + base, xc, bx, by, cx);
+    } else {
+        const int x1 = min((int)(cx + 1) * xc, L.nx);
+        for (int x = cx * xc; x < x1; ++x)
+            moments_node<LIN, S, R, P, SHIFT, false, false>(L, a, c, phio + base, phi + base, nullptr, Ext{}, bx, by,
+                                                            x);
+    }
 }
 
 template <int LIN, int LOUT, class S, class R, class P, bool SHIFT>
-__global__ __launch_bounds__(256) void k_collide_batch(const BoxDev *__restrict__ B, int nbox, int remap, int xc,
+__global__ __launch_bounds__(TZ * KTY<R>) void k_collide_batch(const BoxDev *__restrict__ B, int nbox, int remap, int xc,
                                                        const void *__restrict__ i0, const void *__restrict__ i1,
                                                        void *__restrict__ o0, void *__restrict__ o1,
                                                        const P *__restrict__ phi, const P *__restrict__ phio)
@@ -1099,7 +1291,7 @@ __global__ __launch_bounds__(256) void k_collide_batch(const BoxDev *__restrict_
     const int b = find_box<true>(B, nbox, t);
     const Lat L = B[b].L;
     const unsigned long long base = B[b].base;
-    const unsigned lt = t - B[b].t2, gz = B[b].gz, gy = B[b].gy;
+    const unsigned lt = t - B[b].t2, gz = B[b].gz, gy = B[b].gy2;
     const void *a0 = i0, *a1 = i1, *c0 = o0, *c1 = o1;
     box_state<LIN, S, SHIFT>(a0, a1, base);
     box_state<LOUT, S, SHIFT>(c0, c1, base);
