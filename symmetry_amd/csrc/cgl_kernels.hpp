@@ -850,11 +850,16 @@ __device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, uns
             }
         }
     } else {
+        // the next state is read by the next step's kernels only: stream it
+        // past the caches (FP32 values; FP16 and FP64 use plain stores)
         S *fo = (S *)o0;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             S *fp = fo + (size_t)i * N;
-            StoreOf<S>::st(fp, n, f[i]);
+            if constexpr (SHIFT && sizeof(S) == 4)
+                __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, n));
+            else
+                StoreOf<S>::st(fp, n, f[i]);
         }
         R A = c.A, g = c.gam;
         if (c.evr) { A = 0; g = 0; }
@@ -865,7 +870,9 @@ __device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, uns
             at(sc, 3 * n + 1) = make_double2(nx, ny);
             at(sc, 3 * n + 2) = make_double2(nz, 0.0);
         } else {
-            at((float4 *)o1, n) = make_float4(float(A), float(g * nx), float(g * ny), float(g * nz));
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const f4v v4 = {float(A), float(g * nx), float(g * ny), float(g * nz)};
+            __builtin_nontemporal_store(v4, (f4v *)&at((float4 *)o1, n));
         }
     }
 }
