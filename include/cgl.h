@@ -64,9 +64,11 @@ typedef struct cgl_params {
 } cgl_params;
 
 /* Allocate device state for one box.  Replaces the allocation block of the
- * reference's main (S:144-179).  Faces may be periodic (in pairs) or
- * half-way bounce-back walls; npar > 0 adds moving particles (red/blue
- * layout only).  Mirror faces return CGL_ERR_UNSUPPORTED. */
+ * reference's main (S:144-179).  Faces may be periodic (in pairs),
+ * half-way bounce-back walls (S:1557-1590) or specular mirrors
+ * (S:1443-1542), the last two with a ghost layer; npar > 0 adds moving
+ * particles.  Walls, mirrors and particles run on the red/blue layout;
+ * particles together with mirror faces return CGL_ERR_UNSUPPORTED. */
 int cgl_create(cgl_t **h, const cgl_params *p);
 int cgl_destroy(cgl_t *h);
 

@@ -147,25 +147,30 @@ int cgl_create(cgl_t **out, const cgl_params *p)
         return fail(CGL_ERR_UNSUPPORTED, "box of %.3g nodes exceeds the 4 GiB per-array addressing", nn);
     if (p->precision < CGL_PREC_F64 || p->precision > CGL_PREC_F16)
         return fail(CGL_ERR_ARG, "unknown precision %d", p->precision);
-    int walls = 0;
+    int walls = 0, mirrors = 0;
+    const int nn3[3] = {p->nx, p->ny, p->nz};
     for (int a = 0; a < 3; ++a) {
         for (int s = 0; s < 2; ++s) {
             const int f = p->bc[2 * a + s];
-            if (f == CGL_BC_MIRROR)
-                return fail(CGL_ERR_UNSUPPORTED, "face %d: mirror faces are not implemented on the GPU", 2 * a + s);
-            if (f != CGL_BC_PERIODIC && f != CGL_BC_WALL) return fail(CGL_ERR_ARG, "face %d: unknown type %d", 2 * a + s, f);
+            if (f != CGL_BC_PERIODIC && f != CGL_BC_WALL && f != CGL_BC_MIRROR)
+                return fail(CGL_ERR_ARG, "face %d: unknown type %d", 2 * a + s, f);
             walls |= f == CGL_BC_WALL;
+            mirrors |= f == CGL_BC_MIRROR;
         }
         if ((p->bc[2 * a] == CGL_BC_PERIODIC) != (p->bc[2 * a + 1] == CGL_BC_PERIODIC))
             return fail(CGL_ERR_ARG, "axis %d: periodic faces come in pairs", a);
+        if (p->bc[2 * a] != CGL_BC_PERIODIC && nn3[a] < 4)
+            return fail(CGL_ERR_ARG, "axis %d: a bounded axis needs two ghost layers and two live nodes", a);
     }
     if (p->npar < 0) return fail(CGL_ERR_ARG, "npar must be >= 0");
-    if ((walls || p->npar > 0) && p->layout != CGL_LAYOUT_RB)
-        return fail(CGL_ERR_UNSUPPORTED, "walls and particles are implemented for the red/blue layout");
+    if ((walls || mirrors || p->npar > 0) && p->layout != CGL_LAYOUT_RB)
+        return fail(CGL_ERR_UNSUPPORTED, "walls, mirrors and particles are implemented for the red/blue layout");
+    if (mirrors && p->npar > 0)
+        return fail(CGL_ERR_UNSUPPORTED, "particles with mirror faces are not implemented on the GPU");
     if (!(p->rho0 > 0) || !(p->nu_r > 0) || !(p->nu_b > 0)) return fail(CGL_ERR_ARG, "rho0, nu_r, nu_b must be > 0");
     if (p->layout != CGL_LAYOUT_RB && p->layout != CGL_LAYOUT_FS) return fail(CGL_ERR_ARG, "unknown layout %d", p->layout);
     if (p->reserved != 0) return fail(CGL_ERR_ARG, "reserved field must be 0");
-    return alloc_handle(out, p, (size_t)p->nx * p->ny * p->nz, walls);
+    return alloc_handle(out, p, (size_t)p->nx * p->ny * p->nz, walls | mirrors);
 }
 
 // Allocation of a handle's device state for N nodes (one box, or a packed
@@ -205,7 +210,10 @@ static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls)
     h->npar = p->npar;
     h->ext = walls || p->npar > 0;
     for (int a = 0; a < 3; ++a)
-        for (int s = 0; s < 2; ++s) h->E.wall[a][s] = p->bc[2 * a + s] == CGL_BC_WALL;
+        for (int s = 0; s < 2; ++s) {
+            h->E.wall[a][s] = p->bc[2 * a + s] == CGL_BC_WALL;
+            h->E.mirror[a][s] = p->bc[2 * a + s] == CGL_BC_MIRROR;
+        }
     h->E.npar = p->npar;
     h->P.npar = p->npar;
     int rc = set_device(h);

@@ -61,6 +61,7 @@ struct Lat {
 enum : unsigned { NT_SOLID = 1u, NT_POUT = 2u, NT_NEAR = 4u, NT_REFILL = 8u, NT_PIN = 16u };
 struct Ext {
     int wall[3][2];          // 1: half-way bounce-back wall face with a ghost layer (S:1557-1590)
+    int mirror[3][2];        // 1: specular mirror face with a ghost layer (S:1443-1542)
     int npar;
     int pad;
     const unsigned *ntype;   // node types of the masks in force
@@ -236,10 +237,48 @@ __device__ __forceinline__ void gather_fs(const S *__restrict__ fs, size_t N, co
 struct WallBB {
     unsigned mask = 0;
 };
-__device__ __forceinline__ bool is_ghost(const Ext &E, const Lat &L, int x, int y, int z)
+__device__ __forceinline__ bool is_wall_ghost(const Ext &E, const Lat &L, int x, int y, int z)
 {
     return (E.wall[0][0] && x == 0) || (E.wall[0][1] && x == L.nx - 1) || (E.wall[1][0] && y == 0) ||
            (E.wall[1][1] && y == L.ny - 1) || (E.wall[2][0] && z == 0) || (E.wall[2][1] && z == L.nz - 1);
+}
+// Mirror ghost planes crossed by (x, y, z): bit a set for axis a.
+__device__ __forceinline__ unsigned mirror_axes(const Ext &E, const Lat &L, int x, int y, int z)
+{
+    const int c[3] = {x, y, z}, n[3] = {L.nx, L.ny, L.nz};
+    unsigned A = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if ((E.mirror[a][0] && c[a] == 0) || (E.mirror[a][1] && c[a] == n[a] - 1)) A |= 1u << a;
+    return A;
+}
+// Ghost layer of a wall or a mirror face: not a lattice node of the fluid.
+__device__ __forceinline__ bool is_ghost(const Ext &E, const Lat &L, int x, int y, int z)
+{
+    return is_wall_ghost(E, L, x, y, z) || mirror_axes(E, L, x, y, z) != 0;
+}
+// Mirror ghost layers next to a live node: bit 2a (low side) and 2a+1 (high
+// side) of axis a.
+__device__ __forceinline__ unsigned mirror_adjacent(const Ext &E, const Lat &L, int x, int y, int z)
+{
+    const int c[3] = {x, y, z}, n[3] = {L.nx, L.ny, L.nz};
+    unsigned m = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (E.mirror[a][0] && c[a] == 1) m |= 1u << (2 * a);
+        if (E.mirror[a][1] && c[a] == n[a] - 2) m |= 1u << (2 * a + 1);
+    }
+    return m;
+}
+// Axes along which the upstream node x - c_i of direction i lies in a mirror
+// ghost layer, from the adjacency bits of the node.
+template <int I> __device__ __forceinline__ unsigned mirrored_axes(unsigned madj)
+{
+    unsigned A = 0;
+    if ((CX[I] == 1 && (madj & 1u)) || (CX[I] == -1 && (madj & 2u))) A |= 1u;
+    if ((CY[I] == 1 && (madj & 4u)) || (CY[I] == -1 && (madj & 8u))) A |= 2u;
+    if ((CZ[I] == 1 && (madj & 16u)) || (CZ[I] == -1 && (madj & 32u))) A |= 4u;
+    return A;
 }
 __device__ __forceinline__ unsigned wall_bb_mask(const Ext &E, const Lat &L, int x, int y, int z)
 {
@@ -256,40 +295,60 @@ __device__ __forceinline__ unsigned wall_bb_mask(const Ext &E, const Lat &L, int
     return m;
 }
 
+// Source slot of direction i at a live node in the pull layout: the upstream
+// node x - c_i, or with a wall ghost there the node's own opposite slot
+// (bounce-back, bb), or with mirror ghosts there (madj) the slot of the
+// mirrored direction at the upstream node's mirror image, which is what the
+// reference's ghost fill followed by streaming delivers (S:1443-1542,
+// S:735-768).  Walls take precedence: their bounce-back is written last
+// (S:1557-1590).
+template <int I>
+__device__ __forceinline__ void pull_slot(const unsigned xo[3], const unsigned yo[3], const unsigned zo[3],
+                                          unsigned bb, unsigned madj, unsigned &m, int &d)
+{
+    if ((bb >> I) & 1u) {
+        m = xo[1] + yo[1] + zo[1];
+        d = OPC[I];
+        return;
+    }
+    const unsigned A = mirrored_axes<I>(madj);
+    m = xo[(A & 1u) ? 1 : 1 - CX[I]] + yo[(A & 2u) ? 1 : 1 - CY[I]] + zo[(A & 4u) ? 1 : 1 - CZ[I]];
+    d = A ? REFL.d[A][I] : I;
+}
+
 template <int LIN, class S, class R, bool SHIFT>
 __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void *p1, const unsigned xo[3],
                                        const unsigned yo[3], const unsigned zo[3], R ax, Node<R, SHIFT> &nd,
-                                       unsigned bb = 0)
+                                       unsigned bb = 0, unsigned madj = 0)
 {
     const size_t N = (size_t)L.N;
     const R rho0 = R(L.rho0);
-    const unsigned self = xo[1] + yo[1] + zo[1];
     if constexpr (LIN == RB) {
         const S *rs = (const S *)p0, *bs = (const S *)p1;
         if constexpr (!SHIFT) {
-#pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                const bool w = (bb >> i) & 1u;
-                const unsigned m = w ? self : xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-                const int d = w ? OPC[i] : i;
+            static_for<Q>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                unsigned m;
+                int d;
+                pull_slot<i>(xo, yo, zo, bb, madj, m, d);
                 const S *ri = rs + (size_t)d * N, *bi = bs + (size_t)d * N;
                 nd.r[i] = StoreOf<S>::ld(ri, m);
                 nd.b[i] = StoreOf<S>::ld(bi, m);
-            }
+            });
         } else {
             // deviations from w_i rho0 ax and w_i rho0 (1 - ax): small, summed plainly
             R sr = 0;
-#pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                const bool w = (bb >> i) & 1u;
-                const unsigned m = w ? self : xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-                const int d = w ? OPC[i] : i;
+            static_for<Q>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                unsigned m;
+                int d;
+                pull_slot<i>(xo, yo, zo, bb, madj, m, d);
                 const S *ri = rs + (size_t)d * N, *bi = bs + (size_t)d * N;
                 R rv = StoreOf<S>::ld(ri, m);
                 R bv = StoreOf<S>::ld(bi, m);
                 sr += rv;
                 nd.ft[i] = rv + bv;
-            }
+            });
             nd.sr = sr + rho0 * ax;
         }
     } else {
@@ -418,7 +477,8 @@ __device__ __forceinline__ void moments_node(const Lat &L, const void *__restric
     Node<R, SHIFT> nd;
     unsigned nt = 0;
     if constexpr (EXT) {
-        gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd, wall_bb_mask(E, L, x, y, z));
+        gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd, wall_bb_mask(E, L, x, y, z),
+                                 mirror_adjacent(E, L, x, y, z));
         if (E.ntype) nt = at(E.ntype, n);
     } else {
         gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd);
@@ -953,19 +1013,32 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
     };
     // unit normal of plane p on the (KTY<R>+2) x (TZ+2) halo region (S:498-532);
     // with EXT: zero at solid and ghost nodes, wetting at live fluid nodes
-    auto normals = [&](int p) {
+    // (planes plo..phi are in the phi ring)
+    auto normals = [&](int p, int plo, int phi_) {
         const int s = slot3(p);
         for (int e = tid; e < (KTY<R> + 2) * (TZ + 2); e += TZ * KTY<R>) {
             int jy = e / (TZ + 2), jz = e - jy * (TZ + 2);
             R nx = 0, ny = 0, nz = 0;
             bool skip = false;
             int gxw = 0, gyw = 0, gzw = 0;
-            unsigned nt = 0;
+            unsigned nt = 0, A = 0;
+            int ox = 0, oy = 0, oz = 0; // a mirror ghost takes its image's normal, reflected (S:589-591)
             if constexpr (EXT) {
                 gxw = wrapm(p, L.nx);
                 gyw = wrapm(y0 - 1 + jy, L.ny);
                 gzw = wrapm(z0 - 1 + jz, L.nz);
-                skip = is_ghost(E, L, gxw, gyw, gzw);
+                skip = is_wall_ghost(E, L, gxw, gyw, gzw);
+                if (!skip) {
+                    A = mirror_axes(E, L, gxw, gyw, gzw);
+                    if (A) {
+                        ox = (A & 1u) ? (gxw == 0 ? 1 : -1) : 0;
+                        oy = (A & 2u) ? (gyw == 0 ? 1 : -1) : 0;
+                        oz = (A & 4u) ? (gzw == 0 ? 1 : -1) : 0;
+                        // the image's stencil must lie in the rings (else no live node uses it)
+                        skip = p + ox - 1 < plo || p + ox + 1 > phi_ || jy + oy < 0 || jy + oy > KTY<R> + 1 ||
+                               jz + oz < 0 || jz + oz > TZ + 1;
+                    }
+                }
                 if (!skip && E.ntype) {
                     nt = at(E.ntype, (unsigned)(gxw * sx + gyw * L.nz + gzw));
                     skip = (nt & NT_SOLID) != 0;
@@ -973,8 +1046,11 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
             }
             if (!skip) {
                 R gx, gy, gz;
-                grad19<R>([&](int dx, int dy, int dz) { return sm.phi[slot5(p + dx)][jy + 1 + dy][jz + 1 + dz]; },
-                          gx, gy, gz);
+                grad19<R>(
+                    [&](int dx, int dy, int dz) {
+                        return sm.phi[slot5(p + ox + dx)][jy + 1 + oy + dy][jz + 1 + oz + dz];
+                    },
+                    gx, gy, gz);
                 const R g2 = gx * gx + gy * gy + gz * gz;
                 bool fl = false;
                 if constexpr (sizeof(R) == 8) {
@@ -1001,6 +1077,9 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                         ny = R(nv[1]);
                         nz = R(nv[2]);
                     }
+                    if (A & 1u) nx = -nx;
+                    if (A & 2u) ny = -ny;
+                    if (A & 4u) nz = -nz;
                 }
             }
             sm.n[0][s][jy][jz] = nx;
@@ -1028,25 +1107,26 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
         }
     };
 
-    for (int p = x0 - 2; p <= x0 + 1; ++p) load_phi(p);
+    for (int p = x0 - 2; p <= x0 + 2; ++p) load_phi(p);
     load_scal(x0 - 1);
     load_scal(x0);
     __syncthreads();
-    normals(x0 - 1);
-    normals(x0);
+    normals(x0 - 1, x0 - 2, x0 + 2);
+    normals(x0, x0 - 2, x0 + 2);
 
     for (int x = x0; x < x1; ++x) {
         __syncthreads(); // readers of the slots refilled below are done
-        load_phi(x + 2);
+        if (x > x0) load_phi(x + 2);
         load_scal(x + 1);
         __syncthreads();
-        normals(x + 1);
+        normals(x + 1, x - 2, x + 2);
         __syncthreads();
         if (!active) continue;
-        unsigned nt = 0, bb = 0;
+        unsigned nt = 0, bb = 0, madj = 0;
         if constexpr (EXT) {
             if (is_ghost(E, L, x, y, z)) continue; // ghost layers are not collided
             bb = wall_bb_mask(E, L, x, y, z);
+            madj = mirror_adjacent(E, L, x, y, z);
         }
 
         // gradient, normal and flag at the node (S:498-532)
@@ -1187,7 +1267,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                     [&](int i, unsigned) { return ring.s[(x - CX[i] + 9) % 3][ty + 1 - CY[i]][tz + 1 - CZ[i]]; },
                     nd, R(L.rho0));
             else
-                gather<LIN, S, R, SHIFT>(L, i0, i1, xo, yo, zo, ax, nd, bb);
+                gather<LIN, S, R, SHIFT>(L, i0, i1, xo, yo, zo, ax, nd, bb, madj);
             mo = moments<R, SHIFT>(L, nd);
         }
         const R phin = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
@@ -1380,9 +1460,16 @@ __global__ __launch_bounds__(256) void k_pack(Lat L, const double *__restrict__ 
     for (int i = 0; i < Q; ++i) {
         int src = xo[1 + CX[i]] + yo[1 + CY[i]] + zo[1 + CZ[i]];
         int qi = i;
-        if (live && is_ghost(E, L, x + CX[i], y + CY[i], z + CZ[i])) {
-            src = n;
-            qi = OPC[i];
+        if (live) {
+            const int rx = x + CX[i], ry = y + CY[i], rz = z + CZ[i];
+            if (is_wall_ghost(E, L, rx, ry, rz)) {
+                src = n;
+                qi = OPC[i];
+            } else if (const unsigned A = mirror_axes(E, L, rx, ry, rz)) {
+                // read by the reader's mirror image in the reflected direction
+                src = xo[(A & 1u) ? 1 : 1 + CX[i]] + yo[(A & 2u) ? 1 : 1 + CY[i]] + zo[(A & 4u) ? 1 : 1 + CZ[i]];
+                qi = REFL.d[A][i];
+            }
         }
         double v = aos[(size_t)src * Q + qi];
         if constexpr (SHIFT) v = v - W[i] * (L.rho0 * (double)shift_a(phio[src], red));
@@ -1406,17 +1493,17 @@ __global__ __launch_bounds__(256) void k_unpack(Lat L, const S *__restrict__ soa
     const int n = xo[1] + yo[1] + zo[1];
     const size_t N = (size_t)L.N;
     const bool live = !is_ghost(E, L, x, y, z);
-    for (int i = 0; i < Q; ++i) {
-        int src = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-        int qi = i;
-        if (live && is_ghost(E, L, x - CX[i], y - CY[i], z - CZ[i])) {
-            src = n;
-            qi = OPC[i];
-        }
+    const unsigned bb = live ? wall_bb_mask(E, L, x, y, z) : 0u;
+    const unsigned madj = live ? mirror_adjacent(E, L, x, y, z) : 0u;
+    static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        unsigned src;
+        int qi;
+        pull_slot<i>(xo, yo, zo, bb, madj, src, qi);
         double v = double(StoreOf<S>::ld(soa + (size_t)qi * N + src));
         if constexpr (SHIFT) v = v + W[i] * (L.rho0 * (double)shift_a(phio[n], red));
         aos[(size_t)n * Q + i] = v;
-    }
+    });
 }
 
 // Unpack the factorised state: P_i(x) = r_i or b_i rebuilt at x - c_i, as

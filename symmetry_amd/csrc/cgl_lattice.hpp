@@ -7,6 +7,8 @@
 // that the C ABI accepts, so they have to be the same tables.
 #pragma once
 
+#include <type_traits>
+
 namespace cgl {
 
 constexpr int Q = 19;
@@ -81,6 +83,36 @@ constexpr MTab MT = make_mtab();
 // parities MT.even they halve the work of the MRT transforms.
 constexpr int NPAIR = 9;
 constexpr int PAIR[NPAIR] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
+
+// REFL[A][i]: direction i with the velocity components of the axes in mask A
+// (bit 0 x, bit 1 y, bit 2 z) reversed -- specular reflection in mirror faces.
+struct ReflTab {
+    int d[8][Q];
+};
+constexpr int find_dir(int cx, int cy, int cz)
+{
+    for (int i = 0; i < Q; ++i)
+        if (CX[i] == cx && CY[i] == cy && CZ[i] == cz) return i;
+    return -1;
+}
+constexpr ReflTab make_refl()
+{
+    ReflTab t{};
+    for (int A = 0; A < 8; ++A)
+        for (int i = 0; i < Q; ++i)
+            t.d[A][i] = find_dir((A & 1) ? -CX[i] : CX[i], (A & 2) ? -CY[i] : CY[i], (A & 4) ? -CZ[i] : CZ[i]);
+    return t;
+}
+constexpr ReflTab REFL = make_refl();
+
+// Compile-time loop: f(std::integral_constant<int, 0>) ... f(<N-1>).
+template <int N, class F, int K = 0> __host__ __device__ constexpr void static_for(F &&f)
+{
+    if constexpr (K < N) {
+        f(std::integral_constant<int, K>{});
+        static_for<N, F, K + 1>(static_cast<F &&>(f));
+    }
+}
 
 // Moments of the rest-state weights: M w = (1, -11, 3, 0, ..., 0).
 constexpr double MW0 = 1.0, MW1 = -11.0, MW2 = 3.0;

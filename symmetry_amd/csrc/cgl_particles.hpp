@@ -555,17 +555,19 @@ static __global__ __launch_bounds__(PB) void k_move(Lat L, Ext E, Part P, int wi
     }
 }
 
-// --------------------------------------------- wall-layer phi and derivatives
-// phi of the wall ghost layers = phi of the adjacent layer (S:806-807).  The
-// reference copies whole planes face by face (z, then x, then y), so a ghost
-// node on an edge or corner ends up with phi of the nearest live node; the
-// source index clamps every wall coordinate into the live range, which gives
-// that result without ordering the six faces.
+// --------------------------------------------- ghost-layer phi and derivatives
+// phi of the wall and mirror ghost layers = phi of the adjacent layer
+// (S:806-807, S:1443-1465).  The reference copies whole planes face by face
+// (walls z then x, then mirrors x, y, z), each copy reading planes the
+// earlier ones already refreshed, so a ghost node on an edge or corner ends
+// up with phi of the nearest live node; the source index clamps every ghost
+// coordinate into the live range, which gives that result without ordering
+// the six faces.
 template <class P> __global__ __launch_bounds__(256) void k_wall_phi(Lat L, Ext E, P *phi)
 {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     const int axis = blockIdx.y / 2, side = blockIdx.y % 2;
-    if (!E.wall[axis][side]) return;
+    if (!E.wall[axis][side] && !E.mirror[axis][side]) return;
     const int n3[3] = {L.nx, L.ny, L.nz};
     const int u = (axis + 1) % 3, v = (axis + 2) % 3;
     if (t >= (unsigned)(n3[u] * n3[v])) return;
@@ -576,8 +578,8 @@ template <class P> __global__ __launch_bounds__(256) void k_wall_phi(Lat L, Ext 
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         cs[a] = cd[a];
-        if (E.wall[a][0] && cs[a] < 1) cs[a] = 1;
-        if (E.wall[a][1] && cs[a] > n3[a] - 2) cs[a] = n3[a] - 2;
+        if ((E.wall[a][0] || E.mirror[a][0]) && cs[a] < 1) cs[a] = 1;
+        if ((E.wall[a][1] || E.mirror[a][1]) && cs[a] > n3[a] - 2) cs[a] = n3[a] - 2;
     }
     phi[((size_t)cd[0] * L.ny + cd[1]) * L.nz + cd[2]] = phi[((size_t)cs[0] * L.ny + cs[1]) * L.nz + cs[2]];
 }
@@ -601,7 +603,25 @@ __global__ __launch_bounds__(256) void k_derivs_ext(Lat L, Ext E, const P *__res
     double g[3] = {0, 0, 0}, nv[3] = {0, 0, 0};
     bool fl = false;
     const unsigned nt = E.ntype ? E.ntype[n] : 0u;
-    if (!is_ghost(E, L, x, y, z) && !(nt & NT_SOLID)) {
+    // a mirror ghost holds its image's gradient and normal, reflected
+    // (S:589-594); wall ghosts hold zeros
+    const unsigned A = is_wall_ghost(E, L, x, y, z) ? 0u : mirror_axes(E, L, x, y, z);
+    if (A) {
+        const int xi = (A & 1u) ? (x == 0 ? 1 : x - 1) : x, yi = (A & 2u) ? (y == 0 ? 1 : y - 1) : y,
+                  zi = (A & 4u) ? (z == 0 ? 1 : z - 1) : z;
+        unsigned ax[3], ay[3], az[3];
+        node_offsets(L, xi, yi, zi, ax, ay, az);
+        grad19<double>([&](int dx, int dy, int dz) { return double(phi[ax[1 + dx] + ay[1 + dy] + az[1 + dz]]); },
+                       g[0], g[1], g[2]);
+        const double fnorm = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        if (fnorm > EVAP_LIM)
+            for (int k = 0; k < 3; ++k) nv[k] = g[k] / fnorm;
+        for (int k = 0; k < 3; ++k)
+            if (A & (1u << k)) {
+                g[k] = -g[k];
+                nv[k] = -nv[k];
+            }
+    } else if (!is_ghost(E, L, x, y, z) && !(nt & NT_SOLID)) {
         grad19<double>([&](int dx, int dy, int dz) { return double(phi[xo[1 + dx] + yo[1 + dy] + zo[1 + dz]]); },
                        g[0], g[1], g[2]);
         const double fnorm = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);

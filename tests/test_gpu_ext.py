@@ -29,11 +29,11 @@ def _print(*a):
 
 
 def _live(shape, bc):
-    """Slices of the nodes that are not wall ghost layers."""
+    """Slices of the nodes that are not wall or mirror ghost layers."""
     sl = []
     for a, n in enumerate(shape):
-        lo = 1 if bc[2 * a] == symmetry_amd.BC_WALL else 0
-        hi = n - 1 if bc[2 * a + 1] == symmetry_amd.BC_WALL else n
+        lo = 1 if bc[2 * a] != symmetry_amd.BC_PERIODIC else 0
+        hi = n - 1 if bc[2 * a + 1] != symmetry_amd.BC_PERIODIC else n
         sl.append(slice(lo, hi))
     return tuple(sl)
 
@@ -63,6 +63,13 @@ WALL_CASES = {
     "xyz": (_wall_cfg(20, 24, 28, (1, 1, 1, 1, 1, 1)), (6.0, 7.0, 8.0, 6.5)),
     # walls in x only
     "x": (_wall_cfg(24, 16, 32, (1, 1, 0, 0, 0, 0)), (4.0, 8.5, 16.0, 6.0)),
+    # the reference's own faces: walls at x=1 and z=1, mirrors at x=xl, y=1,
+    # y=yl and z=zl (S:806-807, S:1443-1592); the drop sits on the mirrors
+    "ref": (_wall_cfg(20, 18, 24, (1, 2, 2, 2, 1, 2)), (18.0, 2.0, 12.0, 6.0)),
+    # mirrors all round, drop in a corner
+    "mirror": (_wall_cfg(16, 20, 22, (2, 2, 2, 2, 2, 2)), (3.0, 17.0, 4.0, 5.5)),
+    # mirrors in x, walls in z, periodic y
+    "mixed": (_wall_cfg(18, 16, 26, (2, 2, 0, 0, 1, 1)), (16.5, 8.0, 3.0, 6.0)),
 }
 
 
@@ -104,6 +111,28 @@ def test_walls_f64_bit_exact(case, nsteps):
         assert np.array_equal(mac[k][live], st.arrays[k][live]), k
     # ghost layers of phi hold the adjacent layer (S:806-807)
     assert np.array_equal(mac["phi"], st.phi)
+
+
+@pytest.mark.parametrize("case", ["ref", "mirror", "z"])
+def test_wall_derivatives_match_oracle(case):
+    """Gradient, normal and flag of the derivatives stage (S:452-597) at every
+    node, ghost layers included: zero at wall ghosts, the reflected copy of
+    the image at mirror ghosts (S:589-594)."""
+    cfg, drop = WALL_CASES[case]
+    _print("wall derivatives", case)
+    ri, bi, _ = ic.flat_interface_state(cfg["nx"], cfg["ny"], cfg["nz"], z0=-1e3, rho0=1.0, drop=drop,
+                                        sigma=cfg["sigma"])
+    lat = _lattice(cfg)
+    lat.upload(ri, bi)
+    lat.step(5)
+    st = _oracle(cfg, ri, bi)
+    st.step(5)
+    st.call("calmacros")
+    st.call("derivatives")
+    der = lat.derivatives()
+    for k in ("gx", "gy", "gz", "nxv", "nyv", "nzv"):
+        assert np.array_equal(der[k], st.arrays[k]), k
+    assert np.array_equal(der["flag"], st.flag)
 
 
 def test_walls_f32_close_to_f64():

@@ -78,8 +78,8 @@ def test_create_validates_before_touching_the_device(lib, kw, code):
 
 def test_face_validation(lib):
     h = ctypes.c_void_p()
-    p = _params()
-    p.bc[4] = p.bc[5] = 2  # mirror faces are not implemented on the GPU
+    p = _params(npar=2)
+    p.bc[4] = p.bc[5] = 2  # particles with mirror faces are not implemented on the GPU
     assert lib.cgl_create(ctypes.byref(h), ctypes.byref(p)) == -3
     p = _params()
     p.bc[4] = symmetry_amd.BC_WALL  # a periodic face needs a periodic partner
@@ -87,6 +87,15 @@ def test_face_validation(lib):
     p = _params(layout=1)
     p.bc[4] = p.bc[5] = symmetry_amd.BC_WALL  # walls run on the red/blue layout only
     assert lib.cgl_create(ctypes.byref(h), ctypes.byref(p)) == -3
+    p = _params(layout=1)
+    p.bc[0] = p.bc[1] = 2  # so do mirrors
+    assert lib.cgl_create(ctypes.byref(h), ctypes.byref(p)) == -3
+    p = _params(nz=3)
+    p.bc[4] = p.bc[5] = 2  # a bounded axis needs ghost layers and live nodes
+    assert lib.cgl_create(ctypes.byref(h), ctypes.byref(p)) == -1
+    p = _params()
+    p.bc[4] = 7
+    assert lib.cgl_create(ctypes.byref(h), ctypes.byref(p)) == -1
 
 
 @pytest.mark.skipif(not oracle.ref_available(), reason="reference build absent")
