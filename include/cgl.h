@@ -132,6 +132,15 @@ int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *
  * (zl, yl, xl) layout.  Any pointer may be NULL. */
 int cgl_download_masks(cgl_t *h, int32_t *walls, int32_t *walls_all, int32_t *pin, int32_t *pout);
 
+/* Global FP64 sums on the device over the live fluid nodes (ghost layers and
+ * particle interiors excluded) of the current state's macroscopic fields:
+ * sums = {mass, red mass, blue mass, momentum x, y, z, kinetic energy}
+ * (the mass check and Energy.dat sums of S:1673-1797).  With slice_red,
+ * also the red area sum (1 + phi)/2 of every z slice (nz values), from which
+ * the neck radius of a jet along z is sqrt(area / pi).  Sums run in a fixed
+ * order, so they are reproducible. */
+int cgl_diagnostics(cgl_t *h, double *sums, double *slice_red);
+
 /* ---------------------------------------------------------------- batches
  * Many independent periodic boxes of uneven sizes advanced together: one
  * launch per kernel covers the tiles of every box (nodes packed box after

@@ -56,7 +56,7 @@ EXPORTS = (
     "cgl_device_name", "cgl_last_error", "cgl_probe_copy_bandwidth", "cgl_tuned_xchunk",
     "cgl_set_particles", "cgl_get_particles", "cgl_download_masks",
     "cgl_batch_create", "cgl_box_upload_state", "cgl_box_init_cylinder", "cgl_box_download_state",
-    "cgl_box_calmacros",
+    "cgl_box_calmacros", "cgl_diagnostics",
 )
 
 
@@ -89,6 +89,7 @@ def lib():
         L.cgl_box_download_state.argtypes = [vp, ctypes.c_int32, vp, vp]
         L.cgl_box_init_cylinder.argtypes = [vp, ctypes.c_int32] + [ctypes.c_double] * 4 + [ctypes.c_int32, vp]
         L.cgl_box_calmacros.argtypes = [vp, ctypes.c_int32] + [vp] * 7
+        L.cgl_diagnostics.argtypes = [vp, vp, vp]
         L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.cgl_probe_copy_bandwidth.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                                ctypes.POINTER(ctypes.c_double)]
@@ -183,6 +184,17 @@ class Lattice:
         t = [ctypes.c_double() for _ in range(3)]
         _check(lib().cgl_time_steps(self.h, int(n), *(ctypes.byref(v) for v in t)))
         return tuple(v.value for v in t)
+
+    def diagnostics(self, slices=False):
+        """FP64 device sums {mass, mass_r, mass_b, jx, jy, jz, ekin} over the
+        live fluid nodes, and with slices=True the red area of every z slice."""
+        sums = np.empty(7)
+        area = np.empty(self.shape[2]) if slices else None
+        _check(lib().cgl_diagnostics(self.h, _ptr(sums), _ptr(area)))
+        out = dict(zip(("mass", "mass_r", "mass_b", "jx", "jy", "jz", "ekin"), sums))
+        if slices:
+            out["slice_red"] = area
+        return out
 
     def set_particles(self, center, rad, mass, inertia, cita, fg, up=None, rp=None):
         """center, up, rp: [npar, 3] arrays (passed to the library in the

@@ -685,6 +685,39 @@ int cgl_download_masks(cgl_t *h, int32_t *walls, int32_t *walls_all, int32_t *pi
     return CGL_OK;
 }
 
+int cgl_diagnostics(cgl_t *h, double *sums, double *slice_red)
+{
+    if (!h || !sums) return fail(CGL_ERR_ARG, "null argument");
+    if (h->nbox) return fail(CGL_ERR_ARG, "a batch handle takes the cgl_box_* entry points");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if ((rc = ensure_scratch(h, true, false, false))) return rc;
+    dispatch(h, [&](auto impl) {
+        if (h->ext)
+            decltype(impl)::moments_ext(h, true);
+        else
+            decltype(impl)::moments(h, true);
+    });
+    if ((rc = check_launch("moments"))) return rc;
+    const unsigned per_block = 16384;
+    const unsigned nblocks = (unsigned)((h->N + per_block - 1) / per_block);
+    double *part = nullptr;
+    HIPCHK(hipMalloc((void **)&part, ((size_t)nblocks + 1) * cgl::NDIAG * sizeof(double) + h->p.nz * sizeof(double)));
+    double *out = part + (size_t)nblocks * cgl::NDIAG, *area = out + cgl::NDIAG;
+    const cgl::Ext E = h->npar ? [&] { cgl::Ext e = h->E; e.ntype = h->ntype[h->ntcur]; return e; }() : h->E;
+    cgl::k_reduce<<<nblocks, cgl::RED_B, 0, h->stream>>>(h->L, E, h->mac, part, per_block);
+    cgl::k_reduce_final<<<1, 64, 0, h->stream>>>(part, nblocks, out);
+    if (slice_red) cgl::k_slice_red<<<h->p.nz, cgl::RED_B, 0, h->stream>>>(h->L, E, h->mac, area);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(sums, out, cgl::NDIAG * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && slice_red)
+        e = hipMemcpyAsync(slice_red, area, h->p.nz * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(part);
+    if (e != hipSuccess) return fail(CGL_ERR_HIP, "diagnostics failed: %s", hipGetErrorString(e));
+    return CGL_OK;
+}
+
 // ------------------------------------------------------------------ batches
 
 int cgl_batch_create(cgl_t **out, const cgl_params *p, int32_t nbox, const cgl_box *boxes)

@@ -1296,6 +1296,83 @@ __global__ __launch_bounds__(TZ * KTY<R>) void k_collide(Lat L, const void *__re
     collide_tile<LIN, LOUT, S, R, P, SHIFT, EXT>(L, i0, i1, o0, o1, phi, phio, xc, E, bx, by, bz);
 }
 
+// ------------------------------------------------------------- diagnostics
+// FP64 sums over the live fluid nodes of the macroscopic fields that the
+// moments kernel wrote (rho, rho_R, rho_B, ux, uy, uz, phi; 7 x N): mass,
+// red and blue mass, momentum and kinetic energy.  Each workgroup reduces a
+// fixed range of nodes in a fixed tree order; k_reduce_final adds the
+// partials in workgroup order, so the sums are reproducible.
+constexpr int RED_B = 256;
+constexpr int NDIAG = 7;
+
+__device__ __forceinline__ bool live_fluid(const Ext &E, const Lat &L, unsigned n)
+{
+    const int z = (int)(n % (unsigned)L.nz), y = (int)((n / (unsigned)L.nz) % (unsigned)L.ny),
+              x = (int)(n / ((unsigned)L.nz * (unsigned)L.ny));
+    if (is_ghost(E, L, x, y, z)) return false;
+    return !(E.ntype && (E.ntype[n] & NT_SOLID));
+}
+
+static __global__ __launch_bounds__(RED_B) void k_reduce(Lat L, Ext E, const double *__restrict__ mac,
+                                                  double *__restrict__ part, unsigned per_block)
+{
+    __shared__ double sh[NDIAG][RED_B];
+    const size_t N = (size_t)L.N;
+    double acc[NDIAG] = {0, 0, 0, 0, 0, 0, 0};
+    const unsigned n0 = blockIdx.x * per_block, n1 = min((unsigned)N, n0 + per_block);
+    for (unsigned n = n0 + threadIdx.x; n < n1; n += RED_B) {
+        if (!live_fluid(E, L, n)) continue;
+        const double rho = mac[n], ux = mac[3 * N + n], uy = mac[4 * N + n], uz = mac[5 * N + n];
+        acc[0] += rho;
+        acc[1] += mac[N + n];
+        acc[2] += mac[2 * N + n];
+        acc[3] += rho * ux;
+        acc[4] += rho * uy;
+        acc[5] += rho * uz;
+        acc[6] += 0.5 * rho * (ux * ux + uy * uy + uz * uz);
+    }
+    for (int k = 0; k < NDIAG; ++k) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int st = RED_B / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int k = 0; k < NDIAG; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < NDIAG) part[(size_t)blockIdx.x * NDIAG + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+static __global__ __launch_bounds__(64) void k_reduce_final(const double *__restrict__ part, unsigned nblocks,
+                                                     double *__restrict__ out)
+{
+    if (threadIdx.x >= NDIAG) return;
+    double s = 0.0;
+    for (unsigned b = 0; b < nblocks; ++b) s += part[(size_t)b * NDIAG + threadIdx.x];
+    out[threadIdx.x] = s;
+}
+
+// Red area of each z slice: sum over the slice's live fluid nodes of
+// (1 + phi) / 2, one workgroup per slice (the neck radius of a jet along z is
+// sqrt(area / pi)).
+static __global__ __launch_bounds__(RED_B) void k_slice_red(Lat L, Ext E, const double *__restrict__ mac,
+                                                     double *__restrict__ area)
+{
+    __shared__ double sh[RED_B];
+    const size_t N = (size_t)L.N;
+    const unsigned z = blockIdx.x, nxy = (unsigned)L.nx * (unsigned)L.ny;
+    double acc = 0.0;
+    for (unsigned k = threadIdx.x; k < nxy; k += RED_B) {
+        const unsigned n = k * (unsigned)L.nz + z; // (x, y) = (k / ny, k % ny)
+        if (live_fluid(E, L, n)) acc += 0.5 * (1.0 + mac[6 * N + n]);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = RED_B / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) area[z] = sh[0];
+}
+
 // ------------------------------------------------- batches of independent boxes
 // Boxes are packed one after another in the node index: box b owns nodes
 // [base, base + N_b) of every per-node array and its own SoA block of each
