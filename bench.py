@@ -110,6 +110,43 @@ def run_ensemble(steps, seq_steps):
             "sequential_f64_mlups": round(seq_mlups, 1), "ratio": round(batch_mlups / seq_mlups, 2)}
 
 
+def run_particles(steps):
+    """The particle workloads, red/blue layout (walls and particles run on it):
+    one sphere (R=10, density 1.5, 60 degrees) at a flat interface in a 96^3
+    box with z walls, FP64 and FP32; and 200 spheres (R=8, seed 99) within
+    +-24 of the interface in a 512^3 box with z walls, FP32 fields with FP64
+    particle dynamics.  MLUPS over all nodes, particle kernels included."""
+    import symmetry_amd
+    from symmetry_amd import ic
+
+    out = {}
+    c2 = ic.particle_config2()
+    p2 = ic.sphere_particles([c2["center"]], c2["rad"], c2["rho_p"], c2["cita_deg"], c2["sigma"])
+    c5 = ic.particle_config5()
+    p5 = ic.sphere_particles(c5["centers"], c5["rad"], c5["rho_p"], c5["cita_deg"], c5["sigma"])
+    for name, cfg, parts, precs in (("sphere_96", c2, p2, ("f64", "f32")), ("dense_512", c5, p5, ("f32",))):
+        n = cfg["nx"]
+        for prec in precs:
+            lat = symmetry_amd.Lattice(n, n, n, precision=prec, sigma=cfg["sigma"], rho0=cfg["rho0"],
+                                       nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], bc=cfg["bc"],
+                                       npar=len(parts["rad"]))
+            lat.init_interface(cfg["z0"], 6.0 * cfg["beta"] * 0.134)
+            lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"],
+                              parts["fg"])
+            lat.step(20)
+            lat.sync()
+            t0 = time.perf_counter()
+            lat.step(steps)
+            lat.sync()
+            dt = time.perf_counter() - t0
+            out[f"{name}_{prec}_mlups"] = round(n ** 3 * steps / dt / 1e6, 1)
+            lat.close()
+    out["steps"] = steps
+    out["workloads"] = {"sphere_96": "96^3, z walls, one sphere R=10 at the interface",
+                        "dense_512": "512^3, z walls, 200 spheres R=8 within +-24 of the interface, seed 99"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +164,8 @@ def main():
                     help="timed steps of the 64-box batched ensemble (0 = skip)")
     ap.add_argument("--ensemble-seq-steps", type=int, default=100,
                     help="timed steps per box of the sequential FP64 baseline")
+    ap.add_argument("--particle-steps", type=int, default=200,
+                    help="timed steps of the particle workloads (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design (DESIGN.md), "
@@ -186,6 +225,7 @@ def main():
                 v.close()
 
     ensemble = run_ensemble(args.ensemble_steps, args.ensemble_seq_steps) if args.ensemble_steps > 0 else None
+    particles = run_particles(args.particle_steps) if args.particle_steps > 0 else None
 
     out = {
         "metric": "MLUPS at 256x256x512 D3Q19 CG-MRT, FP32 vs FP64, % of HBM roofline",
@@ -217,6 +257,8 @@ def main():
     }
     if ensemble:
         out["ensemble"] = ensemble
+    if particles:
+        out["particles"] = particles
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -86,6 +86,11 @@ int cgl_download_state(cgl_t *h, double *ri, double *bi);
  * inside, SRT equilibria at rest (the form of initialization, S:850-896). */
 int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, int32_t nmodes, const double *amp);
 
+/* Device initial state of a flat interface: red below z = z0 (1-based
+ * lattice units), phi = tanh(kprof (z0 - z)), density rho0, at rest, with the
+ * SRT equilibria of S:883-889 (the particle-at-interface workloads). */
+int cgl_init_interface(cgl_t *h, double z0, double kprof);
+
 /* Advance nsteps time steps: the loop body S:216-251 (calmacros,
  * derivatives, collision, Recolor, boundary, streaming) without output. */
 int cgl_step(cgl_t *h, int32_t nsteps);

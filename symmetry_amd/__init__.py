@@ -56,7 +56,7 @@ EXPORTS = (
     "cgl_device_name", "cgl_last_error", "cgl_probe_copy_bandwidth", "cgl_tuned_xchunk",
     "cgl_set_particles", "cgl_get_particles", "cgl_download_masks",
     "cgl_batch_create", "cgl_box_upload_state", "cgl_box_init_cylinder", "cgl_box_download_state",
-    "cgl_box_calmacros", "cgl_diagnostics",
+    "cgl_box_calmacros", "cgl_diagnostics", "cgl_init_interface",
 )
 
 
@@ -90,6 +90,7 @@ def lib():
         L.cgl_box_init_cylinder.argtypes = [vp, ctypes.c_int32] + [ctypes.c_double] * 4 + [ctypes.c_int32, vp]
         L.cgl_box_calmacros.argtypes = [vp, ctypes.c_int32] + [vp] * 7
         L.cgl_diagnostics.argtypes = [vp, vp, vp]
+        L.cgl_init_interface.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.cgl_probe_copy_bandwidth.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                                ctypes.POINTER(ctypes.c_double)]
@@ -184,6 +185,9 @@ class Lattice:
         t = [ctypes.c_double() for _ in range(3)]
         _check(lib().cgl_time_steps(self.h, int(n), *(ctypes.byref(v) for v in t)))
         return tuple(v.value for v in t)
+
+    def init_interface(self, z0, kprof):
+        _check(lib().cgl_init_interface(self.h, float(z0), float(kprof)))
 
     def diagnostics(self, slices=False):
         """FP64 device sums {mass, mass_r, mass_b, jx, jy, jz, ekin} over the

@@ -373,20 +373,39 @@ int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, i
     return CGL_OK;
 }
 
+int cgl_init_interface(cgl_t *h, double z0, double kprof)
+{
+    if (h && h->nbox) return fail(CGL_ERR_ARG, "a batch handle takes the cgl_box_* entry points");
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    int rc = set_device(h);
+    if (rc) return rc;
+    cgl::CylIC ic{};
+    ic.mode = 1;
+    ic.z0 = z0;
+    ic.kprof = kprof;
+    h->cur = 0; // initial states are held in RB form in buffer 0
+    h->state_layout = cgl::RB;
+    h->primed = 0;
+    dispatch(h, [&](auto impl) { decltype(impl)::init_cylinder(h, ic); });
+    if ((rc = check_launch("init_interface"))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CGL_OK;
+}
+
 // Capture two steps from the current buffer parity into a graph (kept until
-// the parity or the state layout changes).  Two steps return the buffers to
-// the same parity, so one graph serves every replay.  The steps contain
-// kernel launches only.
+// the parity or the state layout changes).  Two steps return the buffers
+// (populations, phi, particle masks) to the same parity, so one graph serves
+// every replay.  The steps contain kernel launches only.
 static int capture_two_steps(cgl_handle *h)
 {
-    const int key = h->cur | h->phicur << 1 | h->state_layout << 2;
+    const int key = h->cur | h->phicur << 1 | h->state_layout << 2 | h->ntcur << 3;
     if (h->gexec && h->gkey == key) return CGL_OK;
     if (h->gexec) {
         HIPCHK(hipGraphExecDestroy(h->gexec));
         h->gexec = nullptr;
         h->gkey = -1;
     }
-    const int cur = h->cur, phicur = h->phicur, done = h->steps_done;
+    const int cur = h->cur, phicur = h->phicur, done = h->steps_done, ntcur = h->ntcur;
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step(h);
@@ -395,6 +414,7 @@ static int capture_two_steps(cgl_handle *h)
     h->cur = cur; // the captured steps have not run yet
     h->phicur = phicur;
     h->steps_done = done;
+    h->ntcur = ntcur;
     if (e != hipSuccess) return fail(CGL_ERR_HIP, "step capture failed: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -444,14 +464,15 @@ int cgl_step(cgl_t *h, int32_t nsteps)
     int rc = set_device(h);
     if (rc) return rc;
     if ((rc = tune_some(h, &nsteps))) return rc;
-    if (h->use_graph && !h->ext && !h->tuning && nsteps >= 2) {
-        // steady state: replay two captured steps (no host work per step)
-        if (h->state_layout != h->layout) { // the layout conversion step runs eagerly
+    if (h->use_graph && !h->tuning && nsteps >= 2) {
+        // steady state: replay two captured steps (no host work per step).  The
+        // layout conversion and the particles' start-up step run eagerly.
+        while (nsteps > 0 && (h->state_layout != h->layout || (h->npar && (!h->primed || h->cold)))) {
             enqueue_step(h);
             if ((rc = check_launch("step"))) return rc;
             --nsteps;
         }
-        if ((rc = capture_two_steps(h))) return rc;
+        if (nsteps >= 2 && (rc = capture_two_steps(h))) return rc;
         for (int k = 0; k + 1 < nsteps; k += 2) {
             HIPCHK(hipGraphLaunch(h->gexec, h->stream));
             h->steps_done += 2;

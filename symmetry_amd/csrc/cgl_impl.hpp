@@ -312,7 +312,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     {
         k_init_phi<P><<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, ic, (P *)h->phi[h->phicur]);
         k_init_pops<S, P, SHIFT><<<node_grid(h->L), node_block(), 0, h->stream>>>(
-            h->L, ic, (S *)h->pops[h->cur][0], (S *)h->pops[h->cur][1], (const P *)h->phi[h->phicur]);
+            h->L, ic, (S *)h->pops[h->cur][0], (S *)h->pops[h->cur][1], (const P *)h->phi[h->phicur], h->E);
     }
 };
 

@@ -955,9 +955,18 @@ __device__ __forceinline__ int slot3(int p) { return (p + 9) % 3; }
 // Akai wetting correction of a unit normal n at a live fluid node whose
 // particle centre is c (S:541-582): the candidate closer to n of the two
 // normals at contact angle cita to the wall normal nw.  FP64.
-__device__ __forceinline__ void wetting(double xn, double yn, double zn, const double *c, double cita, double n[3])
+__device__ __forceinline__ void wetting(const Lat &L, const Ext &E, double xn, double yn, double zn, const double *c,
+                                        double cita, double n[3])
 {
     double nw[3] = {xn - c[0], yn - c[1], zn - c[2]};
+    // minimum image on periodic axes (a no-op away from the seam)
+    const int n3[3] = {L.nx, L.ny, L.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (!(E.wall[a][0] || E.wall[a][1] || E.mirror[a][0] || E.mirror[a][1])) {
+            if (nw[a] > 0.5 * n3[a]) nw[a] -= n3[a];
+            if (nw[a] < -0.5 * n3[a]) nw[a] += n3[a];
+        }
     double st = sqrt(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
     nw[0] /= st;
     nw[1] /= st;
@@ -1072,7 +1081,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                     if (fl && (nt & NT_POUT)) {
                         const int pp = at(E.pout, (unsigned)(gxw * sx + gyw * L.nz + gzw)) - 1;
                         double nv[3] = {double(nx), double(ny), double(nz)};
-                        wetting(gxw + 1.0, gyw + 1.0, gzw + 1.0, E.center + 3 * pp, E.cita[pp], nv);
+                        wetting(L, E, gxw + 1.0, gyw + 1.0, gzw + 1.0, E.center + 3 * pp, E.cita[pp], nv);
                         nx = R(nv[0]);
                         ny = R(nv[1]);
                         nz = R(nv[2]);
@@ -1621,18 +1630,25 @@ __global__ __launch_bounds__(256) void k_unpack_fs(Lat L, const S *__restrict__ 
 struct CylIC {
     double xc, yc, r0, kprof;
     int nmodes;
-    int pad;
+    int mode; // 0: liquid cylinder along z; 1: flat interface at z = z0, red below
     double amp[16];
+    double z0;
 };
 
-__device__ __forceinline__ double cyl_phi(const Lat &L, const CylIC &ic, int x, int y, int z, double *Rz_out)
+// phi and total density of the initial state at node (x, y, z).
+__device__ __forceinline__ double ic_phi(const Lat &L, const CylIC &ic, int x, int y, int z, double *rho)
 {
+    if (ic.mode == 1) {
+        *rho = L.rho0;
+        return tanh((ic.z0 - (double)(z + 1)) * ic.kprof);
+    }
     const double PI = 3.14159265358979323846;
     double Rz = ic.r0;
     for (int m = 0; m < ic.nmodes; ++m) Rz += ic.amp[m] * cos(2.0 * PI * (m + 1) * (double)z / (double)L.nz);
     double dx = (double)(x + 1) - ic.xc, dy = (double)(y + 1) - ic.yc;
-    *Rz_out = Rz;
-    return tanh((Rz - sqrt(dx * dx + dy * dy)) * ic.kprof);
+    const double ph = tanh((Rz - sqrt(dx * dx + dy * dy)) * ic.kprof);
+    *rho = ph > 0.0 ? L.rho0 + L.sigma / Rz * 3.0 : L.rho0;
+    return ph;
 }
 
 // Pass 1: phi of the initial state, the shift reference of each reader.
@@ -1643,37 +1659,49 @@ __global__ __launch_bounds__(256) void k_init_phi(Lat L, CylIC ic, P *__restrict
     const int y = blockIdx.y * TY + threadIdx.y;
     const int x = blockIdx.z;
     if (z >= L.nz || y >= L.ny) return;
-    double Rz;
-    phio[((size_t)x * L.ny + y) * L.nz + z] = P(cyl_phi(L, ic, x, y, z, &Rz));
+    double rho;
+    phio[((size_t)x * L.ny + y) * L.nz + z] = P(ic_phi(L, ic, x, y, z, &rho));
 }
 
-// Pass 2: node x's equilibria P_i(x) = w_i rho_c go to slot S_i(x - c_i),
-// shifted by node x's own fraction (x reads that slot).
+// Pass 2: slot i of node s holds the equilibrium w_i rho_c of the node that
+// reads it (the pull layout: s + c_i, the node itself through a wall
+// bounce-back, the mirror image through a mirror face), shifted by that
+// reader's fraction.
 template <class S, class P, bool SHIFT>
 __global__ __launch_bounds__(256) void k_init_pops(Lat L, CylIC ic, S *__restrict__ rs, S *__restrict__ bs,
-                                                   const P *__restrict__ phio)
+                                                   const P *__restrict__ phio, Ext E = Ext{})
 {
     const int z = blockIdx.x * TZ + threadIdx.x;
     const int y = blockIdx.y * TY + threadIdx.y;
     const int x = blockIdx.z;
     if (z >= L.nz || y >= L.ny) return;
-    unsigned xo[3], yo[3], zo[3];
-    node_offsets(L, x, y, z, xo, yo, zo);
     const size_t N = (size_t)L.N;
-    double Rz;
-    double ph = cyl_phi(L, ic, x, y, z, &Rz);
-    double rho = ph > 0.0 ? L.rho0 + L.sigma / Rz * 3.0 : L.rho0;
-    double rr = rho * (ph + 1.0) / 2.0, rb = rho - rr;
-    const P phs = P(ph); // the value pass 1 stored for this node
+    const unsigned s = ((unsigned)x * L.ny + y) * L.nz + z;
+    const bool live = !is_ghost(E, L, x, y, z);
     for (int i = 0; i < Q; ++i) {
-        int dst = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+        int r[3] = {wrap1(x + CX[i], L.nx), wrap1(y + CY[i], L.ny), wrap1(z + CZ[i], L.nz)};
+        if (live) {
+            if (is_wall_ghost(E, L, r[0], r[1], r[2])) {
+                r[0] = x;
+                r[1] = y;
+                r[2] = z;
+            } else if (const unsigned A = mirror_axes(E, L, r[0], r[1], r[2])) {
+                if (A & 1u) r[0] = x;
+                if (A & 2u) r[1] = y;
+                if (A & 4u) r[2] = z;
+            }
+        }
+        double rho;
+        const double ph = ic_phi(L, ic, r[0], r[1], r[2], &rho);
+        const double rr = rho * (ph + 1.0) / 2.0, rb = rho - rr;
         double vr = rr * W[i], vb = rb * W[i];
         if constexpr (SHIFT) {
+            const P phs = phio[((size_t)r[0] * L.ny + r[1]) * L.nz + r[2]]; // the value pass 1 stored for the reader
             vr -= W[i] * (L.rho0 * (double)shift_a(phs, 1));
             vb -= W[i] * (L.rho0 * (double)shift_a(phs, 0));
         }
-        StoreOf<S>::st(rs + (size_t)i * N + dst, (decltype(StoreOf<S>::ld(rs)))vr);
-        StoreOf<S>::st(bs + (size_t)i * N + dst, (decltype(StoreOf<S>::ld(bs)))vb);
+        StoreOf<S>::st(rs + (size_t)i * N + s, (decltype(StoreOf<S>::ld(rs)))vr);
+        StoreOf<S>::st(bs + (size_t)i * N + s, (decltype(StoreOf<S>::ld(bs)))vb);
     }
 }
 

@@ -22,8 +22,8 @@
 namespace cgl {
 
 constexpr int PB = 256;        // threads per particle-kernel workgroup
-constexpr int PSLICES = 8;     // workgroups per particle for box kernels
-constexpr int LAG_BLOCKS = 8;  // workgroups per particle for the Lagrange points
+constexpr int PSLICES = 48;    // workgroups per particle for box kernels
+constexpr int LAG_BLOCKS = 64; // workgroups per particle for the Lagrange points (one point per thread)
 constexpr int LA_M = 180, LA_N = 90, LA_NUM = LA_M * (LA_N + 1);
 
 struct Part {
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(256) void k_derivs_ext(Lat L, Ext E, const P *__res
             for (int k = 0; k < 3; ++k) nv[k] = g[k] / fnorm;
             if (nt & NT_POUT) {
                 const int pp = E.pout[n] - 1;
-                wetting(x + 1.0, y + 1.0, z + 1.0, E.center + 3 * pp, E.cita[pp], nv);
+                wetting(L, E, x + 1.0, y + 1.0, z + 1.0, E.center + 3 * pp, E.cita[pp], nv);
                 for (int k = 0; k < 3; ++k) g[k] = fnorm * nv[k];
             }
         }

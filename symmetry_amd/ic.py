@@ -166,3 +166,35 @@ def ensemble_config4(seed=7, nbox=64):
         boxes.append(dict(nx=nx, ny=nx, nz=nz, r0=r0, amps=(amp,), tau=tau, nu_r=nu, nu_b=nu, sigma=0.01,
                           rho0=1.0, beta=0.7))
     return boxes
+
+
+def particle_config5(seed=99, n=512, npar=200, rad=8.0, band=24.0):
+    """Dense particle-laden interface: n^3 box, walls at z = 1 and z = n,
+    periodic in x and y, red fluid below z = n/2.  npar spheres of radius
+    ``rad`` placed without overlap by random sequential addition from numpy's
+    PCG64 generator with the given seed: centres uniform in x, y over the box
+    and in z within +-band of the interface, a draw rejected when it comes
+    closer than 2 rad + 1 to an accepted sphere (minimum image in x and y);
+    then densities uniform in [0.8, 2.0] and contact angles uniform in
+    [30, 150] degrees, in particle order.  tau = 0.8, A = 0.005, beta = 0.7;
+    2000 steps."""
+    rng = np.random.default_rng(seed)
+    z0 = n / 2.0
+    centers = []
+    while len(centers) < npar:
+        c = np.array([rng.uniform(1.0, n + 1.0), rng.uniform(1.0, n + 1.0), rng.uniform(z0 - band, z0 + band)])
+        ok = True
+        for d in centers:
+            dx = np.abs(c - d)
+            dx[:2] = np.minimum(dx[:2], n - dx[:2])
+            if np.sqrt((dx * dx).sum()) < 2.0 * rad + 1.0:
+                ok = False
+                break
+        if ok:
+            centers.append(c)
+    rho_p = rng.uniform(0.8, 2.0, npar)
+    cita = rng.uniform(30.0, 150.0, npar)
+    tau = 0.8
+    return dict(nx=n, ny=n, nz=n, bc=(0, 0, 0, 0, 1, 1), z0=z0, centers=np.array(centers), rad=rad, rho_p=rho_p,
+                cita_deg=cita, sigma=0.005, rho0=1.0, nu_r=(tau - 0.5) / 3.0, nu_b=(tau - 0.5) / 3.0, beta=0.7,
+                steps=2000)

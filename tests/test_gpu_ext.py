@@ -292,3 +292,48 @@ def test_particle_f32_close_to_f64():
     dc = np.max(np.abs(res["f32"][1]["center"] - res["f64"][1]["center"]))
     _print("center diff", dc)
     assert dc < 1e-5
+
+
+def test_particle_trajectory_vs_oracle_and_f32():
+    """A sphere released at rest at the interface (offset from the lattice so
+    no node sits exactly on its surface): FP64 centre and velocity against the
+    oracle after 50 steps, FP32 against FP64 after 300 steps."""
+    cfg, parts, ri, bi = _particle_setup(up=(0.0, 0.0, 0.0), rp=(0.0, 0.0, 0.0))
+    _print("particle trajectory")
+    st = _oracle_particles(cfg, parts, ri, bi)
+    st.step(50, particles=True)
+    traj = {}
+    for prec in ("f64", "f32"):
+        lat = _lattice_particles(cfg, parts, ri, bi, prec)
+        lat.step(50)
+        traj[prec, 50] = lat.get_particles()
+        lat.step(250)
+        traj[prec, 300] = lat.get_particles()
+        lat.close()
+    g = traj["f64", 50]
+    dc = np.max(np.abs(g["center"][0] - st.center[0]))
+    du = np.max(np.abs(g["up"][0] - st.up[0])) / np.max(np.abs(st.up[0]))
+    _print("f64 vs oracle at 50: centre", dc, "velocity rel", du)
+    assert dc < 1e-9 and du < 1e-6
+    a, b = traj["f32", 300], traj["f64", 300]
+    dc = np.max(np.abs(a["center"][0] - b["center"][0]))
+    du = np.max(np.abs(a["up"][0] - b["up"][0])) / np.max(np.abs(b["up"][0]))
+    _print("f32 vs f64 at 300: centre", dc, "velocity rel", du)
+    assert dc < 1e-5 and du < 1e-3
+
+
+@pytest.mark.parametrize("case", ["z", "ref"])
+def test_device_interface_ic_matches_host(case):
+    """cgl_init_interface equals the host flat-interface state (uploaded) at
+    every live node to rounding of tanh, ghost layers included in the pull
+    layout's bounce-back and mirror slots."""
+    cfg, _ = WALL_CASES[case]
+    nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
+    kprof = 6.0 * cfg["beta"] * 0.134
+    ri, bi, _ = ic.flat_interface_state(nx, ny, nz, z0=nz / 2.0, rho0=cfg["rho0"], beta=cfg["beta"])
+    lat = _lattice(cfg)
+    lat.init_interface(nz / 2.0, kprof)
+    r2, b2 = lat.download()
+    live = _live(ri.shape[:3], cfg["bc"])
+    assert np.max(np.abs(r2[live] - ri[live])) < 1e-14
+    assert np.max(np.abs(b2[live] - bi[live])) < 1e-14
