@@ -4,8 +4,10 @@
 Workload: the 256x256x512 periodic Rayleigh-Plateau box (liquid cylinder of
 R=40 along z with 8 seeded cosine modes, tau_red=1.0, tau_blue=0.7,
 sigma=A=0.01, beta=0.7), synthetic initial state generated on the device.
-A step is one full lattice update (moments kernel + fused collide/stream
-kernel) of all 33,554,432 nodes.  The headline precision is FP32 shifted
+A step is one full lattice update of all 33,554,432 nodes: for the periodic
+FP32/FP16 factorised box the fused step kernel (which also sums the next
+phase field in scatter form) and a small phase-field pass; otherwise the
+moments kernel and the fused collide/stream kernel.  The headline precision is FP32 shifted
 storage; --precision selects f64 / f16.
 
 Prints ONE JSON line (rank 0).  Fields beyond the driver contract:
@@ -37,6 +39,10 @@ B_ALG_COLLIDE = {
     ("f64", "rb"): 2 * 19 * 8 * 2 + 8, ("f32", "rb"): 2 * 19 * 4 * 2 + 4, ("f16", "rb"): 2 * 19 * 2 * 2 + 4,
     ("f64", "fs"): 2 * (19 * 8 + 48) + 8, ("f32", "fs"): 2 * (19 * 4 + 16) + 4, ("f16", "fs"): 2 * (19 * 2 + 16) + 4,
 }
+# Scatter form (periodic FP32/FP16 factorised boxes): 19 total populations read
+# and written, phi read once, the node's own fixed-point sums (2 x int32)
+# written once; the recolour scalars are written on a run's last step only.
+B_ALG_COLLIDE_SC = {"f32": 2 * 19 * 4 + 4 + 8, "f16": 2 * 19 * 2 + 4 + 8}
 DTYPE = {"f64": "f64", "f32": "f32", "f16": "f16-storage/f32-compute"}
 
 
@@ -53,11 +59,11 @@ def cpu_baseline(seconds):
         return {"value": None, "error": str(e)}
 
 
-def traffic_from_profile(precision, layout):
+def traffic_from_profile(precision, layout, kernel="k_collide"):
     """HBM bytes per launch of the fused kernel from committed rocprofv3 PMC
     summaries (profiles/), scaled by the copy-kernel calibration; None if
     absent."""
-    path = os.path.join(ROOT, "profiles", f"k_collide_{precision}_{layout}_256x256x512_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{precision}_{layout}_256x256x512_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -199,9 +205,17 @@ def main():
     ev_total, ms_mom, ms_col = lat.time_steps(ksteps)
     avg_col = ms_col / ksteps * 1e-3
     avg_mom = ms_mom / ksteps * 1e-3
-    b_col = B_ALG_COLLIDE[(args.precision, args.layout)]
+    scatter = lat.scatter_form()
+    if scatter:
+        b_col = B_ALG_COLLIDE_SC[args.precision]
+        kname = "k_collide_sc"
+        kdesc = "k_collide_sc (fused derivatives+collision+recolour+streaming, next phase field summed in scatter form)"
+    else:
+        b_col = B_ALG_COLLIDE[(args.precision, args.layout)]
+        kname = "k_collide"
+        kdesc = "k_collide (fused derivatives+collision+recolour+streaming)"
     achieved = b_col * nodes / avg_col
-    traffic = traffic_from_profile(args.precision, args.layout)
+    traffic = traffic_from_profile(args.precision, args.layout, kname)
     lat.close()
 
     # the same workload in every precision and layout (MLUPS only)
@@ -223,6 +237,23 @@ def main():
                 v.sync()
                 variants[key] = round(nodes * args.variant_steps / (time.perf_counter() - t1) / 1e6, 1)
                 v.close()
+        if scatter:
+            # the same box with a moments pass per step (gather form)
+            os.environ["CGL_SCATTER"] = "0"
+            try:
+                v = symmetry_amd.Lattice(nx, ny, nz, precision=args.precision, sigma=cfg["sigma"], rho0=cfg["rho0"],
+                                         nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], layout=args.layout)
+            finally:
+                os.environ.pop("CGL_SCATTER", None)
+            v.init_cylinder(nx / 2 + 0.5, ny / 2 + 0.5, cfg["r0"], 6.0 * cfg["beta"] * 0.134, cfg["amps"])
+            v.step(20)
+            v.sync()
+            t1 = time.perf_counter()
+            v.step(args.variant_steps)
+            v.sync()
+            variants[f"{args.precision}-{args.layout}-gather"] = round(
+                nodes * args.variant_steps / (time.perf_counter() - t1) / 1e6, 1)
+            v.close()
 
     ensemble = run_ensemble(args.ensemble_steps, args.ensemble_seq_steps) if args.ensemble_steps > 0 else None
     particles = run_particles(args.particle_steps) if args.particle_steps > 0 else None
@@ -243,10 +274,11 @@ def main():
         "config": {"workload": f"Rayleigh-Plateau {nx}x{ny}x{nz} periodic D3Q19 colour-gradient MRT, R=40, "
                                "8 modes seed 1234, tau_r=1.0 tau_b=0.7, sigma=0.01, beta=0.7",
                    "precision": args.precision, "layout": args.layout, "nodes": nodes,
-                   "xchunk": xchunk, "xchunk_autotuned": args.xchunk == 0 and not still_tuning},
+                   "xchunk": xchunk, "xchunk_autotuned": args.xchunk == 0 and not still_tuning,
+                   "phase_field": "scatter" if scatter else "gather"},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                     "kernel": "k_collide (fused derivatives+collision+recolour+streaming)",
+                     "kernel": kdesc,
                      "alg_bytes_per_node": b_col,
                      "avg_launch_ms": round(avg_col * 1e3, 4), "moments_avg_launch_ms": round(avg_mom * 1e3, 4)},
         "step_roofline": {"bytes_per_node": B_ALG_STEP[args.precision],

@@ -97,7 +97,8 @@ int cgl_step(cgl_t *h, int32_t nsteps);
 
 /* Moments of the current state (calmacros, S:770-831): rho, rho_R, rho_B,
  * bare velocity ux, uy, uz and phi (rho_N) in (zl, yl, xl) layout.  Any
- * pointer may be NULL. */
+ * pointer may be NULL.  On wall and mirror ghost layers phi is the copy of
+ * the adjacent live layer (S:806-807) and the other fields are 0. */
 int cgl_calmacros(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *ux, double *uy, double *uz,
                   double *phi);
 
@@ -110,6 +111,14 @@ int cgl_derivatives(cgl_t *h, double *gradx, double *grady, double *gradz, doubl
  * is still timing candidates on the first steps.  No counterpart in the
  * reference. */
 int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning);
+
+/* 1 when the handle's steps compute the next phase field in scatter form
+ * (calmacros' sums S:792-799 accumulated by the writers of the populations),
+ * so that a step reads the state once: periodic FP32/FP16 factorised single
+ * boxes with ny a multiple of 8 and nz a multiple of 64, unless the
+ * environment sets CGL_SCATTER=0.  0: a moments pass per step (gather form).
+ * No counterpart in the reference. */
+int cgl_phase_field_form(cgl_t *h, int32_t *scatter);
 
 /* Block until all queued device work of the handle has finished. */
 int cgl_sync(cgl_t *h);

@@ -56,7 +56,7 @@ EXPORTS = (
     "cgl_device_name", "cgl_last_error", "cgl_probe_copy_bandwidth", "cgl_tuned_xchunk",
     "cgl_set_particles", "cgl_get_particles", "cgl_download_masks",
     "cgl_batch_create", "cgl_box_upload_state", "cgl_box_init_cylinder", "cgl_box_download_state",
-    "cgl_box_calmacros", "cgl_diagnostics", "cgl_init_interface",
+    "cgl_box_calmacros", "cgl_diagnostics", "cgl_init_interface", "cgl_phase_field_form",
 )
 
 
@@ -92,6 +92,7 @@ def lib():
         L.cgl_diagnostics.argtypes = [vp, vp, vp]
         L.cgl_init_interface.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+        L.cgl_phase_field_form.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
         L.cgl_probe_copy_bandwidth.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                                ctypes.POINTER(ctypes.c_double)]
         _lib = L
@@ -227,6 +228,13 @@ class Lattice:
         xc, tu = ctypes.c_int32(), ctypes.c_int32()
         _check(lib().cgl_tuned_xchunk(self.h, ctypes.byref(xc), ctypes.byref(tu)))
         return xc.value, bool(tu.value)
+
+    def scatter_form(self):
+        """True when the steps sum the next phase field in scatter form (one
+        read of the state per step)."""
+        v = ctypes.c_int32()
+        _check(lib().cgl_phase_field_form(self.h, ctypes.byref(v)))
+        return bool(v.value)
 
     def device_name(self):
         buf = ctypes.create_string_buffer(256)

@@ -62,8 +62,11 @@ int set_device(cgl_handle *h)
     return CGL_OK;
 }
 
-// One time step: K1 then K2, then swap buffers.
-void enqueue_step(cgl_handle *h)
+// One time step, then swap buffers.  Gather form: K1 then K2.  Scatter form
+// (periodic FP32/FP16 factorised boxes, once the state is factorised): the
+// step kernel and the phase-field pass; `last` = the run's last step, which
+// writes the recolour scalars so that the state can be read back.
+void enqueue_step(cgl_handle *h, bool last = true)
 {
     dispatch(h, [&](auto impl) {
         if (h->nbox) {
@@ -71,14 +74,48 @@ void enqueue_step(cgl_handle *h)
             decltype(impl)::batch_collide(h);
         } else if (h->ext) {
             decltype(impl)::step_ext(h);
+        } else if (h->sc && h->state_layout == cgl::FS) {
+            decltype(impl)::step_sc(h, last);
         } else {
             decltype(impl)::moments(h, false);
             decltype(impl)::collide(h);
+            h->phi_ok = 0;
         }
     });
     h->cur ^= 1;
     h->phicur ^= 1;
     h->steps_done++;
+}
+
+// Steps that must run eagerly before steady graph replays: the layout
+// conversion, the particles' start-up step, and the scatter form's first
+// step (its phase field comes from a moments pass).
+bool needs_eager(const cgl_handle *h)
+{
+    return h->state_layout != h->layout || (h->npar && (!h->primed || h->cold)) || (h->sc && !h->phi_ok);
+}
+
+// Entry points that read the state back need its recolour scalars.
+int state_readable(const cgl_handle *h)
+{
+    if (h->scal_stale) return fail(CGL_ERR_HIP, "internal: the state's recolour scalars were not written");
+    return CGL_OK;
+}
+
+// Fixed-point partial sums of the scatter step, for single periodic FP32/FP16
+// factorised boxes whose y and z extents are whole workgroup boxes.
+int setup_scatter(cgl_handle *h)
+{
+    const char *env = getenv("CGL_SCATTER"); // 0: gather form (moments pass) on every step
+    const bool want = env ? atoi(env) != 0 : true;
+    if (!want || h->ext || h->nbox || h->p.precision == CGL_PREC_F64 || h->p.layout != CGL_LAYOUT_FS ||
+        !cgl::sc_shape_ok(h->p.ny, h->p.nz))
+        return CGL_OK;
+    int2 **bufs[cgl::SC_NBUF] = {&h->sb.own,   &h->sb.fxlo,  &h->sb.fxhi,  &h->sb.fy,    &h->sb.fz,
+                                 &h->sb.exylo, &h->sb.exyhi, &h->sb.exzlo, &h->sb.exzhi, &h->sb.eyz};
+    for (int k = 0; k < cgl::SC_NBUF; ++k) HIPCHK(hipMalloc((void **)bufs[k], h->N * sizeof(int2)));
+    h->sc = 1;
+    return CGL_OK;
 }
 
 // Streaming copy, 16 B per thread per iteration: the reference kernel for the
@@ -170,7 +207,13 @@ int cgl_create(cgl_t **out, const cgl_params *p)
     if (!(p->rho0 > 0) || !(p->nu_r > 0) || !(p->nu_b > 0)) return fail(CGL_ERR_ARG, "rho0, nu_r, nu_b must be > 0");
     if (p->layout != CGL_LAYOUT_RB && p->layout != CGL_LAYOUT_FS) return fail(CGL_ERR_ARG, "unknown layout %d", p->layout);
     if (p->reserved != 0) return fail(CGL_ERR_ARG, "reserved field must be 0");
-    return alloc_handle(out, p, (size_t)p->nx * p->ny * p->nz, walls | mirrors);
+    int rc = alloc_handle(out, p, (size_t)p->nx * p->ny * p->nz, walls | mirrors);
+    if (rc) return rc;
+    if ((rc = setup_scatter(*out))) {
+        cgl_destroy(*out);
+        *out = nullptr;
+    }
+    return rc;
 }
 
 // Allocation of a handle's device state for N nodes (one box, or a packed
@@ -222,7 +265,7 @@ static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls)
         return rc;
     }
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-    for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipEventCreate(&h->ev[k]);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&h->ev[k]);
     // RB: both population buffers.  FS: one RB buffer (staging of uploaded or
     // initial states) plus the factorised double buffer.
     const int nrb = p->layout == CGL_LAYOUT_RB ? 2 : 1;
@@ -296,7 +339,11 @@ int cgl_destroy(cgl_t *h)
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->dboxes) (void)hipFree(h->dboxes);
     delete[] h->hboxes;
-    for (int k = 0; k < 3; ++k)
+    int2 *sbufs[cgl::SC_NBUF] = {h->sb.own,   h->sb.fxlo,  h->sb.fxhi,  h->sb.fy,    h->sb.fz,
+                                 h->sb.exylo, h->sb.exyhi, h->sb.exzlo, h->sb.exzhi, h->sb.eyz};
+    for (int2 *q : sbufs)
+        if (q) (void)hipFree(q);
+    for (int k = 0; k < 4; ++k)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -320,6 +367,8 @@ int cgl_upload_state(cgl_t *h, const double *ri, const double *bi)
         h->cur = 0; // uploaded states are held in RB form in buffer 0
         h->state_layout = cgl::RB;
         h->primed = 0;
+        h->phi_ok = 0;
+        h->scal_stale = 0;
         dispatch(h, [&](auto impl) {
             decltype(impl)::phi_aos(h, h->aos, tmp);
             decltype(impl)::pack(h, h->aos, 1);
@@ -339,6 +388,7 @@ int cgl_download_state(cgl_t *h, double *ri, double *bi)
     if (!h || !ri || !bi) return fail(CGL_ERR_ARG, "null argument");
     int rc = set_device(h);
     if (rc) return rc;
+    if ((rc = state_readable(h))) return rc;
     if ((rc = ensure_scratch(h, false, true, false))) return rc;
     const size_t bytes = (size_t)cgl::Q * h->N * sizeof(double);
     for (int red = 1; red >= 0; --red) {
@@ -367,6 +417,8 @@ int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, i
     h->cur = 0; // initial states are held in RB form in buffer 0
     h->state_layout = cgl::RB;
     h->primed = 0;
+    h->phi_ok = 0;
+    h->scal_stale = 0;
     dispatch(h, [&](auto impl) { decltype(impl)::init_cylinder(h, ic); });
     if ((rc = check_launch("init_cylinder"))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -386,6 +438,8 @@ int cgl_init_interface(cgl_t *h, double z0, double kprof)
     h->cur = 0; // initial states are held in RB form in buffer 0
     h->state_layout = cgl::RB;
     h->primed = 0;
+    h->phi_ok = 0;
+    h->scal_stale = 0;
     dispatch(h, [&](auto impl) { decltype(impl)::init_cylinder(h, ic); });
     if ((rc = check_launch("init_interface"))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -398,7 +452,7 @@ int cgl_init_interface(cgl_t *h, double z0, double kprof)
 // every replay.  The steps contain kernel launches only.
 static int capture_two_steps(cgl_handle *h)
 {
-    const int key = h->cur | h->phicur << 1 | h->state_layout << 2 | h->ntcur << 3;
+    const int key = h->cur | h->phicur << 1 | h->state_layout << 2 | h->ntcur << 3 | h->phi_ok << 4;
     if (h->gexec && h->gkey == key) return CGL_OK;
     if (h->gexec) {
         HIPCHK(hipGraphExecDestroy(h->gexec));
@@ -406,15 +460,18 @@ static int capture_two_steps(cgl_handle *h)
         h->gkey = -1;
     }
     const int cur = h->cur, phicur = h->phicur, done = h->steps_done, ntcur = h->ntcur;
+    const int phi_ok = h->phi_ok, stale = h->scal_stale;
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(h);
-    enqueue_step(h);
+    enqueue_step(h, false);
+    enqueue_step(h, false);
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     h->cur = cur; // the captured steps have not run yet
     h->phicur = phicur;
     h->steps_done = done;
     h->ntcur = ntcur;
+    h->phi_ok = phi_ok;
+    h->scal_stale = stale;
     if (e != hipSuccess) return fail(CGL_ERR_HIP, "step capture failed: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -466,21 +523,28 @@ int cgl_step(cgl_t *h, int32_t nsteps)
     if ((rc = tune_some(h, &nsteps))) return rc;
     if (h->use_graph && !h->tuning && nsteps >= 2) {
         // steady state: replay two captured steps (no host work per step).  The
-        // layout conversion and the particles' start-up step run eagerly.
-        while (nsteps > 0 && (h->state_layout != h->layout || (h->npar && (!h->primed || h->cold)))) {
-            enqueue_step(h);
+        // layout conversion, the particles' start-up step and the scatter
+        // form's first step run eagerly, and so does the scatter form's last
+        // step, which writes the recolour scalars.
+        while (nsteps > 0 && needs_eager(h)) {
+            enqueue_step(h, nsteps == 1);
             if ((rc = check_launch("step"))) return rc;
             --nsteps;
         }
-        if (nsteps >= 2 && (rc = capture_two_steps(h))) return rc;
-        for (int k = 0; k + 1 < nsteps; k += 2) {
-            HIPCHK(hipGraphLaunch(h->gexec, h->stream));
-            h->steps_done += 2;
+        const int pairs = (nsteps - (h->sc ? 1 : 0)) / 2;
+        if (pairs > 0) {
+            if ((rc = capture_two_steps(h))) return rc;
+            for (int k = 0; k < pairs; ++k) HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+            h->steps_done += 2 * pairs;
+            if (h->sc) {
+                h->phi_ok = 1;
+                h->scal_stale = 1;
+            }
+            nsteps -= 2 * pairs;
         }
-        nsteps &= 1;
     }
     for (int t = 0; t < nsteps; ++t) {
-        enqueue_step(h);
+        enqueue_step(h, t == nsteps - 1);
         if ((rc = check_launch("step"))) return rc;
     }
     return CGL_OK;
@@ -494,12 +558,20 @@ int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning)
     return CGL_OK;
 }
 
+int cgl_phase_field_form(cgl_t *h, int32_t *scatter)
+{
+    if (!h || !scatter) return fail(CGL_ERR_ARG, "null argument");
+    *scatter = h->sc;
+    return CGL_OK;
+}
+
 int cgl_calmacros(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *ux, double *uy, double *uz, double *phi)
 {
     if (h && h->nbox) return fail(CGL_ERR_ARG, "a batch handle takes the cgl_box_* entry points");
     if (!h) return fail(CGL_ERR_ARG, "null handle");
     int rc = set_device(h);
     if (rc) return rc;
+    if ((rc = state_readable(h))) return rc;
     if ((rc = ensure_scratch(h, true, false, false))) return rc;
     dispatch(h, [&](auto impl) {
         if (h->ext)
@@ -522,6 +594,7 @@ int cgl_derivatives(cgl_t *h, double *gradx, double *grady, double *gradz, doubl
     if (!h) return fail(CGL_ERR_ARG, "null handle");
     int rc = set_device(h);
     if (rc) return rc;
+    if ((rc = state_readable(h))) return rc;
     if ((rc = ensure_scratch(h, true, false, true))) return rc;
     dispatch(h, [&](auto impl) {
         if (h->ext) {
@@ -555,7 +628,7 @@ int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moment
     HIPCHK(hipStreamSynchronize(h->stream));
     // pass 1: the whole loop between two events (no per-kernel events inside)
     HIPCHK(hipEventRecord(h->ev[0], h->stream));
-    for (int t = 0; t < nsteps; ++t) enqueue_step(h);
+    for (int t = 0; t < nsteps; ++t) enqueue_step(h, t == nsteps - 1);
     HIPCHK(hipEventRecord(h->ev[1], h->stream));
     if ((rc = check_launch("step"))) return rc;
     HIPCHK(hipEventSynchronize(h->ev[1]));
@@ -574,30 +647,43 @@ int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moment
         HIPCHK(hipEventElapsedTime(&a, h->ev[0], h->ev[2]));
         kc += a;
     }
+    // scatter form: the phase-field passes (a moments pass where needed, and
+    // the finishing pass) count as moments, the step kernel as collide
     for (int t = 0; t < nsteps && !h->ext; ++t) {
-        float a = 0.f, b = 0.f;
+        float a = 0.f, b = 0.f, c = 0.f;
+        const bool scs = h->sc && h->state_layout == cgl::FS;
         HIPCHK(hipEventRecord(h->ev[0], h->stream));
         dispatch(h, [&](auto impl) {
-            if (h->nbox)
+            if (scs)
+                decltype(impl)::sc_pre(h);
+            else if (h->nbox)
                 decltype(impl)::batch_moments(h);
             else
                 decltype(impl)::moments(h, false);
         });
         HIPCHK(hipEventRecord(h->ev[1], h->stream));
         dispatch(h, [&](auto impl) {
-            if (h->nbox)
+            if (scs)
+                decltype(impl)::sc_collide(h, t == nsteps - 1);
+            else if (h->nbox)
                 decltype(impl)::batch_collide(h);
             else
                 decltype(impl)::collide(h);
         });
         HIPCHK(hipEventRecord(h->ev[2], h->stream));
+        if (scs)
+            dispatch(h, [&](auto impl) { decltype(impl)::sc_fin(h); });
+        else
+            h->phi_ok = 0;
+        HIPCHK(hipEventRecord(h->ev[3], h->stream));
         h->cur ^= 1;
         h->phicur ^= 1;
         h->steps_done++;
-        HIPCHK(hipEventSynchronize(h->ev[2]));
+        HIPCHK(hipEventSynchronize(h->ev[3]));
         HIPCHK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
         HIPCHK(hipEventElapsedTime(&b, h->ev[1], h->ev[2]));
-        km += a;
+        HIPCHK(hipEventElapsedTime(&c, h->ev[2], h->ev[3]));
+        km += a + (scs ? c : 0.f);
         kc += b;
     }
     if (ms_moments) *ms_moments = km;
@@ -712,6 +798,7 @@ int cgl_diagnostics(cgl_t *h, double *sums, double *slice_red)
     if (h->nbox) return fail(CGL_ERR_ARG, "a batch handle takes the cgl_box_* entry points");
     int rc = set_device(h);
     if (rc) return rc;
+    if ((rc = state_readable(h))) return rc;
     if ((rc = ensure_scratch(h, true, false, false))) return rc;
     dispatch(h, [&](auto impl) {
         if (h->ext)

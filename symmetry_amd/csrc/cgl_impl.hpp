@@ -8,6 +8,7 @@
 #include "../../include/cgl.h"
 #include "cgl_kernels.hpp"
 #include "cgl_particles.hpp"
+#include "cgl_scatter.hpp"
 
 struct cgl_handle {
     cgl_params p;
@@ -34,7 +35,7 @@ struct cgl_handle {
     int *iflag;       // scratch: N ints (lazy)
     size_t scratch_n; // nodes the scratch buffers are sized for
     hipStream_t stream;
-    hipEvent_t ev[3];
+    hipEvent_t ev[4];
     int steps_done;
     // walls and particles
     int ext;             // 1: wall faces or particles present (EXT kernels)
@@ -54,7 +55,12 @@ struct cgl_handle {
     // steady-state steps replayed from a captured graph (two steps per replay)
     int use_graph;       // replay steady-state steps from a captured graph
     hipGraphExec_t gexec;
-    int gkey;            // cur | phicur << 1 | state_layout << 2 at capture, -1 = none
+    int gkey;            // cur | phicur << 1 | state_layout << 2 | ntcur << 3 | phi_ok << 4 at capture, -1 = none
+    // periodic FP32/FP16 factorised boxes: next phase field in scatter form
+    int sc;              // 1: steady steps use k_collide_sc + k_phi_sc
+    int phi_ok;          // phi[phicur ^ 1] holds the current state's phase field (from the last step)
+    int scal_stale;      // the current state's recolour scalars were not written (mid-run only)
+    cgl::ScBufs sb;      // fixed-point partial sums
 };
 
 namespace cgl {
@@ -93,12 +99,16 @@ inline void state_out(const cgl_handle *h, void *&o0, void *&o1)
 constexpr int K1_XC = 8;
 
 template <class S, class R, class P, bool SHIFT> struct Impl {
+    // phi buffer that moments() writes: the step's phi buffer, or the spare one
+    // while that already holds the scatter step's phase field (observing the
+    // state does not change the next step)
+    static int phi_mom(const cgl_handle *h) { return h->phi_ok ? h->phicur : h->phicur ^ 1; }
     static void moments(cgl_handle *h, bool full)
     {
         const void *i0, *i1;
         state_in(h, i0, i1);
         const P *phio = (const P *)h->phi[h->phicur];
-        P *phi = (P *)h->phi[h->phicur ^ 1];
+        P *phi = (P *)h->phi[phi_mom(h)];
         double *mac = full ? h->mac : nullptr;
         auto launch = [&](auto lin) {
             constexpr int LIN = decltype(lin)::value;
@@ -178,8 +188,39 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     // derivatives of the phi written by the last moments() call
     static void derivatives(cgl_handle *h, double *out, int *flag)
     {
-        k_derivatives<P><<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, (const P *)h->phi[h->phicur ^ 1], out,
+        k_derivatives<P><<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, (const P *)h->phi[phi_mom(h)], out,
                                                                          flag);
+    }
+    // ---- scatter-form step (periodic FP32/FP16 factorised boxes)
+    // the current state's phase field in gather form, when no scatter step
+    // has produced it
+    static void sc_pre(cgl_handle *h)
+    {
+        if (!h->phi_ok) moments(h, false);
+    }
+    static void sc_collide(cgl_handle *h, bool last)
+    {
+        if constexpr (SHIFT && std::is_same_v<P, float>) {
+            const S *fin = (const S *)h->fsf[h->cur];
+            S *fout = (S *)h->fsf[h->cur ^ 1];
+            float4 *sout = last ? (float4 *)h->fss[h->cur ^ 1] : nullptr;
+            k_collide_sc<S><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(
+                h->L, fin, fout, sout, (const float *)h->phi[h->phicur ^ 1], h->sb, h->xc);
+            h->scal_stale = !last;
+        }
+    }
+    // the next step's phase field into the spare phi buffer (it becomes the
+    // step's buffer when the parities flip)
+    static void sc_fin(cgl_handle *h)
+    {
+        k_phi_sc<<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, h->sb, h->xc, (float *)h->phi[h->phicur]);
+        h->phi_ok = 1;
+    }
+    static void step_sc(cgl_handle *h, bool last)
+    {
+        sc_pre(h);
+        sc_collide(h, last);
+        sc_fin(h);
     }
     static void phi_aos(cgl_handle *h, const double *ra, const double *ba)
     {

@@ -468,7 +468,17 @@ __device__ __forceinline__ void moments_node(const Lat &L, const void *__restric
     const int x = bz;
     if (z >= L.nz || y >= L.ny) return;
     if constexpr (EXT)
-        if (is_ghost(E, L, x, y, z)) return; // phi copied from the adjacent layer afterwards (S:806)
+        if (is_ghost(E, L, x, y, z)) {
+            // phi is copied from the adjacent layer afterwards (S:806); the
+            // other fields are not computed on ghost layers and read as zero
+            if constexpr (FULL) {
+                const size_t N = (size_t)L.N;
+                const unsigned n = ((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) mac[k * N + n] = 0.0;
+            }
+            return;
+        }
     unsigned xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
     const unsigned n = xo[1] + yo[1] + zo[1];

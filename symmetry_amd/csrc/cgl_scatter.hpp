@@ -1,0 +1,445 @@
+// cgl_scatter.hpp -- the periodic FP32/FP16 factorised step with the next
+// phase field summed in scatter form.
+//
+// The gather form computes phi of every node from its 19 upstream
+// populations (the moments pass, calmacros S:770-831), which reads the whole
+// state a second time per step.  Summed from the other side: right after its
+// collision and recolouring (S:599-732) a node x knows everything it hands
+// downstream.  f_i(x) goes to y = x + c_i, and its red part is
+// r_i = A f_i + w_i G.c_i (S:713-726).  So the step kernel adds, for every
+// downstream node y,
+//     sR(y) += r_i(x),     sB(y) += f_i(x) - r_i(x),
+// which are rho_R(y) and rho_B(y) of the next step (S:792-799), and
+// phi(y) = (sR - sB) / (sR + sB).  The collision then takes
+// rho_R = rho (1 + phi) / 2 with rho from its gathered populations, so the
+// recolour scalars are needed only to rebuild red and blue populations for a
+// download; they are written by the last step of a run.
+//
+// Sums are fixed point: each term is rounded to a multiple of rho0 * 2^-29
+// and added as a 32-bit integer.  Integer sums are exact, so a node's sums do
+// not depend on the order or grouping of their 19 terms, and results are the
+// same for every workgroup box decomposition (x chunk).  Pure phases sum to
+// an exact zero (A = 0 or 1, G = 0), so the EVAP_LIM rules of Recolor
+// (S:720-725) see exact zeros there.  Densities up to 4 rho0 fit.
+//
+// Summation: a workgroup owns a box of TZ (z) x 8 (y) x xc (x) nodes and marches
+// x.  A thread keeps its own node's sums for the destination planes x-1, x,
+// x+1 in registers; z neighbours trade terms by lane shuffles, y neighbours
+// through LDS.  Terms that leave the box go to per-node partial arrays, one
+// per side class (x face low/high, y face, z face, x-y edge low/high, x-z edge
+// low/high, y-z edge).  Every entry that a node's position calls for is written
+// by exactly one workgroup each step, and the finishing pass adds them.
+#pragma once
+
+#include "cgl_kernels.hpp"
+
+namespace cgl {
+
+constexpr float SC_ONE = 536870912.0f; // 2^29 fixed-point units per rho0
+
+struct ScBufs {
+    int2 *own;           // sums from the node's own box
+    int2 *fxlo, *fxhi;   // from the box on the node's low / high x side
+    int2 *fy, *fz;       // from the box across the node's y / z face
+    int2 *exylo, *exyhi; // from the box across an x-y edge, low / high x side
+    int2 *exzlo, *exzhi; // from the box across an x-z edge, low / high x side
+    int2 *eyz;           // from the box across a y-z edge
+};
+constexpr int SC_NBUF = 10;
+
+__device__ __forceinline__ void add2(int2 &a, int2 b)
+{
+    a.x += b.x;
+    a.y += b.y;
+}
+__device__ __forceinline__ int2 shfl2(int2 v, int src) { return make_int2(__shfl(v.x, src), __shfl(v.y, src)); }
+
+// Shapes the scatter step takes: whole workgroup boxes in y and z.
+__host__ __device__ inline bool sc_shape_ok(int ny, int nz) { return ny % KTY<float> == 0 && nz % TZ == 0; }
+
+template <class S>
+__global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *__restrict__ fin, S *__restrict__ fout,
+                                                                float4 *__restrict__ sout,
+                                                                const float *__restrict__ phi, ScBufs B, int xc)
+{
+    using R = float;
+    constexpr int KY = KTY<float>;
+    __shared__ K2Smem<R, float> sm;
+    __shared__ int2 yx[2][3][KY][TZ]; // terms for the row above [0] / below [1], per x offset
+    __shared__ int2 zdx[2][2][KY];    // edge-lane terms for the next row: [above/below][lane 0 / TZ-1][row]
+    unsigned bx, by, bz;
+    tile_of_block(L, bx, by, bz);
+    const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
+    const int z0 = bx * TZ, y0 = by * KY, x0 = bz * xc;
+    const int x1 = min(x0 + xc, L.nx);
+    const int z = z0 + tz, y = y0 + ty;
+    const unsigned sx = (unsigned)L.ny * (unsigned)L.nz;
+    const size_t N = (size_t)L.N;
+    const R rho0 = R(L.rho0), qs = SC_ONE / R(L.rho0);
+
+    // phi ring: each thread loads at most two elements of the (KY+4) x (TZ+4)
+    // plane region; their in-plane offsets are the same for every plane
+    // (ny >= 8 and nz >= 64, so one conditional wrap suffices)
+    constexpr int NPHI = (KY + 4) * (TZ + 4);
+    static_assert(NPHI <= 2 * TZ * KY, "two phi elements per thread");
+    auto wrap_small = [](int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); };
+    int pj0, pj1 = -1;
+    unsigned po0, po1 = 0;
+    {
+        const int e0 = tid, e1 = tid + TZ * KY;
+        const int jy0 = e0 / (TZ + 4), jz0 = e0 - jy0 * (TZ + 4);
+        pj0 = e0;
+        po0 = (unsigned)wrap_small(y0 - 2 + jy0, L.ny) * (unsigned)L.nz + (unsigned)wrap_small(z0 - 2 + jz0, L.nz);
+        if (e1 < NPHI) {
+            const int jy1 = e1 / (TZ + 4), jz1 = e1 - jy1 * (TZ + 4);
+            pj1 = e1;
+            po1 = (unsigned)wrap_small(y0 - 2 + jy1, L.ny) * (unsigned)L.nz +
+                  (unsigned)wrap_small(z0 - 2 + jz1, L.nz);
+        }
+    }
+    auto load_phi = [&](int p) {
+        const unsigned gx = (unsigned)wrapm(p, L.nx) * sx;
+        float *dst = &sm.phi[slot5(p)][0][0];
+        dst[pj0] = at(phi, gx + po0);
+        if (pj1 >= 0) dst[pj1] = at(phi, gx + po1);
+    };
+    // unit normals of plane p on the (KY+2) x (TZ+2) halo region (S:498-532).
+    // Each thread computes its own node first (keeping that node's gradient)
+    // and at most one element of the 1-node halo ring.
+    constexpr int NHALO = 2 * (TZ + 2) + 2 * KY;
+    static_assert(NHALO <= TZ * KY, "one halo element per thread");
+    int hy = -1, hz = 0;
+    if (tid < NHALO) {
+        const int h = tid;
+        if (h < TZ + 2) { hy = 0; hz = h; }
+        else if (h < 2 * (TZ + 2)) { hy = KY + 1; hz = h - (TZ + 2); }
+        else if (h < 2 * (TZ + 2) + KY) { hy = h - 2 * (TZ + 2) + 1; hz = 0; }
+        else { hy = h - 2 * (TZ + 2) - KY + 1; hz = TZ + 1; }
+    }
+    auto normal_at = [&](int p, int jy, int jz, R &gx, R &gy, R &gz) {
+        const int s = slot3(p);
+        grad19<R>([&](int dx, int dy, int dz) { return sm.phi[slot5(p + dx)][jy + 1 + dy][jz + 1 + dz]; }, gx, gy,
+                  gz);
+        const R g2 = gx * gx + gy * gy + gz * gz;
+        R nx = 0, ny = 0, nz = 0;
+        if (g2 > R(EVAP_LIM * EVAP_LIM)) {
+            const R inv = __builtin_amdgcn_rsqf(g2);
+            nx = gx * inv;
+            ny = gy * inv;
+            nz = gz * inv;
+        }
+        sm.n[0][s][jy][jz] = nx;
+        sm.n[1][s][jy][jz] = ny;
+        sm.n[2][s][jy][jz] = nz;
+    };
+    auto normals = [&](int p, R &gx, R &gy, R &gz) {
+        normal_at(p, ty + 1, tz + 1, gx, gy, gz);
+        if (hy >= 0) {
+            R a, b, c;
+            normal_at(p, hy, hz, a, b, c);
+        }
+    };
+
+    // where this thread's box-leaving terms go
+    const bool ylo = ty == 0, yhi = ty == KY - 1, yedge = ylo || yhi;
+    const bool zl = tz == 0, zh = tz == TZ - 1, zedge = zl || zh; // lane 0 holds terms for z0 + TZ, lane TZ-1 for z0 - 1
+    const unsigned yrow = (unsigned)(ylo ? wrapm(y0 - 1, L.ny) : wrapm(y0 + KY, L.ny)) * (unsigned)L.nz;
+    const unsigned zcol = (unsigned)(zl ? wrapm(z0 + TZ, L.nz) : wrapm(z0 - 1, L.nz));
+    const unsigned yself = (unsigned)y * (unsigned)L.nz, zself = (unsigned)z;
+    const int left = (tz + TZ - 1) & (TZ - 1), right = (tz + 1) & (TZ - 1);
+    const int zs = zl ? 0 : 1;
+
+    // sums for destination planes x-1 (m), x (c), x+1 (p): own node column,
+    // the row outside the box next to this row, the column outside next to this lane
+    const int2 Z2 = make_int2(0, 0);
+    int2 am = Z2, ac = Z2, ap = Z2;
+    int2 ym = Z2, yc = Z2, yp = Z2;
+    int2 zm = Z2, zc = Z2, zp = Z2;
+
+    for (int p = x0 - 2; p <= x0 + 2; ++p) load_phi(p);
+    __syncthreads();
+    R gx, gy, gz; // gradient of phi at the node, plane x
+    {
+        R a, b, c;
+        normals(x0 - 1, a, b, c);
+    }
+    normals(x0, gx, gy, gz);
+
+    for (int x = x0; x < x1; ++x) {
+        __syncthreads(); // readers of the slots refilled below are done
+        if (x > x0) load_phi(x + 2);
+        __syncthreads();
+        R ngx, ngy, ngz;
+        normals(x + 1, ngx, ngy, ngz);
+        __syncthreads();
+
+        // gradient, normal and flag at the node (S:498-532)
+        const R g2 = gx * gx + gy * gy + gz * gz;
+        const bool flag = g2 > R(EVAP_LIM * EVAP_LIM);
+        const R nx = sm.n[0][slot3(x)][ty + 1][tz + 1];
+        const R ny = sm.n[1][slot3(x)][ty + 1][tz + 1];
+        const R nz = sm.n[2][slot3(x)][ty + 1][tz + 1];
+        // normal derivatives for the curvature (S:635-660)
+        R d[3][3];
+        if (flag) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    R tmp = R(W[i]) * sm.n[k][slot3(x + CX[i])][ty + 1 + CY[i]][tz + 1 + CZ[i]];
+                    if (CX[i] == 1) d[k][0] = d[k][0] + tmp;
+                    if (CX[i] == -1) d[k][0] = d[k][0] - tmp;
+                    if (CY[i] == 1) d[k][1] = d[k][1] + tmp;
+                    if (CY[i] == -1) d[k][1] = d[k][1] - tmp;
+                    if (CZ[i] == 1) d[k][2] = d[k][2] + tmp;
+                    if (CZ[i] == -1) d[k][2] = d[k][2] - tmp;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
+        }
+
+        // streaming: pull gather of the total populations (deviations from w rho0)
+        unsigned xo[3], yo[3], zo[3];
+        node_offsets(L, x, y, z, xo, yo, zo);
+        const unsigned n = xo[1] + yo[1] + zo[1];
+        R f[Q];
+        R df = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+            f[i] = StoreOf<S>::ld(fin + (size_t)i * N, m);
+        }
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            df += f[i];
+            if (CX[i] == 1) jx += f[i];
+            if (CX[i] == -1) jx -= f[i];
+            if (CY[i] == 1) jy += f[i];
+            if (CY[i] == -1) jy -= f[i];
+            if (CZ[i] == 1) jz += f[i];
+            if (CZ[i] == -1) jz -= f[i];
+        }
+        // moments (S:792-799): rho and j from the gathered populations, the
+        // colour split from the scattered phase field
+        const R phin = sm.phi[slot5(x)][ty + 2][tz + 2];
+        Moments<R, true> mo;
+        mo.rho = rho0 + df;
+        const R irho = fdiv(R(1), mo.rho);
+        mo.sr = R(0.5) * mo.rho * (R(1) + phin);
+        mo.sb = mo.rho - mo.sr;
+        mo.ux = jx * irho;
+        mo.uy = jy * irho;
+        mo.uz = jz * irho;
+        mo.phi = phin;
+        collide<R, true>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, flag, d);
+        const Recol<R> rc = recol<R, true>(L, mo, flag);
+        R A = rc.A, g = rc.gam;
+        if (rc.evr) { A = 0; g = 0; }
+        if (rc.evb) { A = 1; g = 0; }
+        const R Gx = g * nx, Gy = g * ny, Gz = g * nz;
+
+        // the next state: post-collision totals, and the recolour scalars on
+        // a run's last step; streamed past the caches
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            S *fp = fout + (size_t)i * N;
+            if constexpr (sizeof(S) == 4)
+                __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, n));
+            else
+                StoreOf<S>::st(fp, n, f[i]);
+        }
+        if (sout) {
+            typedef float f4v __attribute__((ext_vector_type(4)));
+            const f4v v4 = {float(A), float(Gx), float(Gy), float(Gz)};
+            __builtin_nontemporal_store(v4, (f4v *)&at(sout, n));
+        }
+
+        // terms (rho_R, rho_B) for the downstream nodes, one direction per
+        // destination offset (dx, dy, dz): g00 (dx,0,0), gzp/gzm (dx,0,+-1),
+        // gyp/gym (dx,+-1,0), dpp/dpm/dmp/dmm (0,+-1,+-1).  In fixed-point
+        // units: f_i = f~_i qs + w_i 2^29 and r_i = A f_i + (w_i qs G).c_i,
+        // rounded to the nearest integer (a one-sided rounding would bias
+        // every sum by up to 19 units).
+        int2 g00[3], gzp[3], gzm[3], gyp[3], gym[3], dpp, dpm, dmp, dmm;
+        const R s1 = R(W[1]) * qs, s7 = R(W[7]) * qs;
+        const R G1x = Gx * s1, G1y = Gy * s1, G1z = Gz * s1;
+        const R G7x = Gx * s7, G7y = Gy * s7, G7z = Gz * s7;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            R v = f[i];
+            if constexpr (sizeof(S) != 4) v = R(S(v)); // the value the next step reads
+            const R fq = __builtin_fmaf(v, qs, R(W[i] * double(SC_ONE)));
+            R cq = A * fq;
+            if (i > 0 && i <= 6) cq = __builtin_fmaf(A, fq, dot_c<R>(i, G1x, G1y, G1z));
+            if (i > 6) cq = __builtin_fmaf(A, fq, dot_c<R>(i, G7x, G7y, G7z));
+            const int qf = __float2int_rn(fq), qr = __float2int_rn(cq);
+            const int2 c = make_int2(qr, qf - qr);
+            const int k = CX[i] + 1;
+            if (CY[i] == 0 && CZ[i] == 0) g00[k] = c;
+            else if (CY[i] == 0 && CZ[i] == 1) gzp[k] = c;
+            else if (CY[i] == 0) gzm[k] = c;
+            else if (CZ[i] == 0 && CY[i] == 1) gyp[k] = c;
+            else if (CZ[i] == 0) gym[k] = c;
+            else if (CY[i] == 1 && CZ[i] == 1) dpp = c;
+            else if (CY[i] == 1) dpm = c;
+            else if (CZ[i] == 1) dmp = c;
+            else dmm = c;
+        }
+
+        // z neighbours: lane shuffles.  Lane 0 receives lane TZ-1's +z terms
+        // and lane TZ-1 lane 0's -z terms: those leave the box.
+        int2 own[3] = {g00[0], g00[1], g00[2]};
+        int2 zo3[3] = {Z2, Z2, Z2}, zup = Z2, zdn = Z2;
+        {
+            int2 lz[3], rz[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                lz[k] = shfl2(gzp[k], left);
+                rz[k] = shfl2(gzm[k], right);
+            }
+            const int2 lpp = shfl2(dpp, left), lmp = shfl2(dmp, left);
+            const int2 rpm = shfl2(dpm, right), rmm = shfl2(dmm, right);
+            if (!zl) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) add2(own[k], lz[k]);
+                add2(gyp[1], lpp);
+                add2(gym[1], lmp);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) zo3[k] = lz[k];
+                zup = lpp;
+                zdn = lmp;
+            }
+            if (!zh) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) add2(own[k], rz[k]);
+                add2(gyp[1], rpm);
+                add2(gym[1], rmm);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) zo3[k] = rz[k];
+                zup = rpm;
+                zdn = rmm;
+            }
+        }
+
+        // y neighbours through LDS; y-z edge terms of the corner lanes go out
+        // at once (they have dx = 0 only)
+        const unsigned px = (unsigned)x * sx;
+        if (!yhi) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) yx[0][k][ty][tz] = gyp[k];
+        }
+        if (!ylo) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) yx[1][k][ty][tz] = gym[k];
+        }
+        if (zedge) {
+            if (!yhi) zdx[0][zs][ty] = zup;
+            else at(B.eyz, px + yrow + zcol) = zup;
+            if (!ylo) zdx[1][zs][ty] = zdn;
+            else at(B.eyz, px + yrow + zcol) = zdn;
+        }
+        __syncthreads();
+        if (!ylo) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) add2(own[k], yx[0][k][ty - 1][tz]);
+        }
+        if (!yhi) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) add2(own[k], yx[1][k][ty + 1][tz]);
+        }
+        if (zedge) {
+            if (!ylo) add2(zo3[1], zdx[0][zs][ty - 1]);
+            if (!yhi) add2(zo3[1], zdx[1][zs][ty + 1]);
+        }
+
+        // accumulate over source planes; destination plane x-1 is complete
+        add2(am, own[0]);
+        add2(ac, own[1]);
+        add2(ap, own[2]);
+        if (yedge) {
+            add2(ym, ylo ? gym[0] : gyp[0]);
+            add2(yc, ylo ? gym[1] : gyp[1]);
+            add2(yp, ylo ? gym[2] : gyp[2]);
+        }
+        if (zedge) {
+            add2(zm, zo3[0]);
+            add2(zc, zo3[1]);
+            add2(zp, zo3[2]);
+        }
+        const unsigned pm = (unsigned)wrapm(x - 1, L.nx) * sx;
+        if (x > x0) {
+            at(B.own, pm + yself + zself) = am;
+            if (yedge) at(B.fy, pm + yrow + zself) = ym;
+            if (zedge) at(B.fz, pm + yself + zcol) = zm;
+        } else { // plane x0 - 1 belongs to the box on the low x side
+            at(B.fxhi, pm + yself + zself) = am;
+            if (yedge) at(B.exyhi, pm + yrow + zself) = ym;
+            if (zedge) at(B.exzhi, pm + yself + zcol) = zm;
+        }
+        gx = ngx;
+        gy = ngy;
+        gz = ngz;
+        am = ac;
+        ac = ap;
+        ap = Z2;
+        ym = yc;
+        yc = yp;
+        yp = Z2;
+        zm = zc;
+        zc = zp;
+        zp = Z2;
+    }
+    // plane x1 - 1 (own box) and plane x1 (the box on the high x side)
+    const unsigned pl = (unsigned)wrapm(x1 - 1, L.nx) * sx, pn = (unsigned)wrapm(x1, L.nx) * sx;
+    at(B.own, pl + yself + zself) = am;
+    at(B.fxlo, pn + yself + zself) = ac;
+    if (yedge) {
+        at(B.fy, pl + yrow + zself) = ym;
+        at(B.exylo, pn + yrow + zself) = yc;
+    }
+    if (zedge) {
+        at(B.fz, pl + yself + zcol) = zm;
+        at(B.exzlo, pn + yself + zcol) = zc;
+    }
+}
+
+// Finishing pass: phi(y) = (sR - sB) / (sR + sB) from the node's own-box sums
+// and the partial sums its position in its box calls for.  Pure phases give
+// exactly -1 and 1.
+static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, float *__restrict__ phi)
+{
+    const int z = blockIdx.x * TZ + threadIdx.x, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z;
+    if (z >= L.nz || y >= L.ny) return;
+    const unsigned n = ((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z;
+    int2 s = at(B.own, n);
+    const int cx0 = (x / xc) * xc;
+    const bool xlo = x == cx0, xhi = x == min(cx0 + xc, L.nx) - 1;
+    const int yr = y % KTY<float>, zr = z % TZ;
+    const bool fy = yr == 0 || yr == KTY<float> - 1, fz = zr == 0 || zr == TZ - 1;
+    if (xlo) add2(s, at(B.fxlo, n));
+    if (xhi) add2(s, at(B.fxhi, n));
+    if (fy) add2(s, at(B.fy, n));
+    if (fz) add2(s, at(B.fz, n));
+    if (fy && xlo) add2(s, at(B.exylo, n));
+    if (fy && xhi) add2(s, at(B.exyhi, n));
+    if (fz && xlo) add2(s, at(B.exzlo, n));
+    if (fz && xhi) add2(s, at(B.exzhi, n));
+    if (fy && fz) add2(s, at(B.eyz, n));
+    float ph;
+    if (s.x == 0)
+        ph = -1.0f;
+    else if (s.y == 0)
+        ph = 1.0f;
+    else
+        ph = float(s.x - s.y) / float(s.x + s.y);
+    at(phi, n) = ph;
+}
+
+} // namespace cgl

@@ -1,0 +1,180 @@
+"""GPU: the scatter form of the phase field (periodic FP32/FP16 factorised
+boxes).  The step kernel adds each node's post-collision red and blue
+populations to the sums of its 19 downstream nodes in 32-bit fixed point, so
+the next step needs no moments pass (calmacros' sums, S:792-799, taken from
+the writers' side).
+
+* Against the gather form (a moments pass per step, CGL_SCATTER=0): the two
+  differ only in the rounding of rho_R, so after the first scattered step
+  phi and u agree to 1e-6 (L2-relative).  Over 200 steps both stay within the
+  parity tolerances of the FP64 path (rounding differences grow with the
+  flow, so the long-run check is against FP64, not between the two forms).
+* Fixed-point sums are exact, so results do not depend on the workgroup box
+  decomposition: every x chunk, including chunks of one plane and a single
+  chunk, gives bit-identical states.  Boxes of one workgroup tile in y or z
+  (the sums wrap onto the box itself) agree with the gather form.
+* Reading the state mid-run (calmacros, derivatives, download, diagnostics)
+  and graph replay do not change the trajectory.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import symmetry_amd
+from tests.helpers import rel_err, small_rp
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def _lattice(cfg, precision="f32", scatter=True, graphs=True, **kw):
+    saved = {k: os.environ.get(k) for k in ("CGL_SCATTER", "CGL_GRAPHS")}
+    os.environ["CGL_SCATTER"] = "1" if scatter else "0"
+    os.environ["CGL_GRAPHS"] = "1" if graphs else "0"
+    try:
+        lat = symmetry_amd.Lattice(cfg["nx"], cfg["ny"], cfg["nz"], precision=precision, sigma=cfg["sigma"],
+                                   rho0=cfg["rho0"], nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"],
+                                   layout="fs", **kw)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return lat
+
+
+def _fields(lat):
+    m = lat.calmacros()
+    return m["phi"], np.stack([m["ux"], m["uy"], m["uz"]])
+
+
+@pytest.fixture(scope="module")
+def rp():
+    return small_rp(nx=32, ny=32, nz=128, nu_b=0.1)
+
+
+def test_scatter_form_selection(rp):
+    cfg, _, _ = rp
+    assert _lattice(cfg).scatter_form()
+    assert _lattice(cfg, "f16").scatter_form()
+    assert not _lattice(cfg, scatter=False).scatter_form()
+    assert not _lattice(cfg, "f64").scatter_form()
+    odd = dict(cfg, ny=12)
+    assert not _lattice(odd).scatter_form()  # ny not a multiple of the 8-row workgroup tile
+
+
+def test_scatter_matches_gather_after_three_steps(rp):
+    """Steps 1 and 2 take phi from a moments pass in both forms (layout
+    conversion, then the scatter form's first step); step 3 uses the
+    scattered sums, which differ from the gathered ones in rounding only."""
+    cfg, ri, bi = rp
+    out = []
+    for sc in (True, False):
+        lat = _lattice(cfg, scatter=sc, xchunk=8)
+        assert lat.scatter_form() == sc
+        lat.upload(ri, bi)
+        lat.step(3)
+        out.append(_fields(lat))
+    e_phi, e_u = rel_err(out[0][0], out[1][0]), rel_err(out[0][1], out[1][1])
+    _print("scatter vs gather, 3 steps: phi", e_phi, "u", e_u)
+    assert 0 < e_phi[0] < 1e-6 and e_u[0] < 1e-6
+
+
+@pytest.mark.parametrize("prec,tol_phi,tol_u", [("f32", 1e-5, 1e-5), ("f16", 5e-3, 5e-2)])
+def test_scatter_and_gather_against_f64(prec, tol_phi, tol_u):
+    """Both forms against the FP64 path after 200 steps on the Rayleigh-Plateau
+    box of the parity tests, at the parity tolerances; the scatter form is not
+    further from FP64 than the gather form by more than a factor 2."""
+    cfg, ri, bi = small_rp()
+    ref = _lattice(cfg, "f64")
+    ref.upload(ri, bi)
+    ref.step(200)
+    p64, u64 = _fields(ref)
+    errs = {}
+    for sc in (True, False):
+        lat = _lattice(cfg, prec, scatter=sc)
+        lat.upload(ri, bi)
+        lat.step(200)
+        phi, u = _fields(lat)
+        errs[sc] = (rel_err(phi, p64)[0], rel_err(u, u64)[0])
+    _print(prec, "vs f64 (phi, u): scatter", errs[True], "gather", errs[False])
+    assert errs[True][0] < tol_phi and errs[True][1] < tol_u
+    assert errs[True][0] <= 2 * errs[False][0] + 1e-7 and errs[True][1] <= 2 * errs[False][1] + 1e-7
+
+
+def test_scatter_independent_of_box_decomposition():
+    """x chunks 1, 2, 3 (last chunk one plane), 8, 40 (one chunk) and 64 on a
+    40-plane box: bit-identical states."""
+    cfg, ri, bi = small_rp(nx=40, ny=16, nz=128, r0=5.0, nu_b=0.1)
+    ref = None
+    for xc in (8, 1, 2, 3, 40, 64):
+        _print("decomposition xchunk", xc)
+        lat = _lattice(cfg, xchunk=xc)
+        lat.upload(ri, bi)
+        lat.step(9)
+        r, b = lat.download()
+        if ref is None:
+            ref = (r, b)
+        else:
+            assert np.array_equal(r, ref[0]) and np.array_equal(b, ref[1]), xc
+
+
+@pytest.mark.parametrize("shape", [(6, 8, 64), (10, 8, 128), (12, 24, 64)])
+def test_scatter_single_tile_axes(shape):
+    """One workgroup tile along y (ny = 8) and/or z (nz = 64): the box-leaving
+    sums wrap onto the box itself."""
+    nx, ny, nz = shape
+    cfg, ri, bi = small_rp(nx=nx, ny=ny, nz=nz, r0=2.5, nu_b=0.1)
+    out = []
+    for sc in (True, False):
+        lat = _lattice(cfg, scatter=sc, xchunk=4)
+        lat.upload(ri, bi)
+        lat.step(5)
+        out.append(_fields(lat))
+    e_phi, e_u = rel_err(out[0][0], out[1][0]), rel_err(out[0][1], out[1][1])
+    _print("single tile", shape, "phi", e_phi, "u", e_u)
+    assert e_phi[0] < 1e-6 and e_u[0] < 1e-5
+
+
+def test_observation_does_not_change_the_run(rp):
+    cfg, ri, bi = rp
+    a = _lattice(cfg, xchunk=8)
+    a.upload(ri, bi)
+    a.step(24)
+    ra, ba = a.download()
+    b = _lattice(cfg, xchunk=8)
+    b.upload(ri, bi)
+    b.step(7)
+    b.calmacros()
+    b.step(5)
+    b.derivatives()
+    b.download()
+    b.diagnostics()
+    b.step(12)
+    rb, bb = b.download()
+    assert np.array_equal(ra, rb) and np.array_equal(ba, bb)
+    c = _lattice(cfg, xchunk=8, graphs=False)
+    c.upload(ri, bi)
+    for _ in range(24):
+        c.step(1)
+    rc, bc = c.download()
+    assert np.array_equal(ra, rc) and np.array_equal(ba, bc)
+
+
+def test_timed_steps_leave_a_readable_state(rp):
+    cfg, ri, bi = rp
+    a = _lattice(cfg, xchunk=8)
+    a.upload(ri, bi)
+    a.step(3)
+    a.time_steps(4)
+    ra, ba = a.download()
+    b = _lattice(cfg, xchunk=8)
+    b.upload(ri, bi)
+    b.step(3 + 8)  # time_steps runs its steps twice (loop timing, then per-kernel timing)
+    rb, bb = b.download()
+    assert np.array_equal(ra, rb) and np.array_equal(ba, bb)
