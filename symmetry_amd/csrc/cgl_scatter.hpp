@@ -103,6 +103,19 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *_
         dst[pj0] = at(phi, gx + po0);
         if (pj1 >= 0) dst[pj1] = at(phi, gx + po1);
     };
+    // the ring's next plane is fetched one plane ahead into registers, so its
+    // load latency overlaps a plane of work
+    float pf0 = 0.f, pf1 = 0.f;
+    auto prefetch_phi = [&](int p) {
+        const unsigned gx = (unsigned)wrapm(p, L.nx) * sx;
+        pf0 = at(phi, gx + po0);
+        if (pj1 >= 0) pf1 = at(phi, gx + po1);
+    };
+    auto store_phi = [&](int p) {
+        float *dst = &sm.phi[slot5(p)][0][0];
+        dst[pj0] = pf0;
+        if (pj1 >= 0) dst[pj1] = pf1;
+    };
     // unit normals of plane p on the (KY+2) x (TZ+2) halo region (S:498-532).
     // Each thread computes its own node first (keeping that node's gradient)
     // and at most one element of the 1-node halo ring.
@@ -167,7 +180,8 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *_
 
     for (int x = x0; x < x1; ++x) {
         __syncthreads(); // readers of the slots refilled below are done
-        if (x > x0) load_phi(x + 2);
+        if (x > x0) store_phi(x + 2);
+        prefetch_phi(x + 3);
         __syncthreads();
         R ngx, ngy, ngz;
         normals(x + 1, ngx, ngy, ngz);
