@@ -67,8 +67,7 @@ typedef struct cgl_params {
  * reference's main (S:144-179).  Faces may be periodic (in pairs),
  * half-way bounce-back walls (S:1557-1590) or specular mirrors
  * (S:1443-1542), the last two with a ghost layer; npar > 0 adds moving
- * particles.  Walls, mirrors and particles run on the red/blue layout;
- * particles together with mirror faces return CGL_ERR_UNSUPPORTED. */
+ * particles.  Walls, mirrors and particles run on the red/blue layout. */
 int cgl_create(cgl_t **h, const cgl_params *p);
 int cgl_destroy(cgl_t *h);
 
@@ -133,6 +132,21 @@ int cgl_sync(cgl_t *h);
  * particle set-up of S:76-84 and initia_center (S:926-976). */
 int cgl_set_particles(cgl_t *h, const double *center, const double *up, const double *rp, const double *rad,
                       const double *mass, const double *inertia, const double *cita, double fg);
+
+/* Particle geometry that the reference hard-codes for its own box: the
+ * repulsion planes of the wall force (S:1607-1612), rep_plane[6] / rep_on[6]
+ * per face (x-lo, x-hi, y-lo, y-hi, z-lo, z-hi; 1-based lattice positions);
+ * the box inside which Lagrange points count (S:1832), dom_lo[3],
+ * dom_hi[3] and dom_on[9] = (low bound on, low bound strict, high bound on)
+ * per axis (a point counts if x >= lo, or x > lo when strict, and x < hi);
+ * and sym_rules = 1 for the symmetry-plane rules of movement (S:1272-1335),
+ * which act on particles whose set-up centre lies on the plane x = xl - 1/2,
+ * y = 3/2, y = yl - 1/2 or z = zl - 1/2 (the mirror planes of the reference's
+ * 1/8 domain), and average the x and z components of particle par/3 + 1.
+ * cgl_set_particles resets the defaults: planes half-way between the ghost
+ * layer and the first live layer of every wall face, no box, no rules. */
+int cgl_set_particle_geometry(cgl_t *h, const double *rep_plane, const int32_t *rep_on, const double *dom_lo,
+                              const double *dom_hi, const int32_t *dom_on, int32_t sym_rules);
 
 /* Particle state after the last step: centre, velocity, angular velocity,
  * momentum-exchange force and torque (Fpx.., Tpx.., S:1247-1252), capillary

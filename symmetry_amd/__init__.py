@@ -57,6 +57,7 @@ EXPORTS = (
     "cgl_set_particles", "cgl_get_particles", "cgl_download_masks",
     "cgl_batch_create", "cgl_box_upload_state", "cgl_box_init_cylinder", "cgl_box_download_state",
     "cgl_box_calmacros", "cgl_diagnostics", "cgl_init_interface", "cgl_phase_field_form",
+    "cgl_set_particle_geometry",
 )
 
 
@@ -93,6 +94,7 @@ def lib():
         L.cgl_init_interface.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.cgl_phase_field_form.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+        L.cgl_set_particle_geometry.argtypes = [vp] + [vp] * 5 + [ctypes.c_int32]
         L.cgl_probe_copy_bandwidth.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                                ctypes.POINTER(ctypes.c_double)]
         _lib = L
@@ -209,6 +211,16 @@ class Lattice:
         args = [t(center), t(up), t(rp), f(rad), f(mass), f(inertia), f(cita)]
         self._keep = args
         _check(lib().cgl_set_particles(self.h, *(_ptr(a) for a in args), float(fg)))
+
+    def set_particle_geometry(self, rep_plane, rep_on, dom_lo=(0.0, 0.0, 0.0), dom_hi=(0.0, 0.0, 0.0),
+                              dom_on=(0,) * 9, sym_rules=0):
+        """Repulsion planes (6 faces), Lagrange in-domain box (dom_on = low
+        on, low strict, high on per axis) and the symmetry-plane rules of the
+        reference's own box; call after set_particles."""
+        d = lambda a, n: np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(n))
+        i = lambda a, n: np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(n))
+        args = [d(rep_plane, 6), i(rep_on, 6), d(dom_lo, 3), d(dom_hi, 3), i(dom_on, 9)]
+        _check(lib().cgl_set_particle_geometry(self.h, *(_ptr(a) for a in args), int(sym_rules)))
 
     def get_particles(self):
         n = self.params.npar

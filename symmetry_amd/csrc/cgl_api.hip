@@ -95,6 +95,18 @@ bool needs_eager(const cgl_handle *h)
     return h->state_layout != h->layout || (h->npar && (!h->primed || h->cold)) || (h->sc && !h->phi_ok);
 }
 
+// Captured steps hold their kernels' by-value arguments (particle scalars,
+// geometry): drop the graph when those change.
+int drop_graph(cgl_handle *h)
+{
+    if (h->gexec) {
+        HIPCHK(hipGraphExecDestroy(h->gexec));
+        h->gexec = nullptr;
+        h->gkey = -1;
+    }
+    return CGL_OK;
+}
+
 // Entry points that read the state back need its recolour scalars.
 int state_readable(const cgl_handle *h)
 {
@@ -202,8 +214,6 @@ int cgl_create(cgl_t **out, const cgl_params *p)
     if (p->npar < 0) return fail(CGL_ERR_ARG, "npar must be >= 0");
     if ((walls || mirrors || p->npar > 0) && p->layout != CGL_LAYOUT_RB)
         return fail(CGL_ERR_UNSUPPORTED, "walls, mirrors and particles are implemented for the red/blue layout");
-    if (mirrors && p->npar > 0)
-        return fail(CGL_ERR_UNSUPPORTED, "particles with mirror faces are not implemented on the GPU");
     if (!(p->rho0 > 0) || !(p->nu_r > 0) || !(p->nu_b > 0)) return fail(CGL_ERR_ARG, "rho0, nu_r, nu_b must be > 0");
     if (p->layout != CGL_LAYOUT_RB && p->layout != CGL_LAYOUT_FS) return fail(CGL_ERR_ARG, "unknown layout %d", p->layout);
     if (p->reserved != 0) return fail(CGL_ERR_ARG, "reserved field must be 0");
@@ -289,11 +299,11 @@ static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls)
         if (e == hipSuccess) e = hipMalloc((void **)&h->E.der, 6 * h->N * sizeof(double));
         if (e == hipSuccess) e = hipMemset(h->E.der, 0, 6 * h->N * sizeof(double));
         const size_t n3 = 3 * (size_t)h->npar * sizeof(double), n1 = (size_t)h->npar * sizeof(double);
-        double **v3[10] = {&h->P.center, &h->P.center_old, &h->P.center_prev, &h->P.up, &h->P.rp,
-                           &h->P.fp, &h->P.tp, &h->P.frep, &h->P.ten, &h->P.tent};
+        double **v3[11] = {&h->P.center, &h->P.center_old, &h->P.center_prev, &h->P.up, &h->P.rp,
+                           &h->P.fp, &h->P.tp, &h->P.frep, &h->P.ten, &h->P.tent, &h->P.center0};
         double **v1[5] = {&h->P.rad, &h->P.mass, &h->P.inertia, &h->P.cita, &h->P.shb};
-        for (int k = 0; k < 10 && e == hipSuccess; ++k) e = hipMalloc((void **)v3[k], n3);
-        for (int k = 0; k < 10 && e == hipSuccess; ++k) e = hipMemset(*v3[k], 0, n3);
+        for (int k = 0; k < 11 && e == hipSuccess; ++k) e = hipMalloc((void **)v3[k], n3);
+        for (int k = 0; k < 11 && e == hipSuccess; ++k) e = hipMemset(*v3[k], 0, n3);
         for (int k = 0; k < 5 && e == hipSuccess; ++k) e = hipMalloc((void **)v1[k], n1);
         for (int k = 0; k < 5 && e == hipSuccess; ++k) e = hipMemset(*v1[k], 0, n1);
         if (e == hipSuccess) e = hipMalloc((void **)&h->P.part_lag, (size_t)h->npar * cgl::LAG_BLOCKS * 7 * sizeof(double));
@@ -330,7 +340,7 @@ int cgl_destroy(cgl_t *h)
         void *ptrs[] = {h->ntype[0], h->ntype[1], h->walls_i, h->pin_i, h->pout_i, h->E.mac, h->E.der,
                         h->P.center, h->P.center_old, h->P.center_prev, h->P.up, h->P.rp, h->P.fp, h->P.tp,
                         h->P.frep, h->P.ten, h->P.tent, h->P.rad, h->P.mass, h->P.inertia, h->P.cita, h->P.shb,
-                        h->P.part_lag, h->P.part_mom};
+                        h->P.part_lag, h->P.part_mom, h->P.center0};
         for (void *q : ptrs)
             if (q) (void)hipFree(q);
     }
@@ -702,7 +712,7 @@ int cgl_set_particles(cgl_t *h, const double *center, const double *up, const do
     for (int k = 0; k < h->npar; ++k) {
         if (!(rad[k] > 0) || !(mass[k] > 0) || !(inertia[k] > 0)) return fail(CGL_ERR_ARG, "particle %d: bad radius or mass", k);
         for (int a = 0; a < 3; ++a)
-            if (!h->E.wall[a][0] && 2.0 * (rad[k] + 4.0) + 3.0 > n3[a])
+            if (!h->E.wall[a][0] && !h->E.mirror[a][0] && 2.0 * (rad[k] + 4.0) + 3.0 > n3[a])
                 return fail(CGL_ERR_ARG, "particle %d: box along periodic axis %d is smaller than the particle's reach", k, a);
     }
     const size_t np = (size_t)h->npar, b3 = 3 * np * sizeof(double), b1 = np * sizeof(double);
@@ -717,6 +727,7 @@ int cgl_set_particles(cgl_t *h, const double *center, const double *up, const do
     HIPCHK(hipMemcpy(h->P.center, c3.data(), b3, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->P.center_old, c3.data(), b3, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->P.center_prev, c3.data(), b3, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->P.center0, c3.data(), b3, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->P.up, u3.data(), b3, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->P.rp, w3.data(), b3, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpyAsync(h->P.rad, rad, b1, hipMemcpyHostToDevice, h->stream));
@@ -724,13 +735,18 @@ int cgl_set_particles(cgl_t *h, const double *center, const double *up, const do
     HIPCHK(hipMemcpyAsync(h->P.inertia, inertia, b1, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->P.cita, cita, b1, hipMemcpyHostToDevice, h->stream));
     h->P.fg = fg;
-    // repulsion planes: wall faces sit half-way between ghost and first layer
+    // repulsion planes: wall faces sit half-way between ghost and first
+    // layer; no Lagrange domain box, no symmetry rules (cgl_set_particle_geometry)
     for (int a = 0; a < 3; ++a) {
         h->P.rep_on[2 * a] = h->E.wall[a][0];
         h->P.rep_on[2 * a + 1] = h->E.wall[a][1];
         h->P.rep_plane[2 * a] = 1.5;
         h->P.rep_plane[2 * a + 1] = n3[a] - 0.5;
+        h->P.dom_lo_on[a] = h->P.dom_lo_strict[a] = h->P.dom_hi_on[a] = 0;
+        h->P.dom_lo[a] = h->P.dom_hi[a] = 0.0;
     }
+    h->P.sym_rules = 0;
+    if ((rc = drop_graph(h))) return rc;
     // set_walls of the start-up (S:212): fresh masks
     for (int b = 0; b < 2; ++b) HIPCHK(hipMemsetAsync(h->ntype[b], 0, h->N * sizeof(unsigned), h->stream));
     HIPCHK(hipMemsetAsync(h->walls_i, 0, h->N * sizeof(int), h->stream));
@@ -743,6 +759,27 @@ int cgl_set_particles(cgl_t *h, const double *center, const double *up, const do
     h->cold = 1;
     HIPCHK(hipStreamSynchronize(h->stream));
     return CGL_OK;
+}
+
+int cgl_set_particle_geometry(cgl_t *h, const double *rep_plane, const int32_t *rep_on, const double *dom_lo,
+                              const double *dom_hi, const int32_t *dom_on, int32_t sym_rules)
+{
+    if (!h || !rep_plane || !rep_on || !dom_lo || !dom_hi || !dom_on) return fail(CGL_ERR_ARG, "null argument");
+    if (h->npar == 0) return fail(CGL_ERR_ARG, "the handle was created with npar = 0");
+    if (sym_rules != 0 && sym_rules != 1) return fail(CGL_ERR_ARG, "sym_rules must be 0 or 1");
+    for (int f = 0; f < 6; ++f) {
+        h->P.rep_plane[f] = rep_plane[f];
+        h->P.rep_on[f] = rep_on[f] != 0;
+    }
+    for (int a = 0; a < 3; ++a) {
+        h->P.dom_lo[a] = dom_lo[a];
+        h->P.dom_hi[a] = dom_hi[a];
+        h->P.dom_lo_on[a] = dom_on[3 * a] != 0;
+        h->P.dom_lo_strict[a] = dom_on[3 * a + 1] != 0;
+        h->P.dom_hi_on[a] = dom_on[3 * a + 2] != 0;
+    }
+    h->P.sym_rules = sym_rules;
+    return drop_graph(h); // the captured steps carry the old geometry
 }
 
 int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *force, double *torque,

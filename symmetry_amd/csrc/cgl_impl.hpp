@@ -249,6 +249,29 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         return E;
     }
     static dim3 pgrid(const cgl_handle *h, int slices) { return dim3(h->npar, slices, 1); }
+    static bool mirrors(const cgl_handle *h)
+    {
+        for (int a = 0; a < 3; ++a)
+            if (h->E.mirror[a][0] || h->E.mirror[a][1]) return true;
+        return false;
+    }
+    // launch shape of the face kernels: the largest face, six faces
+    static dim3 face_grid(const cgl_handle *h)
+    {
+        int m = h->L.nx * h->L.ny;
+        if (h->L.ny * h->L.nz > m) m = h->L.ny * h->L.nz;
+        if (h->L.nx * h->L.nz > m) m = h->L.nx * h->L.nz;
+        return dim3((m + 255) / 256, 6, 1);
+    }
+    // mirror fills of phi, and of the band's macroscopic fields, with particles
+    static void mirror_phi(cgl_handle *h, P *phi)
+    {
+        if (mirrors(h)) k_wall_phi<P><<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, phi, 1);
+    }
+    static void mirror_mac(cgl_handle *h)
+    {
+        if (h->npar && mirrors(h)) k_mirror_mac<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->E.mac);
+    }
     static void moments_ext(cgl_handle *h, bool full)
     {
         const void *i0, *i1;
@@ -264,13 +287,11 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                 <<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, nullptr, E);
         wall_phi(h, phi);
         if (full) wall_phi(h, h->mac + 6 * h->N); // the FP64 phi output
+        mirror_mac(h);
     }
     template <class T> static void wall_phi(cgl_handle *h, T *phi)
     {
-        int m = h->L.nx * h->L.ny;
-        if (h->L.ny * h->L.nz > m) m = h->L.ny * h->L.nz;
-        if (h->L.nx * h->L.nz > m) m = h->L.nx * h->L.nz;
-        k_wall_phi<T><<<dim3((m + 255) / 256, 6, 1), 256, 0, h->stream>>>(h->L, h->E, phi);
+        k_wall_phi<T><<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, phi);
     }
     // start-up derivatives (S:214) for the capillary force of the first step
     static void prime(cgl_handle *h)
@@ -279,6 +300,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         P *phi = (P *)h->phi[h->phicur ^ 1];
         const Ext E = ext_with(h, h->ntype[h->ntcur]);
         k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[h->ntcur], h->pin_i, phi);
+        mirror_phi(h, phi);
         k_derivs_ext<P><<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, E, phi, nullptr, nullptr);
     }
     static void derivatives_ext(cgl_handle *h, double *out, int *flag)
@@ -286,8 +308,10 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         moments_ext(h, false);
         P *phi = (P *)h->phi[h->phicur ^ 1];
         const Ext E = ext_with(h, h->ntype[h->ntcur]);
-        if (h->npar)
+        if (h->npar) {
             k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[h->ntcur], h->pin_i, phi);
+            mirror_phi(h, phi);
+        }
         Ext E2 = E;
         E2.der = nullptr;
         k_derivs_ext<P><<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, E2, phi, out, flag);
@@ -299,6 +323,9 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         k_masks_solid<<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[buf], h->walls_i);
         k_masks_links<<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[buf], h->walls_i, h->pin_i,
                                                                h->pout_i);
+        if (mirrors(h))
+            k_mirror_masks<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->ntype[buf], h->walls_i, h->pin_i,
+                                                                h->pout_i);
     }
     // one time step with walls and/or particles (S:216-251)
     static void step_ext(cgl_handle *h)
@@ -329,7 +356,10 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             const Ext En = ext_with(h, h->ntype[nb]);
             k_refill<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[h->ntcur], h->ntype[nb],
                                                                  h->pout_i, phi);
+            mirror_phi(h, phi);
+            mirror_mac(h);
             k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[nb], h->pin_i, phi);
+            mirror_phi(h, phi);
         }
         {
             const void *i0, *i1;
@@ -344,6 +374,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                 S *ro = (S *)o0, *bo = (S *)o1;
                 k_curve<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(
                     h->L, En, h->P, h->ntype[nb], h->walls_i, h->pout_i, ro, bo, SHIFT ? (const float *)phi : nullptr);
+                if (mirrors(h)) k_mirror_der<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->E.der);
             }
         }
         if (h->npar) h->ntcur = nb;

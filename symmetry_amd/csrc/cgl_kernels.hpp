@@ -1089,9 +1089,12 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                 }
                 if constexpr (EXT) {
                     if (fl && (nt & NT_POUT)) {
+                        // wetting at the node, or for a mirror ghost at its image
+                        // (whose masks the ghost holds)
                         const int pp = at(E.pout, (unsigned)(gxw * sx + gyw * L.nz + gzw)) - 1;
                         double nv[3] = {double(nx), double(ny), double(nz)};
-                        wetting(L, E, gxw + 1.0, gyw + 1.0, gzw + 1.0, E.center + 3 * pp, E.cita[pp], nv);
+                        wetting(L, E, gxw + ox + 1.0, gyw + oy + 1.0, gzw + oz + 1.0, E.center + 3 * pp, E.cita[pp],
+                                nv);
                         nx = R(nv[0]);
                         ny = R(nv[1]);
                         nz = R(nv[2]);

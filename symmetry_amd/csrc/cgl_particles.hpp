@@ -32,6 +32,11 @@ struct Part {
     double fg;                   // repulsion scale Fg (S:84)
     double rep_plane[6];         // repulsion planes per face (S:1607-1612)
     int rep_on[6];
+    double dom_lo[3], dom_hi[3]; // Lagrange points count inside this box only (S:1832)
+    int dom_lo_on[3], dom_lo_strict[3], dom_hi_on[3];
+    int sym_rules;               // 1: the reference's symmetry-plane rules in movement (S:1272-1335)
+    int pad2;
+    double *center0;             // [npar][3] centres at set-up (the rules' plane test)
     double *center, *center_old, *center_prev; // [npar][3]: now, last step, the step before
     double *up, *rp;             // velocity, angular velocity
     double *fp, *tp, *frep, *ten, *tent; // momentum-exchange force/torque, repulsion, capillary force/torque
@@ -76,21 +81,26 @@ __device__ __forceinline__ bool box_node(const Lat &L, const Box &b, unsigned k,
     n = ((unsigned)g[0] * L.ny + g[1]) * L.nz + g[2];
     return true;
 }
+__device__ __forceinline__ bool bounded(const Ext &E, int a)
+{
+    return E.wall[a][0] || E.wall[a][1] || E.mirror[a][0] || E.mirror[a][1];
+}
 __device__ __forceinline__ bool box_in_domain(const Ext &E, const Lat &L, const int X[3])
 {
     const int n3[3] = {L.nx, L.ny, L.nz};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-        if ((E.wall[a][0] || E.wall[a][1]) && (X[a] < 0 || X[a] >= n3[a])) return false;
+        if (bounded(E, a) && (X[a] < 0 || X[a] >= n3[a])) return false;
     return true;
 }
+// not in a wall or mirror ghost layer
 __device__ __forceinline__ bool interior(const Ext &E, const Lat &L, const int X[3])
 {
     const int n3[3] = {L.nx, L.ny, L.nz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (E.wall[a][0] && X[a] < 1) return false;
-        if (E.wall[a][1] && X[a] > n3[a] - 2) return false;
+        if ((E.wall[a][0] || E.mirror[a][0]) && X[a] < 1) return false;
+        if ((E.wall[a][1] || E.mirror[a][1]) && X[a] > n3[a] - 2) return false;
     }
     return true;
 }
@@ -100,6 +110,109 @@ __device__ __forceinline__ unsigned nb_index(const Lat &L, const int X[3], int d
     return ((unsigned)gx * L.ny + gy) * L.nz + gz;
 }
 __device__ __forceinline__ unsigned box_size(const Box &b) { return (unsigned)b.w[0] * b.w[1] * b.w[2]; }
+
+// The slot (node m, direction d) that the pull gather of live node X reads
+// for direction i: the upstream node X - c_i, or, where that node lies in
+// mirror ghost layers along axes A (and in no wall ghost layer), its mirror
+// image (ghost coordinates replaced by X's own) in direction i with the A
+// components reversed.  That is the pull form of the mirror fills followed by
+// streaming (S:1443-1542, S:743-753); the moving-wall bounce-back writes, and
+// the momentum exchange reads, the value there.
+__device__ __forceinline__ void read_slot(const Ext &E, const Lat &L, const int X[3], int i, unsigned &m, int &d)
+{
+    const int n3[3] = {L.nx, L.ny, L.nz};
+    const int c[3] = {CX[i], CY[i], CZ[i]};
+    int src[3];
+    unsigned A = 0;
+    bool wall = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        src[a] = X[a] - c[a];
+        const int g = wrapm(src[a], n3[a]);
+        if ((E.wall[a][0] && g == 0) || (E.wall[a][1] && g == n3[a] - 1)) wall = true;
+        if ((E.mirror[a][0] && g == 0) || (E.mirror[a][1] && g == n3[a] - 1)) A |= 1u << a;
+    }
+    d = i;
+    if (A && !wall) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (A & (1u << a)) src[a] = X[a];
+        d = REFL.d[A][i];
+    }
+    m = nb_index(L, src, 0, 0, 0);
+}
+
+// Mirror ghost nodes of face `blockIdx.y` (axis = y / 2, side = y % 2) as
+// (destination, source) index pairs: the source clamps the coordinates of
+// every mirror axis into the live range, which is what the reference's
+// face-by-face plane copies (x, y, z; high face first) leave on edges and
+// corners.  `A` gets the mirror axes the destination lies on.
+__device__ __forceinline__ bool mirror_pair(const Ext &E, const Lat &L, size_t &dst, size_t &src, unsigned &A)
+{
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int axis = blockIdx.y / 2, side = blockIdx.y % 2;
+    if (!E.mirror[axis][side]) return false;
+    const int n3[3] = {L.nx, L.ny, L.nz};
+    const int u = (axis + 1) % 3, v = (axis + 2) % 3;
+    if (t >= (unsigned)(n3[u] * n3[v])) return false;
+    int cd[3], cs[3];
+    cd[u] = (int)(t / n3[v]);
+    cd[v] = (int)(t % n3[v]);
+    cd[axis] = side ? n3[axis] - 1 : 0;
+    A = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        cs[a] = cd[a];
+        if (E.mirror[a][0] && cs[a] < 1) { cs[a] = 1; A |= 1u << a; }
+        if (E.mirror[a][1] && cs[a] > n3[a] - 2) { cs[a] = n3[a] - 2; A |= 1u << a; }
+    }
+    dst = ((size_t)cd[0] * L.ny + cd[1]) * L.nz + cd[2];
+    src = ((size_t)cs[0] * L.ny + cs[1]) * L.nz + cs[2];
+    return true;
+}
+
+// Masks of the mirror ghost layers: copies of the adjacent layer (the
+// mirror-face copies of set_walls, S:375-393).
+static __global__ __launch_bounds__(256) void k_mirror_masks(Lat L, Ext E, unsigned *nt, int *walls, int *pin,
+                                                             int *pout)
+{
+    size_t d, s;
+    unsigned A;
+    if (!mirror_pair(E, L, d, s, A)) return;
+    nt[d] = nt[s];
+    walls[d] = walls[s];
+    pin[d] = pin[s];
+    pout[d] = pout[s];
+}
+
+// Gradient and unit normal of the particles' band on the mirror ghost layers:
+// the image's values with the components normal to the mirrors reversed
+// (S:589-594), for the capillary kernel of the next step.
+static __global__ __launch_bounds__(256) void k_mirror_der(Lat L, Ext E, double *der)
+{
+    size_t d, s;
+    unsigned A;
+    if (!mirror_pair(E, L, d, s, A)) return;
+    const size_t N = (size_t)L.N;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) der[k * N + d] = (A & (1u << (k % 3))) ? -der[k * N + s] : der[k * N + s];
+}
+
+// Macroscopic fields of the particles' band on the mirror ghost layers:
+// densities copied, velocity with the components normal to the mirrors
+// reversed (the mirror block of calmacros and refilling, S:809-828,
+// S:1397-1416).
+static __global__ __launch_bounds__(256) void k_mirror_mac(Lat L, Ext E, double *mac)
+{
+    size_t d, s;
+    unsigned A;
+    if (!mirror_pair(E, L, d, s, A)) return;
+    const size_t N = (size_t)L.N;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mac[k * N + d] = mac[k * N + s];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mac[(3 + k) * N + d] = (A & (1u << k)) ? -mac[(3 + k) * N + s] : mac[(3 + k) * N + s];
+}
 
 // Rigid-body velocity at lattice node X (1-based position X+1) of particle
 // p, from centre c (S:1009-1014).
@@ -256,7 +369,8 @@ __global__ __launch_bounds__(PB) void k_ghost_phi(Lat L, Ext E, Part Pt, const u
 // Moving-wall half-way bounce-back on links from live fluid x into its
 // particle (S:1004-1018):  r_i(x) = r*_opp(x) + 6 w_i rho_R(x) (u_w . c_i).
 // The pull gather of the next step reads direction i of x from the solid
-// node x - c_i, so the value is written into that solid node's slot.
+// node x - c_i (or, across a mirror face, from that node's image), so the
+// value is written into that slot.
 template <class S, bool SHIFT>
 __global__ __launch_bounds__(PB) void k_curve(Lat L, Ext E, Part Pt, const unsigned *nt, const int *walls,
                                               const int *pout, S *ro, S *bo, const float *phis)
@@ -276,6 +390,9 @@ __global__ __launch_bounds__(PB) void k_curve(Lat L, Ext E, Part Pt, const unsig
         for (int i = 1; i < Q; ++i) {
             const unsigned m = nb_index(L, X, -CX[i], -CY[i], -CZ[i]);
             if (walls[m] != p + 1) continue;
+            unsigned ms;
+            int ds;
+            read_slot(E, L, X, i, ms, ds);
             const double star = uw[0] * CX[i] + uw[1] * CY[i] + uw[2] * CZ[i];
             const int o = OPC[i];
             double vr = double(StoreOf<S>::ld(ro + (size_t)o * N, n));
@@ -291,8 +408,8 @@ __global__ __launch_bounds__(PB) void k_curve(Lat L, Ext E, Part Pt, const unsig
                 vr = vr + 6.0 * W[i] * rr * star;
                 vb = vb + 6.0 * W[i] * rbv * star;
             }
-            StoreOf<S>::st(ro + (size_t)i * N, m, (decltype(StoreOf<S>::ld(ro)))vr);
-            StoreOf<S>::st(bo + (size_t)i * N, m, (decltype(StoreOf<S>::ld(bo)))vb);
+            StoreOf<S>::st(ro + (size_t)ds * N, ms, (decltype(StoreOf<S>::ld(ro)))vr);
+            StoreOf<S>::st(bo + (size_t)ds * N, ms, (decltype(StoreOf<S>::ld(bo)))vb);
         }
     }
 }
@@ -314,11 +431,11 @@ template <int K> __device__ __forceinline__ void block_sum(double v[K], double *
 
 // ---------------------------------------------------------- momentum exchange
 // Galilean-invariant momentum exchange on the links of particle p
-// (S:1198-1252): f_i(x) is the bounced-back population (held in the solid
-// slot), ffi_opp(x) the node's post-collision one.  On the first step after
-// the start-up (cold != 0) there is no previous collision: the reference's
-// initialization sets ffi = fi (S:894), the node's own populations, which the
-// pull layout holds at x + c_i.
+// (S:1198-1252): f_i(x) is the bounced-back population (held in the slot the
+// gather of x reads), ffi_opp(x) the node's post-collision one.  On the first
+// step after the start-up (cold != 0) there is no previous collision: the
+// reference's initialization sets ffi = fi (S:894), the node's own
+// populations, which the pull layout holds where x gathers them.
 template <class S, bool SHIFT>
 __global__ __launch_bounds__(PB) void k_momentum(Lat L, Ext E, Part Pt, const unsigned *nt, const int *walls,
                                                  const int *pout, const S *rs, const S *bs, int cold)
@@ -339,9 +456,12 @@ __global__ __launch_bounds__(PB) void k_momentum(Lat L, Ext E, Part Pt, const un
             const unsigned m = nb_index(L, X, -CX[i], -CY[i], -CZ[i]);
             if (walls[m] != p + 1) continue;
             const int o = OPC[i];
-            double fi = double(StoreOf<S>::ld(rs + (size_t)i * N, m)) + double(StoreOf<S>::ld(bs + (size_t)i * N, m));
-            const unsigned no = cold ? nb_index(L, X, CX[i], CY[i], CZ[i]) : n;
-            double fo = double(StoreOf<S>::ld(rs + (size_t)o * N, no)) + double(StoreOf<S>::ld(bs + (size_t)o * N, no));
+            unsigned mi, no = n;
+            int di, dof = o;
+            read_slot(E, L, X, i, mi, di);
+            if (cold) read_slot(E, L, X, o, no, dof);
+            double fi = double(StoreOf<S>::ld(rs + (size_t)di * N, mi)) + double(StoreOf<S>::ld(bs + (size_t)di * N, mi));
+            double fo = double(StoreOf<S>::ld(rs + (size_t)dof * N, no)) + double(StoreOf<S>::ld(bs + (size_t)dof * N, no));
             if constexpr (SHIFT) {
                 fi += W[i] * L.rho0;
                 fo += W[o] * L.rho0;
@@ -401,6 +521,14 @@ __global__ __launch_bounds__(PB) void k_lagrange(Lat L, Ext E, Part Pt, const P 
         Lp[1] = c[1] + R * sin(bb) * sin(aa);
         Lp[2] = c[2] + R * cos(bb);
         const double ds = R * R * sin(bb) * (alpha * alpha);
+        // points outside the in-domain box neither interpolate nor count
+        // (S:1832): lrho = 0, no interface flag
+        bool in = true;
+        for (int a = 0; a < 3; ++a) {
+            if (Pt.dom_lo_on[a] && (Pt.dom_lo_strict[a] ? !(Lp[a] > Pt.dom_lo[a]) : !(Lp[a] >= Pt.dom_lo[a]))) in = false;
+            if (Pt.dom_hi_on[a] && !(Lp[a] < Pt.dom_hi[a])) in = false;
+        }
+        if (!in) continue;
         int c0[3];
         for (int a = 0; a < 3; ++a) c0[a] = (int)Lp[a];
         double wsum = 0.0, lrho = 0.0;
@@ -543,6 +671,53 @@ static __global__ __launch_bounds__(PB) void k_move(Lat L, Ext E, Part P, int wi
         for (int a = 0; a < 3; ++a) P.frep[3 * p + a] = fr[a];
     }
     __syncthreads(); // every repulsion read the old centres
+    if (P.sym_rules) {
+        // the reference's symmetry-plane rules (S:1272-1335): a particle that
+        // started on a mirror plane feels its image, so the in-plane force
+        // components double and the normal ones vanish; particle par/3+1
+        // (1-based) averages its x and z components
+        const double xl = L.nx, yl = L.ny, zl = L.nz;
+        for (int p = threadIdx.x; p < np; p += PB) {
+            double *F = P.fp + 3 * p, *T = P.tp + 3 * p, *Fr = P.frep + 3 * p;
+            double *te = P.ten + 3 * p, *tt = P.tent + 3 * p;
+            const double *c0 = P.center0 + 3 * p;
+            if (fabs(c0[0] - (xl - 0.5)) < (double)0.1f) {
+                F[0] = 0.0; F[1] = 2.0 * F[1]; F[2] = 2.0 * F[2];
+                T[0] = 2.0 * T[0]; T[1] = 0.0; T[2] = 0.0;
+                Fr[0] = 0.0;
+                te[0] = 0.0; te[1] = te[1] * 2.0; te[2] = te[2] * 2.0;
+                tt[0] = 2.0 * tt[0]; tt[1] = 0.0; tt[2] = 0.0;
+            }
+            if (fabs(c0[1] - (yl - 0.5)) < (double)0.1f || fabs(c0[1] - 1.5) < (double)0.1f) {
+                F[0] = 2.0 * F[0]; F[1] = 0.0; F[2] = 2.0 * F[2];
+                T[0] = 0.0; T[1] = 2.0 * T[1]; T[2] = 0.0;
+                Fr[1] = 0.0;
+                te[0] = 2.0 * te[0]; te[1] = 0.0; te[2] = 2.0 * te[2];
+                tt[0] = 0.0; tt[1] = 0.0; tt[2] = 0.0;
+            }
+            if (fabs(c0[2] - (zl - 0.5)) < (double)0.1f) {
+                F[0] = 2.0 * F[0]; F[1] = 2.0 * F[1]; F[2] = 0.0;
+                T[0] = 0.0; T[1] = 0.0; T[2] = 2.0 * T[2];
+                Fr[2] = 0.0;
+                te[0] = te[0] * 2.0; te[1] = te[1] * 2.0; te[2] = 0.0;
+                tt[0] = 0.0; tt[1] = 0.0; tt[2] = 2.0 * tt[2];
+            }
+            if (p == np / 3) {
+                double t = (te[0] + te[2]) / 2.0;
+                te[0] = t;
+                te[2] = t;
+                tt[1] = 0.0;
+                t = (F[0] + F[2]) / 2.0;
+                F[0] = t;
+                F[2] = t;
+                T[1] = 0.0;
+                t = (Fr[0] + Fr[2]) / 2.0;
+                Fr[0] = t;
+                Fr[2] = t;
+            }
+        }
+        __syncthreads();
+    }
     for (int p = threadIdx.x; p < np; p += PB) {
         for (int k = 0; k < 3; ++k) {
             const double ftot = P.fp[3 * p + k] + P.frep[3 * p + k] + P.ten[3 * p + k];
@@ -563,10 +738,19 @@ static __global__ __launch_bounds__(PB) void k_move(Lat L, Ext E, Part P, int wi
 // up with phi of the nearest live node; the source index clamps every ghost
 // coordinate into the live range, which gives that result without ordering
 // the six faces.
-template <class P> __global__ __launch_bounds__(256) void k_wall_phi(Lat L, Ext E, P *phi)
+// With mirror_only, only the mirror faces are copied, clamping mirror axes
+// only: the mirror fills that follow refilling and the ghost phi of live
+// solid nodes (S:1397-1416, S:493-495).
+template <class P> __global__ __launch_bounds__(256) void k_wall_phi(Lat L, Ext E, P *phi, int mirror_only = 0)
 {
     const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     const int axis = blockIdx.y / 2, side = blockIdx.y % 2;
+    if (mirror_only) {
+        size_t d, s;
+        unsigned A;
+        if (mirror_pair(E, L, d, s, A)) phi[d] = phi[s];
+        return;
+    }
     if (!E.wall[axis][side] && !E.mirror[axis][side]) return;
     const int n3[3] = {L.nx, L.ny, L.nz};
     const int u = (axis + 1) % 3, v = (axis + 2) % 3;
@@ -603,36 +787,35 @@ __global__ __launch_bounds__(256) void k_derivs_ext(Lat L, Ext E, const P *__res
     double g[3] = {0, 0, 0}, nv[3] = {0, 0, 0};
     bool fl = false;
     const unsigned nt = E.ntype ? E.ntype[n] : 0u;
-    // a mirror ghost holds its image's gradient and normal, reflected
-    // (S:589-594); wall ghosts hold zeros
-    const unsigned A = is_wall_ghost(E, L, x, y, z) ? 0u : mirror_axes(E, L, x, y, z);
-    if (A) {
+    // a mirror ghost holds its image's gradient and normal (wetting
+    // included), reflected (S:589-594); wall ghosts hold zeros
+    const bool wg = is_wall_ghost(E, L, x, y, z);
+    const unsigned A = wg ? 0u : mirror_axes(E, L, x, y, z);
+    if (!wg && (A || !is_ghost(E, L, x, y, z))) {
         const int xi = (A & 1u) ? (x == 0 ? 1 : x - 1) : x, yi = (A & 2u) ? (y == 0 ? 1 : y - 1) : y,
                   zi = (A & 4u) ? (z == 0 ? 1 : z - 1) : z;
         unsigned ax[3], ay[3], az[3];
         node_offsets(L, xi, yi, zi, ax, ay, az);
-        grad19<double>([&](int dx, int dy, int dz) { return double(phi[ax[1 + dx] + ay[1 + dy] + az[1 + dz]]); },
-                       g[0], g[1], g[2]);
-        const double fnorm = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-        if (fnorm > EVAP_LIM)
-            for (int k = 0; k < 3; ++k) nv[k] = g[k] / fnorm;
-        for (int k = 0; k < 3; ++k)
-            if (A & (1u << k)) {
-                g[k] = -g[k];
-                nv[k] = -nv[k];
+        const unsigned ni = ax[1] + ay[1] + az[1];
+        const unsigned nti = E.ntype ? E.ntype[ni] : 0u;
+        if (!(nti & NT_SOLID)) {
+            grad19<double>([&](int dx, int dy, int dz) { return double(phi[ax[1 + dx] + ay[1 + dy] + az[1 + dz]]); },
+                           g[0], g[1], g[2]);
+            const double fnorm = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+            if (fnorm > EVAP_LIM) {
+                fl = A == 0;
+                for (int k = 0; k < 3; ++k) nv[k] = g[k] / fnorm;
+                if (nti & NT_POUT) {
+                    const int pp = E.pout[ni] - 1;
+                    wetting(L, E, xi + 1.0, yi + 1.0, zi + 1.0, E.center + 3 * pp, E.cita[pp], nv);
+                    for (int k = 0; k < 3; ++k) g[k] = fnorm * nv[k];
+                }
             }
-    } else if (!is_ghost(E, L, x, y, z) && !(nt & NT_SOLID)) {
-        grad19<double>([&](int dx, int dy, int dz) { return double(phi[xo[1 + dx] + yo[1 + dy] + zo[1 + dz]]); },
-                       g[0], g[1], g[2]);
-        const double fnorm = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-        if (fnorm > EVAP_LIM) {
-            fl = true;
-            for (int k = 0; k < 3; ++k) nv[k] = g[k] / fnorm;
-            if (nt & NT_POUT) {
-                const int pp = E.pout[n] - 1;
-                wetting(L, E, x + 1.0, y + 1.0, z + 1.0, E.center + 3 * pp, E.cita[pp], nv);
-                for (int k = 0; k < 3; ++k) g[k] = fnorm * nv[k];
-            }
+            for (int k = 0; k < 3; ++k)
+                if (A & (1u << k)) {
+                    g[k] = -g[k];
+                    nv[k] = -nv[k];
+                }
         }
     }
     if ((nt & NT_NEAR) && E.der)

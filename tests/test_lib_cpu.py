@@ -78,8 +78,8 @@ def test_create_validates_before_touching_the_device(lib, kw, code):
 
 def test_face_validation(lib):
     h = ctypes.c_void_p()
-    p = _params(npar=2)
-    p.bc[4] = p.bc[5] = 2  # particles with mirror faces are not implemented on the GPU
+    p = _params(npar=2, layout=1)
+    p.bc[4] = p.bc[5] = 2  # particles (with mirror faces or not) run on the red/blue layout
     assert lib.cgl_create(ctypes.byref(h), ctypes.byref(p)) == -3
     p = _params()
     p.bc[4] = symmetry_amd.BC_WALL  # a periodic face needs a periodic partner
