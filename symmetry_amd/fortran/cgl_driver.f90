@@ -17,7 +17,11 @@
 ! reference's z=1 kind, S:1557-1590).  npar > 0 adds spheres at
 ! (px, py, pz) of radius rad_p, density rho_p and contact angle cita_deg,
 ! with mass and moment of inertia as S:81-82 and Fg as S:84; their state is
-! appended to PARTICLES.dat at every output step.
+! appended to PARTICLES.dat at every output step.  ref_faces selects the
+! reference's own box: walls at x = 1 and z = 1, mirrors at x = xl, y = 1,
+! y = yl and z = zl (S:1443-1592), and for particles its repulsion planes
+! (S:1607-1612), Lagrange in-domain box (S:1832) and symmetry-plane rules of
+! movement (S:1272-1335).
 program cgl_driver
   use iso_c_binding
   use cgl
@@ -26,6 +30,7 @@ program cgl_driver
   double precision :: sigma = 0.01d0, rho0 = 1.0d0, tau_R = 1.0d0, tau_B = 1.0d0, beta = 0.7d0, Rad_c = 8.0d0
   double precision :: amp(16) = 0.0d0
   logical :: ic_on_device = .false., write_tecplot = .true., dump_binary = .false., zwalls = .false.
+  logical :: ref_faces = .false.
   character(len=256) :: outdir = '.'
   character(len=16) :: ic = 'cylinder'
   integer, parameter :: MAXPAR = 64
@@ -34,7 +39,7 @@ program cgl_driver
   double precision :: px(MAXPAR) = 0.0d0, py(MAXPAR) = 0.0d0, pz(MAXPAR) = 0.0d0
   namelist /cgl/ xl, yl, zl, nsteps, out_every, prec, xchunk, sigma, rho0, tau_R, tau_B, beta, Rad_c, &
                  nmodes, amp, ic_on_device, write_tecplot, dump_binary, outdir, layout, ic, z0, zwalls, &
-                 npar, px, py, pz, rad_p, rho_p, cita_deg
+                 npar, px, py, pz, rad_p, rho_p, cita_deg, ref_faces
 
   double precision, allocatable :: ri(:,:,:,:), bi(:,:,:,:)
   double precision, allocatable :: rho(:,:,:), rho_R(:,:,:), rho_B(:,:,:), rho_N(:,:,:), ux(:,:,:), uy(:,:,:), uz(:,:,:)
@@ -77,6 +82,7 @@ program cgl_driver
   p%nx = xl; p%ny = yl; p%nz = zl
   p%bc = CGL_BC_PERIODIC
   if (zwalls) p%bc(5:6) = CGL_BC_WALL
+  if (ref_faces) p%bc = [CGL_BC_WALL, CGL_BC_MIRROR, CGL_BC_MIRROR, CGL_BC_MIRROR, CGL_BC_WALL, CGL_BC_MIRROR]
   p%precision = prec
   p%npar = npar
   p%xchunk = xchunk
@@ -113,6 +119,10 @@ program cgl_driver
     pcita = cita_deg/180.0d0*PI
     fg = -PI*2.0d0*rad_p*sigma                       ! S:84
     call cgl_check(cgl_set_particles(h, center, up, rp, prad, pmass, pinert, pcita, fg), 'cgl_set_particles')
+    if (ref_faces) call cgl_check(cgl_set_particle_geometry(h, &
+        [1.5d0, dble(xl) - 1.5d0, 1.5d0, dble(yl) - 1.5d0, 1.5d0, dble(zl) - 0.5d0], [1, 1, 1, 1, 1, 1], &
+        [0.0d0, 1.5d0, 0.0d0], [dble(xl) - 0.5d0, dble(yl) - 0.5d0, dble(zl) - 0.5d0], &
+        [0, 0, 1, 1, 0, 1, 0, 0, 1], 1_c_int32_t), 'cgl_set_particle_geometry')
     open(newunit=u, file=trim(outdir)//'/PARTICLES.dat', status='replace')
     write(u, '(a)') 'VARIABLES=T,P,X,Y,Z,U,V,W,FX,FY,FZ,FCX,FCY,FCZ'
     close(u)

@@ -74,6 +74,14 @@ module cgl
       real(c_double), intent(in) :: center(*), up(*), rp(*), rad(*), mass(*), inertia(*), cita(*)
       real(c_double), value :: fg
     end function
+    integer(c_int) function cgl_set_particle_geometry(h, rep_plane, rep_on, dom_lo, dom_hi, dom_on, sym_rules) &
+        bind(C, name="cgl_set_particle_geometry")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: rep_plane(6), dom_lo(3), dom_hi(3)
+      integer(c_int32_t), intent(in) :: rep_on(6), dom_on(9)
+      integer(c_int32_t), value :: sym_rules
+    end function
     integer(c_int) function cgl_get_particles(h, center, up, rp, force, torque, fcap, tcap, frep) &
         bind(C, name="cgl_get_particles")
       import :: c_int, c_ptr, c_double

@@ -85,3 +85,44 @@ def test_fortran_driver_particle_at_interface(tmp_path):
     lines = (tmp_path / f"OUT_{steps}.dat").read_text().splitlines()
     wall_col = np.array([int(l.split()[3]) for l in lines[3:]]).reshape(n, n, n)  # (z, y, x), x fastest
     assert np.array_equal(wall_col.transpose(2, 1, 0), masks["walls_all"])
+
+
+def test_fortran_driver_reference_faces_with_particles(tmp_path):
+    """ref_faces: the reference's own box (walls at x = 1 and z = 1, mirrors at
+    x = xl, y = 1, y = yl, z = zl) with particles on the mirror planes, its
+    repulsion planes, Lagrange box and symmetry-plane rules; the Fortran
+    driver equals the same run through the ctypes binding bit for bit."""
+    import symmetry_amd
+    n, steps, rad = 32, 6, 5.0
+    cs = [(n - 0.5, 15.8, 16.4), (14.3, 1.5, 17.2)]
+    nml = tmp_path / "in.nml"
+    nml.write_text(f"&cgl\n xl={n}, yl={n}, zl={n}, nsteps={steps}, out_every={steps}, prec=0,\n"
+                   f" sigma=0.005d0, rho0=1.0d0, tau_R=0.8d0, tau_B=0.8d0, beta=0.7d0,\n"
+                   f" ic='interface', z0=16.0d0, ref_faces=.true., npar=2,\n"
+                   f" px(1)={cs[0][0]}d0, py(1)={cs[0][1]}d0, pz(1)={cs[0][2]}d0,\n"
+                   f" px(2)={cs[1][0]}d0, py(2)={cs[1][1]}d0, pz(2)={cs[1][2]}d0, rad_p={rad}d0, rho_p=1.5d0,\n"
+                   f" cita_deg=90.0d0, dump_binary=.true., write_tecplot=.false., outdir='{tmp_path}'\n/\n")
+    print("driver reference faces", n, steps, flush=True)
+    r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    raw = np.fromfile(tmp_path / f"FIELDS_{steps}.bin", dtype=np.float64).reshape(7, n, n, n)
+    ri, bi, _ = ic.flat_interface_state(n, n, n, z0=16.0, rho0=1.0, beta=0.7, exact=True)
+    parts = ic.sphere_particles(cs, rad, 1.5, 90.0, 0.005)
+    W, M = symmetry_amd.BC_WALL, symmetry_amd.BC_MIRROR
+    lat = symmetry_amd.Lattice(n, n, n, precision="f64", sigma=0.005, rho0=1.0, nu_r=(0.8 - 0.5) / 3.0,
+                               nu_b=(0.8 - 0.5) / 3.0, beta=0.7, bc=(W, M, M, M, W, M), npar=2)
+    lat.upload(ri, bi)
+    lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"], parts["fg"])
+    lat.set_particle_geometry((1.5, n - 1.5, 1.5, n - 1.5, 1.5, n - 0.5), (1,) * 6, (0.0, 1.5, 0.0),
+                              (n - 0.5, n - 0.5, n - 0.5), (0, 0, 1, 1, 0, 1, 0, 0, 1), 1)
+    lat.step(steps)
+    mac = lat.calmacros()
+    for k, name in enumerate(("rho", "rho_r", "rho_b", "ux", "uy", "uz", "phi")):
+        assert np.array_equal(raw[k], mac[name]), name
+    got = lat.get_particles()
+    rows = [l.split() for l in (tmp_path / "PARTICLES.dat").read_text().splitlines()[1:]]
+    last = [r_ for r_ in rows if int(r_[0]) == steps]
+    for p, row in enumerate(last):
+        vals = [float(v) for v in row[2:]]
+        assert vals[0:3] == list(got["center"][p]) and vals[6:9] == list(got["force"][p])
