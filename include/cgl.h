@@ -76,7 +76,8 @@ int cgl_destroy(cgl_t *h);
 int cgl_upload_state(cgl_t *h, const double *ri, const double *bi);
 
 /* Download the current state in the same layout: equal to the reference's
- * ri, bi after its streaming and curve calls (S:249-251). */
+ * ri, bi after its streaming and curve calls (S:249-251) at live nodes; the
+ * wall and mirror ghost layers are not part of the state and read as 0. */
 int cgl_download_state(cgl_t *h, double *ri, double *bi);
 
 /* Initial condition on the device: liquid cylinder along z through (xc, yc)
@@ -118,6 +119,18 @@ int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning);
  * environment sets CGL_SCATTER=0.  0: a moments pass per step (gather form).
  * No counterpart in the reference. */
 int cgl_phase_field_form(cgl_t *h, int32_t *scatter);
+
+/* Checkpoint and restart (the reference has none).  cgl_checkpoint_save
+ * writes the device state as the handle holds it (storage precision,
+ * layout, both phase-field buffers, particle masks, band fields and particle
+ * state) and the step bookkeeping to the file `path`; cgl_checkpoint_load
+ * restores it into a handle created with the same parameters (box, faces,
+ * precision, layout, npar or batch), after which the run continues bit for
+ * bit as if it had not stopped.  The file is for this library on a host of
+ * the same byte order, not a portable format.  A load that fails after the
+ * header checks leaves the state undefined: upload or load again. */
+int cgl_checkpoint_save(cgl_t *h, const char *path);
+int cgl_checkpoint_load(cgl_t *h, const char *path);
 
 /* Block until all queued device work of the handle has finished. */
 int cgl_sync(cgl_t *h);

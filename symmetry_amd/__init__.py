@@ -57,7 +57,7 @@ EXPORTS = (
     "cgl_set_particles", "cgl_get_particles", "cgl_download_masks",
     "cgl_batch_create", "cgl_box_upload_state", "cgl_box_init_cylinder", "cgl_box_download_state",
     "cgl_box_calmacros", "cgl_diagnostics", "cgl_init_interface", "cgl_phase_field_form",
-    "cgl_set_particle_geometry",
+    "cgl_set_particle_geometry", "cgl_checkpoint_save", "cgl_checkpoint_load",
 )
 
 
@@ -95,6 +95,8 @@ def lib():
         L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.cgl_phase_field_form.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
         L.cgl_set_particle_geometry.argtypes = [vp] + [vp] * 5 + [ctypes.c_int32]
+        L.cgl_checkpoint_save.argtypes = [vp, ctypes.c_char_p]
+        L.cgl_checkpoint_load.argtypes = [vp, ctypes.c_char_p]
         L.cgl_probe_copy_bandwidth.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                                ctypes.POINTER(ctypes.c_double)]
         _lib = L
@@ -248,6 +250,13 @@ class Lattice:
         _check(lib().cgl_phase_field_form(self.h, ctypes.byref(v)))
         return bool(v.value)
 
+    def save(self, path):
+        """Checkpoint of the device state (resumes bit for bit)."""
+        _check(lib().cgl_checkpoint_save(self.h, os.fsencode(path)))
+
+    def load(self, path):
+        _check(lib().cgl_checkpoint_load(self.h, os.fsencode(path)))
+
     def device_name(self):
         buf = ctypes.create_string_buffer(256)
         _check(lib().cgl_device_name(self.h, buf, 256))
@@ -306,6 +315,12 @@ class Batch:
         out = {k: np.empty(self.shapes[b]) for k in names}
         _check(lib().cgl_box_calmacros(self.h, b, *(_ptr(out[k]) for k in names)))
         return out
+
+    def save(self, path):
+        _check(lib().cgl_checkpoint_save(self.h, os.fsencode(path)))
+
+    def load(self, path):
+        _check(lib().cgl_checkpoint_load(self.h, os.fsencode(path)))
 
     def step(self, n=1):
         _check(lib().cgl_step(self.h, int(n)))

@@ -1032,6 +1032,183 @@ int cgl_box_calmacros(cgl_t *h, int32_t box, double *rho, double *rho_r, double 
     return cgl_calmacros(&v, rho, rho_r, rho_b, ux, uy, uz, phi);
 }
 
+// ------------------------------------------------------------ checkpoints
+// A checkpoint holds the device state as the handle keeps it (storage
+// precision and layout, shifted values, both phi buffers, masks, the
+// particles' band fields and state) plus the step bookkeeping, so a handle
+// of the same configuration resumes bit for bit.  Little-endian host order,
+// a fixed header, then the sections in a fixed order.
+
+struct CkptHeader {
+    char magic[8];
+    int32_t version;
+    int32_t nx, ny, nz, precision, layout, npar, nbox;
+    int32_t bc[6];
+    int32_t cur, phicur, state_layout, ntcur, phi_ok, primed, cold, xc, tuning, tune_idx, tune_best_xc;
+    float tune_best_ms;
+    int64_t N;
+    int64_t steps_done;
+    double sigma, rho0, nu_r, nu_b, beta;
+};
+static const char CKPT_MAGIC[8] = {'C', 'G', 'L', 'C', 'K', 'P', 'T', '1'};
+
+// the particles' host-side scalars, written as one block
+struct CkptPart {
+    double fg, rep_plane[6], dom_lo[3], dom_hi[3];
+    int32_t rep_on[6], dom_lo_on[3], dom_lo_strict[3], dom_hi_on[3], sym_rules;
+};
+
+// device buffers of a checkpoint, in file order
+static std::vector<std::pair<void *, size_t>> ckpt_sections(cgl_handle *h)
+{
+    std::vector<std::pair<void *, size_t>> v;
+    const size_t N = h->N, pops = (size_t)cgl::Q * N * h->esize;
+    if (h->state_layout == cgl::RB) {
+        v.push_back({h->pops[h->cur][0], pops});
+        v.push_back({h->pops[h->cur][1], pops});
+    } else {
+        const size_t ns = h->p.precision == CGL_PREC_F64 ? 6 * sizeof(double) : 4 * sizeof(float);
+        v.push_back({h->fsf[h->cur], pops});
+        v.push_back({h->fss[h->cur], N * ns});
+    }
+    v.push_back({h->phi[0], N * h->phisize});
+    v.push_back({h->phi[1], N * h->phisize});
+    if (h->npar) {
+        v.push_back({h->ntype[0], N * sizeof(unsigned)});
+        v.push_back({h->ntype[1], N * sizeof(unsigned)});
+        v.push_back({h->walls_i, N * sizeof(int)});
+        v.push_back({h->pin_i, N * sizeof(int)});
+        v.push_back({h->pout_i, N * sizeof(int)});
+        v.push_back({h->E.mac, 6 * N * sizeof(double)});
+        v.push_back({h->E.der, 6 * N * sizeof(double)});
+        const size_t n3 = 3 * (size_t)h->npar * sizeof(double), n1 = (size_t)h->npar * sizeof(double);
+        double *vec[11] = {h->P.center, h->P.center_old, h->P.center_prev, h->P.center0, h->P.up, h->P.rp,
+                           h->P.fp, h->P.tp, h->P.frep, h->P.ten, h->P.tent};
+        for (double *q : vec) v.push_back({q, n3});
+        double *sc[5] = {h->P.rad, h->P.mass, h->P.inertia, h->P.cita, h->P.shb};
+        for (double *q : sc) v.push_back({q, n1});
+    }
+    return v;
+}
+
+static const size_t CKPT_CHUNK = 64u << 20;
+
+int cgl_checkpoint_save(cgl_t *h, const char *path)
+{
+    if (!h || !path) return fail(CGL_ERR_ARG, "null argument");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if ((rc = state_readable(h))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    CkptHeader H;
+    memset(&H, 0, sizeof H);
+    memcpy(H.magic, CKPT_MAGIC, 8);
+    H.version = 1;
+    H.nx = h->p.nx; H.ny = h->p.ny; H.nz = h->p.nz;
+    H.precision = h->p.precision; H.layout = h->p.layout; H.npar = h->npar; H.nbox = h->nbox;
+    for (int f = 0; f < 6; ++f) H.bc[f] = h->p.bc[f];
+    H.cur = h->cur; H.phicur = h->phicur; H.state_layout = h->state_layout; H.ntcur = h->ntcur;
+    H.phi_ok = h->phi_ok; H.primed = h->primed; H.cold = h->cold; H.xc = h->xc; H.tuning = h->tuning;
+    H.tune_idx = h->tune_idx; H.tune_best_xc = h->tune_best_xc; H.tune_best_ms = h->tune_best_ms;
+    H.N = (int64_t)h->N; H.steps_done = h->steps_done;
+    H.sigma = h->p.sigma; H.rho0 = h->p.rho0; H.nu_r = h->p.nu_r; H.nu_b = h->p.nu_b; H.beta = h->p.beta;
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(CGL_ERR_ARG, "cannot open %s for writing", path);
+    bool ok = fwrite(&H, sizeof H, 1, f) == 1;
+    if (ok && h->npar) {
+        CkptPart Pp;
+        memset(&Pp, 0, sizeof Pp);
+        Pp.fg = h->P.fg;
+        for (int k = 0; k < 6; ++k) { Pp.rep_plane[k] = h->P.rep_plane[k]; Pp.rep_on[k] = h->P.rep_on[k]; }
+        for (int a = 0; a < 3; ++a) {
+            Pp.dom_lo[a] = h->P.dom_lo[a]; Pp.dom_hi[a] = h->P.dom_hi[a];
+            Pp.dom_lo_on[a] = h->P.dom_lo_on[a]; Pp.dom_lo_strict[a] = h->P.dom_lo_strict[a];
+            Pp.dom_hi_on[a] = h->P.dom_hi_on[a];
+        }
+        Pp.sym_rules = h->P.sym_rules;
+        ok = fwrite(&Pp, sizeof Pp, 1, f) == 1;
+    }
+    std::vector<char> buf;
+    hipError_t e = hipSuccess;
+    for (auto &sec : ckpt_sections(h)) {
+        for (size_t off = 0; ok && e == hipSuccess && off < sec.second; off += CKPT_CHUNK) {
+            const size_t n = sec.second - off < CKPT_CHUNK ? sec.second - off : CKPT_CHUNK;
+            buf.resize(n);
+            e = hipMemcpy(buf.data(), (char *)sec.first + off, n, hipMemcpyDeviceToHost);
+            if (e == hipSuccess) ok = fwrite(buf.data(), 1, n, f) == n;
+        }
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (e != hipSuccess) return fail(CGL_ERR_HIP, "checkpoint copy failed: %s", hipGetErrorString(e));
+    if (!ok) return fail(CGL_ERR_ARG, "writing %s failed", path);
+    return CGL_OK;
+}
+
+int cgl_checkpoint_load(cgl_t *h, const char *path)
+{
+    if (!h || !path) return fail(CGL_ERR_ARG, "null argument");
+    int rc = set_device(h);
+    if (rc) return rc;
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(CGL_ERR_ARG, "cannot open %s", path);
+    CkptHeader H;
+    if (fread(&H, sizeof H, 1, f) != 1 || memcmp(H.magic, CKPT_MAGIC, 8) != 0 || H.version != 1) {
+        fclose(f);
+        return fail(CGL_ERR_ARG, "%s is not a libcgl checkpoint", path);
+    }
+    bool same = H.nx == h->p.nx && H.ny == h->p.ny && H.nz == h->p.nz && H.precision == h->p.precision &&
+                H.layout == h->p.layout && H.npar == h->npar && H.nbox == h->nbox && H.N == (int64_t)h->N &&
+                H.sigma == h->p.sigma && H.rho0 == h->p.rho0 && H.beta == h->p.beta &&
+                (h->nbox || (H.nu_r == h->p.nu_r && H.nu_b == h->p.nu_b));
+    for (int k = 0; k < 6; ++k) same = same && H.bc[k] == h->p.bc[k];
+    const bool layout_ok = H.state_layout == cgl::RB ? (h->p.layout == CGL_LAYOUT_RB || H.cur == 0)
+                                                     : h->p.layout == CGL_LAYOUT_FS;
+    if (!same || !layout_ok || H.cur < 0 || H.cur > 1 || H.phicur < 0 || H.phicur > 1 || H.ntcur < 0 ||
+        H.ntcur > 1 || H.xc < 1) {
+        fclose(f);
+        return fail(CGL_ERR_ARG, "%s holds a different configuration", path);
+    }
+    CkptPart Pp;
+    if (h->npar && fread(&Pp, sizeof Pp, 1, f) != 1) {
+        fclose(f);
+        return fail(CGL_ERR_ARG, "%s is truncated", path);
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->cur = H.cur;
+    h->state_layout = H.state_layout;
+    std::vector<char> buf;
+    bool ok = true;
+    hipError_t e = hipSuccess;
+    for (auto &sec : ckpt_sections(h)) {
+        for (size_t off = 0; ok && e == hipSuccess && off < sec.second; off += CKPT_CHUNK) {
+            const size_t n = sec.second - off < CKPT_CHUNK ? sec.second - off : CKPT_CHUNK;
+            buf.resize(n);
+            ok = fread(buf.data(), 1, n, f) == n;
+            if (ok) e = hipMemcpy((char *)sec.first + off, buf.data(), n, hipMemcpyHostToDevice);
+        }
+    }
+    fclose(f);
+    if (e != hipSuccess) return fail(CGL_ERR_HIP, "checkpoint copy failed: %s", hipGetErrorString(e));
+    if (!ok) {
+        h->phi_ok = 0; // the state is incomplete: take phi from a moments pass
+        return fail(CGL_ERR_ARG, "%s is truncated", path);
+    }
+    h->phicur = H.phicur; h->ntcur = H.ntcur; h->phi_ok = H.phi_ok; h->primed = H.primed; h->cold = H.cold;
+    h->xc = H.xc; h->tuning = H.tuning; h->tune_idx = H.tune_idx; h->tune_best_xc = H.tune_best_xc;
+    h->tune_best_ms = H.tune_best_ms; h->steps_done = (int)H.steps_done; h->scal_stale = 0;
+    if (h->npar) {
+        h->P.fg = Pp.fg;
+        for (int k = 0; k < 6; ++k) { h->P.rep_plane[k] = Pp.rep_plane[k]; h->P.rep_on[k] = Pp.rep_on[k]; }
+        for (int a = 0; a < 3; ++a) {
+            h->P.dom_lo[a] = Pp.dom_lo[a]; h->P.dom_hi[a] = Pp.dom_hi[a];
+            h->P.dom_lo_on[a] = Pp.dom_lo_on[a]; h->P.dom_lo_strict[a] = Pp.dom_lo_strict[a];
+            h->P.dom_hi_on[a] = Pp.dom_hi_on[a];
+        }
+        h->P.sym_rules = Pp.sym_rules;
+    }
+    return drop_graph(h);
+}
+
 int cgl_device_name(cgl_t *h, char *buf, int32_t len)
 {
     if (!h || !buf || len < 1) return fail(CGL_ERR_ARG, "bad arguments");

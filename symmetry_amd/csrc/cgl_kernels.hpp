@@ -1232,14 +1232,16 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
             if (nt & NT_SOLID) {
                 // solid nodes are not collided: their streamed populations pass
                 // through to the next streaming (S:622, S:712, S:743)
+                // (pulled as the live nodes pull: bounce-back off walls, the
+                // mirror image across mirror faces)
                 const size_t N = (size_t)L.N;
                 const S *rs = (const S *)i0, *bs = (const S *)i1;
                 S *ro = (S *)o0, *bo = (S *)o1;
-#pragma unroll
-                for (int i = 0; i < Q; ++i) {
-                    const bool w = (bb >> i) & 1u;
-                    const unsigned m = w ? n : xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-                    const int dsrc = w ? OPC[i] : i;
+                static_for<Q>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    unsigned m;
+                    int dsrc;
+                    pull_slot<i>(xo, yo, zo, bb, madj, m, dsrc);
                     R rv = StoreOf<S>::ld(rs + (size_t)dsrc * N, m), bv = StoreOf<S>::ld(bs + (size_t)dsrc * N, m);
                     if constexpr (SHIFT) {
                         const R dsh = R(W[i]) * R(L.rho0) * (ax - aw[i]);
@@ -1248,7 +1250,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                     }
                     StoreOf<S>::st(ro + (size_t)i * N, n, rv);
                     StoreOf<S>::st(bo + (size_t)i * N, n, bv);
-                }
+                });
                 continue;
             }
         }
@@ -1591,9 +1593,14 @@ __global__ __launch_bounds__(256) void k_unpack(Lat L, const S *__restrict__ soa
     node_offsets(L, x, y, z, xo, yo, zo);
     const int n = xo[1] + yo[1] + zo[1];
     const size_t N = (size_t)L.N;
-    const bool live = !is_ghost(E, L, x, y, z);
-    const unsigned bb = live ? wall_bb_mask(E, L, x, y, z) : 0u;
-    const unsigned madj = live ? mirror_adjacent(E, L, x, y, z) : 0u;
+    if (is_ghost(E, L, x, y, z)) {
+        // ghost layers are not part of the state: no kernel writes their slots
+#pragma unroll
+        for (int i = 0; i < Q; ++i) aos[(size_t)n * Q + i] = 0.0;
+        return;
+    }
+    const unsigned bb = wall_bb_mask(E, L, x, y, z);
+    const unsigned madj = mirror_adjacent(E, L, x, y, z);
     static_for<Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
         unsigned src;

@@ -21,7 +21,9 @@
 ! reference's own box: walls at x = 1 and z = 1, mirrors at x = xl, y = 1,
 ! y = yl and z = zl (S:1443-1592), and for particles its repulsion planes
 ! (S:1607-1612), Lagrange in-domain box (S:1832) and symmetry-plane rules of
-! movement (S:1272-1335).
+! movement (S:1272-1335).  ckpt_every > 0 writes a checkpoint CKPT_<t>.bin
+! every ckpt_every steps; restart = 'file' with restart_step = t resumes from
+! one (after the same set-up, which the checkpoint then overwrites).
 program cgl_driver
   use iso_c_binding
   use cgl
@@ -31,6 +33,9 @@ program cgl_driver
   double precision :: amp(16) = 0.0d0
   logical :: ic_on_device = .false., write_tecplot = .true., dump_binary = .false., zwalls = .false.
   logical :: ref_faces = .false.
+  integer :: ckpt_every = 0, restart_step = 0
+  character(len=256) :: restart = ' '
+  character(len=300) :: ckpt_name
   character(len=256) :: outdir = '.'
   character(len=16) :: ic = 'cylinder'
   integer, parameter :: MAXPAR = 64
@@ -39,7 +44,7 @@ program cgl_driver
   double precision :: px(MAXPAR) = 0.0d0, py(MAXPAR) = 0.0d0, pz(MAXPAR) = 0.0d0
   namelist /cgl/ xl, yl, zl, nsteps, out_every, prec, xchunk, sigma, rho0, tau_R, tau_B, beta, Rad_c, &
                  nmodes, amp, ic_on_device, write_tecplot, dump_binary, outdir, layout, ic, z0, zwalls, &
-                 npar, px, py, pz, rad_p, rho_p, cita_deg, ref_faces
+                 npar, px, py, pz, rad_p, rho_p, cita_deg, ref_faces, ckpt_every, restart, restart_step
 
   double precision, allocatable :: ri(:,:,:,:), bi(:,:,:,:)
   double precision, allocatable :: rho(:,:,:), rho_R(:,:,:), rho_B(:,:,:), rho_N(:,:,:), ux(:,:,:), uy(:,:,:), uz(:,:,:)
@@ -129,7 +134,15 @@ program cgl_driver
   end if
 
   t = 0
+  if (len_trim(restart) > 0) then
+    call cgl_check(cgl_checkpoint_load(h, trim(restart)//c_null_char), 'cgl_checkpoint_load')
+    t = restart_step
+  end if
   do
+    if (ckpt_every > 0 .and. t > restart_step .and. mod(t, max(ckpt_every, 1)) == 0) then
+      write(ckpt_name, '(a, "/CKPT_", i0, ".bin")') trim(outdir), t
+      call cgl_check(cgl_checkpoint_save(h, trim(ckpt_name)//c_null_char), 'cgl_checkpoint_save')
+    end if
     if (mod(t, out_every) == 0 .or. t == nsteps) then
       call cgl_check(cgl_calmacros(h, rho, rho_R, rho_B, ux, uy, uz, rho_N), 'cgl_calmacros')
       write(*,'(i8, 3(1x, es22.15))') t, sum(rho_R), sum(rho_B), maxval(sqrt(ux**2 + uy**2 + uz**2))
@@ -139,6 +152,7 @@ program cgl_driver
     end if
     if (t >= nsteps) exit
     nchunk = min(out_every - mod(t, out_every), nsteps - t)
+    if (ckpt_every > 0) nchunk = min(nchunk, ckpt_every - mod(t, ckpt_every))
     call cgl_check(cgl_step(h, int(nchunk, c_int32_t)), 'cgl_step')
     t = t + nchunk
   end do

@@ -82,6 +82,16 @@ module cgl
       integer(c_int32_t), intent(in) :: rep_on(6), dom_on(9)
       integer(c_int32_t), value :: sym_rules
     end function
+    integer(c_int) function cgl_checkpoint_save(h, path) bind(C, name="cgl_checkpoint_save")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)   ! NUL-terminated: trim(name)//c_null_char
+    end function
+    integer(c_int) function cgl_checkpoint_load(h, path) bind(C, name="cgl_checkpoint_load")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+    end function
     integer(c_int) function cgl_get_particles(h, center, up, rp, force, torque, fcap, tcap, frep) &
         bind(C, name="cgl_get_particles")
       import :: c_int, c_ptr, c_double

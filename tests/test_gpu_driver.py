@@ -126,3 +126,31 @@ def test_fortran_driver_reference_faces_with_particles(tmp_path):
     for p, row in enumerate(last):
         vals = [float(v) for v in row[2:]]
         assert vals[0:3] == list(got["center"][p]) and vals[6:9] == list(got["force"][p])
+
+
+def test_fortran_driver_checkpoint_restart(tmp_path):
+    """ckpt_every writes CKPT_<t>.bin; a second driver run restarted from
+    CKPT_4.bin (restart_step = 4) ends on the same fields and particle state
+    bit for bit."""
+    n, steps = 24, 10
+    common = (f" xl={n}, yl={n}, zl={n}, nsteps={steps}, out_every=5, prec=1,\n"
+              f" sigma=0.005d0, rho0=1.0d0, tau_R=0.8d0, tau_B=0.8d0, beta=0.7d0,\n"
+              f" ic='interface', z0=12.0d0, zwalls=.true., npar=1, px(1)=12.3d0, py(1)=11.8d0, pz(1)=12.4d0,\n"
+              f" rad_p=4.0d0, rho_p=1.5d0, cita_deg=90.0d0, dump_binary=.true., write_tecplot=.false.,\n")
+    runs = {}
+    for tag, extra in (("a", " ckpt_every=4,"), ("b", f" restart='{tmp_path}/a/CKPT_4.bin', restart_step=4,")):
+        d = tmp_path / tag
+        d.mkdir()
+        nml = d / "in.nml"
+        nml.write_text("&cgl\n" + common + extra + f" outdir='{d}'\n/\n")
+        print("driver checkpoint run", tag, flush=True)
+        r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=120)
+        print(r.stdout[-1500:], r.stderr[-1500:])
+        assert r.returncode == 0
+        runs[tag] = d
+    assert (runs["a"] / "CKPT_4.bin").exists() and (runs["a"] / "CKPT_8.bin").exists()
+    fa = np.fromfile(runs["a"] / f"FIELDS_{steps}.bin", dtype=np.float64)
+    fb = np.fromfile(runs["b"] / f"FIELDS_{steps}.bin", dtype=np.float64)
+    assert np.array_equal(fa, fb)
+    last = lambda d: [l for l in (d / "PARTICLES.dat").read_text().splitlines()[1:] if int(l.split()[0]) == steps]
+    assert last(runs["a"]) == last(runs["b"])
