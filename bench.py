@@ -111,7 +111,7 @@ def run_ensemble(steps, seq_steps):
     batch_mlups = nodes * steps / t_batch / 1e6
     seq_mlups = nodes * seq_steps / t_seq / 1e6
     return {"workload": "64 periodic Rayleigh-Plateau boxes, nx=ny in {32,48,64,96}, nz in 64..512, seed 7",
-            "boxes": len(boxes), "nodes": nodes, "batched": "f32 factorised, one launch per kernel, graph replay",
+            "boxes": len(boxes), "nodes": nodes, "batched": "f32 factorised, scatter-form phase field, one launch per kernel, graph replay",
             "steps": steps, "mlups": round(batch_mlups, 1), "sequential_f64_steps": seq_steps,
             "sequential_f64_mlups": round(seq_mlups, 1), "ratio": round(batch_mlups / seq_mlups, 2)}
 

@@ -114,9 +114,10 @@ int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning);
 
 /* 1 when the handle's steps compute the next phase field in scatter form
  * (calmacros' sums S:792-799 accumulated by the writers of the populations),
- * so that a step reads the state once: periodic FP32/FP16 factorised single
- * boxes with ny a multiple of 8 and nz a multiple of 64, unless the
- * environment sets CGL_SCATTER=0.  0: a moments pass per step (gather form).
+ * so that a step reads the state once: periodic FP32/FP16 factorised boxes,
+ * single or batched, with ny % 8 != 1 and nz % 64 != 1 (every workgroup tile
+ * has two rows and two lanes), unless the environment sets CGL_SCATTER=0 at
+ * creation.  0: a moments pass per step (gather form).
  * No counterpart in the reference. */
 int cgl_phase_field_form(cgl_t *h, int32_t *scatter);
 

@@ -316,6 +316,11 @@ class Batch:
         _check(lib().cgl_box_calmacros(self.h, b, *(_ptr(out[k]) for k in names)))
         return out
 
+    def scatter_form(self):
+        v = ctypes.c_int32()
+        _check(lib().cgl_phase_field_form(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
     def save(self, path):
         _check(lib().cgl_checkpoint_save(self.h, os.fsencode(path)))
 

@@ -69,13 +69,14 @@ int set_device(cgl_handle *h)
 void enqueue_step(cgl_handle *h, bool last = true)
 {
     dispatch(h, [&](auto impl) {
-        if (h->nbox) {
+        if (h->sc && h->state_layout == cgl::FS) {
+            decltype(impl)::step_sc(h, last);
+        } else if (h->nbox) {
             decltype(impl)::batch_moments(h);
             decltype(impl)::batch_collide(h);
+            h->phi_ok = 0;
         } else if (h->ext) {
             decltype(impl)::step_ext(h);
-        } else if (h->sc && h->state_layout == cgl::FS) {
-            decltype(impl)::step_sc(h, last);
         } else {
             decltype(impl)::moments(h, false);
             decltype(impl)::collide(h);
@@ -114,15 +115,20 @@ int state_readable(const cgl_handle *h)
     return CGL_OK;
 }
 
-// Fixed-point partial sums of the scatter step, for single periodic FP32/FP16
-// factorised boxes whose y and z extents are whole workgroup boxes.
+// Fixed-point partial sums of the scatter step, for periodic FP32/FP16
+// factorised boxes (single, or every box of a batch) whose workgroup boxes
+// have at least two rows and two lanes.
 int setup_scatter(cgl_handle *h)
 {
     const char *env = getenv("CGL_SCATTER"); // 0: gather form (moments pass) on every step
     const bool want = env ? atoi(env) != 0 : true;
-    if (!want || h->ext || h->nbox || h->p.precision == CGL_PREC_F64 || h->p.layout != CGL_LAYOUT_FS ||
-        !cgl::sc_shape_ok(h->p.ny, h->p.nz))
+    if (!want || h->ext || h->p.precision == CGL_PREC_F64 || h->p.layout != CGL_LAYOUT_FS) return CGL_OK;
+    if (h->nbox) {
+        for (int b = 0; b < h->nbox; ++b)
+            if (!cgl::sc_shape_ok(h->hboxes[b].L.ny, h->hboxes[b].L.nz)) return CGL_OK;
+    } else if (!cgl::sc_shape_ok(h->p.ny, h->p.nz)) {
         return CGL_OK;
+    }
     int2 **bufs[cgl::SC_NBUF] = {&h->sb.own,   &h->sb.fxlo,  &h->sb.fxhi,  &h->sb.fy,    &h->sb.fz,
                                  &h->sb.exylo, &h->sb.exyhi, &h->sb.exzlo, &h->sb.exzhi, &h->sb.eyz};
     for (int k = 0; k < cgl::SC_NBUF; ++k) HIPCHK(hipMalloc((void **)bufs[k], h->N * sizeof(int2)));
@@ -947,7 +953,11 @@ int cgl_batch_create(cgl_t **out, const cgl_params *p, int32_t nbox, const cgl_b
         *out = nullptr;
         return rc;
     }
-    return CGL_OK;
+    if ((rc = setup_scatter(h))) {
+        cgl_destroy(h);
+        *out = nullptr;
+    }
+    return rc;
 }
 
 // A single-box handle that aliases box `b` of a batch: same buffers, offset
@@ -990,6 +1000,7 @@ static void box_back(cgl_handle *h, const cgl_handle *v)
     h->cur = v->cur;
     h->phicur = v->phicur;
     h->state_layout = v->state_layout;
+    h->phi_ok = v->phi_ok; // an upload into one box invalidates the batch's carried phi
 }
 
 int cgl_box_upload_state(cgl_t *h, int32_t box, const double *ri, const double *bi)

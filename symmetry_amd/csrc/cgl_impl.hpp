@@ -196,7 +196,11 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     // has produced it
     static void sc_pre(cgl_handle *h)
     {
-        if (!h->phi_ok) moments(h, false);
+        if (h->phi_ok) return;
+        if (h->nbox)
+            batch_moments(h);
+        else
+            moments(h, false);
     }
     static void sc_collide(cgl_handle *h, bool last)
     {
@@ -204,8 +208,14 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             const S *fin = (const S *)h->fsf[h->cur];
             S *fout = (S *)h->fsf[h->cur ^ 1];
             float4 *sout = last ? (float4 *)h->fss[h->cur ^ 1] : nullptr;
-            k_collide_sc<S><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(
-                h->L, fin, fout, sout, (const float *)h->phi[h->phicur ^ 1], h->sb, h->xc);
+            const float *phi = (const float *)h->phi[h->phicur ^ 1];
+            float *phinext = (float *)h->phi[h->phicur];
+            if (h->nbox)
+                k_collide_sc_batch<S><<<h->tiles2, k2_block<R>(), 0, h->stream>>>(
+                    h->dboxes, h->nbox, h->L.xcd_remap, h->xc, fin, fout, sout, phi, phinext, h->sb);
+            else
+                k_collide_sc<S><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(h->L, fin, fout, sout, phi,
+                                                                                         phinext, h->sb, h->xc);
             h->scal_stale = !last;
         }
     }
@@ -213,7 +223,11 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     // step's buffer when the parities flip)
     static void sc_fin(cgl_handle *h)
     {
-        k_phi_sc<<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, h->sb, h->xc, (float *)h->phi[h->phicur]);
+        if (h->nbox)
+            k_phi_sc_batch<<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, K1_XC, h->xc, h->sb,
+                                                                       (float *)h->phi[h->phicur]);
+        else
+            k_phi_sc<<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, h->sb, h->xc, (float *)h->phi[h->phicur]);
         h->phi_ok = 1;
     }
     static void step_sc(cgl_handle *h, bool last)

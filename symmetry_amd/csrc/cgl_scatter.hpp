@@ -54,24 +54,42 @@ __device__ __forceinline__ void add2(int2 &a, int2 b)
 }
 __device__ __forceinline__ int2 shfl2(int2 v, int src) { return make_int2(__shfl(v.x, src), __shfl(v.y, src)); }
 
-// Shapes the scatter step takes: whole workgroup boxes in y and z.
-__host__ __device__ inline bool sc_shape_ok(int ny, int nz) { return ny % KTY<float> == 0 && nz % TZ == 0; }
+// phi from the sums (IEEE division: a one-sided rounding of the colour split
+// is amplified by the dynamics); pure phases give exactly -1 and 1
+__device__ __forceinline__ float phi_of_sums(int2 s)
+{
+    if (s.x == 0) return -1.0f;
+    if (s.y == 0) return 1.0f;
+    return float(s.x - s.y) / float(s.x + s.y);
+}
 
+// Shapes the scatter step takes: every workgroup box has at least two rows
+// and two lanes, so that its low and high faces are different threads.
+__host__ __device__ inline bool sc_shape_ok(int ny, int nz)
+{
+    return ny >= 2 && nz >= 2 && ny % KTY<float> != 1 && nz % TZ != 1;
+}
+
+// The box (z-tile bx, y-tile by, x-chunk bz) of one workgroup.  A tile at
+// the high end of y or z may be partial: its last row / lane is the box face,
+// and the threads beyond it take part in the shuffles and barriers only.
 template <class S>
-__global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *__restrict__ fin, S *__restrict__ fout,
-                                                                float4 *__restrict__ sout,
-                                                                const float *__restrict__ phi, ScBufs B, int xc)
+__device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restrict__ fin, S *__restrict__ fout,
+                                                float4 *__restrict__ sout, const float *__restrict__ phi,
+                                                float *__restrict__ phinext, const ScBufs &B, int xc, unsigned bx,
+                                                unsigned by, unsigned bz)
 {
     using R = float;
     constexpr int KY = KTY<float>;
     __shared__ K2Smem<R, float> sm;
     __shared__ int2 yx[2][3][KY][TZ]; // terms for the row above [0] / below [1], per x offset
-    __shared__ int2 zdx[2][2][KY];    // edge-lane terms for the next row: [above/below][lane 0 / TZ-1][row]
-    unsigned bx, by, bz;
-    tile_of_block(L, bx, by, bz);
+    __shared__ int2 zdx[2][2][KY];    // edge-lane terms for the next row: [above/below][lane 0 / last][row]
+    (void)phinext;
     const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
     const int z0 = bx * TZ, y0 = by * KY, x0 = bz * xc;
     const int x1 = min(x0 + xc, L.nx);
+    const int zlast = min(TZ, L.nz - z0) - 1, ylast = min(KY, L.ny - y0) - 1; // last lane / row in the box
+    const bool active = tz <= zlast && ty <= ylast;
     const int z = z0 + tz, y = y0 + ty;
     const unsigned sx = (unsigned)L.ny * (unsigned)L.nz;
     const size_t N = (size_t)L.N;
@@ -79,22 +97,19 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *_
 
     // phi ring: each thread loads at most two elements of the (KY+4) x (TZ+4)
     // plane region; their in-plane offsets are the same for every plane
-    // (ny >= 8 and nz >= 64, so one conditional wrap suffices)
     constexpr int NPHI = (KY + 4) * (TZ + 4);
     static_assert(NPHI <= 2 * TZ * KY, "two phi elements per thread");
-    auto wrap_small = [](int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); };
     int pj0, pj1 = -1;
     unsigned po0, po1 = 0;
     {
         const int e0 = tid, e1 = tid + TZ * KY;
         const int jy0 = e0 / (TZ + 4), jz0 = e0 - jy0 * (TZ + 4);
         pj0 = e0;
-        po0 = (unsigned)wrap_small(y0 - 2 + jy0, L.ny) * (unsigned)L.nz + (unsigned)wrap_small(z0 - 2 + jz0, L.nz);
+        po0 = (unsigned)wrapm(y0 - 2 + jy0, L.ny) * (unsigned)L.nz + (unsigned)wrapm(z0 - 2 + jz0, L.nz);
         if (e1 < NPHI) {
             const int jy1 = e1 / (TZ + 4), jz1 = e1 - jy1 * (TZ + 4);
             pj1 = e1;
-            po1 = (unsigned)wrap_small(y0 - 2 + jy1, L.ny) * (unsigned)L.nz +
-                  (unsigned)wrap_small(z0 - 2 + jz1, L.nz);
+            po1 = (unsigned)wrapm(y0 - 2 + jy1, L.ny) * (unsigned)L.nz + (unsigned)wrapm(z0 - 2 + jz1, L.nz);
         }
     }
     auto load_phi = [&](int p) {
@@ -153,13 +168,15 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *_
         }
     };
 
-    // where this thread's box-leaving terms go
-    const bool ylo = ty == 0, yhi = ty == KY - 1, yedge = ylo || yhi;
-    const bool zl = tz == 0, zh = tz == TZ - 1, zedge = zl || zh; // lane 0 holds terms for z0 + TZ, lane TZ-1 for z0 - 1
-    const unsigned yrow = (unsigned)(ylo ? wrapm(y0 - 1, L.ny) : wrapm(y0 + KY, L.ny)) * (unsigned)L.nz;
-    const unsigned zcol = (unsigned)(zl ? wrapm(z0 + TZ, L.nz) : wrapm(z0 - 1, L.nz));
-    const unsigned yself = (unsigned)y * (unsigned)L.nz, zself = (unsigned)z;
-    const int left = (tz + TZ - 1) & (TZ - 1), right = (tz + 1) & (TZ - 1);
+    // where this thread's box-leaving terms go (boxes have at least two rows
+    // and two lanes, so the low and high faces are different threads)
+    const bool ylo = ty == 0, yhi = ty == ylast, yedge = active && (ylo || yhi);
+    // lane 0 holds the terms for z0 + zlast + 1, the last lane those for z0 - 1
+    const bool zl = tz == 0, zh = tz == zlast, zedge = active && (zl || zh);
+    const unsigned yrow = (unsigned)(ylo ? wrapm(y0 - 1, L.ny) : wrapm(y0 + ylast + 1, L.ny)) * (unsigned)L.nz;
+    const unsigned zcol = (unsigned)(zl ? wrapm(z0 + zlast + 1, L.nz) : wrapm(z0 - 1, L.nz));
+    const unsigned yself = (unsigned)wrapm(y, L.ny) * (unsigned)L.nz, zself = (unsigned)wrapm(z, L.nz);
+    const int left = zl ? zlast : tz - 1, right = zh ? 0 : (tz + 1) & (TZ - 1);
     const int zs = zl ? 0 : 1;
 
     // sums for destination planes x-1 (m), x (c), x+1 (p): own node column,
@@ -187,126 +204,130 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *_
         normals(x + 1, ngx, ngy, ngz);
         __syncthreads();
 
-        // gradient, normal and flag at the node (S:498-532)
-        const R g2 = gx * gx + gy * gy + gz * gz;
-        const bool flag = g2 > R(EVAP_LIM * EVAP_LIM);
-        const R nx = sm.n[0][slot3(x)][ty + 1][tz + 1];
-        const R ny = sm.n[1][slot3(x)][ty + 1][tz + 1];
-        const R nz = sm.n[2][slot3(x)][ty + 1][tz + 1];
-        // normal derivatives for the curvature (S:635-660)
-        R d[3][3];
-        if (flag) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
-#pragma unroll
-            for (int i = 1; i < Q; ++i) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    R tmp = R(W[i]) * sm.n[k][slot3(x + CX[i])][ty + 1 + CY[i]][tz + 1 + CZ[i]];
-                    if (CX[i] == 1) d[k][0] = d[k][0] + tmp;
-                    if (CX[i] == -1) d[k][0] = d[k][0] - tmp;
-                    if (CY[i] == 1) d[k][1] = d[k][1] + tmp;
-                    if (CY[i] == -1) d[k][1] = d[k][1] - tmp;
-                    if (CZ[i] == 1) d[k][2] = d[k][2] + tmp;
-                    if (CZ[i] == -1) d[k][2] = d[k][2] - tmp;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
-        }
-
-        // streaming: pull gather of the total populations (deviations from w rho0)
-        unsigned xo[3], yo[3], zo[3];
-        node_offsets(L, x, y, z, xo, yo, zo);
-        const unsigned n = xo[1] + yo[1] + zo[1];
-        R f[Q];
-        R df = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-            f[i] = StoreOf<S>::ld(fin + (size_t)i * N, m);
-        }
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            df += f[i];
-            if (CX[i] == 1) jx += f[i];
-            if (CX[i] == -1) jx -= f[i];
-            if (CY[i] == 1) jy += f[i];
-            if (CY[i] == -1) jy -= f[i];
-            if (CZ[i] == 1) jz += f[i];
-            if (CZ[i] == -1) jz -= f[i];
-        }
-        // moments (S:792-799): rho and j from the gathered populations, the
-        // colour split from the scattered phase field
-        const R phin = sm.phi[slot5(x)][ty + 2][tz + 2];
-        Moments<R, true> mo;
-        mo.rho = rho0 + df;
-        const R irho = fdiv(R(1), mo.rho);
-        mo.sr = R(0.5) * mo.rho * (R(1) + phin);
-        mo.sb = mo.rho - mo.sr;
-        mo.ux = jx * irho;
-        mo.uy = jy * irho;
-        mo.uz = jz * irho;
-        mo.phi = phin;
-        collide<R, true>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, flag, d);
-        const Recol<R> rc = recol<R, true>(L, mo, flag);
-        R A = rc.A, g = rc.gam;
-        if (rc.evr) { A = 0; g = 0; }
-        if (rc.evb) { A = 1; g = 0; }
-        const R Gx = g * nx, Gy = g * ny, Gz = g * nz;
-
-        // the next state: post-collision totals, and the recolour scalars on
-        // a run's last step; streamed past the caches
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            S *fp = fout + (size_t)i * N;
-            if constexpr (sizeof(S) == 4)
-                __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, n));
-            else
-                StoreOf<S>::st(fp, n, f[i]);
-        }
-        if (sout) {
-            typedef float f4v __attribute__((ext_vector_type(4)));
-            const f4v v4 = {float(A), float(Gx), float(Gy), float(Gz)};
-            __builtin_nontemporal_store(v4, (f4v *)&at(sout, n));
-        }
-
         // terms (rho_R, rho_B) for the downstream nodes, one direction per
         // destination offset (dx, dy, dz): g00 (dx,0,0), gzp/gzm (dx,0,+-1),
-        // gyp/gym (dx,+-1,0), dpp/dpm/dmp/dmm (0,+-1,+-1).  In fixed-point
-        // units: f_i = f~_i qs + w_i 2^29 and r_i = A f_i + (w_i qs G).c_i,
-        // rounded to the nearest integer (a one-sided rounding would bias
-        // every sum by up to 19 units).
-        int2 g00[3], gzp[3], gzm[3], gyp[3], gym[3], dpp, dpm, dmp, dmm;
-        const R s1 = R(W[1]) * qs, s7 = R(W[7]) * qs;
-        const R G1x = Gx * s1, G1y = Gy * s1, G1z = Gz * s1;
-        const R G7x = Gx * s7, G7y = Gy * s7, G7z = Gz * s7;
+        // gyp/gym (dx,+-1,0), dpp/dpm/dmp/dmm (0,+-1,+-1); zero outside the box
+        int2 g00[3] = {Z2, Z2, Z2}, gzp[3] = {Z2, Z2, Z2}, gzm[3] = {Z2, Z2, Z2}, gyp[3] = {Z2, Z2, Z2},
+             gym[3] = {Z2, Z2, Z2}, dpp = Z2, dpm = Z2, dmp = Z2, dmm = Z2;
+        if (active) {
+            // gradient, normal and flag at the node (S:498-532)
+            const R g2 = gx * gx + gy * gy + gz * gz;
+            const bool flag = g2 > R(EVAP_LIM * EVAP_LIM);
+            const R nx = sm.n[0][slot3(x)][ty + 1][tz + 1];
+            const R ny = sm.n[1][slot3(x)][ty + 1][tz + 1];
+            const R nz = sm.n[2][slot3(x)][ty + 1][tz + 1];
+            // normal derivatives for the curvature (S:635-660)
+            R d[3][3];
+            if (flag) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            R v = f[i];
-            if constexpr (sizeof(S) != 4) v = R(S(v)); // the value the next step reads
-            const R fq = __builtin_fmaf(v, qs, R(W[i] * double(SC_ONE)));
-            R cq = A * fq;
-            if (i > 0 && i <= 6) cq = __builtin_fmaf(A, fq, dot_c<R>(i, G1x, G1y, G1z));
-            if (i > 6) cq = __builtin_fmaf(A, fq, dot_c<R>(i, G7x, G7y, G7z));
-            const int qf = __float2int_rn(fq), qr = __float2int_rn(cq);
-            const int2 c = make_int2(qr, qf - qr);
-            const int k = CX[i] + 1;
-            if (CY[i] == 0 && CZ[i] == 0) g00[k] = c;
-            else if (CY[i] == 0 && CZ[i] == 1) gzp[k] = c;
-            else if (CY[i] == 0) gzm[k] = c;
-            else if (CZ[i] == 0 && CY[i] == 1) gyp[k] = c;
-            else if (CZ[i] == 0) gym[k] = c;
-            else if (CY[i] == 1 && CZ[i] == 1) dpp = c;
-            else if (CY[i] == 1) dpm = c;
-            else if (CZ[i] == 1) dmp = c;
-            else dmm = c;
+                for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
+#pragma unroll
+                for (int i = 1; i < Q; ++i) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        R tmp = R(W[i]) * sm.n[k][slot3(x + CX[i])][ty + 1 + CY[i]][tz + 1 + CZ[i]];
+                        if (CX[i] == 1) d[k][0] = d[k][0] + tmp;
+                        if (CX[i] == -1) d[k][0] = d[k][0] - tmp;
+                        if (CY[i] == 1) d[k][1] = d[k][1] + tmp;
+                        if (CY[i] == -1) d[k][1] = d[k][1] - tmp;
+                        if (CZ[i] == 1) d[k][2] = d[k][2] + tmp;
+                        if (CZ[i] == -1) d[k][2] = d[k][2] - tmp;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
+            }
+
+            // streaming: pull gather of the total populations (deviations from w rho0)
+            unsigned xo[3], yo[3], zo[3];
+            node_offsets(L, x, y, z, xo, yo, zo);
+            const unsigned n = xo[1] + yo[1] + zo[1];
+            R f[Q];
+            R df = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+                f[i] = StoreOf<S>::ld(fin + (size_t)i * N, m);
+            }
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                df += f[i];
+                if (CX[i] == 1) jx += f[i];
+                if (CX[i] == -1) jx -= f[i];
+                if (CY[i] == 1) jy += f[i];
+                if (CY[i] == -1) jy -= f[i];
+                if (CZ[i] == 1) jz += f[i];
+                if (CZ[i] == -1) jz -= f[i];
+            }
+            // moments (S:792-799): rho and j from the gathered populations, the
+            // colour split from the scattered phase field
+            const R phin = sm.phi[slot5(x)][ty + 2][tz + 2];
+            Moments<R, true> mo;
+            mo.rho = rho0 + df;
+            const R irho = fdiv(R(1), mo.rho);
+            mo.sr = R(0.5) * mo.rho * (R(1) + phin);
+            mo.sb = mo.rho - mo.sr;
+            mo.ux = jx * irho;
+            mo.uy = jy * irho;
+            mo.uz = jz * irho;
+            mo.phi = phin;
+            collide<R, true>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, flag, d);
+            const Recol<R> rc = recol<R, true>(L, mo, flag);
+            R A = rc.A, g = rc.gam;
+            if (rc.evr) { A = 0; g = 0; }
+            if (rc.evb) { A = 1; g = 0; }
+            const R Gx = g * nx, Gy = g * ny, Gz = g * nz;
+
+            // the next state: post-collision totals, and the recolour scalars on
+            // a run's last step; streamed past the caches
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                S *fp = fout + (size_t)i * N;
+                if constexpr (sizeof(S) == 4)
+                    __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, n));
+                else
+                    StoreOf<S>::st(fp, n, f[i]);
+            }
+            if (sout) {
+                typedef float f4v __attribute__((ext_vector_type(4)));
+                const f4v v4 = {float(A), float(Gx), float(Gy), float(Gz)};
+                __builtin_nontemporal_store(v4, (f4v *)&at(sout, n));
+            }
+
+            // the terms in fixed-point units: f_i = f~_i qs + w_i 2^29 and
+            // r_i = A f_i + (w_i qs G).c_i, rounded to the nearest integer (a
+            // one-sided rounding would bias every sum by up to 19 units)
+            const R s1 = R(W[1]) * qs, s7 = R(W[7]) * qs;
+            const R G1x = Gx * s1, G1y = Gy * s1, G1z = Gz * s1;
+            const R G7x = Gx * s7, G7y = Gy * s7, G7z = Gz * s7;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                R v = f[i];
+                if constexpr (sizeof(S) != 4) v = R(S(v)); // the value the next step reads
+                const R fq = __builtin_fmaf(v, qs, R(W[i] * double(SC_ONE)));
+                R cq = A * fq;
+                if (i > 0 && i <= 6) cq = __builtin_fmaf(A, fq, dot_c<R>(i, G1x, G1y, G1z));
+                if (i > 6) cq = __builtin_fmaf(A, fq, dot_c<R>(i, G7x, G7y, G7z));
+                const int qf = __float2int_rn(fq), qr = __float2int_rn(cq);
+                const int2 c = make_int2(qr, qf - qr);
+                const int k = CX[i] + 1;
+                if (CY[i] == 0 && CZ[i] == 0) g00[k] = c;
+                else if (CY[i] == 0 && CZ[i] == 1) gzp[k] = c;
+                else if (CY[i] == 0) gzm[k] = c;
+                else if (CZ[i] == 0 && CY[i] == 1) gyp[k] = c;
+                else if (CZ[i] == 0) gym[k] = c;
+                else if (CY[i] == 1 && CZ[i] == 1) dpp = c;
+                else if (CY[i] == 1) dpm = c;
+                else if (CZ[i] == 1) dmp = c;
+                else dmm = c;
+            }
         }
 
-        // z neighbours: lane shuffles.  Lane 0 receives lane TZ-1's +z terms
-        // and lane TZ-1 lane 0's -z terms: those leave the box.
+        // z neighbours: lane shuffles (every lane takes part).  Lane 0 receives
+        // the last lane's +z terms and the last lane lane 0's -z terms: those
+        // leave the box.
         int2 own[3] = {g00[0], g00[1], g00[2]};
         int2 zo3[3] = {Z2, Z2, Z2}, zup = Z2, zdn = Z2;
         {
@@ -345,11 +366,11 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *_
         // y neighbours through LDS; y-z edge terms of the corner lanes go out
         // at once (they have dx = 0 only)
         const unsigned px = (unsigned)x * sx;
-        if (!yhi) {
+        if (active && !yhi) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) yx[0][k][ty][tz] = gyp[k];
         }
-        if (!ylo) {
+        if (active && !ylo) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) yx[1][k][ty][tz] = gym[k];
         }
@@ -360,11 +381,11 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *_
             else at(B.eyz, px + yrow + zcol) = zdn;
         }
         __syncthreads();
-        if (!ylo) {
+        if (active && !ylo) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) add2(own[k], yx[0][k][ty - 1][tz]);
         }
-        if (!yhi) {
+        if (active && !yhi) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) add2(own[k], yx[1][k][ty + 1][tz]);
         }
@@ -388,7 +409,8 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *_
             add2(zp, zo3[2]);
         }
         const unsigned pm = (unsigned)wrapm(x - 1, L.nx) * sx;
-        if (x > x0) {
+        if (!active) {
+        } else if (x > x0) {
             at(B.own, pm + yself + zself) = am;
             if (yedge) at(B.fy, pm + yrow + zself) = ym;
             if (zedge) at(B.fz, pm + yself + zcol) = zm;
@@ -411,6 +433,7 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *_
         zp = Z2;
     }
     // plane x1 - 1 (own box) and plane x1 (the box on the high x side)
+    if (!active) return;
     const unsigned pl = (unsigned)wrapm(x1 - 1, L.nx) * sx, pn = (unsigned)wrapm(x1, L.nx) * sx;
     at(B.own, pl + yself + zself) = am;
     at(B.fxlo, pn + yself + zself) = ac;
@@ -424,19 +447,57 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *_
     }
 }
 
-// Finishing pass: phi(y) = (sR - sB) / (sR + sB) from the node's own-box sums
-// and the partial sums its position in its box calls for.  Pure phases give
-// exactly -1 and 1.
-static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, float *__restrict__ phi)
+template <class S>
+__global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *__restrict__ fin, S *__restrict__ fout,
+                                                                float4 *__restrict__ sout,
+                                                                const float *__restrict__ phi,
+                                                                float *__restrict__ phinext, ScBufs B, int xc)
 {
-    const int z = blockIdx.x * TZ + threadIdx.x, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z;
-    if (z >= L.nz || y >= L.ny) return;
+    unsigned bx, by, bz;
+    tile_of_block(L, bx, by, bz);
+    collide_sc_tile<S>(L, fin, fout, sout, phi, phinext, B, xc, bx, by, bz);
+}
+
+// Partial-sum arrays of box `base` in a packed batch.
+__device__ __forceinline__ ScBufs sc_offset(const ScBufs &B, unsigned long long base)
+{
+    return ScBufs{B.own + base,   B.fxlo + base,  B.fxhi + base,  B.fy + base,    B.fz + base,
+                  B.exylo + base, B.exyhi + base, B.exzlo + base, B.exzhi + base, B.eyz + base};
+}
+
+// Every box of a batch in one launch; a workgroup finds its box by the
+// first-tile table, and the box's arrays start at its packed offset.
+template <class S>
+__global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc_batch(const BoxDev *__restrict__ BX, int nbox, int remap,
+                                                                      int xc, const S *__restrict__ fin,
+                                                                      S *__restrict__ fout,
+                                                                      float4 *__restrict__ sout,
+                                                                      const float *__restrict__ phi,
+                                                                      float *__restrict__ phinext, ScBufs B)
+{
+    const unsigned t = remap ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int b = find_box<true>(BX, nbox, t);
+    const Lat L = BX[b].L;
+    const unsigned long long base = BX[b].base;
+    const unsigned lt = t - BX[b].t2, gz = BX[b].gz, gy = BX[b].gy2;
+    collide_sc_tile<S>(L, fin + (size_t)Q * base, fout + (size_t)Q * base, sout ? sout + base : nullptr,
+                       phi + base, phinext + base, sc_offset(B, base), xc, lt % gz, (lt / gz) % gy,
+                       lt / (gz * gy));
+}
+
+// Finishing pass at node (x, y, z): phi(y) = (sR - sB) / (sR + sB) from the
+// node's own-box sums and the partial sums its position in its workgroup box
+// calls for.
+__device__ __forceinline__ void phi_sc_node(const Lat &L, const ScBufs &B, int xc, float *__restrict__ phi, int x,
+                                            int y, int z)
+{
     const unsigned n = ((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z;
     int2 s = at(B.own, n);
     const int cx0 = (x / xc) * xc;
     const bool xlo = x == cx0, xhi = x == min(cx0 + xc, L.nx) - 1;
     const int yr = y % KTY<float>, zr = z % TZ;
-    const bool fy = yr == 0 || yr == KTY<float> - 1, fz = zr == 0 || zr == TZ - 1;
+    const bool fy = yr == 0 || y == min(y - yr + KTY<float>, L.ny) - 1;
+    const bool fz = zr == 0 || z == min(z - zr + TZ, L.nz) - 1;
     if (xlo) add2(s, at(B.fxlo, n));
     if (xhi) add2(s, at(B.fxhi, n));
     if (fy) add2(s, at(B.fy, n));
@@ -446,14 +507,31 @@ static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, 
     if (fz && xlo) add2(s, at(B.exzlo, n));
     if (fz && xhi) add2(s, at(B.exzhi, n));
     if (fy && fz) add2(s, at(B.eyz, n));
-    float ph;
-    if (s.x == 0)
-        ph = -1.0f;
-    else if (s.y == 0)
-        ph = 1.0f;
-    else
-        ph = float(s.x - s.y) / float(s.x + s.y);
-    at(phi, n) = ph;
+    at(phi, n) = phi_of_sums(s);
+}
+
+static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, float *__restrict__ phi)
+{
+    const int z = blockIdx.x * TZ + threadIdx.x, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z;
+    if (z >= L.nz || y >= L.ny) return;
+    phi_sc_node(L, B, xc, phi, x, y, z);
+}
+
+// The batch's finishing pass over the moments launch's tiles (z tile, y tile,
+// chunk of xs planes) of every box.
+static __global__ __launch_bounds__(256) void k_phi_sc_batch(const BoxDev *__restrict__ BX, int nbox, int xs, int xc,
+                                                             ScBufs B, float *__restrict__ phi)
+{
+    const unsigned t = blockIdx.x;
+    const int b = find_box<false>(BX, nbox, t);
+    const Lat L = BX[b].L;
+    const unsigned long long base = BX[b].base;
+    const unsigned lt = t - BX[b].t1, gz = BX[b].gz, gy = BX[b].gy;
+    const int z = (int)(lt % gz) * TZ + threadIdx.x, y = (int)((lt / gz) % gy) * TY + threadIdx.y;
+    const int xb = (int)(lt / (gz * gy)) * xs;
+    if (z >= L.nz || y >= L.ny) return;
+    const ScBufs Bb = sc_offset(B, base);
+    for (int x = xb; x < min(xb + xs, L.nx); ++x) phi_sc_node(L, Bb, xc, phi + base, x, y, z);
 }
 
 } // namespace cgl

@@ -61,13 +61,16 @@ def test_batch_f64_bit_exact_vs_oracle(layout):
         assert np.array_equal(mac["phi"], st.phi), k
 
 
-def test_batch_f32_equals_single_boxes(monkeypatch):
-    """Batched FP32 boxes equal single boxes stepped in the same (gather) form
-    bit for bit."""
-    _print("batch f32 fs vs single")
+@pytest.mark.parametrize("scatter", ["1", "0"])
+def test_batch_f32_equals_single_boxes(monkeypatch, scatter):
+    """Batched FP32 boxes equal single boxes stepped in the same form bit for
+    bit: the scatter form (its sums do not depend on the tiling) and the
+    gather form."""
+    _print("batch f32 fs vs single, scatter", scatter)
+    monkeypatch.setenv("CGL_SCATTER", scatter)
     bt, cfgs, states = _batch("f32", "fs", xchunk=8)
+    assert bt.scatter_form() == (scatter == "1")
     bt.step(20)
-    monkeypatch.setenv("CGL_SCATTER", "0")  # batches sum phi with a moments pass
     for k, c in enumerate(cfgs):
         lat = symmetry_amd.Lattice(c["nx"], c["ny"], c["nz"], precision="f32", sigma=c["sigma"], rho0=c["rho0"],
                                    nu_r=c["nu_r"], nu_b=c["nu_b"], beta=c["beta"], layout="fs", xchunk=2)

@@ -64,8 +64,9 @@ def test_scatter_form_selection(rp):
     assert _lattice(cfg, "f16").scatter_form()
     assert not _lattice(cfg, scatter=False).scatter_form()
     assert not _lattice(cfg, "f64").scatter_form()
-    odd = dict(cfg, ny=12)
-    assert not _lattice(odd).scatter_form()  # ny not a multiple of the 8-row workgroup tile
+    assert _lattice(dict(cfg, ny=12, nz=96)).scatter_form()  # partial workgroup tiles
+    assert not _lattice(dict(cfg, ny=9)).scatter_form()  # a one-row tile: faces on one thread
+    assert not _lattice(dict(cfg, nz=129)).scatter_form()  # a one-lane tile
 
 
 def test_scatter_matches_gather_after_three_steps(rp):
@@ -107,10 +108,11 @@ def test_scatter_and_gather_against_f64(prec, tol_phi, tol_u):
     assert errs[True][0] <= 2 * errs[False][0] + 1e-7 and errs[True][1] <= 2 * errs[False][1] + 1e-7
 
 
-def test_scatter_independent_of_box_decomposition():
+@pytest.mark.parametrize("ny,nz", [(16, 128), (12, 96)])
+def test_scatter_independent_of_box_decomposition(ny, nz):
     """x chunks 1, 2, 3 (last chunk one plane), 8, 40 (one chunk) and 64 on a
-    40-plane box: bit-identical states."""
-    cfg, ri, bi = small_rp(nx=40, ny=16, nz=128, r0=5.0, nu_b=0.1)
+    40-plane box, with whole and partial y/z tiles: bit-identical states."""
+    cfg, ri, bi = small_rp(nx=40, ny=ny, nz=nz, r0=min(ny, 16) / 3.2, nu_b=0.1)
     ref = None
     for xc in (8, 1, 2, 3, 40, 64):
         _print("decomposition xchunk", xc)
@@ -124,10 +126,12 @@ def test_scatter_independent_of_box_decomposition():
             assert np.array_equal(r, ref[0]) and np.array_equal(b, ref[1]), xc
 
 
-@pytest.mark.parametrize("shape", [(6, 8, 64), (10, 8, 128), (12, 24, 64)])
+@pytest.mark.parametrize("shape", [(6, 8, 64), (10, 8, 128), (12, 24, 64), (8, 12, 96), (6, 10, 130), (5, 6, 40)])
 def test_scatter_single_tile_axes(shape):
     """One workgroup tile along y (ny = 8) and/or z (nz = 64): the box-leaving
-    sums wrap onto the box itself."""
+    sums wrap onto the box itself.  Partial tiles: a 4-row and a 32-lane tile
+    (12 x 96), a 2-row and a 2-lane tile (10 x 130), one partial tile on both
+    axes (6 x 40)."""
     nx, ny, nz = shape
     cfg, ri, bi = small_rp(nx=nx, ny=ny, nz=nz, r0=2.5, nu_b=0.1)
     out = []
