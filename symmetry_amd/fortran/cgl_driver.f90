@@ -48,7 +48,7 @@ program cgl_driver
 
   double precision, allocatable :: ri(:,:,:,:), bi(:,:,:,:)
   double precision, allocatable :: rho(:,:,:), rho_R(:,:,:), rho_B(:,:,:), rho_N(:,:,:), ux(:,:,:), uy(:,:,:), uz(:,:,:)
-  integer(c_int32_t), allocatable, target :: walls_all(:,:,:)
+  integer(c_int32_t), allocatable, target :: walls_all(:,:,:), pout(:,:,:)
   ! particle state in the reference's layout: center(par,3) etc. (S:45-50)
   double precision, allocatable :: center(:,:), up(:,:), rp(:,:), fp(:,:), tp(:,:), fcap(:,:), tcap(:,:), frep(:,:)
   double precision, allocatable :: prad(:), pmass(:), pinert(:), pcita(:)
@@ -98,8 +98,9 @@ program cgl_driver
   call cgl_check(cgl_create(h, p), 'cgl_create')
 
   allocate(rho(zl,yl,xl), rho_R(zl,yl,xl), rho_B(zl,yl,xl), rho_N(zl,yl,xl), ux(zl,yl,xl), uy(zl,yl,xl), uz(zl,yl,xl))
-  allocate(walls_all(zl,yl,xl))
+  allocate(walls_all(zl,yl,xl), pout(zl,yl,xl))
   walls_all = 0
+  pout = 0
   if (ic_on_device .and. trim(ic) == 'cylinder') then
     call cgl_check(cgl_init_cylinder(h, dble(xl)/2.0d0 + 0.5d0, dble(yl)/2.0d0 + 0.5d0, Rad_c, &
                    6.0d0*beta*0.134d0, int(nmodes, c_int32_t), amp), 'cgl_init_cylinder')
@@ -147,6 +148,8 @@ program cgl_driver
       call cgl_check(cgl_calmacros(h, rho, rho_R, rho_B, ux, uy, uz, rho_N), 'cgl_calmacros')
       write(*,'(i8, 3(1x, es22.15))') t, sum(rho_R), sum(rho_B), maxval(sqrt(ux**2 + uy**2 + uz**2))
       if (npar > 0) call output_particles(t)
+      if (npar > 0) call output_result2(t)
+      if (ref_faces) call output3(t)
       if (write_tecplot) call output_results(t)
       if (dump_binary) call dump_fields(t)
     end if
@@ -213,13 +216,77 @@ contains
     integer, intent(in) :: tt
     integer :: k, uo
     call cgl_check(cgl_get_particles(h, center, up, rp, fp, tp, fcap, tcap, frep), 'cgl_get_particles')
-    call cgl_check(cgl_download_masks(h, c_null_ptr, c_loc(walls_all), c_null_ptr, c_null_ptr), 'cgl_download_masks')
+    call cgl_check(cgl_download_masks(h, c_null_ptr, c_loc(walls_all), c_null_ptr, c_loc(pout)), 'cgl_download_masks')
     open(newunit=uo, file=trim(outdir)//'/PARTICLES.dat', status='old', position='append')
     do k = 1, npar
       write(uo, '(i8, 1x, i4, 12(1x, es24.16e3))') tt, k, center(k,:), up(k,:), fp(k,:), fcap(k,:)
     end do
     close(uo)
   end subroutine output_particles
+
+  ! output_result2 (S:1420-1441): one history file P_<p>.dat per particle,
+  ! the reference's columns and format; appended at every output step (the
+  ! reference rewrites the file from its start)
+  subroutine output_result2(tt)
+    integer, intent(in) :: tt
+    integer :: k, uo
+    character(len=50) :: filename
+    double precision :: tr
+    tr = sqrt(Rad_c**3*rho0/sigma)                   ! S:92
+    do k = 1, npar
+      write(filename, *) k
+      if (tt == 0) then
+        open(newunit=uo, file=trim(outdir)//'/P_'//trim(adjustl(filename))//'.dat', status='replace')
+        write(uo, '(a)') ' VARIABLES=t,t*,cx,cy,cz,upx,upy,upz,rpx,rpy,rpz,Fpx,Fpy,Fpz,Tpx,Tpy,Tpz,Fcx,Fcy,Fcz,'// &
+                         'Tcx,Tcy,Tcz,Frex,Frey,Frez'
+      else
+        open(newunit=uo, file=trim(outdir)//'/P_'//trim(adjustl(filename))//'.dat', status='unknown', &
+             position='append')
+      end if
+      write(uo, "(1(1xI6),25(1xF12.6))") tt, tt/tr, center(k,:), up(k,:), rp(k,:), fp(k,:), tp(k,:), fcap(k,:), &
+            tcap(k,:), frep(k,:)
+      close(uo)
+    end do
+  end subroutine output_result2
+
+  ! output3 (S:1673-1715): thinnest fluid thread on the plane x = xl - 1 of
+  ! the reference's box -- per y, the last sign change of rho_N along z
+  ! between live fluid nodes, h = (zl - 1/2) - that height -- with its y;
+  ! appended to hmin.dat
+  subroutine output3(tt)
+    integer, intent(in) :: tt
+    integer :: x, y, z, zn, y0, uo
+    double precision :: bottom(yl), h, hmin, tr, lamda_d
+    tr = sqrt(Rad_c**3*rho0/sigma)
+    lamda_d = (dble(yl) - 2.0d0)*2.0d0                ! S:62
+    x = xl - 1
+    hmin = dble(zl)
+    y0 = 0
+    bottom = 1.0d0
+    do y = 2, yl - 1
+      do z = 2, zl - 1
+        zn = z + 1
+        if (pout(z,y,x) == 0 .and. walls_all(z,y,x) == 0 .and. pout(zn,y,x) == 0 .and. walls_all(zn,y,x) == 0) then
+          if (rho_N(z,y,x)*rho_N(zn,y,x) < 0.0d0) bottom(y) = dble(zn) - rho_N(zn,y,x)/(rho_N(zn,y,x) - rho_N(z,y,x))
+        end if
+      end do
+    end do
+    do y = 2, yl - 1
+      h = (dble(zl) - 0.5d0) - bottom(y)
+      if (h < hmin) then
+        hmin = h
+        y0 = y
+      end if
+    end do
+    if (tt == 0) then
+      open(newunit=uo, file=trim(outdir)//'/hmin.dat', status='replace')
+      write(uo, '(a)') ' VARIABLES=t,t*,h,y0'
+    else
+      open(newunit=uo, file=trim(outdir)//'/hmin.dat', status='unknown', position='append')
+    end if
+    write(uo, "(1(1xI8),3(1xF12.6))") tt, tt/tr, hmin/Rad_c, y0/lamda_d
+    close(uo)
+  end subroutine output3
 
   ! output_results (S:898-924): Tecplot POINT zone, x fastest
   subroutine output_results(tt)

@@ -126,6 +126,15 @@ def test_fortran_driver_reference_faces_with_particles(tmp_path):
     for p, row in enumerate(last):
         vals = [float(v) for v in row[2:]]
         assert vals[0:3] == list(got["center"][p]) and vals[6:9] == list(got["force"][p])
+    # the reference's per-particle histories (S:1420-1441) and neck file (S:1673-1715)
+    for p in range(2):
+        lines = (tmp_path / f"P_{p + 1}.dat").read_text().splitlines()
+        assert lines[0].split()[0].startswith("VARIABLES=t,t*,cx") and len(lines) == 3  # t = 0 and t = steps
+        cols = [float(v) for v in lines[-1].split()]
+        assert len(cols) == 26 and int(cols[0]) == steps
+        assert np.allclose(cols[2:5], got["center"][p], atol=1e-6) and np.allclose(cols[11:14], got["force"][p], atol=1e-6)
+    hmin = (tmp_path / "hmin.dat").read_text().splitlines()
+    assert hmin[0].strip() == "VARIABLES=t,t*,h,y0" and len(hmin) == 3 and len(hmin[-1].split()) == 4
 
 
 def test_fortran_driver_checkpoint_restart(tmp_path):

@@ -140,6 +140,25 @@ def test_ensemble_draws_match_fixture():
         assert 0.01 * b["r0"] <= b["amps"][0] <= 0.1 * b["r0"]
 
 
+def test_particle_draws_match_fixture():
+    """The 200 spheres of the dense interface workload (seed 99) are committed
+    data; the draw rules hold: no two spheres closer than 2R + 1 (minimum
+    image in x and y), centres within +-24 of the interface, densities and
+    contact angles in their ranges."""
+    with open(os.path.join(ROOT, "tests", "golden", "particle_config5.json")) as f:
+        fix = json.load(f)
+    c = ic.particle_config5()
+    assert np.array_equal(c["centers"], np.array(fix["centers"]))
+    assert np.array_equal(c["rho_p"], np.array(fix["rho_p"])) and np.array_equal(c["cita_deg"], np.array(fix["cita_deg"]))
+    n, rad, ctr = c["nx"], c["rad"], c["centers"]
+    assert len(ctr) == 200 and np.all(np.abs(ctr[:, 2] - n / 2.0) <= 24.0)
+    d = np.abs(ctr[:, None, :] - ctr[None, :, :])
+    d[..., :2] = np.minimum(d[..., :2], n - d[..., :2])
+    dist = np.sqrt((d * d).sum(-1)) + np.eye(len(ctr)) * 1e9
+    assert dist.min() >= 2.0 * rad + 1.0
+    assert np.all((0.8 <= c["rho_p"]) & (c["rho_p"] <= 2.0)) and np.all((30 <= c["cita_deg"]) & (c["cita_deg"] <= 150))
+
+
 def test_batch_create_validates(lib):
     boxes = (symmetry_amd.Box * 2)()
     boxes[0].nx, boxes[0].ny, boxes[0].nz, boxes[0].nu_r, boxes[0].nu_b = 8, 8, 8, 0.1, 0.1
