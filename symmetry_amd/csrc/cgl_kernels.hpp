@@ -44,6 +44,11 @@ constexpr int TY = 4;               // workgroup extent along y
 // arithmetic (two 512-thread workgroups fit a CU's LDS), 4 for FP64, whose
 // larger LDS rings would leave room for only one 512-thread workgroup.
 template <class R> constexpr int KTY = sizeof(R) == 8 ? 4 : 8;
+// Waves per SIMD the fused collide kernel is compiled for: 4 for FP32/FP16
+// without particle stages, so that it stays within 128 VGPRs and two
+// 512-thread workgroups share a CU; no bound otherwise (the particle-stage
+// variant needs about 210 VGPRs, FP64 runs 256-thread workgroups).
+template <class R, bool EXT> constexpr int KWAVES = (sizeof(R) == 8 || EXT) ? 1 : 4;
 
 struct Lat {
     int nx, ny, nz;
@@ -375,6 +380,7 @@ __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void 
 //  shifted: compensated sums of the deviations.
 template <class R, bool SHIFT> struct Moments {
     R sr, sb, rho, ux, uy, uz, phi;
+    R drho; // shifted storage: the sum of the deviations, rho - rho0
 };
 
 template <class R, bool SHIFT>
@@ -429,6 +435,7 @@ __device__ __forceinline__ Moments<R, SHIFT> moments(const Lat &L, const Node<R,
             if (CZ[i] == -1) jz -= f;
         }
         m.rho = R(L.rho0) + df;
+        m.drho = df;
         m.sr = nd.sr;
         m.sb = m.rho - m.sr;
         const R irho = fdiv(R(1), m.rho);
@@ -684,31 +691,41 @@ __device__ __forceinline__ void collide(const Lat &L, R f[Q], const Moments<R, S
         //          + (1 - su/2) G+_i + (1 - sq/2) G-_i
         // with f+- the half sum and difference of the pair.  Here on shifted
         // values f~ = f - w rho0 (the rho0 part is its own equilibrium).
-        R drho = f[0];
-#pragma unroll
-        for (int i = 1; i < Q; ++i) drho += f[i];
+        const R drho = mo.drho; // sum of the deviations (the moments' own sum)
         const R usqr = ux * ux + uy * uy + uz * uz;
         const R uf = ux * fx + uy * fy + uz * fz;
         const R ep0 = drho - R(1.5) * rho * usqr; // w-less even equilibrium part
-        const R r45 = R(4.5) * rho, r3 = R(3) * rho;
         const R ap = R(1) - R(0.5) * su, am = R(1) - R(0.5) * sq;
         const R gp0 = R(-3) * uf;
         const R hsu = R(0.5) * su, hsq = R(0.5) * sq;
         // rest population: f0 += -su (f0 - feq0) + (1 - su/2) G0
         f[0] = f[0] - su * (f[0] - R(W[0]) * ep0) + ap * (R(W[0]) * gp0);
+        // per weight class (w = 1/18 axis pairs, 1/36 diagonal pairs), with
+        // feq+ = w (ep0 + 4.5 rho cu^2), feq- = 3 w rho cu,
+        // G+ = w (gp0 + 9 cu cf), G- = 3 w cf:
+        //   A = su w ep0 + ap w gp0 + (su w 4.5 rho) cu^2 + (9 ap w) cu cf - su/2 sp
+        //   B = (3 sq w rho) cu + (3 am w) cf - sq/2 dm
+        R kE[2], kQ[2], kX[2], kM[2], kF[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const R w = R(c ? W[7] : W[1]);
+            kE[c] = w * (su * ep0 + ap * gp0);
+            kQ[c] = (w * su) * (R(4.5) * rho);
+            kX[c] = (w * ap) * R(9);
+            kM[c] = (w * sq) * (R(3) * rho);
+            kF[c] = (w * am) * R(3);
+        }
 #pragma unroll
         for (int q = 0; q < NPAIR; ++q) {
-            const int i = PAIR[q], o = OPC[i];
+            const int i = PAIR[q], o = OPC[i], c = i <= 6 ? 0 : 1;
             const R cu = dot_c<R>(i, ux, uy, uz);
             const R cf = dot_c<R>(i, fx, fy, fz);
-            const R w = R(W[i]);
-            const R feqp = w * (ep0 + r45 * cu * cu);
-            const R feqm = w * (r3 * cu);
-            const R gp = w * (gp0 + R(9) * cu * cf);
-            const R gm = w * (R(3) * cf);
             const R sp = f[i] + f[o], dm = f[i] - f[o];
-            const R A = su * feqp - hsu * sp + ap * gp;
-            const R B = sq * feqm - hsq * dm + am * gm;
+            R A = kE[c] + kQ[c] * (cu * cu);
+            A = A + kX[c] * (cu * cf);
+            A = A - hsu * sp;
+            R B = kM[c] * cu + kF[c] * cf;
+            B = B - hsq * dm;
             f[i] = f[i] + (A + B);
             f[o] = f[o] + (A - B);
         }
@@ -1280,6 +1297,12 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
             mo.sr = R(rr);
             mo.sb = R(rbv);
             mo.rho = R(E.mac[n]);
+            if constexpr (SHIFT) {
+                R s = nd.ft[0];
+#pragma unroll
+                for (int i = 1; i < Q; ++i) s += nd.ft[i];
+                mo.drho = s;
+            }
             mo.ux = R(ux);
             mo.uy = R(uy);
             mo.uz = R(uz);
@@ -1310,7 +1333,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
 }
 
 template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT = false>
-__global__ __launch_bounds__(TZ * KTY<R>) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
+__global__ __launch_bounds__(TZ * KTY<R>, (KWAVES<R, EXT>)) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
                                                  void *__restrict__ o0, void *__restrict__ o1,
                                                  const P *__restrict__ phi, const P *__restrict__ phio, int xc,
                                                  Ext E = Ext{})
@@ -1470,7 +1493,7 @@ __global__ __launch_bounds__(256) void k_moments_batch(const BoxDev *__restrict_
 }
 
 template <int LIN, int LOUT, class S, class R, class P, bool SHIFT>
-__global__ __launch_bounds__(TZ * KTY<R>) void k_collide_batch(const BoxDev *__restrict__ B, int nbox, int remap, int xc,
+__global__ __launch_bounds__(TZ * KTY<R>, (KWAVES<R, false>)) void k_collide_batch(const BoxDev *__restrict__ B, int nbox, int remap, int xc,
                                                        const void *__restrict__ i0, const void *__restrict__ i1,
                                                        void *__restrict__ o0, void *__restrict__ o1,
                                                        const P *__restrict__ phi, const P *__restrict__ phio)

@@ -47,6 +47,16 @@ struct ScBufs {
 };
 constexpr int SC_NBUF = 10;
 
+// floor(v + 0.5) as an int in one instruction; its result equals the exact
+// floor(v + 0.5) on gfx950 for every value tried (1M floats: |v| < 2^30,
+// exact halves, integers above 2^23 where v + 0.5 is not a float).
+__device__ __forceinline__ int round_half_up(float v)
+{
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
 __device__ __forceinline__ void add2(int2 &a, int2 b)
 {
     a.x += b.x;
@@ -266,6 +276,7 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             const R phin = sm.phi[slot5(x)][ty + 2][tz + 2];
             Moments<R, true> mo;
             mo.rho = rho0 + df;
+            mo.drho = df;
             const R irho = fdiv(R(1), mo.rho);
             mo.sr = R(0.5) * mo.rho * (R(1) + phin);
             mo.sb = mo.rho - mo.sr;
@@ -297,8 +308,9 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             }
 
             // the terms in fixed-point units: f_i = f~_i qs + w_i 2^29 and
-            // r_i = A f_i + (w_i qs G).c_i, rounded to the nearest integer (a
-            // one-sided rounding would bias every sum by up to 19 units)
+            // r_i = A f_i + (w_i qs G).c_i, rounded to the nearest integer,
+            // halves up (a one-sided rounding such as truncation would bias
+            // every sum by up to 19 units)
             const R s1 = R(W[1]) * qs, s7 = R(W[7]) * qs;
             const R G1x = Gx * s1, G1y = Gy * s1, G1z = Gz * s1;
             const R G7x = Gx * s7, G7y = Gy * s7, G7z = Gz * s7;
@@ -310,7 +322,7 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
                 R cq = A * fq;
                 if (i > 0 && i <= 6) cq = __builtin_fmaf(A, fq, dot_c<R>(i, G1x, G1y, G1z));
                 if (i > 6) cq = __builtin_fmaf(A, fq, dot_c<R>(i, G7x, G7y, G7z));
-                const int qf = __float2int_rn(fq), qr = __float2int_rn(cq);
+                const int qf = round_half_up(fq), qr = round_half_up(cq);
                 const int2 c = make_int2(qr, qf - qr);
                 const int k = CX[i] + 1;
                 if (CY[i] == 0 && CZ[i] == 0) g00[k] = c;
