@@ -382,6 +382,8 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             state_out(h, o0, o1);
             const P *phio = (const P *)h->phi[h->phicur];
             const Ext En = ext_with(h, h->ntype[nb]);
+            if (h->npar) // wetted normals next to the particles, read by the collide kernel
+                k_wet_normals<R, P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, phi);
             k_collide<RB, RB, S, R, P, SHIFT, true>
                 <<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
             if (h->npar) {

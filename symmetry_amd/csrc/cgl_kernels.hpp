@@ -44,11 +44,12 @@ constexpr int TY = 4;               // workgroup extent along y
 // arithmetic (two 512-thread workgroups fit a CU's LDS), 4 for FP64, whose
 // larger LDS rings would leave room for only one 512-thread workgroup.
 template <class R> constexpr int KTY = sizeof(R) == 8 ? 4 : 8;
-// Waves per SIMD the fused collide kernel is compiled for: 4 for FP32/FP16
-// without particle stages, so that it stays within 128 VGPRs and two
-// 512-thread workgroups share a CU; no bound otherwise (the particle-stage
-// variant needs about 210 VGPRs, FP64 runs 256-thread workgroups).
-template <class R, bool EXT> constexpr int KWAVES = (sizeof(R) == 8 || EXT) ? 1 : 4;
+// Waves per SIMD the fused collide kernel is compiled for: 4 for FP32/FP16,
+// so that it stays within 128 VGPRs and two 512-thread workgroups share a CU
+// (the wall/particle variant spills about 20 registers for it and still runs
+// 1.2x faster than at one workgroup per CU); no bound for FP64, which runs
+// 256-thread workgroups.
+template <class R> constexpr int KWAVES = sizeof(R) == 8 ? 1 : 4;
 
 struct Lat {
     int nx, ny, nz;
@@ -317,8 +318,18 @@ __device__ __forceinline__ void pull_slot(const unsigned xo[3], const unsigned y
         return;
     }
     const unsigned A = mirrored_axes<I>(madj);
-    m = xo[(A & 1u) ? 1 : 1 - CX[I]] + yo[(A & 2u) ? 1 : 1 - CY[I]] + zo[(A & 4u) ? 1 : 1 - CZ[I]];
-    d = A ? REFL.d[A][I] : I;
+    // the upstream slot, moved back onto the node's own coordinate along the
+    // mirrored axes (differences of constant-index offsets, so the offset
+    // arrays stay in registers)
+    const unsigned dx = xo[1] - xo[1 - CX[I]], dy = yo[1] - yo[1 - CY[I]], dz = zo[1] - zo[1 - CZ[I]];
+    m = xo[1 - CX[I]] + yo[1 - CY[I]] + zo[1 - CZ[I]];
+    m += ((A & 1u) ? dx : 0u) + ((A & 2u) ? dy : 0u) + ((A & 4u) ? dz : 0u);
+    // the reflected direction by selects over the seven masks (a runtime
+    // index into the table would place a copy of its column in scratch)
+    d = I;
+#pragma unroll
+    for (unsigned a = 1; a < 8; ++a)
+        if (A == a) d = REFL.d[a][I];
 }
 
 template <int LIN, class S, class R, bool SHIFT>
@@ -645,6 +656,29 @@ __device__ __forceinline__ void grad19(F phi_at, R &gx, R &gy, R &gz)
     gx = R(3) * gx;
     gy = R(3) * gy;
     gz = R(3) * gz;
+}
+
+// Unit normal n = grad phi / |grad phi| and the interface flag |grad phi| >
+// 1e-8 (S:498-532); n = 0 without the flag.  FP64 divides by the square root
+// as the reference does, FP32 scales by the hardware reciprocal square root.
+template <class R> __device__ __forceinline__ bool unit_normal(R gx, R gy, R gz, R &nx, R &ny, R &nz)
+{
+    const R g2 = gx * gx + gy * gy + gz * gz;
+    nx = ny = nz = 0;
+    if constexpr (sizeof(R) == 8) {
+        const R fnorm = fsqrt(g2);
+        if (!(fnorm > R(EVAP_LIM))) return false;
+        nx = gx / fnorm;
+        ny = gy / fnorm;
+        nz = gz / fnorm;
+    } else {
+        if (!(g2 > R(EVAP_LIM * EVAP_LIM))) return false;
+        const R inv = __builtin_amdgcn_rsqf(g2);
+        nx = gx * inv;
+        ny = gy * inv;
+        nz = gz * inv;
+    }
+    return true;
 }
 
 // ------------------------------------------------------------- collision (a)
@@ -1087,34 +1121,17 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                         return sm.phi[slot5(p + ox + dx)][jy + 1 + oy + dy][jz + 1 + oz + dz];
                     },
                     gx, gy, gz);
-                const R g2 = gx * gx + gy * gy + gz * gz;
-                bool fl = false;
-                if constexpr (sizeof(R) == 8) {
-                    const R fnorm = fsqrt(g2);
-                    if (fnorm > R(EVAP_LIM)) {
-                        nx = gx / fnorm;
-                        ny = gy / fnorm;
-                        nz = gz / fnorm;
-                        fl = true;
-                    }
-                } else if (g2 > R(EVAP_LIM * EVAP_LIM)) {
-                    const R inv = __builtin_amdgcn_rsqf(g2);
-                    nx = gx * inv;
-                    ny = gy * inv;
-                    nz = gz * inv;
-                    fl = true;
-                }
+                const bool fl = unit_normal<R>(gx, gy, gz, nx, ny, nz);
                 if constexpr (EXT) {
                     if (fl && (nt & NT_POUT)) {
-                        // wetting at the node, or for a mirror ghost at its image
-                        // (whose masks the ghost holds)
-                        const int pp = at(E.pout, (unsigned)(gxw * sx + gyw * L.nz + gzw)) - 1;
-                        double nv[3] = {double(nx), double(ny), double(nz)};
-                        wetting(L, E, gxw + ox + 1.0, gyw + oy + 1.0, gzw + oz + 1.0, E.center + 3 * pp, E.cita[pp],
-                                nv);
-                        nx = R(nv[0]);
-                        ny = R(nv[1]);
-                        nz = R(nv[2]);
+                        // the wetted normal that k_wet_normals stored for the
+                        // node, or for a mirror ghost for its image (whose
+                        // masks the ghost holds)
+                        const size_t N = (size_t)L.N;
+                        const unsigned im = (unsigned)((gxw + ox) * sx + (gyw + oy) * L.nz + gzw + oz);
+                        nx = R(E.der[3 * N + im]);
+                        ny = R(E.der[4 * N + im]);
+                        nz = R(E.der[5 * N + im]);
                     }
                     if (A & 1u) nx = -nx;
                     if (A & 2u) ny = -ny;
@@ -1279,33 +1296,42 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
             // refilled node (S:1364-1391): SRT equilibria of the refill densities
             // and the rigid-body velocity; macros as refilling set them
             const size_t N = (size_t)L.N;
-            const double rr = E.mac[N + n], rbv = E.mac[2 * N + n];
-            const double ux = E.mac[3 * N + n], uy = E.mac[4 * N + n], uz = E.mac[5 * N + n];
-            const double usqr = ux * ux + uy * uy + uz * uz;
+            if constexpr (!SHIFT) {
+                const double rr = E.mac[N + n], rbv = E.mac[2 * N + n];
+                const double ux = E.mac[3 * N + n], uy = E.mac[4 * N + n], uz = E.mac[5 * N + n];
+                const double usqr = ux * ux + uy * uy + uz * uz;
 #pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                const double uc = ux * CX[i] + uy * CY[i] + uz * CZ[i];
-                const double poly = (1.0 + 3.0 * uc + 4.5 * uc * uc - 1.5 * usqr);
-                const double er = rr * W[i] * poly, eb = rbv * W[i] * poly;
-                if constexpr (!SHIFT) {
-                    nd.r[i] = er;
-                    nd.b[i] = eb;
-                } else {
-                    nd.ft[i] = R((er + eb) - W[i] * L.rho0);
+                for (int i = 0; i < Q; ++i) {
+                    const double uc = ux * CX[i] + uy * CY[i] + uz * CZ[i];
+                    const double poly = (1.0 + 3.0 * uc + 4.5 * uc * uc - 1.5 * usqr);
+                    nd.r[i] = rr * W[i] * poly;
+                    nd.b[i] = rbv * W[i] * poly;
                 }
-            }
-            mo.sr = R(rr);
-            mo.sb = R(rbv);
-            mo.rho = R(E.mac[n]);
-            if constexpr (SHIFT) {
-                R s = nd.ft[0];
+                mo.ux = R(ux);
+                mo.uy = R(uy);
+                mo.uz = R(uz);
+            } else {
+                // shifted storage: the deviations w_i (rho poly_i - rho0), in
+                // the working precision
+                const R rt = R(E.mac[N + n] + E.mac[2 * N + n]);
+                const R ux = R(E.mac[3 * N + n]), uy = R(E.mac[4 * N + n]), uz = R(E.mac[5 * N + n]);
+                const R usqr = ux * ux + uy * uy + uz * uz;
+                R s = 0;
 #pragma unroll
-                for (int i = 1; i < Q; ++i) s += nd.ft[i];
+                for (int i = 0; i < Q; ++i) {
+                    const R uc = ux * R(CX[i]) + uy * R(CY[i]) + uz * R(CZ[i]);
+                    const R poly = R(1) + R(3) * uc + R(4.5) * uc * uc - R(1.5) * usqr;
+                    nd.ft[i] = R(W[i]) * (rt * poly - R(L.rho0));
+                    s += nd.ft[i];
+                }
                 mo.drho = s;
+                mo.ux = ux;
+                mo.uy = uy;
+                mo.uz = uz;
             }
-            mo.ux = R(ux);
-            mo.uy = R(uy);
-            mo.uz = R(uz);
+            mo.sr = R(E.mac[N + n]);
+            mo.sb = R(E.mac[2 * N + n]);
+            mo.rho = R(E.mac[n]);
             mo.phi = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
         } else {
             if constexpr (RING)
@@ -1333,7 +1359,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
 }
 
 template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT = false>
-__global__ __launch_bounds__(TZ * KTY<R>, (KWAVES<R, EXT>)) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
+__global__ __launch_bounds__(TZ * KTY<R>, KWAVES<R>) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
                                                  void *__restrict__ o0, void *__restrict__ o1,
                                                  const P *__restrict__ phi, const P *__restrict__ phio, int xc,
                                                  Ext E = Ext{})
@@ -1493,7 +1519,7 @@ __global__ __launch_bounds__(256) void k_moments_batch(const BoxDev *__restrict_
 }
 
 template <int LIN, int LOUT, class S, class R, class P, bool SHIFT>
-__global__ __launch_bounds__(TZ * KTY<R>, (KWAVES<R, false>)) void k_collide_batch(const BoxDev *__restrict__ B, int nbox, int remap, int xc,
+__global__ __launch_bounds__(TZ * KTY<R>, KWAVES<R>) void k_collide_batch(const BoxDev *__restrict__ B, int nbox, int remap, int xc,
                                                        const void *__restrict__ i0, const void *__restrict__ i1,
                                                        void *__restrict__ o0, void *__restrict__ o1,
                                                        const P *__restrict__ phi, const P *__restrict__ phio)

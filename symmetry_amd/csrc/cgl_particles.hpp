@@ -366,6 +366,42 @@ __global__ __launch_bounds__(PB) void k_ghost_phi(Lat L, Ext E, Part Pt, const u
 }
 
 // --------------------------------------------------------------- bounce-back
+// Wetted unit normals at the live fluid nodes next to particle p (pout = p+1,
+// S:536-587), from the phase field the collide kernel reads this step, into
+// the normal slots of the band fields (E.der rows 3-5); the collide kernel
+// takes them from there (a mirror ghost from its image) instead of applying
+// the correction itself.  Same gradient and normal arithmetic as the collide
+// kernel's normals pass; zero without the interface flag.  The collide
+// kernel writes the same values into these slots at band nodes.
+template <class R, class P>
+__global__ __launch_bounds__(PB) void k_wet_normals(Lat L, Ext E, Part Pt, const P *__restrict__ phi)
+{
+    const int p = blockIdx.x;
+    const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
+    const size_t N = (size_t)L.N;
+    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+        int X[3];
+        unsigned n;
+        box_node(L, b, k, X, n);
+        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
+        if (!(E.ntype[n] & NT_POUT) || E.pout[n] != p + 1) continue;
+        R gx, gy, gz, nx, ny, nz;
+        grad19<R>([&](int dx, int dy, int dz) { return phi[nb_index(L, X, dx, dy, dz)]; }, gx, gy, gz);
+        if (unit_normal<R>(gx, gy, gz, nx, ny, nz)) {
+            // the node's position in the box (wetting takes the minimum image)
+            double nv[3] = {double(nx), double(ny), double(nz)};
+            wetting(L, E, wrapm(X[0], L.nx) + 1.0, wrapm(X[1], L.ny) + 1.0, wrapm(X[2], L.nz) + 1.0,
+                    Pt.center + 3 * p, Pt.cita[p], nv);
+            nx = R(nv[0]);
+            ny = R(nv[1]);
+            nz = R(nv[2]);
+        }
+        E.der[3 * N + n] = double(nx);
+        E.der[4 * N + n] = double(ny);
+        E.der[5 * N + n] = double(nz);
+    }
+}
+
 // Moving-wall half-way bounce-back on links from live fluid x into its
 // particle (S:1004-1018):  r_i(x) = r*_opp(x) + 6 w_i rho_R(x) (u_w . c_i).
 // The pull gather of the next step reads direction i of x from the solid
