@@ -170,6 +170,12 @@ int cgl_set_particle_geometry(cgl_t *h, const double *rep_plane, const int32_t *
 int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *force, double *torque,
                       double *fcap, double *tcap, double *frep);
 
+/* Shb of each particle after the last step: the area of its sphere that
+ * lies in the red fluid, the sum of the Lagrange points' surface elements
+ * dS where phi > 0 (Lagrange_points, S:1864-1866), unscaled (output4 scales
+ * it by 8 / (pi Rad_c^2), S:1793).  npar values. */
+int cgl_get_particle_shb(cgl_t *h, double *shb);
+
 /* Masks of set_walls in force (walls, walls_all, pin, pout; S:326-395),
  * (zl, yl, xl) layout.  Any pointer may be NULL. */
 int cgl_download_masks(cgl_t *h, int32_t *walls, int32_t *walls_all, int32_t *pin, int32_t *pout);

@@ -54,6 +54,8 @@ PINS = {
     "or_repulsive": "compiled reference (S:1594-1671)",
     "or_movement": "compiled reference (S:1263-1349)",
     "or_step": "compiled reference, whole steps with and without particles",
+    "energy.energy (output4)": "compiled reference's own output4 at 152^3 with its 12 particles: Energy.dat line, "
+                               "Wcap_old and Shb (S:1719-1797)",
 }
 
 
@@ -215,7 +217,7 @@ REF_CALL = {
     "calmacros": 1, "derivatives": 2, "collision": 3, "recolor": 4, "boundary": 5,
     "streaming_r": 6, "streaming_b": 7, "curve": 8, "set_walls": 9, "refilling": 10,
     "momentum": 11, "lagrange": 12, "capillary": 13, "repulsive": 14, "movement": 15,
-    "initialization": 16, "initia_center": 17, "tables": 18,
+    "initialization": 16, "initia_center": 17, "tables": 18, "output4": 19,
 }
 
 # The reference's geometry: x=1 and z=1 walls, mirrors at x=xl, y=1, y=yl, z=zl
@@ -305,6 +307,11 @@ class RefHarness:
 
     def call(self, name):
         self._run(self.lib.ref_call, REF_CALL[name])
+
+    def energy_line(self):
+        """The line the last output4 call wrote to Energy.dat."""
+        with open(os.path.join(_scratch_dir(), "Energy.dat")) as f:
+            return f.read().splitlines()[-1]
 
     def step(self, nsteps=1, particles=False):
         self._run(self.lib.ref_step, int(nsteps), int(bool(particles)))

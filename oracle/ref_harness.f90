@@ -26,7 +26,7 @@ module ref_harness
   double precision, target :: upx(par), upy(par), upz(par), rpx(par), rpy(par), rpz(par)
   double precision, target :: Shb(par)
   double precision, target :: Mmat(0:18,0:18), Minv(0:18,0:18)
-  double precision, target :: scal(8)   ! Fg, mass, inertia, lamda_d, Wcap_old
+  double precision, target :: scal(8)   ! Fg, mass, inertia, lamda_d, Wcap_old, tr, t
 contains
 
   subroutine ref_alloc() bind(C, name="ref_alloc")
@@ -184,6 +184,13 @@ contains
     case (16); call initialization(scal(4),rho,rho_N,rho_R,rho_B,ux,uy,uz,fi,ffi,ri,bi)
     case (17); call initia_center(center,center0,scal(4))
     case (18); call tables(next_x, next_y, next_z)
+    ! output4 with t = scal(7), tr = scal(6), mass = scal(2), inertia =
+    ! scal(3), Wcap_old = scal(5) (updated); unit 400 closed afterwards so
+    ! that Energy.dat holds this call's line
+    case (19)
+      call output4(rho_N,int(scal(7)),scal(6),walls_all,pout,center0,ux,uy,uz,rho,upx,upy,upz,rpx,rpy,rpz, &
+                   tension,tension_t,scal(2),scal(3),scal(5),Shb)
+      close(400)
     end select
   end subroutine ref_call
 

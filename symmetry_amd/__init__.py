@@ -54,7 +54,7 @@ EXPORTS = (
     "cgl_create", "cgl_destroy", "cgl_upload_state", "cgl_download_state", "cgl_init_cylinder",
     "cgl_step", "cgl_calmacros", "cgl_derivatives", "cgl_sync", "cgl_time_steps",
     "cgl_device_name", "cgl_last_error", "cgl_probe_copy_bandwidth", "cgl_tuned_xchunk",
-    "cgl_set_particles", "cgl_get_particles", "cgl_download_masks",
+    "cgl_set_particles", "cgl_get_particles", "cgl_get_particle_shb", "cgl_download_masks",
     "cgl_batch_create", "cgl_box_upload_state", "cgl_box_init_cylinder", "cgl_box_download_state",
     "cgl_box_calmacros", "cgl_diagnostics", "cgl_init_interface", "cgl_phase_field_form",
     "cgl_set_particle_geometry", "cgl_checkpoint_save", "cgl_checkpoint_load",
@@ -84,6 +84,7 @@ def lib():
         L.cgl_device_name.argtypes = [vp, ctypes.c_char_p, ctypes.c_int32]
         L.cgl_set_particles.argtypes = [vp] + [vp] * 7 + [ctypes.c_double]
         L.cgl_get_particles.argtypes = [vp] + [vp] * 8
+        L.cgl_get_particle_shb.argtypes = [vp, vp]
         L.cgl_download_masks.argtypes = [vp] + [vp] * 4
         L.cgl_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(Params), ctypes.c_int32, ctypes.POINTER(Box)]
         L.cgl_box_upload_state.argtypes = [vp, ctypes.c_int32, vp, vp]
@@ -229,7 +230,11 @@ class Lattice:
         names = ("center", "up", "rp", "force", "torque", "fcap", "tcap", "frep")
         out = {k: np.empty((3, n)) for k in names}
         _check(lib().cgl_get_particles(self.h, *(_ptr(out[k]) for k in names)))
-        return {k: np.ascontiguousarray(v.T) for k, v in out.items()}
+        res = {k: np.ascontiguousarray(v.T) for k, v in out.items()}
+        shb = np.empty(n)
+        _check(lib().cgl_get_particle_shb(self.h, _ptr(shb)))
+        res["shb"] = shb
+        return res
 
     def masks(self):
         names = ("walls", "walls_all", "pin", "pout")

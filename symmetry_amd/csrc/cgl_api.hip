@@ -809,6 +809,17 @@ int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *
     return CGL_OK;
 }
 
+int cgl_get_particle_shb(cgl_t *h, double *shb)
+{
+    if (!h || !shb) return fail(CGL_ERR_ARG, "null argument");
+    if (h->npar == 0) return fail(CGL_ERR_ARG, "the handle was created with npar = 0");
+    int rc = set_device(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(shb, h->P.shb, (size_t)h->npar * sizeof(double), hipMemcpyDeviceToHost));
+    return CGL_OK;
+}
+
 __global__ void k_masks_out(const unsigned *nt, size_t N, int *walls_all)
 {
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -59,6 +59,7 @@ program cgl_driver
   type(c_ptr) :: h
   character(len=512) :: arg
   integer :: t, nchunk, u, ios
+  double precision :: wcap_old = 0.0d0   ! output4's accumulated capillary work (S:202)
 
   amp(1) = 0.8d0
   if (command_argument_count() >= 1) then
@@ -150,6 +151,7 @@ program cgl_driver
       if (npar > 0) call output_particles(t)
       if (npar > 0) call output_result2(t)
       if (ref_faces) call output3(t)
+      call output4(t)
       if (write_tecplot) call output_results(t)
       if (dump_binary) call dump_fields(t)
     end if
@@ -287,6 +289,100 @@ contains
     write(uo, "(1(1xI8),3(1xF12.6))") tt, tt/tr, hmin/Rad_c, y0/lamda_d
     close(uo)
   end subroutine output3
+
+  ! output4 (S:1719-1797): energy budget appended to Energy.dat -- surface
+  ! energy (sigma / 2 per live node with a live neighbour of the other
+  ! colour), pressure term and kinetic energy of the red fluid, particle
+  ! kinetic energy, capillary work accumulated over the output steps, and
+  ! the wetted area Shb, each scaled by 8 / (pi Rad_c^2 sigma) (Shb by
+  ! 8 / (pi Rad_c^2)), particles set up on a symmetry plane counted half.
+  ! Sums over x, y, z = 2 .. n-1.  The reference reads walls_all, rho and uz
+  ! at x index z (walls_all(z,y,z), rho(z,y,z), uz(z,y,z), S:1745-1750); so
+  ! does this where z <= xl, which covers the reference's cube, and at the
+  ! node's own x where z > xl.  A restart starts the work sum from zero.
+  subroutine output4(tt)
+    integer, intent(in) :: tt
+    integer, parameter :: cx(18) = [1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0]
+    integer, parameter :: cy(18) = [0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1]
+    integer, parameter :: cz(18) = [0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1]
+    integer :: i, x, y, z, xq, k, uo
+    double precision :: tr, er, epv, emov, esur, area, emov_p, wcap, emov_psum, wcap_sum, s_emov, s_wcap, s_shb
+    double precision, allocatable :: shb(:)
+    tr = sqrt(Rad_c**3*rho0/sigma)
+    er = PI*Rad_c**2*sigma
+    epv = 0.0d0
+    emov = 0.0d0
+    area = 0.0d0
+    do x = 2, xl - 1
+      do y = 2, yl - 1
+        do z = 2, zl - 1
+          xq = z
+          if (xq > xl) xq = x
+          if (walls_all(z,y,xq) <= 0) then
+            if (rho_N(z,y,x) > 0.0d0) then
+              epv = epv + (rho(z,y,x) - rho0)/3.0d0
+              emov = emov + 0.5d0*rho(z,y,xq)*(ux(z,y,x)**2 + uy(z,y,x)**2 + uz(z,y,xq)**2)
+            end if
+            do i = 1, 18
+              if (rho_N(z,y,x)*rho_N(z+cz(i),y+cy(i),x+cx(i)) < 0.0d0 .and. &
+                  walls_all(z+cz(i),y+cy(i),x+cx(i)) <= 0) then
+                area = area + 1.0d0
+                exit
+              end if
+            end do
+          end if
+        end do
+      end do
+    end do
+    epv = epv/er*8.0d0
+    emov = emov/er*8.0d0
+    esur = area/2.0d0*sigma/er*8.0d0
+    s_emov = 0.0d0
+    s_wcap = 0.0d0
+    s_shb = 0.0d0
+    if (npar > 0) then
+      allocate(shb(npar))
+      call cgl_check(cgl_get_particle_shb(h, shb), 'cgl_get_particle_shb')
+      do k = 1, npar
+        emov_p = 0.5d0*pmass(k)*(up(k,1)**2 + up(k,2)**2 + up(k,3)**2) + &
+                 0.5d0*pinert(k)*(rp(k,1)**2 + rp(k,2)**2 + rp(k,3)**2)
+        wcap = fcap(k,1)*up(k,1) + fcap(k,2)*up(k,2) + fcap(k,3)*up(k,3) + &
+               tcap(k,1)*rp(k,1) + tcap(k,2)*rp(k,2) + tcap(k,3)*rp(k,3)
+        if (abs(py(k) - (dble(yl) - 0.5d0)) < 0.1 .or. abs(py(k) - 1.5d0) < 0.1) then
+          emov_p = emov_p*0.5d0
+          wcap = wcap*0.5d0
+          shb(k) = shb(k)*0.5d0
+        end if
+        if (abs(px(k) - (dble(xl) - 0.5d0)) < 0.1) then
+          emov_p = emov_p*0.5d0
+          wcap = wcap*0.5d0
+          shb(k) = shb(k)*0.5d0
+        end if
+        if (abs(pz(k) - (dble(zl) - 0.5d0)) < 0.1) then
+          emov_p = emov_p*0.5d0
+          wcap = wcap*0.5d0
+          shb(k) = shb(k)*0.5d0
+        end if
+        s_emov = s_emov + emov_p
+        s_wcap = s_wcap + wcap
+      end do
+      do k = 1, npar
+        s_shb = s_shb + shb(k)/(PI*Rad_c*Rad_c)*8.0d0
+      end do
+    end if
+    emov_psum = s_emov/er*8.0d0
+    wcap_sum = wcap_old + s_wcap/er*8.0d0
+    wcap_old = wcap_sum
+    if (tt == 0) then
+      open(newunit=uo, file=trim(outdir)//'/Energy.dat', status='replace')
+      write(uo, '(a)') ' VARIABLES=t,t*,Esur,Shb,Efree,Emov,Wcap,Emov_p,Esum'
+    else
+      open(newunit=uo, file=trim(outdir)//'/Energy.dat', status='unknown', position='append')
+    end if
+    write(uo, "(1(1xI8),8(1xF12.6))") tt, tt/tr, esur, s_shb, esur - epv, emov, wcap_sum, emov_psum, &
+          (esur - epv + emov + emov_psum)
+    close(uo)
+  end subroutine output4
 
   ! output_results (S:898-924): Tecplot POINT zone, x fastest
   subroutine output_results(tt)

@@ -98,6 +98,11 @@ module cgl
       type(c_ptr), value :: h
       real(c_double), intent(out) :: center(*), up(*), rp(*), force(*), torque(*), fcap(*), tcap(*), frep(*)
     end function
+    integer(c_int) function cgl_get_particle_shb(h, shb) bind(C, name="cgl_get_particle_shb")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: shb(*)
+    end function
     ! each mask by c_loc of an int32 (zl, yl, xl) array, or c_null_ptr to skip it
     integer(c_int) function cgl_download_masks(h, walls, walls_all, pin, pout) bind(C, name="cgl_download_masks")
       import :: c_int, c_ptr

@@ -9,6 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from oracle import energy as en
 from symmetry_amd import ic
 from tests.helpers import oracle_from
 
@@ -91,7 +92,9 @@ def test_fortran_driver_reference_faces_with_particles(tmp_path):
     """ref_faces: the reference's own box (walls at x = 1 and z = 1, mirrors at
     x = xl, y = 1, y = yl, z = zl) with particles on the mirror planes, its
     repulsion planes, Lagrange box and symmetry-plane rules; the Fortran
-    driver equals the same run through the ctypes binding bit for bit."""
+    driver equals the same run through the ctypes binding bit for bit, and
+    its Energy.dat lines (output4) equal the oracle's restatement of output4
+    on the binding's state."""
     import symmetry_amd
     n, steps, rad = 32, 6, 5.0
     cs = [(n - 0.5, 15.8, 16.4), (14.3, 1.5, 17.2)]
@@ -116,7 +119,19 @@ def test_fortran_driver_reference_faces_with_particles(tmp_path):
     lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"], parts["fg"])
     lat.set_particle_geometry((1.5, n - 1.5, 1.5, n - 1.5, 1.5, n - 0.5), (1,) * 6, (0.0, 1.5, 0.0),
                               (n - 0.5, n - 0.5, n - 0.5), (0, 0, 1, 1, 0, 1, 0, 0, 1), 1)
+
+    def energy(wcap_old):
+        """output4 restated from the binding's state (which equals the
+        driver's bit for bit); the driver's Rad_c is its default 8."""
+        m, mk, g = lat.calmacros(), lat.masks(), lat.get_particles()
+        return en.energy(m["phi"], m["rho"], m["ux"], m["uy"], m["uz"], mk["walls_all"], rho0=1.0, sigma=0.005,
+                         rad_c=8.0, up=g["up"], rp=g["rp"], tension=g["fcap"], tension_t=g["tcap"],
+                         mass=parts["mass"], inertia=parts["inertia"], center0=np.array(cs), shb=g["shb"],
+                         wcap_old=wcap_old)
+
+    e0 = energy(0.0)
     lat.step(steps)
+    e1 = energy(e0["wcap_old"])
     mac = lat.calmacros()
     for k, name in enumerate(("rho", "rho_r", "rho_b", "ux", "uy", "uz", "phi")):
         assert np.array_equal(raw[k], mac[name]), name
@@ -135,6 +150,12 @@ def test_fortran_driver_reference_faces_with_particles(tmp_path):
         assert np.allclose(cols[2:5], got["center"][p], atol=1e-6) and np.allclose(cols[11:14], got["force"][p], atol=1e-6)
     hmin = (tmp_path / "hmin.dat").read_text().splitlines()
     assert hmin[0].strip() == "VARIABLES=t,t*,h,y0" and len(hmin) == 3 and len(hmin[-1].split()) == 4
+    # the energy budget (S:1719-1797), both particles on mirror planes (half counts)
+    lines = (tmp_path / "Energy.dat").read_text().splitlines()
+    tr = (8.0 ** 3 * 1.0 / 0.005) ** 0.5
+    assert lines[0].strip() == "VARIABLES=t,t*,Esur,Shb,Efree,Emov,Wcap,Emov_p,Esum" and len(lines) == 3
+    assert lines[1] == en.line(0, tr, e0) and lines[2] == en.line(steps, tr, e1)
+    assert e1["Shb"] > 0 and e1["Emov_p"] > 0 and e1["Esur"] > 0
 
 
 def test_fortran_driver_checkpoint_restart(tmp_path):

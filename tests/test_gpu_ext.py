@@ -238,7 +238,7 @@ def test_particle_steps_f64(nsteps):
     got = lat.get_particles()
     worst = {}
     for k, ok in (("center", "center"), ("up", "up"), ("rp", "rp"), ("force", "fp"), ("torque", "tp"),
-                  ("fcap", "tension"), ("tcap", "tension_t"), ("frep", "frep")):
+                  ("fcap", "tension"), ("tcap", "tension_t"), ("frep", "frep"), ("shb", "shb")):
         d = np.max(np.abs(got[k] - st.arrays[ok][:1]))
         scale = max(np.max(np.abs(st.arrays[ok][:1])), 1e-12)
         _print(k, got[k][0], st.arrays[ok][0], "rel", d / scale)

@@ -115,7 +115,7 @@ def test_mirror_particle_steps_f64(nsteps):
     npar = len(parts["rad"])
     worst = {}
     for k, ok in (("center", "center"), ("up", "up"), ("rp", "rp"), ("force", "fp"), ("torque", "tp"),
-                  ("fcap", "tension"), ("tcap", "tension_t"), ("frep", "frep")):
+                  ("fcap", "tension"), ("tcap", "tension_t"), ("frep", "frep"), ("shb", "shb")):
         ref = st.arrays[ok][:npar]
         d = np.max(np.abs(got[k] - ref))
         scale = max(np.max(np.abs(ref)), 1e-12)

@@ -163,3 +163,44 @@ def test_pair_repulsion_bit_exact(ref):
     st.call("repulsive")
     assert mismatches(ref, st) == []
     assert np.any(ref.part("Frey") != 0.0)
+
+
+def test_output4_energy_line(ref):
+    """The reference's own output4 (S:1719-1797) at its box with its 12
+    particles, which initia_center places on the mirror planes (so the
+    half-count rules act), with seeded rho, u, particle velocities, capillary
+    forces, Shb and Wcap_old: its Energy.dat line equals the restatement's
+    line (format (1(1xI8),8(1xF12.6))), and Wcap_old and Shb, which output4
+    updates in place, agree bit for bit."""
+    from oracle import energy as en
+
+    ref.call("initia_center")
+    ref.call("set_walls")
+    ref.call("initialization")
+    rng = np.random.default_rng(5)
+    shape = ref.field("rho").shape
+    ref.field("rho")[...] = 1.0 + 0.01 * rng.standard_normal(shape)
+    for k in ("ux", "uy", "uz"):
+        ref.field(k)[...] = 0.01 * rng.standard_normal(shape)
+    for names in (PV["up"], PV["rp"]):
+        for n in names:
+            ref.part(n)[:] = 0.05 * rng.standard_normal(ref.par)
+    ref.part("tension")[...] = rng.standard_normal((ref.par, 3))
+    ref.part("tension_t")[...] = rng.standard_normal((ref.par, 3))
+    ref.part("Shb")[:] = rng.uniform(10.0, 100.0, ref.par)
+    tr, t, wcap_old = 123.25, 400, 0.3
+    sc = ref.scal()
+    sc[4], sc[5], sc[6] = wcap_old, tr, float(t)
+    c0 = ref.part("center0").copy()
+    assert np.any(np.abs(c0[:, 1] - 1.5) < 0.1) and np.any(np.abs(c0[:, 0] - (ref.xl - 0.5)) < 0.1)
+    e = en.energy(ref.field("phi").copy(), ref.field("rho").copy(), ref.field("ux").copy(), ref.field("uy").copy(),
+                  ref.field("uz").copy(), ref.field("walls_all").copy(), rho0=RHO0, sigma=SIGMA, rad_c=RAD_C,
+                  up=np.stack([ref.part(n) for n in PV["up"]], 1), rp=np.stack([ref.part(n) for n in PV["rp"]], 1),
+                  tension=ref.part("tension").copy(), tension_t=ref.part("tension_t").copy(),
+                  mass=[MASS] * ref.par, inertia=[INERTIA] * ref.par, center0=c0, shb=ref.part("Shb").copy(),
+                  wcap_old=wcap_old)
+    assert np.count_nonzero(ref.field("walls_all")) > 0 and e["Esur"] > 0 and e["Emov"] > 0
+    ref.call("output4")
+    assert ref.energy_line() == en.line(t, tr, e)
+    assert sc[4] == e["wcap_old"]
+    assert np.array_equal(ref.part("Shb"), e["shb"])
