@@ -209,13 +209,12 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             S *fout = (S *)h->fsf[h->cur ^ 1];
             float4 *sout = last ? (float4 *)h->fss[h->cur ^ 1] : nullptr;
             const float *phi = (const float *)h->phi[h->phicur ^ 1];
-            float *phinext = (float *)h->phi[h->phicur];
             if (h->nbox)
                 k_collide_sc_batch<S><<<h->tiles2, k2_block<R>(), 0, h->stream>>>(
-                    h->dboxes, h->nbox, h->L.xcd_remap, h->xc, fin, fout, sout, phi, phinext, h->sb);
+                    h->dboxes, h->nbox, h->L.xcd_remap, h->xc, fin, fout, sout, phi, h->sb);
             else
                 k_collide_sc<S><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(h->L, fin, fout, sout, phi,
-                                                                                         phinext, h->sb, h->xc);
+                                                                                         h->sb, h->xc);
             h->scal_stale = !last;
         }
     }

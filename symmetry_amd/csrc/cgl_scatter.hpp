@@ -86,15 +86,13 @@ __host__ __device__ inline bool sc_shape_ok(int ny, int nz)
 template <class S>
 __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restrict__ fin, S *__restrict__ fout,
                                                 float4 *__restrict__ sout, const float *__restrict__ phi,
-                                                float *__restrict__ phinext, const ScBufs &B, int xc, unsigned bx,
-                                                unsigned by, unsigned bz)
+                                                const ScBufs &B, int xc, unsigned bx, unsigned by, unsigned bz)
 {
     using R = float;
     constexpr int KY = KTY<float>;
     __shared__ K2Smem<R, float> sm;
     __shared__ int2 yx[2][3][KY][TZ]; // terms for the row above [0] / below [1], per x offset
     __shared__ int2 zdx[2][2][KY];    // edge-lane terms for the next row: [above/below][lane 0 / last][row]
-    (void)phinext;
     const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
     const int z0 = bx * TZ, y0 = by * KY, x0 = bz * xc;
     const int x1 = min(x0 + xc, L.nx);
@@ -462,12 +460,11 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
 template <class S>
 __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *__restrict__ fin, S *__restrict__ fout,
                                                                 float4 *__restrict__ sout,
-                                                                const float *__restrict__ phi,
-                                                                float *__restrict__ phinext, ScBufs B, int xc)
+                                                                const float *__restrict__ phi, ScBufs B, int xc)
 {
     unsigned bx, by, bz;
     tile_of_block(L, bx, by, bz);
-    collide_sc_tile<S>(L, fin, fout, sout, phi, phinext, B, xc, bx, by, bz);
+    collide_sc_tile<S>(L, fin, fout, sout, phi, B, xc, bx, by, bz);
 }
 
 // Partial-sum arrays of box `base` in a packed batch.
@@ -484,8 +481,7 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc_batch(const BoxDe
                                                                       int xc, const S *__restrict__ fin,
                                                                       S *__restrict__ fout,
                                                                       float4 *__restrict__ sout,
-                                                                      const float *__restrict__ phi,
-                                                                      float *__restrict__ phinext, ScBufs B)
+                                                                      const float *__restrict__ phi, ScBufs B)
 {
     const unsigned t = remap ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x;
     const int b = find_box<true>(BX, nbox, t);
@@ -493,7 +489,7 @@ __global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc_batch(const BoxDe
     const unsigned long long base = BX[b].base;
     const unsigned lt = t - BX[b].t2, gz = BX[b].gz, gy = BX[b].gy2;
     collide_sc_tile<S>(L, fin + (size_t)Q * base, fout + (size_t)Q * base, sout ? sout + base : nullptr,
-                       phi + base, phinext + base, sc_offset(B, base), xc, lt % gz, (lt / gz) % gy,
+                       phi + base, sc_offset(B, base), xc, lt % gz, (lt / gz) % gy,
                        lt / (gz * gy));
 }
 
