@@ -1,0 +1,114 @@
+"""GPU FP64 against the unmodified reference on the reference's own set-up.
+
+The compiled reference (oracle/_ref/libref.so, built from Source1.f90 with
+flang, driven through oracle/ref_harness.f90) runs its start-up at its
+compile-time box: 152^3, walls at x = 1 and z = 1, mirrors at x = xl, y = 1,
+y = yl and z = zl, and its 12 particles placed by initia_center on the
+mirror planes (S:926-976).  The library gets the reference's initial
+populations and particles, the reference's repulsion planes, Lagrange box
+and symmetry-plane rules (S:1607-1612, S:1832, S:1272-1335), and both run the
+same whole steps (S:216-251) with the particles pushed so that nodes are
+uncovered and refilled.
+
+Masks must agree exactly; every particle quantity within 1e-9 of its scale
+(the FP64 force and torque sums are block reductions on the GPU and serial
+or OpenMP sums in the reference, which round differently in the last bits),
+and the macroscopic fields at live fluid nodes within 1e-11 of each field's
+largest magnitude.  The pushed particles (up to 0.6 lattice units per step)
+leave a few nodes near vacuum (rho of order 1e-4), where u = j / rho and
+phi = (rho_R - rho_B) / rho reach O(100) and carry the last-bit differences
+of j and rho at that scale (measured: 1e-12 of the value).
+
+Skipped where the reference build is absent.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import symmetry_amd
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref/libref.so not built")]
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "d3q19_tables.json")
+
+# the reference's parameters (S:58-84)
+OH, LAM, RAD_C, SIGMA, RHO0, RAD = 0.1, 10.0, 30.0, 0.05, 0.05, 7.5
+NU_R = OH / math.sqrt(RHO0 / (RAD_C * SIGMA))
+NU_B = NU_R / LAM
+PI = 4.0 * math.atan(1.0)
+MASS = 4.0 / 3.0 * RHO0 * PI * RAD ** 3
+INERTIA = 2.0 / 5.0 * MASS * RAD ** 2
+FG = -PI * 2.0 * RAD * SIGMA
+MACROS = ("rho", "rho_r", "rho_b", "ux", "uy", "uz", "phi")
+PV = {"up": ("upx", "upy", "upz"), "rp": ("rpx", "rpy", "rpz"), "force": ("Fpx", "Fpy", "Fpz"),
+      "torque": ("Tpx", "Tpy", "Tpz"), "frep": ("Frex", "Frey", "Frez")}
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def test_reference_box_with_its_particles():
+    nsteps = 3
+    with open(GOLD) as f:
+        M = np.array(json.load(f)["M"], dtype=float)
+    ref = oracle.RefHarness(M, SIGMA, RHO0, NU_R, NU_B, RAD, RAD_C)
+    sc = ref.scal()
+    sc[0], sc[1], sc[2], sc[3] = FG, MASS, INERTIA, (ref.yl - 2) * 2.0
+    n, npar = ref.xl, ref.par
+    _print("reference box", n, "particles", npar, "steps", nsteps)
+    ref.call("initia_center")
+    ref.call("set_walls")
+    ref.call("initialization")
+    ref.call("derivatives")
+    up = np.stack([np.linspace(-0.6, 0.6, npar), np.full(npar, 0.45), np.zeros(npar)], 1)
+    rp = np.stack([np.zeros(npar), np.zeros(npar), np.full(npar, 0.01)], 1)
+    for k, names in (("up", PV["up"]), ("rp", PV["rp"])):
+        arr = up if k == "up" else rp
+        for a, nm in enumerate(names):
+            ref.part(nm)[:] = arr[:, a]
+    center = ref.part("center").copy()
+
+    lat = symmetry_amd.Lattice(n, ref.yl, ref.zl, precision="f64", sigma=SIGMA, rho0=RHO0, nu_r=NU_R, nu_b=NU_B,
+                               beta=0.7, bc=oracle.REF_BC, npar=npar)
+    lat.upload(ref.field("ri").copy(), ref.field("bi").copy())
+    lat.set_particles(center, np.full(npar, RAD), np.full(npar, MASS), np.full(npar, INERTIA),
+                      np.full(npar, 90.0 / 180.0 * PI), FG, up=up, rp=rp)
+    lat.set_particle_geometry((1.5, n - 1.5, 1.5, ref.yl - 1.5, 1.5, ref.zl - 0.5), (1,) * 6,
+                              (0.0, 1.5, 0.0), (n - 0.5, ref.yl - 0.5, ref.zl - 0.5), (0, 0, 1, 1, 0, 1, 0, 0, 1), 1)
+
+    ref.step(nsteps, particles=True)
+    lat.step(nsteps)
+
+    masks = lat.masks()
+    bad = [k for k in ("walls", "walls_all", "pin", "pout") if not np.array_equal(masks[k], ref.field(k))]
+    assert bad == [], bad
+    assert np.count_nonzero(masks["pout"]) > 0
+    # nodes solid before the last step and fluid after it were refilled
+    assert np.count_nonzero((ref.field("walls_old") >= 1) & (ref.field("walls") == 0)) > 0
+    got = lat.get_particles()
+    refp = {k: np.stack([ref.part(nm) for nm in names], 1) for k, names in PV.items()}
+    refp.update(center=ref.part("center"), fcap=ref.part("tension"), tcap=ref.part("tension_t"))
+    for k, r in refp.items():
+        d = np.max(np.abs(got[k] - r))
+        scale = max(np.max(np.abs(r)), 1e-12)
+        _print(k, "max abs diff", d, "scale", scale)
+        assert d <= 1e-9 * scale + 1e-12, k
+    d = np.max(np.abs(got["shb"] - ref.part("Shb")))
+    _print("shb max abs diff", d)
+    assert d <= 1e-9 * np.max(np.abs(ref.part("Shb"))) + 1e-12
+    ref.call("calmacros")
+    mac = lat.calmacros()
+    live = (slice(1, n - 1), slice(1, ref.yl - 1), slice(1, ref.zl - 1))
+    fluid = ref.field("walls_all")[live] == 0
+    for k in MACROS:
+        r = ref.field(k)[live][fluid]
+        d = np.max(np.abs(mac[k][live][fluid] - r))
+        scale = max(np.max(np.abs(r)), 1.0)
+        _print(k, "max abs diff", d, "scale", scale)
+        assert d <= 1e-11 * scale, k
