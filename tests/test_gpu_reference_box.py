@@ -126,6 +126,8 @@ def test_reference_box_fluid_bit_exact():
     with open(GOLD) as f:
         M = np.array(json.load(f)["M"], dtype=float)
     ref = oracle.RefHarness(M, SIGMA, RHO0, NU_R, NU_B, RAD, RAD_C)
+    sc = ref.scal()
+    sc[0], sc[1], sc[2], sc[3] = FG, MASS, INERTIA, (ref.yl - 2) * 2.0  # lamda_d enters initialization
     n = ref.xl
     _print("reference box fluid", n, "steps", nsteps)
     ref.part("center")[:] = -1000.0
@@ -139,9 +141,47 @@ def test_reference_box_fluid_bit_exact():
     lat.step(nsteps)
     live = (slice(1, n - 1), slice(1, ref.yl - 1), slice(1, ref.zl - 1))
     r, b = lat.download()
+    assert np.all(np.isfinite(r[live])) and np.all(np.isfinite(b[live]))
     assert np.array_equal(r[live], ref.field("ri")[live]) and np.array_equal(b[live], ref.field("bi")[live])
     ref.call("calmacros")
     mac = lat.calmacros()
     for k in MACROS:
         assert np.array_equal(mac[k][live], ref.field(k)[live]), k
     assert np.count_nonzero(np.abs(ref.field("phi")[live]) < 0.9) > 0  # an interface is present
+
+
+@pytest.mark.parametrize("prec,tol_phi,tol_u", [("f32", 1e-5, 5e-5), ("f16", 5e-3, 5e-2)])
+def test_reference_box_storage_precision(prec, tol_phi, tol_u):
+    """The reference's initial state in its box (walls and mirrors, no
+    particles), 200 steps in FP32 or FP16 storage against FP64: phi and u
+    (L2-relative over live nodes).  The box's rho0 is 0.05 and its velocities
+    stay below 0.006, so u = j / rho carries 20x the momentum sum's fixed FP32
+    rounding of a rho0 = 1 box: measured 4e-7 (phi) and 2.2e-5 (u) in FP32,
+    1.2e-4 and 1.9e-2 in FP16."""
+    from tests.helpers import rel_err
+
+    with open(GOLD) as f:
+        M = np.array(json.load(f)["M"], dtype=float)
+    ref = oracle.RefHarness(M, SIGMA, RHO0, NU_R, NU_B, RAD, RAD_C)
+    sc = ref.scal()
+    sc[0], sc[1], sc[2], sc[3] = FG, MASS, INERTIA, (ref.yl - 2) * 2.0
+    ref.part("center")[:] = -1000.0
+    ref.call("set_walls")
+    ref.call("initialization")
+    ri, bi = ref.field("ri").copy(), ref.field("bi").copy()
+    n = ref.xl
+    out = {}
+    for p in ("f64", prec):
+        _print("reference box", p, "200 steps")
+        lat = symmetry_amd.Lattice(n, ref.yl, ref.zl, precision=p, sigma=SIGMA, rho0=RHO0, nu_r=NU_R, nu_b=NU_B,
+                                   beta=0.7, bc=oracle.REF_BC)
+        lat.upload(ri, bi)
+        lat.step(200)
+        out[p] = lat.calmacros()
+        lat.close()
+    live = (slice(1, n - 1), slice(1, ref.yl - 1), slice(1, ref.zl - 1))
+    e_phi = rel_err(out[prec]["phi"][live], out["f64"]["phi"][live])[0]
+    u = [np.stack([out[p][k][live] for k in ("ux", "uy", "uz")]) for p in (prec, "f64")]
+    e_u = rel_err(u[0], u[1])[0]
+    _print(prec, "vs f64: phi", e_phi, "u", e_u)
+    assert np.all(np.isfinite(u[1])) and e_phi < tol_phi and e_u < tol_u
