@@ -12,7 +12,8 @@ storage; --precision selects f64 / f16.
 
 Prints ONE JSON line (rank 0).  Fields beyond the driver contract:
   roofline      the dominant kernel (fused collide) against the 8 TB/s HBM
-                peak, with its algorithmic bytes per node
+                peak, with its algorithmic bytes per node (every stored word
+                the kernel needs read once and written once)
   cpu_baseline  the reference's own FP64 step on the host cores
   step_roofline MLUPS x counted bytes per node (312 B FP32 / 624 B FP64 /
                 160 B FP16) / 8 TB/s
@@ -174,7 +175,7 @@ def main():
                     help="timed steps of the particle workloads (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design (DESIGN.md), "
+        print("bench.py: this implementation runs on one GPU only; sharding over GPUs is designed, "
               "not built. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 

@@ -30,7 +30,7 @@ extern "C" {
 #define CGL_OK 0
 #define CGL_ERR_ARG (-1)         /* invalid argument or size */
 #define CGL_ERR_HIP (-2)         /* HIP runtime error (no GPU, out of memory, launch failure) */
-#define CGL_ERR_UNSUPPORTED (-3) /* valid request this build does not implement */
+#define CGL_ERR_UNSUPPORTED (-3) /* valid request the library does not implement */
 
 #define CGL_BC_PERIODIC 0 /* periodic face pair (the reference's next_* tables, S:295-324) */
 #define CGL_BC_WALL 1     /* half-way bounce-back wall with a ghost layer (S:1557-1590) */

@@ -31,7 +31,7 @@ def test_moment_matrix_closed_forms():
     wi, c, opc, sym, M, Minv = oracle.tables()
     G = M @ M.T
     norms = np.diag(G)
-    # Rows are orthogonal with the norms the survey derived from S:105-123.
+    # Rows are orthogonal, with the norms M M^T of the reference's matrix (S:105-123).
     assert np.allclose(G - np.diag(norms), 0.0)
     assert list(norms) == [19, 2394, 252, 10, 40, 10, 40, 10, 40, 36, 72, 12, 24, 4, 4, 4, 8, 8, 8]
     assert np.count_nonzero(M) == 213
