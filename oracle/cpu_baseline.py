@@ -20,6 +20,20 @@ import sys
 import time
 
 
+def cpu_model():
+    """The host CPU's model name and its logical CPU count."""
+    name = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name}, {os.cpu_count()} logical CPUs visible"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
@@ -55,7 +69,8 @@ def main():
         nodes = ref.xl * ref.yl * ref.zl
         out = dict(value=nodes * nsteps / dt / 1e6, unit="MLUPS", cores=args.threads, kind="reference",
                    sample=f"reference Source1.f90 (flang -O2, OpenMP) no-particle FP64 step at its compiled "
-                          f"{ref.xl}x{ref.yl}x{ref.zl} box, {nsteps} steps after 1 warm-up, {dt:.1f} s")
+                          f"{ref.xl}x{ref.yl}x{ref.zl} box, {nsteps} steps after 1 warm-up, {dt:.1f} s, "
+                          f"{args.threads} threads on {cpu_model()}")
     else:
         nx, ny, nz = 64, 64, 128
         cfg = ic.rp_config3()
@@ -73,7 +88,7 @@ def main():
         dt = time.perf_counter() - t0
         out = dict(value=nx * ny * nz * nsteps / dt / 1e6, unit="MLUPS", cores=1, kind="port",
                    sample=f"FP64 C restatement of the reference step, periodic {nx}x{ny}x{nz} slice of the "
-                          f"256x256x512 workload, {nsteps} steps, {dt:.1f} s")
+                          f"256x256x512 workload, {nsteps} steps, {dt:.1f} s, one thread on {cpu_model()}")
     print(json.dumps(out), flush=True)
 
 
