@@ -9,8 +9,10 @@ Fortran arrays read in C order).  The sums run over the nodes x, y, z =
 2..n-1 (1-based) whose ``walls_all`` is <= 0, and the reference reads
 ``walls_all``, ``rho`` and ``uz`` there at x index = z (``walls_all(z,y,z)``,
 ``rho(z,y,z)``, ``uz(z,y,z)``, S:1745-1750) while ``rho_N``, ``ux`` and ``uy``
-are read at the node itself; this restatement reads them the same way, so it
-needs nz <= nx + 1 (the reference's own box is a cube).
+are read at the node itself; this restatement reads them the same way where
+z <= nx (1-based; always in the reference's cube) and, like the Fortran
+driver, at the node itself where z > nx, which the reference's expression
+does not define.
 """
 from __future__ import annotations
 
@@ -32,15 +34,21 @@ def field_terms(rho_n, rho, ux, uy, uz, walls_all, rho0):
     kinetic energy over red live nodes, and the count of live nodes with a
     live neighbour of the other colour."""
     nx, ny, nz = rho_n.shape
-    if nz > nx + 1:
-        raise ValueError("the reference's (z, y, z) reads need nz <= nx + 1")
     sx, sy, sz = slice(1, nx - 1), slice(1, ny - 1), slice(1, nz - 1)
+    shape = (nx - 2, ny - 2, nz - 2)
+    # values read at x index = z (0-based z < nx), else at the node
+    zq = min(nz - 1, nx)  # interior z of 1..zq-1 take the (z, y, z) reads
     y = np.arange(1, ny - 1)[:, None]
-    z = np.arange(1, nz - 1)[None, :]
-    # values read at x index = z: depend on (y, z) only
-    live = (walls_all[z, y, z] <= 0)[None, :, :]
-    rho_zyz = rho[z, y, z][None, :, :]
-    uz_zyz = uz[z, y, z][None, :, :]
+    z = np.arange(1, zq)[None, :]
+
+    def quirk(a):
+        out = np.array(np.broadcast_to(a[sx, sy, sz], shape))
+        out[:, :, : zq - 1] = a[z, y, z][None, :, :]
+        return out
+
+    live = quirk(walls_all) <= 0
+    rho_zyz = quirk(rho)
+    uz_zyz = quirk(uz)
     phi = rho_n[sx, sy, sz]
     red = live & (phi > 0.0)
     epv = np.sum(np.where(red, (rho[sx, sy, sz] - rho0) / 3.0, 0.0))

@@ -44,6 +44,14 @@ def test_fortran_driver_matches_oracle(tmp_path):
     assert [int(v) for v in first[:4]] == [1, 1, 1, 0]
     assert abs(float(first[4]) - st.phi[0, 0, 0]) < 1e-8
     assert len(lines) == 3 + nx * ny * nz
+    # the energy budget (output4, S:1719-1797) of a box taller than wide and
+    # without particles: the reads at x index z stop at z = xl
+    e = en.energy(raw[6], raw[0], raw[3], raw[4], raw[5], np.zeros((nx, ny, nz), np.int32), rho0=1.0, sigma=0.01,
+                  rad_c=4.0, up=[], rp=[], tension=[], tension_t=[], mass=[], inertia=[], center0=[], shb=[],
+                  wcap_old=0.0)
+    elines = (tmp_path / "Energy.dat").read_text().splitlines()
+    tr = (4.0 ** 3 * 1.0 / 0.01) ** 0.5
+    assert len(elines) == 3 and elines[2] == en.line(steps, tr, e) and e["Esur"] > 0
 
 
 def test_fortran_driver_particle_at_interface(tmp_path):
