@@ -212,6 +212,9 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             if (h->nbox)
                 k_collide_sc_batch<S><<<h->tiles2, k2_block<R>(), 0, h->stream>>>(
                     h->dboxes, h->nbox, h->L.xcd_remap, h->xc, fin, fout, sout, phi, h->sb);
+            else if ((double)Q * (double)h->N * sizeof(S) < 4294967296.0) // one base for all directions
+                k_collide_sc<S, true><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(
+                    h->L, fin, fout, sout, phi, h->sb, h->xc);
             else
                 k_collide_sc<S><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(h->L, fin, fout, sout, phi,
                                                                                          h->sb, h->xc);

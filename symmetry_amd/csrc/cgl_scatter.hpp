@@ -83,7 +83,10 @@ __host__ __device__ inline bool sc_shape_ok(int ny, int nz)
 // The box (z-tile bx, y-tile by, x-chunk bz) of one workgroup.  A tile at
 // the high end of y or z may be partial: its last row / lane is the box face,
 // and the threads beyond it take part in the shuffles and barriers only.
-template <class S>
+// ONE: all 19 direction arrays of a state lie within 4 GiB of its first, so
+// every load and store takes the one base and a 32-bit offset (no per-direction
+// 64-bit bases held in scalar registers).
+template <class S, bool ONE = false>
 __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restrict__ fin, S *__restrict__ fout,
                                                 float4 *__restrict__ sout, const float *__restrict__ phi,
                                                 const ScBufs &B, int xc, unsigned bx, unsigned by, unsigned bz)
@@ -257,7 +260,10 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-                f[i] = StoreOf<S>::ld(fin + (size_t)i * N, m);
+                if constexpr (ONE)
+                    f[i] = StoreOf<S>::ld(fin, m + (unsigned)i * (unsigned)N);
+                else
+                    f[i] = StoreOf<S>::ld(fin + (size_t)i * N, m);
             }
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
@@ -293,11 +299,12 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             // a run's last step; streamed past the caches
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                S *fp = fout + (size_t)i * N;
+                S *fp = ONE ? fout : fout + (size_t)i * N;
+                const unsigned k = ONE ? n + (unsigned)i * (unsigned)N : n;
                 if constexpr (sizeof(S) == 4)
-                    __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, n));
+                    __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, k));
                 else
-                    StoreOf<S>::st(fp, n, f[i]);
+                    StoreOf<S>::st(fp, k, f[i]);
             }
             if (sout) {
                 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -457,14 +464,14 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     }
 }
 
-template <class S>
-__global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc(Lat L, const S *__restrict__ fin, S *__restrict__ fout,
+template <class S, bool ONE = false>
+__global__ __launch_bounds__(TZ *KTY<float>, 4) void k_collide_sc(Lat L, const S *__restrict__ fin, S *__restrict__ fout,
                                                                 float4 *__restrict__ sout,
                                                                 const float *__restrict__ phi, ScBufs B, int xc)
 {
     unsigned bx, by, bz;
     tile_of_block(L, bx, by, bz);
-    collide_sc_tile<S>(L, fin, fout, sout, phi, B, xc, bx, by, bz);
+    collide_sc_tile<S, ONE>(L, fin, fout, sout, phi, B, xc, bx, by, bz);
 }
 
 // Partial-sum arrays of box `base` in a packed batch.
