@@ -87,6 +87,21 @@ template <class T> __device__ __forceinline__ T &at(T *base, unsigned idx)
     return *(T *)((char *)base + idx * (unsigned)sizeof(T));
 }
 
+// Direction arrays in groups of four: direction i lives in the group that
+// starts at base + (i & ~3) N, at element (i & 3) N + k.  Four arrays span
+// less than 4 GiB for every box cgl_create accepts (N < 2^28 for values of
+// 4 bytes or less, N < 89 M for 8-byte ones), so a kernel holds five 64-bit
+// bases per state array instead of nineteen (for a compile-time i the group
+// bases are shared), which keeps them out of scalar-register spills.
+template <class T> __device__ __forceinline__ T *grp(T *base, int i, size_t N)
+{
+    return base + (size_t)(i & ~3) * N;
+}
+__device__ __forceinline__ unsigned grp_off(int i, size_t N, unsigned k)
+{
+    return (unsigned)(i & 3) * (unsigned)N + k;
+}
+
 template <class S> struct StoreOf;
 template <> struct StoreOf<double> {
     __device__ static double ld(const double *p) { return *p; }
@@ -204,7 +219,7 @@ __device__ __forceinline__ void gather_fs(const S *__restrict__ fs, size_t N, co
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-            R f = StoreOf<S>::ld(fs + (size_t)i * N, m);
+            R f = StoreOf<S>::ld(grp(fs, i, N), grp_off(i, N, m));
             const FsScal64 sc = scal(i, m);
             R r = sc.a * f;
             if (i > 0) r = r + (sc.g * R(W[i])) * dot_c<R>(i, sc.nx, sc.ny, sc.nz);
@@ -216,7 +231,7 @@ __device__ __forceinline__ void gather_fs(const S *__restrict__ fs, size_t N, co
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-            R f = StoreOf<S>::ld(fs + (size_t)i * N, m);
+            R f = StoreOf<S>::ld(grp(fs, i, N), grp_off(i, N, m));
             const float4 g = scal(i, m);
             dev += R(g.x) * f;
             R gc = 0;
@@ -347,9 +362,9 @@ __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void 
                 unsigned m;
                 int d;
                 pull_slot<i>(xo, yo, zo, bb, madj, m, d);
-                const S *ri = rs + (size_t)d * N, *bi = bs + (size_t)d * N;
-                nd.r[i] = StoreOf<S>::ld(ri, m);
-                nd.b[i] = StoreOf<S>::ld(bi, m);
+                const unsigned k = grp_off(d, N, m);
+                nd.r[i] = StoreOf<S>::ld(grp(rs, d, N), k);
+                nd.b[i] = StoreOf<S>::ld(grp(bs, d, N), k);
             });
         } else {
             // deviations from w_i rho0 ax and w_i rho0 (1 - ax): small, summed plainly
@@ -359,9 +374,9 @@ __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void 
                 unsigned m;
                 int d;
                 pull_slot<i>(xo, yo, zo, bb, madj, m, d);
-                const S *ri = rs + (size_t)d * N, *bi = bs + (size_t)d * N;
-                R rv = StoreOf<S>::ld(ri, m);
-                R bv = StoreOf<S>::ld(bi, m);
+                const unsigned k = grp_off(d, N, m);
+                R rv = StoreOf<S>::ld(grp(rs, d, N), k);
+                R bv = StoreOf<S>::ld(grp(bs, d, N), k);
                 sr += rv;
                 nd.ft[i] = rv + bv;
             });
@@ -952,9 +967,9 @@ __device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, uns
                 if (i > 0 && c.gam != R(0)) r = r + c.gam * R(W[i]) * dot_c<R>(i, nx, ny, nz);
                 if (c.evr) r = 0;
                 if (c.evb) r = f[i];
-                S *rp = ro + (size_t)i * N, *bp = bo + (size_t)i * N;
-                StoreOf<S>::st(rp, n, r);
-                StoreOf<S>::st(bp, n, f[i] - r);
+                const unsigned k = grp_off(i, N, n);
+                StoreOf<S>::st(grp(ro, i, N), k, r);
+                StoreOf<S>::st(grp(bo, i, N), k, f[i] - r);
             }
         } else {
             const R rho0 = R(L.rho0);
@@ -965,9 +980,9 @@ __device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, uns
                 R v = c.A * f[i] + R(W[i]) * rho0 * (c.A - a) + c.gam * R(W[i]) * dot;
                 if (c.evr) v = -R(W[i]) * rho0 * a;
                 if (c.evb) v = f[i] + R(W[i]) * rho0 * (R(1) - a);
-                S *rp = ro + (size_t)i * N, *bp = bo + (size_t)i * N;
-                StoreOf<S>::st(rp, n, v);
-                StoreOf<S>::st(bp, n, f[i] - v);
+                const unsigned k = grp_off(i, N, n);
+                StoreOf<S>::st(grp(ro, i, N), k, v);
+                StoreOf<S>::st(grp(bo, i, N), k, f[i] - v);
             }
         }
     } else {
@@ -976,11 +991,12 @@ __device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, uns
         S *fo = (S *)o0;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            S *fp = fo + (size_t)i * N;
+            S *fp = grp(fo, i, N);
+            const unsigned k = grp_off(i, N, n);
             if constexpr (SHIFT && sizeof(S) == 4)
-                __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, n));
+                __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, k));
             else
-                StoreOf<S>::st(fp, n, f[i]);
+                StoreOf<S>::st(fp, k, f[i]);
         }
         R A = c.A, g = c.gam;
         if (c.evr) { A = 0; g = 0; }
@@ -1276,14 +1292,15 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                     unsigned m;
                     int dsrc;
                     pull_slot<i>(xo, yo, zo, bb, madj, m, dsrc);
-                    R rv = StoreOf<S>::ld(rs + (size_t)dsrc * N, m), bv = StoreOf<S>::ld(bs + (size_t)dsrc * N, m);
+                    const unsigned ks = grp_off(dsrc, N, m), kd = grp_off(i, N, n);
+                    R rv = StoreOf<S>::ld(grp(rs, dsrc, N), ks), bv = StoreOf<S>::ld(grp(bs, dsrc, N), ks);
                     if constexpr (SHIFT) {
                         const R dsh = R(W[i]) * R(L.rho0) * (ax - aw[i]);
                         rv += dsh;
                         bv -= dsh;
                     }
-                    StoreOf<S>::st(ro + (size_t)i * N, n, rv);
-                    StoreOf<S>::st(bo + (size_t)i * N, n, bv);
+                    StoreOf<S>::st(grp(ro, i, N), kd, rv);
+                    StoreOf<S>::st(grp(bo, i, N), kd, bv);
                 });
                 continue;
             }
