@@ -263,7 +263,7 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
                 if constexpr (ONE)
                     f[i] = StoreOf<S>::ld(fin, m + (unsigned)i * (unsigned)N);
                 else
-                    f[i] = StoreOf<S>::ld(fin + (size_t)i * N, m);
+                    f[i] = StoreOf<S>::ld(grp(fin, i, N), grp_off(i, N, m));
             }
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
@@ -299,8 +299,8 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             // a run's last step; streamed past the caches
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                S *fp = ONE ? fout : fout + (size_t)i * N;
-                const unsigned k = ONE ? n + (unsigned)i * (unsigned)N : n;
+                S *fp = ONE ? fout : grp(fout, i, N);
+                const unsigned k = ONE ? n + (unsigned)i * (unsigned)N : grp_off(i, N, n);
                 if constexpr (sizeof(S) == 4)
                     __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, k));
                 else
@@ -484,7 +484,7 @@ __device__ __forceinline__ ScBufs sc_offset(const ScBufs &B, unsigned long long 
 // Every box of a batch in one launch; a workgroup finds its box by the
 // first-tile table, and the box's arrays start at its packed offset.
 template <class S>
-__global__ __launch_bounds__(TZ *KTY<float>) void k_collide_sc_batch(const BoxDev *__restrict__ BX, int nbox, int remap,
+__global__ __launch_bounds__(TZ *KTY<float>, 4) void k_collide_sc_batch(const BoxDev *__restrict__ BX, int nbox, int remap,
                                                                       int xc, const S *__restrict__ fin,
                                                                       S *__restrict__ fout,
                                                                       float4 *__restrict__ sout,
