@@ -228,6 +228,9 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         if (h->nbox)
             k_phi_sc_batch<<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, K1_XC, h->xc, h->sb,
                                                                        (float *)h->phi[h->phicur]);
+        else if (h->L.nz % 2 == 0) // two nodes per thread
+            k_phi_sc2<<<dim3((h->L.nz / 2 + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, h->L.nx), node_block(), 0,
+                        h->stream>>>(h->L, h->sb, h->xc, (float *)h->phi[h->phicur]);
         else
             k_phi_sc<<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, h->sb, h->xc, (float *)h->phi[h->phicur]);
         h->phi_ok = 1;

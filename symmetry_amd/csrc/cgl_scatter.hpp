@@ -532,6 +532,48 @@ static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, 
     phi_sc_node(L, B, xc, phi, x, y, z);
 }
 
+// The finishing pass two nodes at a time (z and z + 1, z even, nz even): the
+// pair's own sums in one 16-byte load, the x-face and y-face sums (which
+// call for both nodes of the pair or for neither) likewise, the z-face sums
+// per node, phi in one 8-byte store.  Integer sums are exact, so the order of
+// the additions does not change phi.
+static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc, float *__restrict__ phi)
+{
+    const int z = (blockIdx.x * TZ + threadIdx.x) * 2, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z;
+    if (z >= L.nz || y >= L.ny) return;
+    const unsigned n = ((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z;
+    const unsigned h = n >> 1; // pair index: n is even
+    const int4 o = at((const int4 *)B.own, h);
+    int2 s[2] = {make_int2(o.x, o.y), make_int2(o.z, o.w)};
+    auto add_pair = [&](const int2 *a) {
+        const int4 v = at((const int4 *)a, h);
+        s[0].x += v.x;
+        s[0].y += v.y;
+        s[1].x += v.z;
+        s[1].y += v.w;
+    };
+    const int cx0 = (x / xc) * xc;
+    const bool xlo = x == cx0, xhi = x == min(cx0 + xc, L.nx) - 1;
+    const int yr = y % KTY<float>;
+    const bool fy = yr == 0 || y == min(y - yr + KTY<float>, L.ny) - 1;
+    if (xlo) add_pair(B.fxlo);
+    if (xhi) add_pair(B.fxhi);
+    if (fy) add_pair(B.fy);
+    if (fy && xlo) add_pair(B.exylo);
+    if (fy && xhi) add_pair(B.exyhi);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int zk = z + k, zr = zk % TZ;
+        if (zr == 0 || zk == min(zk - zr + TZ, L.nz) - 1) {
+            add2(s[k], at(B.fz, n + k));
+            if (xlo) add2(s[k], at(B.exzlo, n + k));
+            if (xhi) add2(s[k], at(B.exzhi, n + k));
+            if (fy) add2(s[k], at(B.eyz, n + k));
+        }
+    }
+    at((float2 *)phi, h) = make_float2(phi_of_sums(s[0]), phi_of_sums(s[1]));
+}
+
 // The batch's finishing pass over the moments launch's tiles (z tile, y tile,
 // chunk of xs planes) of every box.
 static __global__ __launch_bounds__(256) void k_phi_sc_batch(const BoxDev *__restrict__ BX, int nbox, int xs, int xc,

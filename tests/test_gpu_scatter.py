@@ -126,12 +126,14 @@ def test_scatter_independent_of_box_decomposition(ny, nz):
             assert np.array_equal(r, ref[0]) and np.array_equal(b, ref[1]), xc
 
 
-@pytest.mark.parametrize("shape", [(6, 8, 64), (10, 8, 128), (12, 24, 64), (8, 12, 96), (6, 10, 130), (5, 6, 40)])
+@pytest.mark.parametrize("shape", [(6, 8, 64), (10, 8, 128), (12, 24, 64), (8, 12, 96), (6, 10, 130), (5, 6, 40),
+                                   (7, 12, 95)])
 def test_scatter_single_tile_axes(shape):
     """One workgroup tile along y (ny = 8) and/or z (nz = 64): the box-leaving
     sums wrap onto the box itself.  Partial tiles: a 4-row and a 32-lane tile
     (12 x 96), a 2-row and a 2-lane tile (10 x 130), one partial tile on both
-    axes (6 x 40)."""
+    axes (6 x 40); an odd nz (95), whose finishing pass takes one node per
+    thread (even nz: two)."""
     nx, ny, nz = shape
     cfg, ri, bi = small_rp(nx=nx, ny=ny, nz=nz, r0=2.5, nu_b=0.1)
     out = []
