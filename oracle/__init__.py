@@ -47,8 +47,10 @@ PINS = {
     "or_refilling": "compiled reference (S:1351-1418)",
     "or_momentum": "compiled reference (S:1178-1261)",
     "or_lagrange_capillary": "compiled reference (S:1799-2028) at the reference's contact angle of 90 degrees; "
-                             "other angles take the S:2015-2019 else-branch, which the reference never runs: "
-                             "parity unpinned for contact angles other than 90 degrees",
+                             "other angles (the S:2015-2019 else-branch, which the reference never runs) by "
+                             "closed forms of the force direction and size (tests/test_oracle_contact_angle.py)",
+    "wetting at other angles": "closed form of the corrected normal on a linear phase field, 30-150 degrees "
+                               "(tests/test_oracle_contact_angle.py)",
     "periodic faces": "translation invariance of a step (tests/test_oracle_invariance.py); the reference has none",
     "high-side wall faces": "reflection symmetry of a step between two walls (tests/test_oracle_invariance.py)",
     "or_repulsive": "compiled reference (S:1594-1671)",

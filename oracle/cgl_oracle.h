@@ -21,9 +21,10 @@
  * Pins: every stage is checked bit for bit against the compiled reference on
  * its own geometry (tests/test_oracle_vs_ref.py); periodic faces and
  * high-side walls, which the reference lacks, by translation and reflection
- * symmetry (tests/test_oracle_invariance.py).  Parity unpinned: the capillary
- * force of or_lagrange_capillary for contact angles other than 90 degrees
- * (the S:2015-2019 else-branch; the reference's angle is a compile-time 90).
+ * symmetry (tests/test_oracle_invariance.py).  Contact angles other than the
+ * reference's compile-time 90 degrees (the wetting correction, S:536-587, and
+ * the S:2015-2019 else-branch of the capillary force) are pinned by closed
+ * forms of their geometry (tests/test_oracle_contact_angle.py).
  */
 #ifndef CGL_ORACLE_H
 #define CGL_ORACLE_H
