@@ -37,6 +37,11 @@ namespace cgl {
 
 constexpr float SC_ONE = 536870912.0f; // 2^29 fixed-point units per rho0
 
+// Arrays of N entries.  The first five kinds are indexed by node; the sums
+// across a z face (fz, exzlo, exzhi, eyz) exist only for the first and last
+// node of each 64-node z tile and are packed two per (x, y, z tile), at
+// zslot(): a z-face read or write then touches 16 bytes per tile row instead
+// of a cache line at each end of it.
 struct ScBufs {
     int2 *own;           // sums from the node's own box
     int2 *fxlo, *fxhi;   // from the box on the node's low / high x side
@@ -46,6 +51,15 @@ struct ScBufs {
     int2 *eyz;           // from the box across a y-z edge
 };
 constexpr int SC_NBUF = 10;
+
+// Entries per (x, y) column of the packed z-face arrays: two per z tile (the
+// tile's first node, side 0, and its last, side 1; tiles have two nodes or
+// more), nx * ny * zcol_len <= N.
+__device__ __forceinline__ unsigned zcol_len(const Lat &L) { return 2u * (unsigned)((L.nz + TZ - 1) / TZ); }
+__device__ __forceinline__ unsigned zslot(const Lat &L, unsigned x, unsigned y, unsigned tile, unsigned side)
+{
+    return ((x * (unsigned)L.ny + y) * zcol_len(L)) + 2u * tile + side;
+}
 
 // floor(v + 0.5) as an int in one instruction; its result equals the exact
 // floor(v + 0.5) on gfx950 for every value tried (1M floats: |v| < 2^30,
@@ -184,11 +198,19 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     const bool ylo = ty == 0, yhi = ty == ylast, yedge = active && (ylo || yhi);
     // lane 0 holds the terms for z0 + zlast + 1, the last lane those for z0 - 1
     const bool zl = tz == 0, zh = tz == zlast, zedge = active && (zl || zh);
-    const unsigned yrow = (unsigned)(ylo ? wrapm(y0 - 1, L.ny) : wrapm(y0 + ylast + 1, L.ny)) * (unsigned)L.nz;
-    const unsigned zcol = (unsigned)(zl ? wrapm(z0 + zlast + 1, L.nz) : wrapm(z0 - 1, L.nz));
-    const unsigned yself = (unsigned)wrapm(y, L.ny) * (unsigned)L.nz, zself = (unsigned)wrapm(z, L.nz);
     const int left = zl ? zlast : tz - 1, right = zh ? 0 : (tz + 1) & (TZ - 1);
     const int zs = zl ? 0 : 1;
+    // packed z-face slot of the destination: lane 0 feeds the first node of
+    // the next z tile, the last lane the last node of the previous one
+    const unsigned nzt = (unsigned)(L.nz + TZ - 1) / TZ;
+    const unsigned zc2 = zl ? 2u * (bx + 1 == nzt ? 0u : bx + 1) : 2u * (bx == 0 ? nzt - 1 : bx - 1) + 1u;
+    const unsigned yw = (unsigned)wrapm(y, L.ny), yo = (unsigned)(ylo ? wrapm(y0 - 1, L.ny) : wrapm(y0 + ylast + 1, L.ny));
+    // in-plane offsets: this node, the node outside the y face next to it, and
+    // the packed z-face slots of this row and of that outside row
+    const unsigned nself = yw * (unsigned)L.nz + (unsigned)wrapm(z, L.nz);
+    const unsigned nrow = yo * (unsigned)L.nz + (unsigned)wrapm(z, L.nz);
+    const unsigned cself = yw * zcol_len(L) + zc2, crow = yo * zcol_len(L) + zc2;
+    const unsigned csx = (unsigned)L.ny * zcol_len(L); // packed z-face plane stride
 
     // sums for destination planes x-1 (m), x (c), x+1 (p): own node column,
     // the row outside the box next to this row, the column outside next to this lane
@@ -382,7 +404,6 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
 
         // y neighbours through LDS; y-z edge terms of the corner lanes go out
         // at once (they have dx = 0 only)
-        const unsigned px = (unsigned)x * sx;
         if (active && !yhi) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) yx[0][k][ty][tz] = gyp[k];
@@ -393,9 +414,9 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
         }
         if (zedge) {
             if (!yhi) zdx[0][zs][ty] = zup;
-            else at(B.eyz, px + yrow + zcol) = zup;
+            else at(B.eyz, (unsigned)x * csx + crow) = zup;
             if (!ylo) zdx[1][zs][ty] = zdn;
-            else at(B.eyz, px + yrow + zcol) = zdn;
+            else at(B.eyz, (unsigned)x * csx + crow) = zdn;
         }
         __syncthreads();
         if (active && !ylo) {
@@ -428,13 +449,13 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
         const unsigned pm = (unsigned)wrapm(x - 1, L.nx) * sx;
         if (!active) {
         } else if (x > x0) {
-            at(B.own, pm + yself + zself) = am;
-            if (yedge) at(B.fy, pm + yrow + zself) = ym;
-            if (zedge) at(B.fz, pm + yself + zcol) = zm;
+            at(B.own, pm + nself) = am;
+            if (yedge) at(B.fy, pm + nrow) = ym;
+            if (zedge) at(B.fz, (unsigned)wrapm(x - 1, L.nx) * csx + cself) = zm;
         } else { // plane x0 - 1 belongs to the box on the low x side
-            at(B.fxhi, pm + yself + zself) = am;
-            if (yedge) at(B.exyhi, pm + yrow + zself) = ym;
-            if (zedge) at(B.exzhi, pm + yself + zcol) = zm;
+            at(B.fxhi, pm + nself) = am;
+            if (yedge) at(B.exyhi, pm + nrow) = ym;
+            if (zedge) at(B.exzhi, (unsigned)wrapm(x - 1, L.nx) * csx + cself) = zm;
         }
         gx = ngx;
         gy = ngy;
@@ -452,15 +473,15 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     // plane x1 - 1 (own box) and plane x1 (the box on the high x side)
     if (!active) return;
     const unsigned pl = (unsigned)wrapm(x1 - 1, L.nx) * sx, pn = (unsigned)wrapm(x1, L.nx) * sx;
-    at(B.own, pl + yself + zself) = am;
-    at(B.fxlo, pn + yself + zself) = ac;
+    at(B.own, pl + nself) = am;
+    at(B.fxlo, pn + nself) = ac;
     if (yedge) {
-        at(B.fy, pl + yrow + zself) = ym;
-        at(B.exylo, pn + yrow + zself) = yc;
+        at(B.fy, pl + nrow) = ym;
+        at(B.exylo, pn + nrow) = yc;
     }
     if (zedge) {
-        at(B.fz, pl + yself + zcol) = zm;
-        at(B.exzlo, pn + yself + zcol) = zc;
+        at(B.fz, (unsigned)wrapm(x1 - 1, L.nx) * csx + cself) = zm;
+        at(B.exzlo, (unsigned)wrapm(x1, L.nx) * csx + cself) = zc;
     }
 }
 
@@ -516,12 +537,15 @@ __device__ __forceinline__ void phi_sc_node(const Lat &L, const ScBufs &B, int x
     if (xlo) add2(s, at(B.fxlo, n));
     if (xhi) add2(s, at(B.fxhi, n));
     if (fy) add2(s, at(B.fy, n));
-    if (fz) add2(s, at(B.fz, n));
     if (fy && xlo) add2(s, at(B.exylo, n));
     if (fy && xhi) add2(s, at(B.exyhi, n));
-    if (fz && xlo) add2(s, at(B.exzlo, n));
-    if (fz && xhi) add2(s, at(B.exzhi, n));
-    if (fy && fz) add2(s, at(B.eyz, n));
+    if (fz) {
+        const unsigned c = zslot(L, (unsigned)x, (unsigned)y, (unsigned)(z / TZ), zr == 0 ? 0u : 1u);
+        add2(s, at(B.fz, c));
+        if (xlo) add2(s, at(B.exzlo, c));
+        if (xhi) add2(s, at(B.exzhi, c));
+        if (fy) add2(s, at(B.eyz, c));
+    }
     at(phi, n) = phi_of_sums(s);
 }
 
@@ -565,10 +589,11 @@ static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc,
     for (int k = 0; k < 2; ++k) {
         const int zk = z + k, zr = zk % TZ;
         if (zr == 0 || zk == min(zk - zr + TZ, L.nz) - 1) {
-            add2(s[k], at(B.fz, n + k));
-            if (xlo) add2(s[k], at(B.exzlo, n + k));
-            if (xhi) add2(s[k], at(B.exzhi, n + k));
-            if (fy) add2(s[k], at(B.eyz, n + k));
+            const unsigned c = zslot(L, (unsigned)x, (unsigned)y, (unsigned)(zk / TZ), zr == 0 ? 0u : 1u);
+            add2(s[k], at(B.fz, c));
+            if (xlo) add2(s[k], at(B.exzlo, c));
+            if (xhi) add2(s[k], at(B.exzhi, c));
+            if (fy) add2(s[k], at(B.eyz, c));
         }
     }
     at((float2 *)phi, h) = make_float2(phi_of_sums(s[0]), phi_of_sums(s[1]));
