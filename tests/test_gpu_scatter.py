@@ -65,7 +65,7 @@ def test_scatter_form_selection(rp):
     assert not _lattice(cfg, scatter=False).scatter_form()
     assert not _lattice(cfg, "f64").scatter_form()
     assert _lattice(dict(cfg, ny=12, nz=96)).scatter_form()  # partial workgroup tiles
-    assert not _lattice(dict(cfg, ny=9)).scatter_form()  # a one-row tile: faces on one thread
+    assert not _lattice(dict(cfg, ny=17)).scatter_form()  # a one-row tile: faces on one thread
     assert not _lattice(dict(cfg, nz=129)).scatter_form()  # a one-lane tile
 
 
