@@ -279,6 +279,10 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             const unsigned n = xo[1] + yo[1] + zo[1];
             R f[Q];
             R df = 0, jx = 0, jy = 0, jz = 0;
+            // FP32 storage: the wave issues its 19 loads at raised priority, so
+            // that the other waves of the SIMD do not delay them (+0.8 % MLUPS
+            // at 256x256x512; neutral for FP16 storage)
+            if constexpr (sizeof(S) == 4) __builtin_amdgcn_s_setprio(2);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
@@ -287,6 +291,7 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
                 else
                     f[i] = StoreOf<S>::ld(grp(fin, i, N), grp_off(i, N, m));
             }
+            if constexpr (sizeof(S) == 4) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 df += f[i];
