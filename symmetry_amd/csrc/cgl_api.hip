@@ -259,6 +259,7 @@ static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls)
     {
         const char *env = getenv("CGL_XCD_REMAP");
         h->L.xcd_remap = env ? atoi(env) : 1;
+        h->remap_env = env != nullptr;
     }
     h->esize = p->precision == CGL_PREC_F64 ? 8 : (p->precision == CGL_PREC_F32 ? 4 : 2);
     h->phisize = p->precision == CGL_PREC_F64 ? 8 : 4;

@@ -58,6 +58,7 @@ struct cgl_handle {
     int gkey;            // cur | phicur << 1 | state_layout << 2 | ntcur << 3 | phi_ok << 4 at capture, -1 = none
     // periodic FP32/FP16 factorised boxes: next phase field in scatter form
     int sc;              // 1: steady steps use k_collide_sc + k_phi_sc
+    int remap_env;       // 1: CGL_XCD_REMAP was set, and its value holds for every kernel
     int phi_ok;          // phi[phicur ^ 1] holds the current state's phase field (from the last step)
     int scal_stale;      // the current state's recolour scalars were not written (mid-run only)
     cgl::ScBufs sb;      // fixed-point partial sums
@@ -209,14 +210,20 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             S *fout = (S *)h->fsf[h->cur ^ 1];
             float4 *sout = last ? (float4 *)h->fss[h->cur ^ 1] : nullptr;
             const float *phi = (const float *)h->phi[h->phicur ^ 1];
+            // tile order: the XCD-contiguous order helps FP16 storage (issue-
+            // bound, +1.4 % at 256x256x512) and costs FP32 storage 1.5 % (2.6 %
+            // batched), so FP32 takes tiles in launch order unless
+            // CGL_XCD_REMAP says otherwise
+            Lat Ls = h->L;
+            if (sizeof(S) == 4 && !h->remap_env) Ls.xcd_remap = 0;
             if (h->nbox)
                 k_collide_sc_batch<S><<<h->tiles2, k2_block<R>(), 0, h->stream>>>(
-                    h->dboxes, h->nbox, h->L.xcd_remap, h->xc, fin, fout, sout, phi, h->sb);
+                    h->dboxes, h->nbox, Ls.xcd_remap, h->xc, fin, fout, sout, phi, h->sb);
             else if ((double)Q * (double)h->N * sizeof(S) < 4294967296.0) // one base for all directions
                 k_collide_sc<S, true><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(
-                    h->L, fin, fout, sout, phi, h->sb, h->xc);
+                    Ls, fin, fout, sout, phi, h->sb, h->xc);
             else
-                k_collide_sc<S><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(h->L, fin, fout, sout, phi,
+                k_collide_sc<S><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(Ls, fin, fout, sout, phi,
                                                                                          h->sb, h->xc);
             h->scal_stale = !last;
         }
