@@ -215,10 +215,11 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             // batched), so FP32 takes tiles in launch order unless
             // CGL_XCD_REMAP says otherwise
             Lat Ls = h->L;
-            if (sizeof(S) == 4 && !h->remap_env) Ls.xcd_remap = 0;
+            if (sizeof(S) == 4 && !h->remap_env) Ls.xcd_remap = 2;
             if (h->nbox)
                 k_collide_sc_batch<S><<<h->tiles2, k2_block<R>(), 0, h->stream>>>(
-                    h->dboxes, h->nbox, Ls.xcd_remap, h->xc, fin, fout, sout, phi, h->sb);
+                    h->dboxes, h->nbox, sizeof(S) == 4 && !h->remap_env ? 0 : h->L.xcd_remap, h->xc, fin, fout,
+                    sout, phi, h->sb);
             else if ((double)Q * (double)h->N * sizeof(S) < 4294967296.0) // one base for all directions
                 k_collide_sc<S, true><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(
                     Ls, fin, fout, sout, phi, h->sb, h->xc);

@@ -153,8 +153,19 @@ __device__ __forceinline__ void tile_of_block(const Lat &L, unsigned &bx, unsign
 {
     const unsigned gx = gridDim.x, gy = gridDim.y, nb = gx * gy * gridDim.z;
     unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-    const unsigned full = nb - nb % NXCD;
-    if (L.xcd_remap && lin < full) lin = (lin % NXCD) * (full / NXCD) + lin / NXCD;
+    if (L.xcd_remap == 2) {
+        // whole z columns of tiles (gx tiles sharing misaligned halo lines) on
+        // one XCD, consecutive columns dealt out over the XCDs: XCD k's j-th
+        // block takes column (j / gx) * 8 + k, tile j % gx of it
+        const unsigned span = NXCD * gx, fullg = nb - nb % span;
+        if (lin < fullg) {
+            const unsigned k = lin % NXCD, j = lin / NXCD;
+            lin = ((j / gx) * NXCD + k) * gx + j % gx;
+        }
+    } else if (L.xcd_remap) {
+        const unsigned full = nb - nb % NXCD;
+        if (lin < full) lin = (lin % NXCD) * (full / NXCD) + lin / NXCD;
+    }
     bx = lin % gx;
     by = (lin / gx) % gy;
     bz = lin / (gx * gy);
