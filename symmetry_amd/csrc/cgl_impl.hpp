@@ -210,12 +210,12 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             S *fout = (S *)h->fsf[h->cur ^ 1];
             float4 *sout = last ? (float4 *)h->fss[h->cur ^ 1] : nullptr;
             const float *phi = (const float *)h->phi[h->phicur ^ 1];
-            // tile order: the XCD-contiguous order helps FP16 storage (issue-
-            // bound, +1.4 % at 256x256x512) and costs FP32 storage 1.5 % (2.6 %
-            // batched), so FP32 takes tiles in launch order unless
-            // CGL_XCD_REMAP says otherwise
+            // tile order: whole z columns of tiles on one XCD, consecutive
+            // columns over the XCDs (at 256x256x512 +2 % for FP32 and +1.4 %
+            // for FP16 storage over the XCD-contiguous order), unless
+            // CGL_XCD_REMAP says otherwise; the FP32 batch takes launch order
             Lat Ls = h->L;
-            if (sizeof(S) == 4 && !h->remap_env) Ls.xcd_remap = 2;
+            if (!h->remap_env) Ls.xcd_remap = 2;
             if (h->nbox)
                 k_collide_sc_batch<S><<<h->tiles2, k2_block<R>(), 0, h->stream>>>(
                     h->dboxes, h->nbox, sizeof(S) == 4 && !h->remap_env ? 0 : h->L.xcd_remap, h->xc, fin, fout,
