@@ -44,7 +44,24 @@ B_ALG_COLLIDE = {
 # and written, phi read once, the node's own fixed-point sums (2 x int32)
 # written once; the recolour scalars are written on a run's last step only.
 B_ALG_COLLIDE_SC = {"f32": 2 * 19 * 4 + 4 + 8, "f16": 2 * 19 * 2 + 4 + 8}
+# Moments kernel of the red/blue layout: 38 populations read and phi written;
+# shifted storage also reads the previous phi (the shift reference).
+B_ALG_MOMENTS_RB = {"f64": 2 * 19 * 8 + 8, "f32": 2 * 19 * 4 + 2 * 4, "f16": 2 * 19 * 2 + 2 * 4}
 DTYPE = {"f64": "f64", "f32": "f32", "f16": "f16-storage/f32-compute"}
+
+
+def finish_tuning(lat, max_calls=64):
+    """Run the x-chunk autotuner to its end outside any timed region (it
+    times each candidate on real steps), then four steady steps so that the
+    step graph is captured before the clock starts."""
+    for _ in range(max_calls):
+        if not lat.tuned_xchunk()[1]:
+            break
+        lat.step(2)
+    if lat.tuned_xchunk()[1]:
+        raise RuntimeError("x-chunk autotuner still running after the warm-up")
+    lat.step(4)
+    lat.sync()
 
 
 def cpu_baseline(seconds):
@@ -69,6 +86,78 @@ def traffic_from_profile(precision, layout, kernel="k_collide"):
         return None
     with open(path) as f:
         return json.load(f).get("bytes_per_launch")
+
+
+def _lattice(nx, ny, nz, cfg, prec, layout, scatter=None):
+    saved = os.environ.get("CGL_SCATTER")
+    if scatter is not None:
+        os.environ["CGL_SCATTER"] = "1" if scatter else "0"
+    try:
+        import symmetry_amd
+        lat = symmetry_amd.Lattice(nx, ny, nz, precision=prec, sigma=cfg["sigma"], rho0=cfg["rho0"],
+                                   nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], layout=layout)
+    finally:
+        if saved is None:
+            os.environ.pop("CGL_SCATTER", None)
+        else:
+            os.environ["CGL_SCATTER"] = saved
+    lat.init_cylinder(nx / 2 + 0.5, ny / 2 + 0.5, cfg["r0"], 6.0 * cfg["beta"] * 0.134, cfg["amps"])
+    return lat
+
+
+def collide_kernel(lat, prec, layout):
+    """(name, algorithmic bytes per node) of the step's dominant kernel."""
+    if lat.scatter_form():
+        return "k_collide_sc", B_ALG_COLLIDE_SC[prec]
+    return "k_collide", B_ALG_COLLIDE[(prec, layout)]
+
+
+def time_variant(nx, ny, nz, cfg, prec, layout, scatter, steps):
+    """MLUPS of one precision/layout on the bench box (host clock around
+    `steps` graph-replayed steps), and its dominant kernel's average launch
+    time (HIP events on the library's stream) against the HBM roofline."""
+    v = _lattice(nx, ny, nz, cfg, prec, layout, scatter)
+    v.step(4)
+    finish_tuning(v)
+    t1 = time.perf_counter()
+    v.step(steps)
+    v.sync()
+    nodes = nx * ny * nz
+    mlups = nodes * steps / (time.perf_counter() - t1) / 1e6
+    k = max(10, min(steps, 50))
+    _, _, ms_col = v.time_steps(k)
+    name, b = collide_kernel(v, prec, layout)
+    v.close()
+    avg = ms_col / k * 1e-3
+    return round(mlups, 1), {"kernel": name, "alg_bytes_per_node": b, "avg_launch_ms": round(avg * 1e3, 4),
+                             "frac": round(b * nodes / avg / HBM_PEAK, 4)}
+
+
+def run_accuracy(nx, ny, nz, cfg, steps):
+    """The bench box advanced `steps` steps from the same device initial state
+    in plain FP64 (red/blue), FP32 and FP16 shifted storage (factorised,
+    scatter-form phase field): L2- and Linf-relative errors of rho - rho0,
+    phi and u against FP64."""
+    import numpy as np
+
+    def fields(prec, layout):
+        lat = _lattice(nx, ny, nz, cfg, prec, layout)
+        lat.step(steps)
+        m = lat.calmacros()
+        lat.close()
+        return {"rho": m["rho"] - cfg["rho0"], "phi": m["phi"], "u": np.stack([m["ux"], m["uy"], m["uz"]])}
+
+    def rel(a, b):
+        d = a - b
+        return [float(np.linalg.norm(d) / np.linalg.norm(b)), float(np.max(np.abs(d)) / np.max(np.abs(b)))]
+
+    ref = fields("f64", "rb")
+    out = {"steps": steps, "reference": "f64-rb", "norms": "[L2-relative, Linf-relative]"}
+    for prec in ("f32", "f16"):
+        m = fields(prec, "fs")
+        out[f"{prec}-fs"] = {k: rel(m[k], ref[k]) for k in ("rho", "phi", "u")}
+        del m
+    return out
 
 
 def run_ensemble(steps, seq_steps):
@@ -117,12 +206,16 @@ def run_ensemble(steps, seq_steps):
             "sequential_f64_mlups": round(seq_mlups, 1), "ratio": round(batch_mlups / seq_mlups, 2)}
 
 
-def run_particles(steps):
+def run_particles(steps, f64_dense_steps):
     """The particle workloads, red/blue layout (walls and particles run on it):
     one sphere (R=10, density 1.5, 60 degrees) at a flat interface in a 96^3
-    box with z walls, FP64 and FP32; and 200 spheres (R=8, seed 99) within
-    +-24 of the interface in a 512^3 box with z walls, FP32 fields with FP64
-    particle dynamics.  MLUPS over all nodes, particle kernels included."""
+    box with z walls, FP64 and FP32; and 200 spheres (R=8, seed 99, densities
+    in [0.8, 2.0], contact angles in [30, 150] degrees) within +-24 of the
+    interface in a 512^3 box with z walls, FP32 fields with FP64 particle
+    dynamics, and FP64.  MLUPS over all nodes, particle kernels included;
+    per workload the step against its counted bytes (312 B/node FP32, 624
+    FP64), and the fused collide kernel and the moments kernel (HIP events on
+    the library's stream) against the HBM roofline at their algorithmic bytes."""
     import symmetry_amd
     from symmetry_amd import ic
 
@@ -131,24 +224,42 @@ def run_particles(steps):
     p2 = ic.sphere_particles([c2["center"]], c2["rad"], c2["rho_p"], c2["cita_deg"], c2["sigma"])
     c5 = ic.particle_config5()
     p5 = ic.sphere_particles(c5["centers"], c5["rad"], c5["rho_p"], c5["cita_deg"], c5["sigma"])
-    for name, cfg, parts, precs in (("sphere_96", c2, p2, ("f64", "f32")), ("dense_512", c5, p5, ("f32",))):
+    runs = [("sphere_96", c2, p2, "f64", steps), ("sphere_96", c2, p2, "f32", steps),
+            ("dense_512", c5, p5, "f32", steps)]
+    if f64_dense_steps > 0:
+        runs.append(("dense_512", c5, p5, "f64", f64_dense_steps))
+    roof = {}
+    for name, cfg, parts, prec, nsteps in runs:
         n = cfg["nx"]
-        for prec in precs:
-            lat = symmetry_amd.Lattice(n, n, n, precision=prec, sigma=cfg["sigma"], rho0=cfg["rho0"],
-                                       nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], bc=cfg["bc"],
-                                       npar=len(parts["rad"]))
-            lat.init_interface(cfg["z0"], 6.0 * cfg["beta"] * 0.134)
-            lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"],
-                              parts["fg"])
-            lat.step(20)
-            lat.sync()
-            t0 = time.perf_counter()
-            lat.step(steps)
-            lat.sync()
-            dt = time.perf_counter() - t0
-            out[f"{name}_{prec}_mlups"] = round(n ** 3 * steps / dt / 1e6, 1)
-            lat.close()
-    out["steps"] = steps
+        lat = symmetry_amd.Lattice(n, n, n, precision=prec, sigma=cfg["sigma"], rho0=cfg["rho0"],
+                                   nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], bc=cfg["bc"],
+                                   npar=len(parts["rad"]))
+        lat.init_interface(cfg["z0"], 6.0 * cfg["beta"] * 0.134)
+        lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"],
+                          parts["fg"])
+        lat.step(20)
+        finish_tuning(lat)
+        t0 = time.perf_counter()
+        lat.step(nsteps)
+        lat.sync()
+        dt = time.perf_counter() - t0
+        nodes = n ** 3
+        mlups = nodes * nsteps / dt / 1e6
+        out[f"{name}_{prec}_mlups"] = round(mlups, 1)
+        k = 10
+        _, ms_mom, ms_col = lat.time_steps(k)
+        lat.close()
+        b_col, b_mom = B_ALG_COLLIDE[(prec, "rb")], B_ALG_MOMENTS_RB[prec]
+        col, mom = ms_col / k * 1e-3, ms_mom / k * 1e-3
+        roof[f"{name}_{prec}"] = {
+            "steps": nsteps, "step_ms": round(dt / nsteps * 1e3, 4),
+            "step_frac": round(mlups * 1e6 * B_ALG_STEP[prec] / HBM_PEAK, 4),
+            "collide": {"kernel": "k_collide (walls/particles)", "alg_bytes_per_node": b_col,
+                        "avg_launch_ms": round(col * 1e3, 4), "frac": round(b_col * nodes / col / HBM_PEAK, 4)},
+            "moments": {"kernel": "k_moments (walls/particles)", "alg_bytes_per_node": b_mom,
+                        "avg_launch_ms": round(mom * 1e3, 4), "frac": round(b_mom * nodes / mom / HBM_PEAK, 4)},
+            "particle_kernels_ms": round(dt / nsteps * 1e3 - (col + mom) * 1e3, 4)}
+    out["roofline"] = roof
     out["workloads"] = {"sphere_96": "96^3, z walls, one sphere R=10 at the interface",
                         "dense_512": "512^3, z walls, 200 spheres R=8 within +-24 of the interface, seed 99"}
     return out
@@ -171,8 +282,12 @@ def main():
                     help="timed steps of the 64-box batched ensemble (0 = skip)")
     ap.add_argument("--ensemble-seq-steps", type=int, default=100,
                     help="timed steps per box of the sequential FP64 baseline")
+    ap.add_argument("--accuracy-steps", type=int, default=1000,
+                    help="steps of the FP64 / FP32 / FP16 accuracy comparison on the bench box (0 = skip)")
     ap.add_argument("--particle-steps", type=int, default=200,
                     help="timed steps of the particle workloads (0 = skip)")
+    ap.add_argument("--particle-f64-dense-steps", type=int, default=40,
+                    help="timed steps of the 512^3 200-sphere workload in FP64 (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this implementation runs on one GPU only; sharding over GPUs is designed, "
@@ -192,7 +307,7 @@ def main():
     copy_gbps = symmetry_amd.copy_bandwidth(2 << 30, 10)
     lat.init_cylinder(nx / 2 + 0.5, ny / 2 + 0.5, cfg["r0"], 6.0 * cfg["beta"] * 0.134, cfg["amps"])
     lat.step(args.warmup)  # with --xchunk 0 the first warm-up steps also autotune the x chunk
-    lat.sync()
+    finish_tuning(lat)
     xchunk, still_tuning = lat.tuned_xchunk()
     nodes = nx * ny * nz
     # timed region: exactly `steps` steps, synchronised on both sides
@@ -219,45 +334,26 @@ def main():
     traffic = traffic_from_profile(args.precision, args.layout, kname)
     lat.close()
 
-    # the same workload in every precision and layout (MLUPS only)
-    variants = {}
+    # the same workload in every precision and layout: MLUPS, and the
+    # dominant kernel against the HBM roofline at its algorithmic bytes
+    variants, variants_roofline = {}, {}
     if args.variant_steps > 0:
-        for prec in ("f64", "f32", "f16"):
-            for lay in ("rb", "fs"):
-                key = f"{prec}-{lay}"
-                if prec == args.precision and lay == args.layout:
-                    variants[key] = round(mlups, 1)
-                    continue
-                v = symmetry_amd.Lattice(nx, ny, nz, precision=prec, sigma=cfg["sigma"], rho0=cfg["rho0"],
-                                         nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], layout=lay)
-                v.init_cylinder(nx / 2 + 0.5, ny / 2 + 0.5, cfg["r0"], 6.0 * cfg["beta"] * 0.134, cfg["amps"])
-                v.step(20)
-                v.sync()
-                t1 = time.perf_counter()
-                v.step(args.variant_steps)
-                v.sync()
-                variants[key] = round(nodes * args.variant_steps / (time.perf_counter() - t1) / 1e6, 1)
-                v.close()
+        keys = [(p, l, None) for p in ("f64", "f32", "f16") for l in ("rb", "fs")]
         if scatter:
-            # the same box with a moments pass per step (gather form)
-            os.environ["CGL_SCATTER"] = "0"
-            try:
-                v = symmetry_amd.Lattice(nx, ny, nz, precision=args.precision, sigma=cfg["sigma"], rho0=cfg["rho0"],
-                                         nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], layout=args.layout)
-            finally:
-                os.environ.pop("CGL_SCATTER", None)
-            v.init_cylinder(nx / 2 + 0.5, ny / 2 + 0.5, cfg["r0"], 6.0 * cfg["beta"] * 0.134, cfg["amps"])
-            v.step(20)
-            v.sync()
-            t1 = time.perf_counter()
-            v.step(args.variant_steps)
-            v.sync()
-            variants[f"{args.precision}-{args.layout}-gather"] = round(
-                nodes * args.variant_steps / (time.perf_counter() - t1) / 1e6, 1)
-            v.close()
+            keys.append((args.precision, args.layout, False))  # a moments pass per step (gather form)
+        for prec, lay, sc in keys:
+            key = f"{prec}-{lay}" + ("-gather" if sc is False else "")
+            if prec == args.precision and lay == args.layout and sc is None:
+                variants[key] = round(mlups, 1)
+                variants_roofline[key] = {"kernel": kname, "alg_bytes_per_node": b_col,
+                                          "avg_launch_ms": round(avg_col * 1e3, 4),
+                                          "frac": round(achieved / HBM_PEAK, 4)}
+                continue
+            variants[key], variants_roofline[key] = time_variant(nx, ny, nz, cfg, prec, lay, sc, args.variant_steps)
 
+    accuracy = run_accuracy(nx, ny, nz, cfg, args.accuracy_steps) if args.accuracy_steps > 0 else None
     ensemble = run_ensemble(args.ensemble_steps, args.ensemble_seq_steps) if args.ensemble_steps > 0 else None
-    particles = run_particles(args.particle_steps) if args.particle_steps > 0 else None
+    particles = run_particles(args.particle_steps, args.particle_f64_dense_steps) if args.particle_steps > 0 else None
 
     out = {
         "metric": "MLUPS at 256x256x512 D3Q19 CG-MRT, FP32 vs FP64, % of HBM roofline",
@@ -287,7 +383,10 @@ def main():
                           "event_mlups": round(nodes * ksteps / (ev_total * 1e-3) / 1e6, 1)},
         "hbm_copy_gbps": round(copy_gbps, 1),
         "variants_mlups": variants,
+        "variants_roofline": variants_roofline,
     }
+    if accuracy:
+        out["accuracy"] = accuracy
     if ensemble:
         out["ensemble"] = ensemble
     if particles:

@@ -216,8 +216,10 @@ int cgl_box_calmacros(cgl_t *h, int32_t box, double *rho, double *rho_r, double 
                       double *uz, double *phi);
 
 /* Time nsteps steps on the device with HIP events recorded on the handle's
- * stream: total elapsed ms, and ms summed over the moments and fused-collide
- * kernels.  No counterpart in the reference (it has no timers). */
+ * stream: total elapsed ms, and ms summed over the moments (scatter form: the
+ * phase-field pass) and fused-collide kernels; with walls or particles, the
+ * moments and collide kernels alone (the particle kernels between them are in
+ * neither).  No counterpart in the reference (it has no timers). */
 int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moments, double *ms_collide);
 
 /* Achievable HBM bandwidth of a device: a streaming copy of `bytes` bytes,

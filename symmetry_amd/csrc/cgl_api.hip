@@ -56,6 +56,22 @@ int ensure_scratch(cgl_handle *h, bool mac, bool aos, bool iflag)
     return CGL_OK;
 }
 
+// Bytes per node of the largest array the kernels address through a 32-bit
+// byte offset from one base: a group of four direction arrays (grp/grp_off),
+// and in the factorised layout the per-node recolour scalars (16 B in
+// FP32/FP16, 48 B in FP64).  The largest offset, bytes x N minus one element,
+// fits 32 bits when bytes x N <= 4 GiB.
+double addressed_bytes(const cgl_params *p)
+{
+    const double es = p->precision == CGL_PREC_F64 ? 8.0 : (p->precision == CGL_PREC_F32 ? 4.0 : 2.0);
+    double b = 4.0 * es;
+    if (p->layout == CGL_LAYOUT_FS) {
+        const double ns = p->precision == CGL_PREC_F64 ? 48.0 : 16.0;
+        if (ns > b) b = ns;
+    }
+    return b;
+}
+
 int set_device(cgl_handle *h)
 {
     HIPCHK(hipSetDevice(h->p.device));
@@ -195,10 +211,7 @@ int cgl_create(cgl_t **out, const cgl_params *p)
     if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(CGL_ERR_ARG, "box must be at least 1x1x1");
     double nn = (double)p->nx * p->ny * p->nz;
     if (nn > 2.0e9) return fail(CGL_ERR_ARG, "box of %.3g nodes exceeds the 32-bit node index", nn);
-    // kernels address each per-direction array (and the 16-48 B per-node
-    // recolour scalars) through 32-bit byte offsets
-    const double eb = p->precision == CGL_PREC_F64 ? 48.0 : 16.0;
-    if (nn * eb >= 4294967296.0)
+    if (nn * addressed_bytes(p) > 4294967296.0)
         return fail(CGL_ERR_UNSUPPORTED, "box of %.3g nodes exceeds the 4 GiB per-array addressing", nn);
     if (p->precision < CGL_PREC_F64 || p->precision > CGL_PREC_F16)
         return fail(CGL_ERR_ARG, "unknown precision %d", p->precision);
@@ -510,6 +523,16 @@ static const int TUNE_STEPS = 2;
 static int tune_some(cgl_handle *h, int32_t *left)
 {
     while (h->tuning && *left >= TUNE_STEPS) {
+        // the layout conversion, the particles' start-up step and the scatter
+        // form's first step (a moments pass) are not steady steps: run them
+        // before a candidate's timing, so every candidate times the same work
+        while (*left > 0 && needs_eager(h)) {
+            enqueue_step(h);
+            int rc = check_launch("step");
+            if (rc) return rc;
+            --*left;
+        }
+        if (*left < TUNE_STEPS) break;
         h->xc = TUNE_XC[h->tune_idx];
         HIPCHK(hipEventRecord(h->ev[0], h->stream));
         for (int t = 0; t < TUNE_STEPS; ++t) enqueue_step(h);
@@ -655,14 +678,17 @@ int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moment
     // pass 2: per-kernel durations, one event pair around each launch
     double km = 0.0, kc = 0.0;
     for (int t = 0; t < nsteps && h->ext; ++t) {
-        // with walls/particles: one event pair around the whole step
-        float a = 0.f;
-        HIPCHK(hipEventRecord(h->ev[0], h->stream));
+        // with walls/particles: event pairs around the step's moments and
+        // collide kernels, recorded by the step itself
+        float a = 0.f, b = 0.f;
+        h->tev = h->ev;
         enqueue_step(h);
-        HIPCHK(hipEventRecord(h->ev[2], h->stream));
-        HIPCHK(hipEventSynchronize(h->ev[2]));
-        HIPCHK(hipEventElapsedTime(&a, h->ev[0], h->ev[2]));
-        kc += a;
+        h->tev = nullptr;
+        HIPCHK(hipEventSynchronize(h->ev[3]));
+        HIPCHK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
+        HIPCHK(hipEventElapsedTime(&b, h->ev[2], h->ev[3]));
+        km += a;
+        kc += b;
     }
     // scatter form: the phase-field passes (a moments pass where needed, and
     // the finishing pass) count as moments, the step kernel as collide
@@ -895,7 +921,7 @@ int cgl_batch_create(cgl_t **out, const cgl_params *p, int32_t nbox, const cgl_b
     if (p->npar != 0) return fail(CGL_ERR_UNSUPPORTED, "batched boxes carry no particles");
     if (!(p->rho0 > 0)) return fail(CGL_ERR_ARG, "rho0 must be > 0");
     if (p->reserved != 0) return fail(CGL_ERR_ARG, "reserved field must be 0");
-    const double eb = p->precision == CGL_PREC_F64 ? 48.0 : 16.0;
+    const double eb = addressed_bytes(p);
     size_t total = 0, nmax = 0;
     for (int b = 0; b < nbox; ++b) {
         const cgl_box &B = boxes[b];
@@ -903,7 +929,7 @@ int cgl_batch_create(cgl_t **out, const cgl_params *p, int32_t nbox, const cgl_b
         if (!(B.nu_r > 0) || !(B.nu_b > 0)) return fail(CGL_ERR_ARG, "box %d: nu_r, nu_b must be > 0", b);
         if (B.reserved != 0) return fail(CGL_ERR_ARG, "box %d: reserved field must be 0", b);
         const size_t n = (size_t)B.nx * B.ny * B.nz;
-        if ((double)n * eb >= 4294967296.0) return fail(CGL_ERR_UNSUPPORTED, "box %d exceeds the 4 GiB per-array addressing", b);
+        if ((double)n * eb > 4294967296.0) return fail(CGL_ERR_UNSUPPORTED, "box %d exceeds the 4 GiB per-array addressing", b);
         total += n;
         if (n > nmax) nmax = n;
     }
@@ -1074,6 +1100,14 @@ struct CkptHeader {
     double sigma, rho0, nu_r, nu_b, beta;
 };
 static const char CKPT_MAGIC[8] = {'C', 'G', 'L', 'C', 'K', 'P', 'T', '1'};
+static const int CKPT_VERSION = 2;
+
+// a batch's boxes, written after the header: a load must find the same
+// boxes (sizes and viscosities) in the same order
+struct CkptBox {
+    int32_t nx, ny, nz, pad;
+    double nu_r, nu_b;
+};
 
 // the particles' host-side scalars, written as one block
 struct CkptPart {
@@ -1126,7 +1160,7 @@ int cgl_checkpoint_save(cgl_t *h, const char *path)
     CkptHeader H;
     memset(&H, 0, sizeof H);
     memcpy(H.magic, CKPT_MAGIC, 8);
-    H.version = 1;
+    H.version = CKPT_VERSION;
     H.nx = h->p.nx; H.ny = h->p.ny; H.nz = h->p.nz;
     H.precision = h->p.precision; H.layout = h->p.layout; H.npar = h->npar; H.nbox = h->nbox;
     for (int f = 0; f < 6; ++f) H.bc[f] = h->p.bc[f];
@@ -1138,6 +1172,11 @@ int cgl_checkpoint_save(cgl_t *h, const char *path)
     FILE *f = fopen(path, "wb");
     if (!f) return fail(CGL_ERR_ARG, "cannot open %s for writing", path);
     bool ok = fwrite(&H, sizeof H, 1, f) == 1;
+    for (int b = 0; ok && b < h->nbox; ++b) {
+        const cgl::Lat &Lb = h->hboxes[b].L;
+        const CkptBox cb{Lb.nx, Lb.ny, Lb.nz, 0, Lb.nu_r, Lb.nu_b};
+        ok = fwrite(&cb, sizeof cb, 1, f) == 1;
+    }
     if (ok && h->npar) {
         CkptPart Pp;
         memset(&Pp, 0, sizeof Pp);
@@ -1175,9 +1214,9 @@ int cgl_checkpoint_load(cgl_t *h, const char *path)
     FILE *f = fopen(path, "rb");
     if (!f) return fail(CGL_ERR_ARG, "cannot open %s", path);
     CkptHeader H;
-    if (fread(&H, sizeof H, 1, f) != 1 || memcmp(H.magic, CKPT_MAGIC, 8) != 0 || H.version != 1) {
+    if (fread(&H, sizeof H, 1, f) != 1 || memcmp(H.magic, CKPT_MAGIC, 8) != 0 || H.version != CKPT_VERSION) {
         fclose(f);
-        return fail(CGL_ERR_ARG, "%s is not a libcgl checkpoint", path);
+        return fail(CGL_ERR_ARG, "%s is not a libcgl checkpoint of this version", path);
     }
     bool same = H.nx == h->p.nx && H.ny == h->p.ny && H.nz == h->p.nz && H.precision == h->p.precision &&
                 H.layout == h->p.layout && H.npar == h->npar && H.nbox == h->nbox && H.N == (int64_t)h->N &&
@@ -1186,10 +1225,22 @@ int cgl_checkpoint_load(cgl_t *h, const char *path)
     for (int k = 0; k < 6; ++k) same = same && H.bc[k] == h->p.bc[k];
     const bool layout_ok = H.state_layout == cgl::RB ? (h->p.layout == CGL_LAYOUT_RB || H.cur == 0)
                                                      : h->p.layout == CGL_LAYOUT_FS;
+    // a batch's tile tables were built for its x chunk at creation; a single
+    // box takes any chunk, and the autotuner's state must index its table
+    const bool tune_ok = H.tuning == 0 || (H.tuning == 1 && H.tune_idx >= 0 && H.tune_idx < TUNE_N);
     if (!same || !layout_ok || H.cur < 0 || H.cur > 1 || H.phicur < 0 || H.phicur > 1 || H.ntcur < 0 ||
-        H.ntcur > 1 || H.xc < 1) {
+        H.ntcur > 1 || H.xc < 1 || H.xc > h->p.nx + 64 || (h->nbox && (H.xc != h->xc || H.tuning)) || !tune_ok) {
         fclose(f);
         return fail(CGL_ERR_ARG, "%s holds a different configuration", path);
+    }
+    for (int b = 0; b < h->nbox; ++b) {
+        CkptBox cb;
+        const cgl::Lat &Lb = h->hboxes[b].L;
+        if (fread(&cb, sizeof cb, 1, f) != 1 || cb.nx != Lb.nx || cb.ny != Lb.ny || cb.nz != Lb.nz ||
+            cb.nu_r != Lb.nu_r || cb.nu_b != Lb.nu_b) {
+            fclose(f);
+            return fail(CGL_ERR_ARG, "%s holds a batch of other boxes (box %d)", path, b);
+        }
     }
     CkptPart Pp;
     if (h->npar && fread(&Pp, sizeof Pp, 1, f) != 1) {

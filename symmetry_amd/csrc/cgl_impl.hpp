@@ -62,6 +62,7 @@ struct cgl_handle {
     int phi_ok;          // phi[phicur ^ 1] holds the current state's phase field (from the last step)
     int scal_stale;      // the current state's recolour scalars were not written (mid-run only)
     cgl::ScBufs sb;      // fixed-point partial sums
+    hipEvent_t *tev;     // timing runs: events around the wall/particle step's moments and collide kernels
 };
 
 namespace cgl {
@@ -104,7 +105,8 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     // while that already holds the scatter step's phase field (observing the
     // state does not change the next step)
     static int phi_mom(const cgl_handle *h) { return h->phi_ok ? h->phicur : h->phicur ^ 1; }
-    static void moments(cgl_handle *h, bool full)
+    // enc: the scatter form's phase-field buffer (signed minority fractions)
+    static void moments(cgl_handle *h, bool full, bool enc = false)
     {
         const void *i0, *i1;
         state_in(h, i0, i1);
@@ -125,6 +127,12 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         } else if (!full) {
             const int xc = K1_XC;
             dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, (h->L.nx + xc - 1) / xc);
+            if constexpr (SHIFT && std::is_same_v<P, float>) {
+                if (enc) {
+                    k_moments_fsx<S, R, P, SHIFT, true><<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, xc);
+                    return;
+                }
+            }
             k_moments_fsx<S, R, P, SHIFT><<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, xc);
         } else {
             launch(std::integral_constant<int, FS>{});
@@ -152,7 +160,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         h->state_layout = h->layout;
     }
     // ---- batches: one launch per kernel over the tiles of every box
-    static void batch_moments(cgl_handle *h)
+    static void batch_moments(cgl_handle *h, bool enc = false)
     {
         const void *i0, *i1;
         state_in(h, i0, i1);
@@ -162,9 +170,17 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         if (h->state_layout == RB)
             k_moments_batch<RB, S, R, P, SHIFT>
                 <<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, remap, K1_XC, i0, i1, phio, phi);
-        else
+        else {
+            if constexpr (SHIFT && std::is_same_v<P, float>) {
+                if (enc) {
+                    k_moments_batch<FS, S, R, P, SHIFT, true><<<h->tiles1, node_block(), 0, h->stream>>>(
+                        h->dboxes, h->nbox, remap, K1_XC, i0, i1, phio, phi);
+                    return;
+                }
+            }
             k_moments_batch<FS, S, R, P, SHIFT>
                 <<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, remap, K1_XC, i0, i1, phio, phi);
+        }
     }
     static void batch_collide(cgl_handle *h)
     {
@@ -199,9 +215,9 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     {
         if (h->phi_ok) return;
         if (h->nbox)
-            batch_moments(h);
+            batch_moments(h, true);
         else
-            moments(h, false);
+            moments(h, false, true);
     }
     static void sc_collide(cgl_handle *h, bool last)
     {
@@ -361,7 +377,9 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             prime(h);
             h->primed = 1;
         }
+        if (h->tev) (void)hipEventRecord(h->tev[0], h->stream);
         moments_ext(h, false); // calmacros with the masks in force
+        if (h->tev) (void)hipEventRecord(h->tev[1], h->stream);
         P *phi = (P *)h->phi[h->phicur ^ 1];
         const int nb = h->ntcur ^ 1;
         if (h->npar) {
@@ -397,8 +415,10 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             const Ext En = ext_with(h, h->ntype[nb]);
             if (h->npar) // wetted normals next to the particles, read by the collide kernel
                 k_wet_normals<R, P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, phi);
+            if (h->tev) (void)hipEventRecord(h->tev[2], h->stream);
             k_collide<RB, RB, S, R, P, SHIFT, true>
                 <<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
+            if (h->tev) (void)hipEventRecord(h->tev[3], h->stream);
             if (h->npar) {
                 S *ro = (S *)o0, *bo = (S *)o1;
                 k_curve<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(

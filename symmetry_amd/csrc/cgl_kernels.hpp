@@ -89,8 +89,8 @@ template <class T> __device__ __forceinline__ T &at(T *base, unsigned idx)
 
 // Direction arrays in groups of four: direction i lives in the group that
 // starts at base + (i & ~3) N, at element (i & 3) N + k.  Four arrays span
-// less than 4 GiB for every box cgl_create accepts (N < 2^28 for values of
-// 4 bytes or less, N < 89 M for 8-byte ones), so a kernel holds five 64-bit
+// at most 4 GiB for every box cgl_create accepts (N <= 2^28 for values of
+// 4 bytes or less, N <= 2^27 for 8-byte ones), so a kernel holds five 64-bit
 // bases per state array instead of nineteen (for a compile-time i the group
 // bases are shared), which keeps them out of scalar-register spills.
 template <class T> __device__ __forceinline__ T *grp(T *base, int i, size_t N)
@@ -199,15 +199,73 @@ template <class R> __device__ __forceinline__ R dot_c(int i, R nx, R ny, R nz)
 
 // Gathered node values.
 //  FP64: r, b = the reference's post-stream red and blue populations.
-//  shifted: ft_i = f_i - w_i rho0 (deviation of the total) and sr = rho_R.
+//  shifted: ft_i = f_i - w_i rho0 (deviation of the total), sr = rho_R and
+//  sb = rho_B, each summed directly (never as the difference rho - rho_R, which
+//  would leave the minority colour of a nearly pure node to FP32 rounding).
 template <class R, bool SHIFT> struct Node;
 template <class R> struct Node<R, false> {
     R r[Q], b[Q];
 };
 template <class R> struct Node<R, true> {
     R ft[Q];
-    R sr;
+    R sr, sb;
 };
+
+// Shifted storage (FP32/FP16) holds each colour's populations as deviations
+// from w_i rho0 s of the node that reads them, with the red and blue shift
+// fractions s_red + s_blue = 1 taken from that node's phase field: the
+// minority colour's fraction hm = (1 - |phi|)/2 (exact: 1 - |phi| has no
+// rounding for |phi| >= 1/2) and the majority's 1 - hm.  A colour that is
+// nearly absent at a node thus keeps FP32's relative precision through the
+// shifts; pure phases shift by exactly 0 and 1.
+template <class R> struct Shift {
+    R hm;        // minority fraction
+    bool redmaj; // phi >= 0: red is the majority colour
+};
+template <class R, class P> __device__ __forceinline__ Shift<R> shift_of(P ph)
+{
+    Shift<R> s;
+    s.redmaj = ph >= P(0);
+    s.hm = R(0.5) * (R(1) - R(ph < P(0) ? -ph : ph));
+    return s;
+}
+// Shift fractions in FP64 (exact for an FP32 phase field): host-layout
+// conversions and the particle kernels.
+__device__ __forceinline__ double shift_d(float ph, int red)
+{
+    const double hm = 0.5 * (1.0 - (double)(ph < 0.0f ? -ph : ph));
+    const double sred = ph >= 0.0f ? 1.0 - hm : hm;
+    return red ? sred : 1.0 - sred;
+}
+
+// Signed colour fraction of the factorised layout's FP32 recolour scalars:
+// +A (red fraction) where blue is the majority, -B (blue fraction) where red
+// is, so that the minority fraction is stored to FP32's relative precision;
+// a pure red node stores -0, a pure blue one +0.
+template <class R> __device__ __forceinline__ void decode_frac(float x, R &A, R &B);
+
+// The scatter form's phase-field buffer holds the signed minority fraction q
+// of each node's colour split: -rho_B/rho where red is the majority (-0 for a
+// pure red node), +rho_R/rho where blue is (+0 for pure blue), so that a
+// nearly absent colour keeps FP32's relative precision; phi = 1 + 2q or
+// 2q - 1.
+template <class R> __device__ __forceinline__ float minority_q(R sr, R sb, R irho)
+{
+    if (sb <= sr) return -__builtin_fmaxf(float(sb * irho), 0.0f);
+    return __builtin_fmaxf(float(sr * irho), 0.0f);
+}
+__device__ __forceinline__ float phi_of_q(float q) { return __builtin_signbitf(q) ? 1.0f + 2.0f * q : 2.0f * q - 1.0f; }
+
+template <class R> __device__ __forceinline__ void decode_frac(float x, R &A, R &B)
+{
+    if (__builtin_signbitf(x)) {
+        B = R(-x);
+        A = R(1) - B;
+    } else {
+        A = R(x);
+        B = R(1) - A;
+    }
+}
 
 // Recolour scalars of one node in the FP64 factorised layout.
 struct FsScal64 {
@@ -238,13 +296,16 @@ __device__ __forceinline__ void gather_fs(const S *__restrict__ fs, size_t N, co
             nd.b[i] = f - r;
         }
     } else {
-        R a0 = 0, a1 = 0, a2 = 0, dev = 0, g1 = 0, g2 = 0;
+        R a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0, dva = 0, dvb = 0, g1 = 0, g2 = 0;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
             R f = StoreOf<S>::ld(grp(fs, i, N), grp_off(i, N, m));
             const float4 g = scal(i, m);
-            dev += R(g.x) * f;
+            R A, B;
+            decode_frac(g.x, A, B);
+            dva += A * f;
+            dvb += B * f;
             R gc = 0;
             if (CX[i] == 1) gc += R(g.y);
             if (CX[i] == -1) gc -= R(g.y);
@@ -252,17 +313,20 @@ __device__ __forceinline__ void gather_fs(const S *__restrict__ fs, size_t N, co
             if (CY[i] == -1) gc -= R(g.z);
             if (CZ[i] == 1) gc += R(g.w);
             if (CZ[i] == -1) gc -= R(g.w);
-            if (i == 0) a0 = R(g.x);
-            else if (i <= 6) { a1 += R(g.x); g1 += gc; }
-            else { a2 += R(g.x); g2 += gc; }
+            if (i == 0) { a0 = A; b0 = B; }
+            else if (i <= 6) { a1 += A; b1 += B; g1 += gc; }
+            else { a2 += A; b2 += B; g2 += gc; }
             nd.ft[i] = f;
         }
-        nd.sr = rho0 * (a0 * R(W[0]) + a1 * R(W[1]) + a2 * R(W[7])) + dev + (g1 * R(W[1]) + g2 * R(W[7]));
+        const R gw = g1 * R(W[1]) + g2 * R(W[7]);
+        nd.sr = rho0 * (a0 * R(W[0]) + a1 * R(W[1]) + a2 * R(W[7])) + dva + gw;
+        nd.sb = rho0 * (b0 * R(W[0]) + b1 * R(W[1]) + b2 * R(W[7])) + dvb - gw;
     }
 }
 
 // Pull gather at node (x, y, z) from x - c_i (streaming, S:743-753).
-// ax: red shift fraction of this node (RB shifted storage only).
+// ax: this node's phase field of the stored state, which fixes its shift
+// fractions (RB shifted storage only).
 // Half-way wall bounce-back in pull form: a population whose upstream node is
 // a wall ghost layer is the node's own opposite population (boundary then
 // streaming, S:1557-1590 and S:743-753).  bbmask bit i set = direction i.
@@ -378,8 +442,9 @@ __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void 
                 nd.b[i] = StoreOf<S>::ld(grp(bs, d, N), k);
             });
         } else {
-            // deviations from w_i rho0 ax and w_i rho0 (1 - ax): small, summed plainly
-            R sr = 0;
+            // deviations from w_i rho0 s_red and w_i rho0 s_blue of this node:
+            // small, summed plainly, each colour by itself
+            R sr = 0, sb = 0;
             static_for<Q>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 unsigned m;
@@ -389,9 +454,13 @@ __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void 
                 R rv = StoreOf<S>::ld(grp(rs, d, N), k);
                 R bv = StoreOf<S>::ld(grp(bs, d, N), k);
                 sr += rv;
+                sb += bv;
                 nd.ft[i] = rv + bv;
             });
-            nd.sr = sr + rho0 * ax;
+            const Shift<R> sh = shift_of<R>(ax);
+            const R smaj = R(1) - sh.hm;
+            nd.sr = sr + rho0 * (sh.redmaj ? smaj : sh.hm);
+            nd.sb = sb + rho0 * (sh.redmaj ? sh.hm : smaj);
         }
     } else {
         const S *fs = (const S *)p0;
@@ -474,12 +543,13 @@ __device__ __forceinline__ Moments<R, SHIFT> moments(const Lat &L, const Node<R,
         m.rho = R(L.rho0) + df;
         m.drho = df;
         m.sr = nd.sr;
-        m.sb = m.rho - m.sr;
+        m.sb = nd.sb;
         const R irho = fdiv(R(1), m.rho);
         m.ux = jx * irho;
         m.uy = jy * irho;
         m.uz = jz * irho;
-        m.phi = (m.sr - m.sb) * irho;
+        // phi = (rho_R - rho_B)/rho from the minority colour (S:799)
+        m.phi = m.sb <= m.sr ? R(1) - R(2) * m.sb * irho : R(2) * m.sr * irho - R(1);
     }
     return m;
 }
@@ -526,8 +596,8 @@ __device__ __forceinline__ void moments_node(const Lat &L, const void *__restric
     unsigned xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
     const unsigned n = xo[1] + yo[1] + zo[1];
-    R ax = 0;
-    if constexpr (SHIFT && LIN == RB) ax = R(0.5) * (R(1) + R(at(phio, n)));
+    R ax = 0; // the node's phase field of the stored state: its shift fractions
+    if constexpr (SHIFT && LIN == RB) ax = R(at(phio, n));
     Node<R, SHIFT> nd;
     unsigned nt = 0;
     if constexpr (EXT) {
@@ -590,7 +660,9 @@ template <int ROWS> struct FsxRing<false, ROWS> {
     double s[3][5][ROWS + 2][TZ + 2];
 };
 
-template <class S, class R, class P, bool SHIFT>
+// ENC: write the signed minority fraction (the scatter form's phase-field
+// buffer) instead of phi.
+template <class S, class R, class P, bool SHIFT, bool ENC = false>
 __device__ __forceinline__ void moments_fsx_tile(const Lat &L, const void *__restrict__ p0,
                                                  const void *__restrict__ p1, const P *__restrict__ phio,
                                                  P *__restrict__ phi, int xc, unsigned bx, unsigned by, unsigned bz)
@@ -645,19 +717,22 @@ __device__ __forceinline__ void moments_fsx_tile(const Lat &L, const void *__res
             };
             gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, scal, nd, R(L.rho0));
             const Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
-            at(phi, xo[1] + yo[1] + zo[1]) = P(mo.phi);
+            if constexpr (ENC)
+                at(phi, xo[1] + yo[1] + zo[1]) = P(minority_q(mo.sr, mo.sb, fdiv(R(1), mo.rho)));
+            else
+                at(phi, xo[1] + yo[1] + zo[1]) = P(mo.phi);
         }
         __syncthreads(); // the next load overwrites the oldest slot
     }
 }
 
-template <class S, class R, class P, bool SHIFT>
+template <class S, class R, class P, bool SHIFT, bool ENC = false>
 __global__ __launch_bounds__(256) void k_moments_fsx(Lat L, const void *__restrict__ p0, const void *__restrict__ p1,
                                                      const P *__restrict__ phio, P *__restrict__ phi, int xc)
 {
     unsigned bx, by, bz;
     tile_of_block(L, bx, by, bz);
-    moments_fsx_tile<S, R, P, SHIFT>(L, p0, p1, phio, phi, xc, bx, by, bz);
+    moments_fsx_tile<S, R, P, SHIFT, ENC>(L, p0, p1, phio, phi, xc, bx, by, bz);
 }
 
 // ----------------------------------------------------- gradient of phi (b)
@@ -948,25 +1023,51 @@ __device__ __forceinline__ void collide(const Lat &L, R f[Q], const Moments<R, S
 template <class R> struct Recol {
     R A, gam;
     bool evr, evb;
+    R M;         // shifted storage: the minority colour's fraction (B where red is the majority, else A)
+    bool redmaj; // shifted storage: rho_R >= rho_B
 };
 template <class R, bool SHIFT>
 __device__ __forceinline__ Recol<R> recol(const Lat &L, const Moments<R, SHIFT> &mo, bool flag)
 {
     Recol<R> c;
-    c.A = fdiv(mo.sr, mo.rho);
+    if constexpr (SHIFT) {
+        c.redmaj = mo.sb <= mo.sr;
+        c.M = fdiv(c.redmaj ? mo.sb : mo.sr, mo.rho);
+        c.A = c.redmaj ? R(1) - c.M : c.M;
+    } else {
+        c.A = fdiv(mo.sr, mo.rho);
+    }
     c.gam = flag ? fdiv(mo.sr * mo.sb, mo.rho) * R(L.beta) : R(0);
     c.evr = mo.sr <= R(EVAP_LIM);
     c.evb = mo.sb <= R(EVAP_LIM);
     return c;
 }
 
+// Signed colour fraction (see decode_frac) after the EVAP_LIM overrides.
+template <class R> __device__ __forceinline__ float encode_frac(const Recol<R> &c)
+{
+    if (c.evb) return -0.0f;
+    if (c.evr) return 0.0f;
+    return c.redmaj ? -float(c.M) : float(c.M);
+}
+
+// Red shift fraction of a node with shift sh minus that of a node with shift
+// t, with the rounding of a difference of two small fractions where both
+// nodes have the same majority colour.
+template <class R> __device__ __forceinline__ R red_shift_diff(const Shift<R> &sh, const Shift<R> &t)
+{
+    if (sh.redmaj == t.redmaj) return sh.redmaj ? t.hm - sh.hm : sh.hm - t.hm;
+    return sh.redmaj ? (R(1) - sh.hm) - t.hm : sh.hm - (R(1) - t.hm);
+}
+
 // Output writers.  RB: the recoloured red and blue populations (S:713-726);
-// shifted RB stores deviations from w_i rho0 aw_i and w_i rho0 (1 - aw_i),
-// aw_i = red fraction of the node that reads direction i next.  FS: the
-// post-collision f and the node's recolour scalars.
+// shifted RB stores deviations from w_i rho0 s_red and w_i rho0 s_blue of the
+// node that reads direction i next (its phase field phw_i), the node's
+// minority colour formed directly and the majority colour as the complement
+// of the total.  FS: the post-collision f and the node's recolour scalars.
 template <int LOUT, class S, class R, bool SHIFT>
 __device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, unsigned n, const R f[Q],
-                                           const Recol<R> &c, R nx, R ny, R nz, const R aw[Q])
+                                           const Recol<R> &c, R nx, R ny, R nz, const R phw[Q])
 {
     const size_t N = (size_t)L.N;
     if constexpr (LOUT == RB) {
@@ -984,16 +1085,36 @@ __device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, uns
             }
         } else {
             const R rho0 = R(L.rho0);
+            const Shift<R> me{c.M, c.redmaj};
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 const R dot = R(CX[i]) * nx + R(CY[i]) * ny + R(CZ[i]) * nz;
-                const R a = aw[i];
-                R v = c.A * f[i] + R(W[i]) * rho0 * (c.A - a) + c.gam * R(W[i]) * dot;
-                if (c.evr) v = -R(W[i]) * rho0 * a;
-                if (c.evb) v = f[i] + R(W[i]) * rho0 * (R(1) - a);
+                const Shift<R> rd = shift_of<R>(phw[i]);
+                const R wr = R(W[i]) * rho0;
+                // r~ = A f~ + w rho0 (A - s_red) + g w n.c and b~ = f~ - r~, the
+                // minority one formed first
+                const R dsh = red_shift_diff(me, rd);
+                const R gd = c.gam * R(W[i]) * dot;
+                R vr, vb;
+                if (c.redmaj) {
+                    vb = c.M * f[i] - wr * dsh - gd;
+                    vr = f[i] - vb;
+                } else {
+                    vr = c.M * f[i] + wr * dsh + gd;
+                    vb = f[i] - vr;
+                }
+                const R sred = rd.redmaj ? R(1) - rd.hm : rd.hm, sblue = rd.redmaj ? rd.hm : R(1) - rd.hm;
+                if (c.evr) { // r = 0
+                    vr = -wr * sred;
+                    vb = f[i] - vr;
+                }
+                if (c.evb) { // b = 0
+                    vb = -wr * sblue;
+                    vr = f[i] - vb;
+                }
                 const unsigned k = grp_off(i, N, n);
-                StoreOf<S>::st(grp(ro, i, N), k, v);
-                StoreOf<S>::st(grp(bo, i, N), k, f[i] - v);
+                StoreOf<S>::st(grp(ro, i, N), k, vr);
+                StoreOf<S>::st(grp(bo, i, N), k, vb);
             }
         }
     } else {
@@ -1019,7 +1140,7 @@ __device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, uns
             at(sc, 3 * n + 2) = make_double2(nz, 0.0);
         } else {
             typedef float f4v __attribute__((ext_vector_type(4)));
-            const f4v v4 = {float(A), float(g * nx), float(g * ny), float(g * nz)};
+            const f4v v4 = {encode_frac(c), float(g * nx), float(g * ny), float(g * nz)};
             __builtin_nontemporal_store(v4, (f4v *)&at((float4 *)o1, n));
         }
     }
@@ -1276,15 +1397,16 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                 for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
         }
 
-        R ax = 0, aw[Q];
-        if constexpr (SHIFT && LIN == RB) ax = R(0.5) * (R(1) + R(at(phio, n)));
+        R ax = 0, aw[Q]; // phase fields fixing the shift fractions: this node's of the stored
+                         // state, and aw_i of the node that reads direction i next
+        if constexpr (SHIFT && LIN == RB) ax = R(at(phio, n));
         if constexpr (SHIFT && LOUT == RB) {
             // shift of the slot read by x + c_i at the next step: that node's phi
             // now; a slot that bounces back off a wall is read by x itself
-            const R aself = R(0.5) * (R(1) + R(sm.phi[slot5(x)][ty + 2][tz + 2]));
+            const R aself = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                aw[i] = R(0.5) * (R(1) + R(sm.phi[slot5(x + CX[i])][ty + 2 + CY[i]][tz + 2 + CZ[i]]));
+                aw[i] = R(sm.phi[slot5(x + CX[i])][ty + 2 + CY[i]][tz + 2 + CZ[i]]);
                 if constexpr (EXT)
                     if ((bb >> OPC[i]) & 1u) aw[i] = aself;
             }
@@ -1306,7 +1428,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                     const unsigned ks = grp_off(dsrc, N, m), kd = grp_off(i, N, n);
                     R rv = StoreOf<S>::ld(grp(rs, dsrc, N), ks), bv = StoreOf<S>::ld(grp(bs, dsrc, N), ks);
                     if constexpr (SHIFT) {
-                        const R dsh = R(W[i]) * R(L.rho0) * (ax - aw[i]);
+                        const R dsh = R(W[i]) * R(L.rho0) * red_shift_diff(shift_of<R>(ax), shift_of<R>(aw[i]));
                         rv += dsh;
                         bv -= dsh;
                     }
@@ -1523,7 +1645,7 @@ __device__ __forceinline__ void box_state(const void *&a, const void *&b, unsign
 
 // Moments over x chunks of xc planes: the x-marching kernel for the
 // factorised layout, plane by plane for red/blue.
-template <int LIN, class S, class R, class P, bool SHIFT>
+template <int LIN, class S, class R, class P, bool SHIFT, bool ENC = false>
 __global__ __launch_bounds__(256) void k_moments_batch(const BoxDev *__restrict__ B, int nbox, int remap, int xc,
                                                        const void *__restrict__ p0, const void *__restrict__ p1,
                                                        const P *__restrict__ phio, P *__restrict__ phi)
@@ -1537,7 +1659,7 @@ __global__ __launch_bounds__(256) void k_moments_batch(const BoxDev *__restrict_
     const void *a = p0, *c = p1;
     box_state<LIN, S, SHIFT>(a, c, base);
     if constexpr (LIN == FS) {
-        moments_fsx_tile<S, R, P, SHIFT>(L, a, c, phio + base, phi + base, xc, bx, by, cx);
+        moments_fsx_tile<S, R, P, SHIFT, ENC>(L, a, c, phio + base, phi + base, xc, bx, by, cx);
     } else {
         const int x1 = min((int)(cx + 1) * xc, L.nx);
         for (int x = cx * xc; x < x1; ++x)
@@ -1610,14 +1732,7 @@ __global__ __launch_bounds__(256) void k_phi_aos(Lat L, const double *__restrict
     phi[n] = P((sr - sb) / (sr + sb));
 }
 
-// Shift reference a of a stored slot, computed exactly as the readers do.
-template <class P> __device__ __forceinline__ float shift_a(P ph, int red)
-{
-    float a = 0.5f * (1.0f + float(ph));
-    return red ? a : 1.0f - a;
-}
-
-// Pack one colour: S_i(y) = P_i(y + c_i) - w_i rho0 a(y + c_i) (inverse of
+// Pack one colour: S_i(y) = P_i(y + c_i) - w_i rho0 s(y + c_i) (inverse of
 // the periodic streaming permutation, so that the next pull gather returns P;
 // the shift is that of the reading node y + c_i).
 // With wall faces, a slot whose reader x + c_i is a ghost node holds the
@@ -1650,7 +1765,7 @@ __global__ __launch_bounds__(256) void k_pack(Lat L, const double *__restrict__ 
             }
         }
         double v = aos[(size_t)src * Q + qi];
-        if constexpr (SHIFT) v = v - W[i] * (L.rho0 * (double)shift_a(phio[src], red));
+        if constexpr (SHIFT) v = v - W[i] * (L.rho0 * shift_d(float(phio[src]), red));
         StoreOf<S>::st(soa + (size_t)i * N + n, (decltype(StoreOf<S>::ld(soa)))v);
     }
 }
@@ -1684,7 +1799,7 @@ __global__ __launch_bounds__(256) void k_unpack(Lat L, const S *__restrict__ soa
         int qi;
         pull_slot<i>(xo, yo, zo, bb, madj, src, qi);
         double v = double(StoreOf<S>::ld(soa + (size_t)qi * N + src));
-        if constexpr (SHIFT) v = v + W[i] * (L.rho0 * (double)shift_a(phio[n], red));
+        if constexpr (SHIFT) v = v + W[i] * (L.rho0 * shift_d(float(phio[n]), red));
         aos[(size_t)n * Q + i] = v;
     });
 }
@@ -1712,8 +1827,16 @@ __global__ __launch_bounds__(256) void k_unpack_fs(Lat L, const S *__restrict__ 
             if (i > 0) r = r + (sc[1] * W[i]) * dot_c<double>(i, sc[2], sc[3], sc[4]);
         } else {
             const float4 g = ((const float4 *)scv)[m];
+            double A, B;
+            decode_frac(g.x, A, B); // exact in FP64
             f = f + W[i] * L.rho0;
-            r = double(g.x) * f + W[i] * (CX[i] * double(g.y) + CY[i] * double(g.z) + CZ[i] * double(g.w));
+            const double gd = W[i] * (CX[i] * double(g.y) + CY[i] * double(g.z) + CZ[i] * double(g.w));
+            if (__builtin_signbitf(g.x)) { // red majority: the blue part first
+                const double b = B * f - gd;
+                r = f - b;
+            } else {
+                r = A * f + gd;
+            }
         }
         aos[(size_t)n * Q + i] = red ? r : f - r;
     }
@@ -1794,8 +1917,8 @@ __global__ __launch_bounds__(256) void k_init_pops(Lat L, CylIC ic, S *__restric
         double vr = rr * W[i], vb = rb * W[i];
         if constexpr (SHIFT) {
             const P phs = phio[((size_t)r[0] * L.ny + r[1]) * L.nz + r[2]]; // the value pass 1 stored for the reader
-            vr -= W[i] * (L.rho0 * (double)shift_a(phs, 1));
-            vb -= W[i] * (L.rho0 * (double)shift_a(phs, 0));
+            vr -= W[i] * (L.rho0 * shift_d(float(phs), 1));
+            vb -= W[i] * (L.rho0 * shift_d(float(phs), 0));
         }
         StoreOf<S>::st(rs + (size_t)i * N + s, (decltype(StoreOf<S>::ld(rs)))vr);
         StoreOf<S>::st(bs + (size_t)i * N + s, (decltype(StoreOf<S>::ld(bs)))vb);

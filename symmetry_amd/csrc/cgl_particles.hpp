@@ -435,11 +435,10 @@ __global__ __launch_bounds__(PB) void k_curve(Lat L, Ext E, Part Pt, const unsig
             double vb = double(StoreOf<S>::ld(bo + (size_t)o * N, n));
             if constexpr (SHIFT) {
                 // slot o of x is read by x + c_o = m; slot i of m is read by x
-                const double am = 0.5 * (1.0 + double(phis[m])), ax = 0.5 * (1.0 + double(phis[n]));
-                vr += W[o] * L.rho0 * am;
-                vb += W[o] * L.rho0 * (1.0 - am);
-                vr = vr + 6.0 * W[i] * rr * star - W[i] * L.rho0 * ax;
-                vb = vb + 6.0 * W[i] * rbv * star - W[i] * L.rho0 * (1.0 - ax);
+                vr += W[o] * L.rho0 * shift_d(phis[m], 1);
+                vb += W[o] * L.rho0 * shift_d(phis[m], 0);
+                vr = vr + 6.0 * W[i] * rr * star - W[i] * L.rho0 * shift_d(phis[n], 1);
+                vb = vb + 6.0 * W[i] * rbv * star - W[i] * L.rho0 * shift_d(phis[n], 0);
             } else {
                 vr = vr + 6.0 * W[i] * rr * star;
                 vb = vb + 6.0 * W[i] * rbv * star;

@@ -18,9 +18,12 @@
 // Sums are fixed point: each term is rounded to a multiple of rho0 * 2^-29
 // and added as a 32-bit integer.  Integer sums are exact, so a node's sums do
 // not depend on the order or grouping of their 19 terms, and results are the
-// same for every workgroup box decomposition (x chunk).  Pure phases sum to
-// an exact zero (A = 0 or 1, G = 0), so the EVAP_LIM rules of Recolor
-// (S:720-725) see exact zeros there.  Densities up to 4 rho0 fit.
+// same for every workgroup box decomposition (x chunk).  Each term's
+// minority colour is formed directly and the majority colour is the integer
+// complement of the total, so a nearly pure node's small colour density is
+// not swamped by the rounding of the large one; pure phases sum to an exact
+// zero (A = 0 or 1, G = 0), so the EVAP_LIM rules of Recolor (S:720-725) see
+// exact zeros there.  Densities up to 4 rho0 fit.
 //
 // Summation: a workgroup owns a box of TZ (z) x 8 (y) x xc (x) nodes and marches
 // x.  A thread keeps its own node's sums for the destination planes x-1, x,
@@ -78,13 +81,17 @@ __device__ __forceinline__ void add2(int2 &a, int2 b)
 }
 __device__ __forceinline__ int2 shfl2(int2 v, int src) { return make_int2(__shfl(v.x, src), __shfl(v.y, src)); }
 
-// phi from the sums (IEEE division: a one-sided rounding of the colour split
-// is amplified by the dynamics); pure phases give exactly -1 and 1
-__device__ __forceinline__ float phi_of_sums(int2 s)
+// The phase-field buffer entry (signed minority fraction, minority_q) of a
+// node from its sums: the minority sum over the total, IEEE division (a
+// one-sided rounding of the colour split is amplified by the dynamics).  Pure
+// phases give exactly -0 (red) and +0 (blue), phi = 1 and -1.
+__device__ __forceinline__ float q_of_sums(int2 s)
 {
-    if (s.x == 0) return -1.0f;
-    if (s.y == 0) return 1.0f;
-    return float(s.x - s.y) / float(s.x + s.y);
+    const int r = s.x > 0 ? s.x : 0, b = s.y > 0 ? s.y : 0;
+    if (r == 0) return 0.0f;
+    if (b == 0) return -0.0f;
+    const float tot = float(r + b);
+    return b <= r ? -(float(b) / tot) : float(r) / tot;
 }
 
 // Shapes the scatter step takes: every workgroup box has at least two rows
@@ -140,8 +147,8 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     auto load_phi = [&](int p) {
         const unsigned gx = (unsigned)wrapm(p, L.nx) * sx;
         float *dst = &sm.phi[slot5(p)][0][0];
-        dst[pj0] = at(phi, gx + po0);
-        if (pj1 >= 0) dst[pj1] = at(phi, gx + po1);
+        dst[pj0] = phi_of_q(at(phi, gx + po0));
+        if (pj1 >= 0) dst[pj1] = phi_of_q(at(phi, gx + po1));
     };
     // the ring's next plane is fetched one plane ahead into registers, so its
     // load latency overlaps a plane of work
@@ -153,8 +160,8 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     };
     auto store_phi = [&](int p) {
         float *dst = &sm.phi[slot5(p)][0][0];
-        dst[pj0] = pf0;
-        if (pj1 >= 0) dst[pj1] = pf1;
+        dst[pj0] = phi_of_q(pf0);
+        if (pj1 >= 0) dst[pj1] = phi_of_q(pf1);
     };
     // unit normals of plane p on the (KY+2) x (TZ+2) halo region (S:498-532).
     // Each thread computes its own node first (keeping that node's gradient)
@@ -304,23 +311,36 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             }
             // moments (S:792-799): rho and j from the gathered populations, the
             // colour split from the scattered phase field
-            const R phin = sm.phi[slot5(x)][ty + 2][tz + 2];
+            // the colour split: the minority colour's fraction hm from the
+            // phase-field buffer (its signed minority fraction), so that a
+            // nearly pure node keeps its small colour density to FP32's
+            // relative precision; the majority colour is the complement
+            const float qn = at(phi, (unsigned)x * sx + nself);
+            const bool redmaj = __builtin_signbitf(qn);
+            const R hm = __builtin_fabsf(qn);
+            const R phin = phi_of_q(qn);
             Moments<R, true> mo;
             mo.rho = rho0 + df;
             mo.drho = df;
             const R irho = fdiv(R(1), mo.rho);
-            mo.sr = R(0.5) * mo.rho * (R(1) + phin);
-            mo.sb = mo.rho - mo.sr;
+            const R smin = mo.rho * hm;
+            mo.sr = redmaj ? mo.rho - smin : smin;
+            mo.sb = redmaj ? smin : mo.rho - smin;
             mo.ux = jx * irho;
             mo.uy = jy * irho;
             mo.uz = jz * irho;
             mo.phi = phin;
             collide<R, true>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, flag, d);
             const Recol<R> rc = recol<R, true>(L, mo, flag);
-            R A = rc.A, g = rc.gam;
-            if (rc.evr) { A = 0; g = 0; }
-            if (rc.evb) { A = 1; g = 0; }
+            // recolouring r_i = A f_i + g w_i n.c_i (S:713-726), with the
+            // EVAP_LIM overrides; the minority colour's part of f_i is
+            // Mf f_i + w_i (Mg n).c_i (red: A = hm, g; blue: 1 - A = hm, -g)
+            R g = rc.gam;
+            R Mf = hm;
+            if (rc.evr) { g = 0; Mf = redmaj ? R(1) : R(0); }
+            if (rc.evb) { g = 0; Mf = redmaj ? R(0) : R(1); }
             const R Gx = g * nx, Gy = g * ny, Gz = g * nz;
+            const R Mg = redmaj ? -g : g;
 
             // the next state: post-collision totals, and the recolour scalars on
             // a run's last step; streamed past the caches
@@ -335,27 +355,37 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             }
             if (sout) {
                 typedef float f4v __attribute__((ext_vector_type(4)));
-                const f4v v4 = {float(A), float(Gx), float(Gy), float(Gz)};
+                Recol<R> ce = rc;
+                ce.M = hm;
+                ce.redmaj = redmaj;
+                const f4v v4 = {encode_frac(ce), float(Gx), float(Gy), float(Gz)};
                 __builtin_nontemporal_store(v4, (f4v *)&at(sout, n));
             }
 
-            // the terms in fixed-point units: f_i = f~_i qs + w_i 2^29 and
-            // r_i = A f_i + (w_i qs G).c_i, rounded to the nearest integer,
+            // the terms in fixed-point units, rounded to the nearest integer,
             // halves up (a one-sided rounding such as truncation would bias
-            // every sum by up to 19 units)
+            // every sum by up to 19 units): the total f_i = f~_i qs + w_i 2^29,
+            // the minority colour's part Mf f_i + (w_i qs Mg n).c_i, small where
+            // the node is nearly pure, and the majority colour's part as the
+            // integer complement.  (At 2^29 units an FP32 value resolves only
+            // 2-16 units, so a majority term formed in FP32 would leave a
+            // minority complement of noise.)  Both colour parts take the same
+            // FP32 total f_i, so the rounding of its constant w_i 2^29 splits in
+            // proportion to the colour fractions and leaves phi unbiased.
             const R s1 = R(W[1]) * qs, s7 = R(W[7]) * qs;
-            const R G1x = Gx * s1, G1y = Gy * s1, G1z = Gz * s1;
-            const R G7x = Gx * s7, G7y = Gy * s7, G7z = Gz * s7;
+            const R Mgx = Mg * nx, Mgy = Mg * ny, Mgz = Mg * nz;
+            const R G1x = Mgx * s1, G1y = Mgy * s1, G1z = Mgz * s1;
+            const R G7x = Mgx * s7, G7y = Mgy * s7, G7z = Mgz * s7;
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
                 R v = f[i];
                 if constexpr (sizeof(S) != 4) v = R(S(v)); // the value the next step reads
                 const R fq = __builtin_fmaf(v, qs, R(W[i] * double(SC_ONE)));
-                R cq = A * fq;
-                if (i > 0 && i <= 6) cq = __builtin_fmaf(A, fq, dot_c<R>(i, G1x, G1y, G1z));
-                if (i > 6) cq = __builtin_fmaf(A, fq, dot_c<R>(i, G7x, G7y, G7z));
-                const int qf = round_half_up(fq), qr = round_half_up(cq);
-                const int2 c = make_int2(qr, qf - qr);
+                R cq = Mf * fq;
+                if (i > 0 && i <= 6) cq = __builtin_fmaf(Mf, fq, dot_c<R>(i, G1x, G1y, G1z));
+                if (i > 6) cq = __builtin_fmaf(Mf, fq, dot_c<R>(i, G7x, G7y, G7z));
+                const int qf = round_half_up(fq), qm = round_half_up(cq);
+                const int2 c = redmaj ? make_int2(qf - qm, qm) : make_int2(qm, qf - qm);
                 const int k = CX[i] + 1;
                 if (CY[i] == 0 && CZ[i] == 0) g00[k] = c;
                 else if (CY[i] == 0 && CZ[i] == 1) gzp[k] = c;
@@ -551,7 +581,7 @@ __device__ __forceinline__ void phi_sc_node(const Lat &L, const ScBufs &B, int x
         if (xhi) add2(s, at(B.exzhi, c));
         if (fy) add2(s, at(B.eyz, c));
     }
-    at(phi, n) = phi_of_sums(s);
+    at(phi, n) = q_of_sums(s);
 }
 
 static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, float *__restrict__ phi)
@@ -601,7 +631,7 @@ static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc,
             if (fy) add2(s[k], at(B.eyz, c));
         }
     }
-    at((float2 *)phi, h) = make_float2(phi_of_sums(s[0]), phi_of_sums(s[1]));
+    at((float2 *)phi, h) = make_float2(q_of_sums(s[0]), q_of_sums(s[1]));
 }
 
 // The batch's finishing pass over the moments launch's tiles (z tile, y tile,

@@ -119,6 +119,28 @@ def test_batch_restart_bit_exact(tmp_path):
         assert _same(a.download(k), b.download(k)), k
 
 
+def test_batch_load_rejects_other_chunk_or_boxes(tmp_path):
+    """A batch's tile tables belong to its creation x chunk, and its packed
+    state to its box list: a checkpoint loads only into the same batch."""
+    a, cfgs, _ = _batch("f32", "fs", xchunk=8)
+    a.step(3)
+    path = tmp_path / "b.bin"
+    a.save(path)
+    other_chunk, _, _ = _batch("f32", "fs", xchunk=16)
+    with pytest.raises(symmetry_amd.CglError):
+        other_chunk.load(path)
+    boxes = [(c["nx"], c["ny"], c["nz"], c["nu_r"], c["nu_b"]) for c in cfgs]
+    permuted = symmetry_amd.Batch(boxes[1:] + boxes[:1], precision="f32", layout="fs", xchunk=8)
+    with pytest.raises(symmetry_amd.CglError):
+        permuted.load(path)
+    same, _, _ = _batch("f32", "fs", xchunk=8)
+    same.load(path)
+    same.step(2)
+    a.step(2)
+    for k in range(len(cfgs)):
+        assert _same(a.download(k), same.download(k)), k
+
+
 def test_load_rejects_other_configurations(tmp_path):
     cfg, ri, bi = small_rp(nx=32, ny=16, nz=128)
     a = _fluid(cfg, "f32", "fs")

@@ -69,10 +69,13 @@ def _params(**kw):
 
 @pytest.mark.parametrize("kw,code", [({"nx": 0}, -1), ({"precision": 7}, -1), ({"npar": -1}, -1),
                                      ({"npar": 3, "layout": 1}, -3), ({"rho0": 0.0}, -1),
-                                     # largest boxes: 2^28 nodes in FP32 and 512^3 in FP64 exceed the
-                                     # 4 GiB per-array addressing; 2048 x 1024 x 1024 the node index
-                                     ({"nx": 1024, "ny": 512, "nz": 512, "precision": 1}, -3),
-                                     ({"nx": 512, "ny": 512, "nz": 512}, -3),
+                                     # largest boxes: above 2^28 nodes in FP32 and 2^27 (512^3) in FP64
+                                     # a group of four direction arrays exceeds the 4 GiB per-array
+                                     # addressing, and so do the FP64 factorised layout's 48 B of
+                                     # recolour scalars per node at 512^3; 2048 x 1024 x 1024 the node index
+                                     ({"nx": 1024, "ny": 512, "nz": 513, "precision": 1}, -3),
+                                     ({"nx": 512, "ny": 512, "nz": 513}, -3),
+                                     ({"nx": 512, "ny": 512, "nz": 512, "layout": 1}, -3),
                                      ({"nx": 2048, "ny": 1024, "nz": 1024, "precision": 1}, -1)])
 def test_create_validates_before_touching_the_device(lib, kw, code):
     h = ctypes.c_void_p()
