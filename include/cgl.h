@@ -189,6 +189,20 @@ int cgl_download_masks(cgl_t *h, int32_t *walls, int32_t *walls_all, int32_t *pi
  * order, so they are reproducible. */
 int cgl_diagnostics(cgl_t *h, double *sums, double *slice_red);
 
+/* cgl_diagnostics without any device allocation: the same sums and slice
+ * areas (equal to rounding: the sums run in another fixed order), formed by
+ * kernels that read the state and write their partial sums to a static
+ * device buffer of CGL_NOALLOC_SCRATCH_DOUBLES doubles (4 MiB per GPU, held
+ * until the process ends); no hipMalloc or other allocating call.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid it).
+ * A box needs 8 + nz + 7 gz gy + gy nz doubles (gz = ceil(nz/64),
+ * gy = ceil(ny/4)): larger boxes get CGL_ERR_UNSUPPORTED.  Every call on one
+ * GPU shares that GPU's buffer, so calls that overlap on one GPU (from other
+ * threads or streams) race; a call returns after its results are copied. */
+#define CGL_NOALLOC_SCRATCH_DOUBLES (1 << 19)
+int cgl_diagnostics_noalloc(cgl_t *h, double *sums, double *slice_red);
+
 /* ---------------------------------------------------------------- batches
  * Many independent periodic boxes of uneven sizes advanced together: one
  * launch per kernel covers the tiles of every box (nodes packed box after

@@ -57,7 +57,7 @@ EXPORTS = (
     "cgl_set_particles", "cgl_get_particles", "cgl_get_particle_shb", "cgl_download_masks",
     "cgl_batch_create", "cgl_box_upload_state", "cgl_box_init_cylinder", "cgl_box_download_state",
     "cgl_box_calmacros", "cgl_diagnostics", "cgl_init_interface", "cgl_phase_field_form",
-    "cgl_set_particle_geometry", "cgl_checkpoint_save", "cgl_checkpoint_load",
+    "cgl_set_particle_geometry", "cgl_checkpoint_save", "cgl_checkpoint_load", "cgl_diagnostics_noalloc",
 )
 
 
@@ -92,6 +92,7 @@ def lib():
         L.cgl_box_init_cylinder.argtypes = [vp, ctypes.c_int32] + [ctypes.c_double] * 4 + [ctypes.c_int32, vp]
         L.cgl_box_calmacros.argtypes = [vp, ctypes.c_int32] + [vp] * 7
         L.cgl_diagnostics.argtypes = [vp, vp, vp]
+        L.cgl_diagnostics_noalloc.argtypes = [vp, vp, vp]
         L.cgl_init_interface.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.cgl_phase_field_form.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
@@ -195,12 +196,14 @@ class Lattice:
     def init_interface(self, z0, kprof):
         _check(lib().cgl_init_interface(self.h, float(z0), float(kprof)))
 
-    def diagnostics(self, slices=False):
+    def diagnostics(self, slices=False, noalloc=False):
         """FP64 device sums {mass, mass_r, mass_b, jx, jy, jz, ekin} over the
-        live fluid nodes, and with slices=True the red area of every z slice."""
+        live fluid nodes, and with slices=True the red area of every z slice.
+        noalloc=True takes the allocation-free entry point."""
         sums = np.empty(7)
         area = np.empty(self.shape[2]) if slices else None
-        _check(lib().cgl_diagnostics(self.h, _ptr(sums), _ptr(area)))
+        fn = lib().cgl_diagnostics_noalloc if noalloc else lib().cgl_diagnostics
+        _check(fn(self.h, _ptr(sums), _ptr(area)))
         out = dict(zip(("mass", "mass_r", "mass_b", "jx", "jy", "jz", "ekin"), sums))
         if slices:
             out["slice_red"] = area

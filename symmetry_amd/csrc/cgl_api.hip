@@ -4,6 +4,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -904,6 +905,62 @@ int cgl_diagnostics(cgl_t *h, double *sums, double *slice_red)
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(part);
     if (e != hipSuccess) return fail(CGL_ERR_HIP, "diagnostics failed: %s", hipGetErrorString(e));
+    return CGL_OK;
+}
+
+// ------------------------------------------- diagnostics without allocation
+// The scratch of cgl_diagnostics_noalloc: a file-scope device array of
+// CGL_NOALLOC_SCRATCH_DOUBLES doubles (4 MiB), one copy per GPU, held until
+// the process ends.  Host code never passes the array's name to a kernel or a
+// copy (on the host it is only a stand-in for the symbol): it takes the
+// device address hipGetSymbolAddress reports, looked up once per device with
+// that device current and kept per device.
+static __device__ __align__(256) double scratch[CGL_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int SCR_MAXDEV = 64;
+std::mutex g_scr_mu;
+double *g_scr[SCR_MAXDEV] = {};
+
+// the device address of `scratch` on the current device `dev`
+int scratch_ptr(int dev, double **p)
+{
+    if (dev < 0 || dev >= SCR_MAXDEV) return fail(CGL_ERR_UNSUPPORTED, "device %d beyond the scratch table", dev);
+    std::lock_guard<std::mutex> lk(g_scr_mu);
+    if (!g_scr[dev]) {
+        void *a = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !a)
+            return fail(CGL_ERR_HIP, "scratch address on device %d: %s", dev, hipGetErrorString(e));
+        g_scr[dev] = (double *)a;
+    }
+    *p = g_scr[dev];
+    return CGL_OK;
+}
+} // namespace
+
+int cgl_diagnostics_noalloc(cgl_t *h, double *sums, double *slice_red)
+{
+    if (!h || !sums) return fail(CGL_ERR_ARG, "null argument");
+    if (h->nbox) return fail(CGL_ERR_ARG, "a batch handle takes the cgl_box_* entry points");
+    int rc = set_device(h);
+    if (rc) return rc;
+    if ((rc = state_readable(h))) return rc;
+    size_t need = 0;
+    dispatch(h, [&](auto impl) { need = decltype(impl)::diag_scratch_doubles(h); });
+    if (need > (size_t)CGL_NOALLOC_SCRATCH_DOUBLES)
+        return fail(CGL_ERR_UNSUPPORTED,
+                    "box %dx%dx%d needs %zu doubles of diagnostics scratch; the static buffer holds %d "
+                    "(use cgl_diagnostics)",
+                    h->p.nx, h->p.ny, h->p.nz, need, CGL_NOALLOC_SCRATCH_DOUBLES);
+    double *scr = nullptr;
+    if ((rc = scratch_ptr(h->p.device, &scr))) return rc;
+    dispatch(h, [&](auto impl) { decltype(impl)::diag_fused(h, scr); });
+    if ((rc = check_launch("diagnostics"))) return rc;
+    HIPCHK(hipMemcpyAsync(sums, scr, cgl::NDIAG * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (slice_red)
+        HIPCHK(hipMemcpyAsync(slice_red, scr + 8, h->p.nz * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return CGL_OK;
 }
 

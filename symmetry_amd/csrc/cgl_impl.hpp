@@ -346,6 +346,33 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         mirror_phi(h, phi);
         k_derivs_ext<P><<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, E, phi, nullptr, nullptr);
     }
+    // diagnostics sums without a field buffer, into the caller's scratch:
+    // scr[0..NDIAG) the sums, scr[8..8+nz) the slice areas, then the partials
+    // (diag_scratch_doubles() of them in all)
+    static size_t diag_scratch_doubles(const cgl_handle *h)
+    {
+        const size_t gz = (h->L.nz + TZ - 1) / TZ, gy = (h->L.ny + TY - 1) / TY;
+        return 8 + (size_t)h->L.nz + NDIAG * gz * gy + gy * (size_t)h->L.nz;
+    }
+    static void diag_fused(cgl_handle *h, double *scr)
+    {
+        const void *i0, *i1;
+        state_in(h, i0, i1);
+        const P *phio = (const P *)h->phi[h->phicur];
+        const dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, 1);
+        double *out = scr, *area = scr + 8, *part = area + h->L.nz, *spart = part + (size_t)NDIAG * grid.x * grid.y;
+        const Ext E = ext_with(h, h->npar ? h->ntype[h->ntcur] : nullptr);
+        if (h->ext)
+            k_diag_tiles<RB, S, R, P, SHIFT, true>
+                <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, E, part, spart);
+        else if (h->state_layout == RB)
+            k_diag_tiles<RB, S, R, P, SHIFT, false>
+                <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, E, part, spart);
+        else
+            k_diag_tiles<FS, S, R, P, SHIFT, false>
+                <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, E, part, spart);
+        k_diag_final<<<1, 256, 0, h->stream>>>(part, grid.x * grid.y, spart, grid.y, h->L.nz, out, area);
+    }
     static void derivatives_ext(cgl_handle *h, double *out, int *flag)
     {
         moments_ext(h, false);

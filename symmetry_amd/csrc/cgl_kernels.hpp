@@ -571,6 +571,41 @@ __device__ __forceinline__ void node_offsets(const Lat &L, int x, int y, int z, 
 // Writes phi of the step.  With FULL it also writes the macroscopic fields
 // rho, rho_R, rho_B, ux, uy, uz, phi (bare velocity, as calmacros) as FP64.
 // One node of tile (bx, by) of plane bz.
+// Moments of live node (x, y, z) of the stored state as calmacros forms them
+// (S:770-831), with its node index n and node type nt; false on wall and
+// mirror ghost layers, where calmacros computes none.
+template <int LIN, class S, class R, class P, bool SHIFT, bool EXT>
+__device__ __forceinline__ bool node_moments(const Lat &L, const void *__restrict__ p0, const void *__restrict__ p1,
+                                             const P *__restrict__ phio, const Ext &E, int x, int y, int z,
+                                             unsigned &n, Moments<R, SHIFT> &mo, unsigned &nt)
+{
+    if constexpr (EXT)
+        if (is_ghost(E, L, x, y, z)) return false;
+    unsigned xo[3], yo[3], zo[3];
+    node_offsets(L, x, y, z, xo, yo, zo);
+    n = xo[1] + yo[1] + zo[1];
+    R ax = 0; // the node's phase field of the stored state: its shift fractions
+    if constexpr (SHIFT && LIN == RB) ax = R(at(phio, n));
+    Node<R, SHIFT> nd;
+    nt = 0;
+    if constexpr (EXT) {
+        gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd, wall_bb_mask(E, L, x, y, z),
+                                 mirror_adjacent(E, L, x, y, z));
+        if (E.ntype) nt = at(E.ntype, n);
+    } else {
+        gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd);
+    }
+    mo = moments<R, SHIFT>(L, nd);
+    if constexpr (EXT) {
+        if (nt & NT_SOLID) { // calmacros at solid nodes (S:784-789); rho_R, rho_B untouched
+            mo.phi = 0;
+            mo.rho = R(L.rho0);
+            mo.ux = mo.uy = mo.uz = 0;
+        }
+    }
+    return true;
+}
+
 template <int LIN, class S, class R, class P, bool SHIFT, bool FULL, bool EXT>
 __device__ __forceinline__ void moments_node(const Lat &L, const void *__restrict__ p0, const void *__restrict__ p1,
                                              const P *__restrict__ phio, P *__restrict__ phi,
@@ -581,39 +616,20 @@ __device__ __forceinline__ void moments_node(const Lat &L, const void *__restric
     const int y = by * TY + threadIdx.y;
     const int x = bz;
     if (z >= L.nz || y >= L.ny) return;
-    if constexpr (EXT)
-        if (is_ghost(E, L, x, y, z)) {
-            // phi is copied from the adjacent layer afterwards (S:806); the
-            // other fields are not computed on ghost layers and read as zero
-            if constexpr (FULL) {
-                const size_t N = (size_t)L.N;
-                const unsigned n = ((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z;
+    unsigned n, nt;
+    Moments<R, SHIFT> mo;
+    if (!node_moments<LIN, S, R, P, SHIFT, EXT>(L, p0, p1, phio, E, x, y, z, n, mo, nt)) {
+        // phi is copied from the adjacent layer afterwards (S:806); the
+        // other fields are not computed on ghost layers and read as zero
+        if constexpr (FULL) {
+            const size_t N = (size_t)L.N;
+            n = ((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z;
 #pragma unroll
-                for (int k = 0; k < 6; ++k) mac[k * N + n] = 0.0;
-            }
-            return;
+            for (int k = 0; k < 6; ++k) mac[k * N + n] = 0.0;
         }
-    unsigned xo[3], yo[3], zo[3];
-    node_offsets(L, x, y, z, xo, yo, zo);
-    const unsigned n = xo[1] + yo[1] + zo[1];
-    R ax = 0; // the node's phase field of the stored state: its shift fractions
-    if constexpr (SHIFT && LIN == RB) ax = R(at(phio, n));
-    Node<R, SHIFT> nd;
-    unsigned nt = 0;
-    if constexpr (EXT) {
-        gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd, wall_bb_mask(E, L, x, y, z),
-                                 mirror_adjacent(E, L, x, y, z));
-        if (E.ntype) nt = at(E.ntype, n);
-    } else {
-        gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd);
+        return;
     }
-    Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
     if constexpr (EXT) {
-        if (nt & NT_SOLID) { // calmacros at solid nodes (S:784-789); rho_R, rho_B untouched
-            mo.phi = 0;
-            mo.rho = R(L.rho0);
-            mo.ux = mo.uy = mo.uz = 0;
-        }
         if (nt & NT_NEAR) {
             const size_t N = (size_t)L.N;
             E.mac[n] = mo.rho;
@@ -1594,6 +1610,75 @@ static __global__ __launch_bounds__(RED_B) void k_slice_red(Lat L, Ext E, const 
         __syncthreads();
     }
     if (threadIdx.x == 0) area[z] = sh[0];
+}
+
+// Diagnostics without a field buffer: each thread forms the moments of its
+// (y, z) column's nodes as calmacros does (node_moments) while marching x, and
+// sums over the live fluid nodes the k_reduce quantities and the red area
+// (1 + phi)/2 of its column.  A workgroup (one 64 x 4 tile of (z, y)) writes
+// its NDIAG sums to part[tile] (fixed-order tree) and the red area of its
+// 64 z slices summed over its 4 rows to spart[y tile][z]; k_diag_final adds
+// them in a fixed order.  Nothing but the two partial arrays is written.
+template <int LIN, class S, class R, class P, bool SHIFT, bool EXT>
+__global__ __launch_bounds__(TZ *TY) void k_diag_tiles(Lat L, const void *__restrict__ p0,
+                                                        const void *__restrict__ p1, const P *__restrict__ phio,
+                                                        Ext E, double *__restrict__ part, double *__restrict__ spart)
+{
+    __shared__ double sh[NDIAG][TZ * TY];
+    __shared__ double sa[TY][TZ];
+    const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
+    const int z = blockIdx.x * TZ + tz, y = blockIdx.y * TY + ty;
+    double acc[NDIAG] = {0, 0, 0, 0, 0, 0, 0}, area = 0.0;
+    if (z < L.nz && y < L.ny) {
+        for (int x = 0; x < L.nx; ++x) {
+            unsigned n, nt;
+            Moments<R, SHIFT> mo;
+            if (!node_moments<LIN, S, R, P, SHIFT, EXT>(L, p0, p1, phio, E, x, y, z, n, mo, nt)) continue;
+            if (nt & NT_SOLID) continue;
+            const double rho = double(mo.rho), ux = double(mo.ux), uy = double(mo.uy), uz = double(mo.uz);
+            acc[0] += rho;
+            acc[1] += double(mo.sr);
+            acc[2] += double(mo.sb);
+            acc[3] += rho * ux;
+            acc[4] += rho * uy;
+            acc[5] += rho * uz;
+            acc[6] += 0.5 * rho * (ux * ux + uy * uy + uz * uz);
+            area += 0.5 * (1.0 + double(mo.phi));
+        }
+    }
+    for (int k = 0; k < NDIAG; ++k) sh[k][tid] = acc[k];
+    sa[ty][tz] = area;
+    __syncthreads();
+    for (int st = TZ * TY / 2; st > 0; st >>= 1) {
+        if (tid < st)
+            for (int k = 0; k < NDIAG; ++k) sh[k][tid] += sh[k][tid + st];
+        __syncthreads();
+    }
+    const unsigned tile = blockIdx.y * gridDim.x + blockIdx.x;
+    if (tid < NDIAG) part[(size_t)tile * NDIAG + tid] = sh[tid][0];
+    if (ty == 0 && z < L.nz) {
+        double s = 0.0;
+        for (int r = 0; r < TY; ++r) s += sa[r][tz];
+        spart[(size_t)blockIdx.y * L.nz + z] = s;
+    }
+}
+
+// Fixed-order sums of k_diag_tiles' partials: the NDIAG totals into out and
+// the red area of every z slice into area.
+static __global__ __launch_bounds__(256) void k_diag_final(const double *__restrict__ part, unsigned ntiles,
+                                                           const double *__restrict__ spart, unsigned gy, int nz,
+                                                           double *__restrict__ out, double *__restrict__ area)
+{
+    if (threadIdx.x < NDIAG) {
+        double s = 0.0;
+        for (unsigned t = 0; t < ntiles; ++t) s += part[(size_t)t * NDIAG + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+        double s = 0.0;
+        for (unsigned b = 0; b < gy; ++b) s += spart[(size_t)b * nz + z];
+        area[z] = s;
+    }
 }
 
 // ------------------------------------------------- batches of independent boxes
