@@ -311,26 +311,27 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             }
             // moments (S:792-799): rho and j from the gathered populations, the
             // colour split from the scattered phase field
-            // the colour split: the minority colour's fraction hm from the
-            // phase-field buffer (its signed minority fraction), so that a
-            // nearly pure node keeps its small colour density to FP32's
-            // relative precision; the majority colour is the complement
-            const float qn = at(phi, (unsigned)x * sx + nself);
-            const bool redmaj = __builtin_signbitf(qn);
-            const R hm = __builtin_fabsf(qn);
-            const R phin = phi_of_q(qn);
+            const R phin = sm.phi[slot5(x)][ty + 2][tz + 2];
             Moments<R, true> mo;
             mo.rho = rho0 + df;
             mo.drho = df;
             const R irho = fdiv(R(1), mo.rho);
-            const R smin = mo.rho * hm;
-            mo.sr = redmaj ? mo.rho - smin : smin;
-            mo.sb = redmaj ? smin : mo.rho - smin;
             mo.ux = jx * irho;
             mo.uy = jy * irho;
             mo.uz = jz * irho;
             mo.phi = phin;
             collide<R, true>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, flag, d);
+            // the colour split (the collision needs only rho, u and phi): the
+            // minority colour's fraction hm from the phase-field buffer (its
+            // signed minority fraction), so that a nearly pure node keeps its
+            // small colour density to FP32's relative precision; the majority
+            // colour is the complement
+            const float qn = at(phi, (unsigned)x * sx + nself);
+            const bool redmaj = __builtin_signbitf(qn);
+            const R hm = __builtin_fabsf(qn);
+            const R smin = mo.rho * hm;
+            mo.sr = redmaj ? mo.rho - smin : smin;
+            mo.sb = redmaj ? smin : mo.rho - smin;
             const Recol<R> rc = recol<R, true>(L, mo, flag);
             // recolouring r_i = A f_i + g w_i n.c_i (S:713-726), with the
             // EVAP_LIM overrides; the minority colour's part of f_i is
