@@ -206,7 +206,7 @@ def run_ensemble(steps, seq_steps):
             "sequential_f64_mlups": round(seq_mlups, 1), "ratio": round(batch_mlups / seq_mlups, 2)}
 
 
-def run_particles(steps, f64_dense_steps):
+def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512")):
     """The particle workloads, red/blue layout (walls and particles run on it):
     one sphere (R=10, density 1.5, 60 degrees) at a flat interface in a 96^3
     box with z walls, FP64 and FP32; and 200 spheres (R=8, seed 99, densities
@@ -228,6 +228,7 @@ def run_particles(steps, f64_dense_steps):
             ("dense_512", c5, p5, "f32", steps)]
     if f64_dense_steps > 0:
         runs.append(("dense_512", c5, p5, "f64", f64_dense_steps))
+    runs = [r for r in runs if r[0] in which]
     roof = {}
     for name, cfg, parts, prec, nsteps in runs:
         n = cfg["nx"]
@@ -288,6 +289,8 @@ def main():
                     help="timed steps of the particle workloads (0 = skip)")
     ap.add_argument("--particle-f64-dense-steps", type=int, default=40,
                     help="timed steps of the 512^3 200-sphere workload in FP64 (0 = skip)")
+    ap.add_argument("--particle-workloads", default="sphere_96,dense_512",
+                    help="comma-separated particle workloads to run (sphere_96, dense_512)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this implementation runs on one GPU only; sharding over GPUs is designed, "
@@ -353,7 +356,8 @@ def main():
 
     accuracy = run_accuracy(nx, ny, nz, cfg, args.accuracy_steps) if args.accuracy_steps > 0 else None
     ensemble = run_ensemble(args.ensemble_steps, args.ensemble_seq_steps) if args.ensemble_steps > 0 else None
-    particles = run_particles(args.particle_steps, args.particle_f64_dense_steps) if args.particle_steps > 0 else None
+    particles = (run_particles(args.particle_steps, args.particle_f64_dense_steps, args.particle_workloads.split(","))
+                 if args.particle_steps > 0 else None)
 
     out = {
         "metric": "MLUPS at 256x256x512 D3Q19 CG-MRT, FP32 vs FP64, % of HBM roofline",
