@@ -46,14 +46,34 @@ int check_launch(const char *what)
     return CGL_OK;
 }
 
+// Device allocation of the handle's buffers.  CGL_POISON=<mask> in the
+// environment fills each new buffer of the classes in the mask with 0xff
+// bytes (NaN in FP16, FP32 and FP64, -1 in integers) before use, so that a
+// kernel which reads memory no step has written shows it in its results
+// instead of reading whatever the allocator hands back (zero pages in a
+// fresh process).  Classes: 1 populations, 2 phase field, 4 factorised
+// state, 8 particle fields and masks, 16 particle state and partial sums,
+// 32 scratch of downloads and diagnostics, 64 scatter-form sums, 128 tables.
+enum : unsigned { PZ_POPS = 1, PZ_PHI = 2, PZ_FS = 4, PZ_PFIELD = 8, PZ_PART = 16, PZ_SCR = 32, PZ_SC = 64, PZ_TAB = 128 };
+hipError_t dmalloc(void **p, size_t bytes, unsigned cls, hipStream_t st)
+{
+    static const unsigned poison = [] {
+        const char *env = getenv("CGL_POISON");
+        return env ? (unsigned)strtoul(env, nullptr, 0) : 0u;
+    }();
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess && (poison & cls)) e = hipMemsetAsync(*p, 0xff, bytes, st);
+    return e;
+}
+
 // Scratch for host-layout conversions, sized for the largest box a call can
 // address (scratch_n nodes).
 int ensure_scratch(cgl_handle *h, bool mac, bool aos, bool iflag)
 {
     const size_t n = h->scratch_n;
-    if (mac && !h->mac) HIPCHK(hipMalloc((void **)&h->mac, 7 * n * sizeof(double)));
-    if (aos && !h->aos) HIPCHK(hipMalloc((void **)&h->aos, (size_t)cgl::Q * n * sizeof(double)));
-    if (iflag && !h->iflag) HIPCHK(hipMalloc((void **)&h->iflag, n * sizeof(int)));
+    if (mac && !h->mac) HIPCHK(dmalloc((void **)&h->mac, 7 * n * sizeof(double), PZ_SCR, h->stream));
+    if (aos && !h->aos) HIPCHK(dmalloc((void **)&h->aos, (size_t)cgl::Q * n * sizeof(double), PZ_SCR, h->stream));
+    if (iflag && !h->iflag) HIPCHK(dmalloc((void **)&h->iflag, n * sizeof(int), PZ_SCR, h->stream));
     return CGL_OK;
 }
 
@@ -148,7 +168,7 @@ int setup_scatter(cgl_handle *h)
     }
     int2 **bufs[cgl::SC_NBUF] = {&h->sb.own,   &h->sb.fxlo,  &h->sb.fxhi,  &h->sb.fy,    &h->sb.fz,
                                  &h->sb.exylo, &h->sb.exyhi, &h->sb.exzlo, &h->sb.exzhi, &h->sb.eyz};
-    for (int k = 0; k < cgl::SC_NBUF; ++k) HIPCHK(hipMalloc((void **)bufs[k], h->N * sizeof(int2)));
+    for (int k = 0; k < cgl::SC_NBUF; ++k) HIPCHK(dmalloc((void **)bufs[k], h->N * sizeof(int2), PZ_SC, h->stream));
     h->sc = 1;
     return CGL_OK;
 }
@@ -301,40 +321,44 @@ static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls)
     // initial states) plus the factorised double buffer.
     const int nrb = p->layout == CGL_LAYOUT_RB ? 2 : 1;
     for (int b = 0; b < nrb && e == hipSuccess; ++b)
-        for (int c = 0; c < 2 && e == hipSuccess; ++c) e = hipMalloc(&h->pops[b][c], (size_t)cgl::Q * h->N * h->esize);
+        for (int c = 0; c < 2 && e == hipSuccess; ++c) e = dmalloc(&h->pops[b][c], (size_t)cgl::Q * h->N * h->esize, PZ_POPS, h->stream);
     if (p->layout == CGL_LAYOUT_FS) {
         const size_t ns = p->precision == CGL_PREC_F64 ? 6 * sizeof(double) : 4 * sizeof(float);
-        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc(&h->fsf[b], (size_t)cgl::Q * h->N * h->esize);
-        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc(&h->fss[b], h->N * ns);
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = dmalloc(&h->fsf[b], (size_t)cgl::Q * h->N * h->esize, PZ_FS, h->stream);
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = dmalloc(&h->fss[b], h->N * ns, PZ_FS, h->stream);
     }
-    for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc(&h->phi[b], h->N * h->phisize);
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) e = dmalloc(&h->phi[b], h->N * h->phisize, PZ_PHI, h->stream);
     if (h->npar > 0) {
         // masks, NEAR-band macroscopic fields and gradients, particle state
-        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMalloc((void **)&h->ntype[b], h->N * sizeof(unsigned));
-        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMemset(h->ntype[b], 0, h->N * sizeof(unsigned));
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = dmalloc((void **)&h->ntype[b], h->N * sizeof(unsigned), PZ_PFIELD, h->stream);
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) e = hipMemsetAsync(h->ntype[b], 0, h->N * sizeof(unsigned), h->stream);
         int **ints[3] = {&h->walls_i, &h->pin_i, &h->pout_i};
-        for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMalloc((void **)ints[k], h->N * sizeof(int));
-        for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMemset(*ints[k], 0, h->N * sizeof(int));
-        if (e == hipSuccess) e = hipMalloc((void **)&h->E.mac, 6 * h->N * sizeof(double));
-        if (e == hipSuccess) e = hipMemset(h->E.mac, 0, 6 * h->N * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc((void **)&h->E.der, 6 * h->N * sizeof(double));
-        if (e == hipSuccess) e = hipMemset(h->E.der, 0, 6 * h->N * sizeof(double));
+        for (int k = 0; k < 3 && e == hipSuccess; ++k) e = dmalloc((void **)ints[k], h->N * sizeof(int), PZ_PFIELD, h->stream);
+        for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMemsetAsync(*ints[k], 0, h->N * sizeof(int), h->stream);
+        if (e == hipSuccess) e = dmalloc((void **)&h->E.mac, 6 * h->N * sizeof(double), PZ_PFIELD, h->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(h->E.mac, 0, 6 * h->N * sizeof(double), h->stream);
+        if (e == hipSuccess) e = dmalloc((void **)&h->E.der, 6 * h->N * sizeof(double), PZ_PFIELD, h->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(h->E.der, 0, 6 * h->N * sizeof(double), h->stream);
         const size_t n3 = 3 * (size_t)h->npar * sizeof(double), n1 = (size_t)h->npar * sizeof(double);
         double **v3[11] = {&h->P.center, &h->P.center_old, &h->P.center_prev, &h->P.up, &h->P.rp,
                            &h->P.fp, &h->P.tp, &h->P.frep, &h->P.ten, &h->P.tent, &h->P.center0};
         double **v1[5] = {&h->P.rad, &h->P.mass, &h->P.inertia, &h->P.cita, &h->P.shb};
-        for (int k = 0; k < 11 && e == hipSuccess; ++k) e = hipMalloc((void **)v3[k], n3);
-        for (int k = 0; k < 11 && e == hipSuccess; ++k) e = hipMemset(*v3[k], 0, n3);
-        for (int k = 0; k < 5 && e == hipSuccess; ++k) e = hipMalloc((void **)v1[k], n1);
-        for (int k = 0; k < 5 && e == hipSuccess; ++k) e = hipMemset(*v1[k], 0, n1);
-        if (e == hipSuccess) e = hipMalloc((void **)&h->P.part_lag, (size_t)h->npar * cgl::LAG_BLOCKS * 7 * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc((void **)&h->P.part_mom, (size_t)h->npar * cgl::PSLICES * 6 * sizeof(double));
+        for (int k = 0; k < 11 && e == hipSuccess; ++k) e = dmalloc((void **)v3[k], n3, PZ_PART, h->stream);
+        for (int k = 0; k < 11 && e == hipSuccess; ++k) e = hipMemsetAsync(*v3[k], 0, n3, h->stream);
+        for (int k = 0; k < 5 && e == hipSuccess; ++k) e = dmalloc((void **)v1[k], n1, PZ_PART, h->stream);
+        for (int k = 0; k < 5 && e == hipSuccess; ++k) e = hipMemsetAsync(*v1[k], 0, n1, h->stream);
+        if (e == hipSuccess) e = dmalloc((void **)&h->P.part_lag, (size_t)h->npar * cgl::LAG_BLOCKS * 7 * sizeof(double), PZ_PART, h->stream);
+        if (e == hipSuccess) e = dmalloc((void **)&h->P.part_mom, (size_t)h->npar * cgl::PSLICES * 6 * sizeof(double), PZ_PART, h->stream);
         h->E.pout = h->pout_i;
         h->E.center = h->P.center;
         h->E.cita = h->P.cita;
     }
-    if (e == hipSuccess) e = hipMemset(h->phi[0], 0, h->N * h->phisize);
-    if (e == hipSuccess) e = hipMemset(h->phi[1], 0, h->N * h->phisize);
+    if (e == hipSuccess) e = hipMemsetAsync(h->phi[0], 0, h->N * h->phisize, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->phi[1], 0, h->N * h->phisize, h->stream);
+    // every device operation of a handle runs on its stream, which does not
+    // order itself with the null stream (hipStreamNonBlocking): the clears
+    // above must be complete before the caller's first entry point
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
         rc = fail(CGL_ERR_HIP, "device allocation failed: %s", hipGetErrorString(e));
         cgl_destroy(h);
@@ -391,7 +415,7 @@ int cgl_upload_state(cgl_t *h, const double *ri, const double *bi)
     const size_t bytes = (size_t)cgl::Q * h->N * sizeof(double);
     // the shift reference of the uploaded state: phi of each node
     double *tmp = nullptr;
-    HIPCHK(hipMalloc((void **)&tmp, bytes));
+    HIPCHK(dmalloc((void **)&tmp, bytes, PZ_SCR, h->stream));
     hipError_t e = hipMemcpyAsync(h->aos, ri, bytes, hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(tmp, bi, bytes, hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) {
@@ -758,12 +782,16 @@ int cgl_set_particles(cgl_t *h, const double *center, const double *up, const do
             if (up) u3[3 * p + k] = up[k * np + p];
             if (rp) w3[3 * p + k] = rp[k * np + p];
         }
-    HIPCHK(hipMemcpy(h->P.center, c3.data(), b3, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->P.center_old, c3.data(), b3, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->P.center_prev, c3.data(), b3, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->P.center0, c3.data(), b3, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->P.up, u3.data(), b3, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->P.rp, w3.data(), b3, hipMemcpyHostToDevice));
+    // the steps queued before this call read the particle arrays: they finish
+    // first, and the copies run on the handle's stream (synchronised below,
+    // before the host vectors go out of scope)
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.center, c3.data(), b3, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.center_old, c3.data(), b3, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.center_prev, c3.data(), b3, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.center0, c3.data(), b3, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.up, u3.data(), b3, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->P.rp, w3.data(), b3, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->P.rad, rad, b1, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->P.mass, mass, b1, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->P.inertia, inertia, b1, hipMemcpyHostToDevice, h->stream));
@@ -830,7 +858,8 @@ int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *
     std::vector<double> tmp(3 * np);
     for (int v = 0; v < 8; ++v) {
         if (!dst[v]) continue;
-        HIPCHK(hipMemcpy(tmp.data(), src[v], b3, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(tmp.data(), src[v], b3, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
         for (size_t p = 0; p < np; ++p) // [npar][3] -> (npar, 3) component-major
             for (int k = 0; k < 3; ++k) dst[v][k * np + p] = tmp[3 * p + k];
     }
@@ -844,7 +873,8 @@ int cgl_get_particle_shb(cgl_t *h, double *shb)
     int rc = set_device(h);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipMemcpy(shb, h->P.shb, (size_t)h->npar * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(shb, h->P.shb, (size_t)h->npar * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return CGL_OK;
 }
 
@@ -892,7 +922,7 @@ int cgl_diagnostics(cgl_t *h, double *sums, double *slice_red)
     const unsigned per_block = 16384;
     const unsigned nblocks = (unsigned)((h->N + per_block - 1) / per_block);
     double *part = nullptr;
-    HIPCHK(hipMalloc((void **)&part, ((size_t)nblocks + 1) * cgl::NDIAG * sizeof(double) + h->p.nz * sizeof(double)));
+    HIPCHK(dmalloc((void **)&part, ((size_t)nblocks + 1) * cgl::NDIAG * sizeof(double) + h->p.nz * sizeof(double), PZ_SCR, h->stream));
     double *out = part + (size_t)nblocks * cgl::NDIAG, *area = out + cgl::NDIAG;
     const cgl::Ext E = h->npar ? [&] { cgl::Ext e = h->E; e.ntype = h->ntype[h->ntcur]; return e; }() : h->E;
     cgl::k_reduce<<<nblocks, cgl::RED_B, 0, h->stream>>>(h->L, E, h->mac, part, per_block);
@@ -1040,8 +1070,10 @@ int cgl_batch_create(cgl_t **out, const cgl_params *p, int32_t nbox, const cgl_b
     }
     h->tiles1 = (unsigned)t1;
     h->tiles2 = (unsigned)t2;
-    hipError_t e = hipMalloc((void **)&h->dboxes, nbox * sizeof(cgl::BoxDev));
-    if (e == hipSuccess) e = hipMemcpy(h->dboxes, h->hboxes, nbox * sizeof(cgl::BoxDev), hipMemcpyHostToDevice);
+    hipError_t e = dmalloc((void **)&h->dboxes, nbox * sizeof(cgl::BoxDev), PZ_TAB, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h->dboxes, h->hboxes, nbox * sizeof(cgl::BoxDev), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
         rc = fail(CGL_ERR_HIP, "box table: %s", hipGetErrorString(e));
         cgl_destroy(h);
@@ -1253,7 +1285,8 @@ int cgl_checkpoint_save(cgl_t *h, const char *path)
         for (size_t off = 0; ok && e == hipSuccess && off < sec.second; off += CKPT_CHUNK) {
             const size_t n = sec.second - off < CKPT_CHUNK ? sec.second - off : CKPT_CHUNK;
             buf.resize(n);
-            e = hipMemcpy(buf.data(), (char *)sec.first + off, n, hipMemcpyDeviceToHost);
+            e = hipMemcpyAsync(buf.data(), (char *)sec.first + off, n, hipMemcpyDeviceToHost, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
             if (e == hipSuccess) ok = fwrite(buf.data(), 1, n, f) == n;
         }
     }
@@ -1315,7 +1348,8 @@ int cgl_checkpoint_load(cgl_t *h, const char *path)
             const size_t n = sec.second - off < CKPT_CHUNK ? sec.second - off : CKPT_CHUNK;
             buf.resize(n);
             ok = fread(buf.data(), 1, n, f) == n;
-            if (ok) e = hipMemcpy((char *)sec.first + off, buf.data(), n, hipMemcpyHostToDevice);
+            if (ok) e = hipMemcpyAsync((char *)sec.first + off, buf.data(), n, hipMemcpyHostToDevice, h->stream);
+            if (ok && e == hipSuccess) e = hipStreamSynchronize(h->stream);
         }
     }
     fclose(f);
