@@ -31,6 +31,7 @@ extern "C" {
 #define CGL_ERR_ARG (-1)         /* invalid argument or size */
 #define CGL_ERR_HIP (-2)         /* HIP runtime error (no GPU, out of memory, launch failure) */
 #define CGL_ERR_UNSUPPORTED (-3) /* valid request the library does not implement */
+#define CGL_ERR_RANGE (-4)       /* a density left the range the scatter form's fixed-point sums hold */
 
 #define CGL_BC_PERIODIC 0 /* periodic face pair (the reference's next_* tables, S:295-324) */
 #define CGL_BC_WALL 1     /* half-way bounce-back wall with a ghost layer (S:1557-1590) */
@@ -133,7 +134,15 @@ int cgl_phase_field_form(cgl_t *h, int32_t *scatter);
 int cgl_checkpoint_save(cgl_t *h, const char *path);
 int cgl_checkpoint_load(cgl_t *h, const char *path);
 
-/* Block until all queued device work of the handle has finished. */
+/* Block until all queued device work of the handle has finished.
+ * Periodic FP32/FP16 factorised boxes sum the next step's colour densities in
+ * 32-bit fixed point at 2^29 units per rho0 (the scatter form): a colour
+ * density of 4 rho0 or more does not fit.  Such a step raises a flag that
+ * cgl_sync, cgl_time_steps and every call that reads the state (calmacros,
+ * downloads, diagnostics, checkpoint save) report as CGL_ERR_RANGE; the
+ * state is then lost.  Installing a state (upload, init, checkpoint load)
+ * clears the flag; CGL_SCATTER=0 in the environment selects the gather form,
+ * which has no such limit. */
 int cgl_sync(cgl_t *h);
 
 /* Particles (npar of the handle), in the reference's layout: per-particle

@@ -24,7 +24,14 @@ Q = 19
 
 
 class CglError(RuntimeError):
-    pass
+    """A libcgl call returned an error; ``code`` is its CGL_ERR_* value."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+ERR_ARG, ERR_HIP, ERR_UNSUPPORTED, ERR_RANGE = -1, -2, -3, -4
 
 
 class Params(ctypes.Structure):
@@ -107,7 +114,7 @@ def lib():
 
 def _check(rc):
     if rc != 0:
-        raise CglError(f"libcgl error {rc}: {lib().cgl_last_error().decode()}")
+        raise CglError(f"libcgl error {rc}: {lib().cgl_last_error().decode()}", rc)
 
 
 def _ptr(a):

@@ -146,10 +146,34 @@ int drop_graph(cgl_handle *h)
 }
 
 // Entry points that read the state back need its recolour scalars.
+// The scatter form's sums are 32-bit fixed point at 2^29 units per rho0: a
+// colour density of 4 rho0 or more wraps, and the finishing pass raises the
+// handle's range flag.  Checked wherever the host reads or waits on the
+// state; cleared when a new state is installed.
+int check_range(const cgl_handle *h)
+{
+    if (!h->sc || !h->sb.range) return CGL_OK;
+    int v = 0;
+    HIPCHK(hipMemcpyAsync(&v, h->sb.range, sizeof v, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (v)
+        return fail(CGL_ERR_RANGE, "a colour density reached 4 rho0, beyond the fixed-point sums of the scatter "
+                                   "form; the state is lost (run with CGL_SCATTER=0 for such densities)");
+    return CGL_OK;
+}
+
+// (Callers that install a state ignore its result: a HIP error here shows
+// again at the next call.)
+int clear_range(cgl_handle *h)
+{
+    if (h->sb.range) HIPCHK(hipMemsetAsync(h->sb.range, 0, sizeof(int), h->stream));
+    return CGL_OK;
+}
+
 int state_readable(const cgl_handle *h)
 {
     if (h->scal_stale) return fail(CGL_ERR_HIP, "internal: the state's recolour scalars were not written");
-    return CGL_OK;
+    return check_range(h);
 }
 
 // Fixed-point partial sums of the scatter step, for periodic FP32/FP16
@@ -169,8 +193,9 @@ int setup_scatter(cgl_handle *h)
     int2 **bufs[cgl::SC_NBUF] = {&h->sb.own,   &h->sb.fxlo,  &h->sb.fxhi,  &h->sb.fy,    &h->sb.fz,
                                  &h->sb.exylo, &h->sb.exyhi, &h->sb.exzlo, &h->sb.exzhi, &h->sb.eyz};
     for (int k = 0; k < cgl::SC_NBUF; ++k) HIPCHK(dmalloc((void **)bufs[k], h->N * sizeof(int2), PZ_SC, h->stream));
+    HIPCHK(dmalloc((void **)&h->sb.range, 256, PZ_SC, h->stream));
     h->sc = 1;
-    return CGL_OK;
+    return clear_range(h);
 }
 
 // Streaming copy, 16 B per thread per iteration: the reference kernel for the
@@ -398,6 +423,7 @@ int cgl_destroy(cgl_t *h)
                                  h->sb.exylo, h->sb.exyhi, h->sb.exzlo, h->sb.exzhi, h->sb.eyz};
     for (int2 *q : sbufs)
         if (q) (void)hipFree(q);
+    if (h->sb.range) (void)hipFree(h->sb.range);
     for (int k = 0; k < 4; ++k)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -424,6 +450,7 @@ int cgl_upload_state(cgl_t *h, const double *ri, const double *bi)
         h->primed = 0;
         h->phi_ok = 0;
         h->scal_stale = 0;
+        (void)clear_range(h);
         dispatch(h, [&](auto impl) {
             decltype(impl)::phi_aos(h, h->aos, tmp);
             decltype(impl)::pack(h, h->aos, 1);
@@ -474,6 +501,7 @@ int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, i
     h->primed = 0;
     h->phi_ok = 0;
     h->scal_stale = 0;
+    (void)clear_range(h);
     dispatch(h, [&](auto impl) { decltype(impl)::init_cylinder(h, ic); });
     if ((rc = check_launch("init_cylinder"))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -495,6 +523,7 @@ int cgl_init_interface(cgl_t *h, double z0, double kprof)
     h->primed = 0;
     h->phi_ok = 0;
     h->scal_stale = 0;
+    (void)clear_range(h);
     dispatch(h, [&](auto impl) { decltype(impl)::init_cylinder(h, ic); });
     if ((rc = check_launch("init_interface"))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -682,7 +711,7 @@ int cgl_sync(cgl_t *h)
 {
     if (!h) return fail(CGL_ERR_ARG, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
-    return CGL_OK;
+    return check_range(h);
 }
 
 int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moments, double *ms_collide)
@@ -756,7 +785,7 @@ int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moment
     }
     if (ms_moments) *ms_moments = km;
     if (ms_collide) *ms_collide = kc;
-    return CGL_OK;
+    return check_range(h);
 }
 
 int cgl_set_particles(cgl_t *h, const double *center, const double *up, const double *rp, const double *rad,
@@ -1361,6 +1390,7 @@ int cgl_checkpoint_load(cgl_t *h, const char *path)
     h->phicur = H.phicur; h->ntcur = H.ntcur; h->phi_ok = H.phi_ok; h->primed = H.primed; h->cold = H.cold;
     h->xc = H.xc; h->tuning = H.tuning; h->tune_idx = H.tune_idx; h->tune_best_xc = H.tune_best_xc;
     h->tune_best_ms = H.tune_best_ms; h->steps_done = (int)H.steps_done; h->scal_stale = 0;
+    (void)clear_range(h);
     if (h->npar) {
         h->P.fg = Pp.fg;
         for (int k = 0; k < 6; ++k) { h->P.rep_plane[k] = Pp.rep_plane[k]; h->P.rep_on[k] = Pp.rep_on[k]; }

@@ -52,6 +52,7 @@ struct ScBufs {
     int2 *exylo, *exyhi; // from the box across an x-y edge, low / high x side
     int2 *exzlo, *exzhi; // from the box across an x-z edge, low / high x side
     int2 *eyz;           // from the box across a y-z edge
+    int *range;          // set to 1 by a finishing pass that finds a colour sum wrapped (sticky)
 };
 constexpr int SC_NBUF = 10;
 
@@ -74,11 +75,16 @@ __device__ __forceinline__ int round_half_up(float v)
     return r;
 }
 
+// Sums wrap modulo 2^32 (unsigned arithmetic: defined behaviour).  A colour
+// density below 4 rho0 never wraps; one that reaches it lands below -rho0/2,
+// where no sum of rounded terms of a real density can be, and sums_wrapped
+// catches it.
 __device__ __forceinline__ void add2(int2 &a, int2 b)
 {
-    a.x += b.x;
-    a.y += b.y;
+    a.x = (int)((unsigned)a.x + (unsigned)b.x);
+    a.y = (int)((unsigned)a.y + (unsigned)b.y);
 }
+__device__ __forceinline__ bool sums_wrapped(int2 s) { return min(s.x, s.y) < -(1 << 28); }
 __device__ __forceinline__ int2 shfl2(int2 v, int src) { return make_int2(__shfl(v.x, src), __shfl(v.y, src)); }
 
 // The phase-field buffer entry (signed minority fraction, minority_q) of a
@@ -90,7 +96,7 @@ __device__ __forceinline__ float q_of_sums(int2 s)
     const int r = s.x > 0 ? s.x : 0, b = s.y > 0 ? s.y : 0;
     if (r == 0) return 0.0f;
     if (b == 0) return -0.0f;
-    const float tot = float(r + b);
+    const float tot = float((unsigned)r + (unsigned)b); // exact below 2^32
     return b <= r ? -(float(b) / tot) : float(r) / tot;
 }
 
@@ -535,7 +541,7 @@ __global__ __launch_bounds__(TZ *KTY<float>, 4) void k_collide_sc(Lat L, const S
 __device__ __forceinline__ ScBufs sc_offset(const ScBufs &B, unsigned long long base)
 {
     return ScBufs{B.own + base,   B.fxlo + base,  B.fxhi + base,  B.fy + base,    B.fz + base,
-                  B.exylo + base, B.exyhi + base, B.exzlo + base, B.exzhi + base, B.eyz + base};
+                  B.exylo + base, B.exyhi + base, B.exzlo + base, B.exzhi + base, B.eyz + base, B.range};
 }
 
 // Every box of a batch in one launch; a workgroup finds its box by the
@@ -582,6 +588,7 @@ __device__ __forceinline__ void phi_sc_node(const Lat &L, const ScBufs &B, int x
         if (xhi) add2(s, at(B.exzhi, c));
         if (fy) add2(s, at(B.eyz, c));
     }
+    if (sums_wrapped(s)) *B.range = 1;
     at(phi, n) = q_of_sums(s);
 }
 
@@ -607,10 +614,8 @@ static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc,
     int2 s[2] = {make_int2(o.x, o.y), make_int2(o.z, o.w)};
     auto add_pair = [&](const int2 *a) {
         const int4 v = at((const int4 *)a, h);
-        s[0].x += v.x;
-        s[0].y += v.y;
-        s[1].x += v.z;
-        s[1].y += v.w;
+        add2(s[0], make_int2(v.x, v.y));
+        add2(s[1], make_int2(v.z, v.w));
     };
     const int cx0 = (x / xc) * xc;
     const bool xlo = x == cx0, xhi = x == min(cx0 + xc, L.nx) - 1;
@@ -632,6 +637,7 @@ static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc,
             if (fy) add2(s[k], at(B.eyz, c));
         }
     }
+    if (sums_wrapped(s[0]) || sums_wrapped(s[1])) *B.range = 1;
     at((float2 *)phi, h) = make_float2(q_of_sums(s[0]), q_of_sums(s[1]));
 }
 

@@ -12,6 +12,8 @@ module cgl
   integer(c_int32_t), parameter :: CGL_BC_PERIODIC = 0, CGL_BC_WALL = 1, CGL_BC_MIRROR = 2
   integer(c_int32_t), parameter :: CGL_PREC_F64 = 0, CGL_PREC_F32 = 1, CGL_PREC_F16 = 2
   integer(c_int32_t), parameter :: CGL_LAYOUT_RB = 0, CGL_LAYOUT_FS = 1
+  integer(c_int32_t), parameter :: CGL_OK = 0, CGL_ERR_ARG = -1, CGL_ERR_HIP = -2, CGL_ERR_UNSUPPORTED = -3, &
+                                   CGL_ERR_RANGE = -4
 
   type, bind(C) :: cgl_params
     integer(c_int32_t) :: nx, ny, nz

@@ -22,6 +22,7 @@ import numpy as np
 import pytest
 
 import symmetry_amd
+from symmetry_amd import ic
 from tests.helpers import rel_err, small_rp
 
 pytestmark = pytest.mark.gpu
@@ -184,3 +185,41 @@ def test_timed_steps_leave_a_readable_state(rp):
     b.step(3 + 8)  # time_steps runs its steps twice (loop timing, then per-kernel timing)
     rb, bb = b.download()
     assert np.array_equal(ra, rb) and np.array_equal(ba, bb)
+
+
+def _ball_state(cfg, rho_in):
+    """Pure blue at rho0 around a pure red ball (radius 6, centred) of
+    density rho_in, at rest: the populations are w_i times the densities."""
+    nx, ny, nz = cfg["nx"], cfg["ny"], cfg["nz"]
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ball = (x - nx / 2) ** 2 + (y - ny / 2) ** 2 + (z - nz / 2) ** 2 <= 36.0
+    w = ic.W
+    return np.where(ball, rho_in, 0.0)[..., None] * w, np.where(ball, 0.0, cfg["rho0"])[..., None] * w
+
+
+@pytest.mark.parametrize("rho_in", [3.0, 5.0])
+def test_density_beyond_the_fixed_point_range_is_reported(rp, rho_in):
+    """The scatter form sums colour densities in 32-bit fixed point at 2^29
+    units per rho0.  A red ball at 3 rho0 steps normally; one at 5 rho0
+    wraps the sums, and sync, calmacros, download and diagnostics report
+    the range error until a new state is installed."""
+    cfg, ri, bi = rp
+    lat = _lattice(cfg)
+    assert lat.scatter_form()
+    lat.upload(*_ball_state(cfg, rho_in))
+    lat.step(4)
+    if rho_in < 4.0:
+        lat.sync()
+        m = lat.calmacros()
+        _print("ball at", rho_in, "rho0: max rho after 4 steps", float(m["rho"].max()))
+        assert np.isfinite(m["phi"]).all() and float(m["rho"].max()) > 2.5
+        return
+    for call in (lat.sync, lat.calmacros, lat.download, lat.diagnostics):
+        with pytest.raises(symmetry_amd.CglError) as e:
+            call()
+        assert e.value.code == symmetry_amd.ERR_RANGE, call
+    lat.upload(ri, bi)
+    lat.step(4)
+    lat.sync()
+    phi, u = _fields(lat)
+    assert np.isfinite(phi).all() and np.isfinite(u).all()
