@@ -27,6 +27,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+REF_O3_PATH = os.path.join(HERE, "_ref", "o3", "libref.so")  # the CPU baseline's -O3 build
 REF_SOURCE = "/root/reference/Source1.f90"
 
 BC_PERIODIC, BC_WALL, BC_MIRROR = 0, 1, 2
@@ -219,7 +220,7 @@ REF_CALL = {
     "calmacros": 1, "derivatives": 2, "collision": 3, "recolor": 4, "boundary": 5,
     "streaming_r": 6, "streaming_b": 7, "curve": 8, "set_walls": 9, "refilling": 10,
     "momentum": 11, "lagrange": 12, "capillary": 13, "repulsive": 14, "movement": 15,
-    "initialization": 16, "initia_center": 17, "tables": 18, "output4": 19,
+    "initialization": 16, "initia_center": 17, "tables": 18, "output4": 19, "output3": 20,
 }
 
 # The reference's geometry: x=1 and z=1 walls, mirrors at x=xl, y=1, y=yl, z=zl
@@ -245,11 +246,11 @@ class RefHarness:
     reference's streaming keeps a whole population array as a stack local
     (S:740)."""
 
-    def __init__(self, M, sigma, rho0, nu_r, nu_b, rad, rad_c):
-        if not ref_available():
-            raise FileNotFoundError(REF_PATH)
+    def __init__(self, M, sigma, rho0, nu_r, nu_b, rad, rad_c, path=REF_PATH):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
         os.environ.setdefault("OMP_NUM_THREADS", "1")
-        self.lib = ctypes.CDLL(REF_PATH)
+        self.lib = ctypes.CDLL(path)
         self.lib.ref_ptr.restype = ctypes.c_void_p
         self.lib.ref_setup.argtypes = [ctypes.c_double] * 6 + [ctypes.c_void_p]
         Mf = np.asfortranarray(np.asarray(M, dtype=np.float64))
@@ -314,6 +315,11 @@ class RefHarness:
         """The line the last output4 call wrote to Energy.dat."""
         with open(os.path.join(_scratch_dir(), "Energy.dat")) as f:
             return f.read().splitlines()[-1]
+
+    def hmin_lines(self):
+        """The lines of hmin.dat after the last output3 call."""
+        with open(os.path.join(_scratch_dir(), "hmin.dat")) as f:
+            return f.read().splitlines()
 
     def step(self, nsteps=1, particles=False):
         self._run(self.lib.ref_step, int(nsteps), int(bool(particles)))

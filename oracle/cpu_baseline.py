@@ -3,8 +3,9 @@
 Times the reference's own no-particle time step -- calmacros, derivatives,
 collision, Recolor, boundary, streaming x2, curve (Source1.f90 S:216-251 with
 the particle block switched off as its commented if(1==2) does) -- from the
-unmodified reference compiled into oracle/_ref/libref.so, with OpenMP over
-all requested threads.  The reference's box is its compile-time 152^3.  When
+unmodified reference compiled at -O3 with OpenMP into oracle/_ref/o3/libref.so
+(the -O2, no-FMA checker build oracle/_ref/libref.so where that is absent),
+over all requested threads.  The reference's box is its compile-time 152^3.  When
 the reference build is absent, times the FP64 C restatement
 (liboracle.so, one thread) on a periodic box instead.
 
@@ -48,12 +49,14 @@ def main():
     import oracle
     from symmetry_amd import ic
 
-    if oracle.ref_available() and not args.port:
+    ref_path = oracle.REF_O3_PATH if os.path.exists(oracle.REF_O3_PATH) else oracle.REF_PATH
+    flags = "-O3" if ref_path == oracle.REF_O3_PATH else "-O2 -ffp-contract=off"
+    if os.path.exists(ref_path) and not args.port:
         with open(os.path.join(oracle.HERE, "..", "tests", "golden", "d3q19_tables.json")) as f:
             M = np.array(json.load(f)["M"], dtype=float)
         rad_c, sigma, rho0 = 30.0, 0.05, 0.05
         nu_r = 0.1 / math.sqrt(rho0 / (rad_c * sigma))
-        ref = oracle.RefHarness(M, sigma, rho0, nu_r, nu_r / 10.0, 7.5, rad_c)
+        ref = oracle.RefHarness(M, sigma, rho0, nu_r, nu_r / 10.0, 7.5, rad_c, path=ref_path)
         ref.scal()[3] = (ref.yl - 2) * 2.0
         ref.part("center")[:] = -1000.0  # no particles in the box
         ref.call("set_walls")
@@ -68,7 +71,7 @@ def main():
         dt = time.perf_counter() - t0
         nodes = ref.xl * ref.yl * ref.zl
         out = dict(value=nodes * nsteps / dt / 1e6, unit="MLUPS", cores=args.threads, kind="reference",
-                   sample=f"reference Source1.f90 (flang -O2, OpenMP) no-particle FP64 step at its compiled "
+                   sample=f"reference Source1.f90 (flang {flags}, OpenMP) no-particle FP64 step at its compiled "
                           f"{ref.xl}x{ref.yl}x{ref.zl} box, {nsteps} steps after 1 warm-up, {dt:.1f} s, "
                           f"{args.threads} threads on {cpu_model()}")
     else:

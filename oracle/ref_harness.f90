@@ -191,6 +191,11 @@ contains
       call output4(rho_N,int(scal(7)),scal(6),walls_all,pout,center0,ux,uy,uz,rho,upx,upy,upz,rpx,rpy,rpz, &
                    tension,tension_t,scal(2),scal(3),scal(5),Shb)
       close(400)
+    ! output3 with t = scal(7), tr = scal(6), lamda_d = scal(4); unit 300
+    ! closed afterwards so that hmin.dat holds this call's line
+    case (20)
+      call output3(rho_N,int(scal(7)),scal(6),walls_all,pout,scal(4))
+      close(300)
     end select
   end subroutine ref_call
 

@@ -3,6 +3,7 @@ libcgl through its bind(C) module exactly where the reference's time loop
 calls its subroutines.  Run it on a small box and check its dumped fields
 bit for bit against the FP64 oracle fed the same initial state, and its
 Tecplot dump against the reference's output_results format (S:910-921)."""
+import math
 import os
 import subprocess
 
@@ -10,6 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import energy as en
+from oracle import hmin as hm
 from symmetry_amd import ic
 from tests.helpers import oracle_from
 
@@ -137,9 +139,17 @@ def test_fortran_driver_reference_faces_with_particles(tmp_path):
                          mass=parts["mass"], inertia=parts["inertia"], center0=np.array(cs), shb=g["shb"],
                          wcap_old=wcap_old)
 
-    e0 = energy(0.0)
+    tr = math.sqrt(8.0 ** 3 * 1.0 / 0.005)
+
+    def hmin_line(t):
+        """output3 restated from the binding's state (Rad_c 8, lamda_d = (yl - 2) 2)."""
+        m, mk = lat.calmacros(), lat.masks()
+        h, y0 = hm.hmin(m["phi"], mk["walls_all"], mk["pout"])
+        return hm.line(t, tr, h, y0, 8.0, (n - 2) * 2.0)
+
+    e0, h0 = energy(0.0), hmin_line(0)
     lat.step(steps)
-    e1 = energy(e0["wcap_old"])
+    e1, h1 = energy(e0["wcap_old"]), hmin_line(steps)
     mac = lat.calmacros()
     for k, name in enumerate(("rho", "rho_r", "rho_b", "ux", "uy", "uz", "phi")):
         assert np.array_equal(raw[k], mac[name]), name
@@ -157,10 +167,10 @@ def test_fortran_driver_reference_faces_with_particles(tmp_path):
         assert len(cols) == 26 and int(cols[0]) == steps
         assert np.allclose(cols[2:5], got["center"][p], atol=1e-6) and np.allclose(cols[11:14], got["force"][p], atol=1e-6)
     hmin = (tmp_path / "hmin.dat").read_text().splitlines()
-    assert hmin[0].strip() == "VARIABLES=t,t*,h,y0" and len(hmin) == 3 and len(hmin[-1].split()) == 4
+    assert hmin[0].strip() == "VARIABLES=t,t*,h,y0" and len(hmin) == 3
+    assert hmin[1] == h0 and hmin[2] == h1, (hmin, h0, h1)
     # the energy budget (S:1719-1797), both particles on mirror planes (half counts)
     lines = (tmp_path / "Energy.dat").read_text().splitlines()
-    tr = (8.0 ** 3 * 1.0 / 0.005) ** 0.5
     assert lines[0].strip() == "VARIABLES=t,t*,Esur,Shb,Efree,Emov,Wcap,Emov_p,Esum" and len(lines) == 3
     assert lines[1] == en.line(0, tr, e0) and lines[2] == en.line(steps, tr, e1)
     assert e1["Shb"] > 0 and e1["Emov_p"] > 0 and e1["Esur"] > 0

@@ -204,3 +204,51 @@ def test_output4_energy_line(ref):
     assert ref.energy_line() == en.line(t, tr, e)
     assert sc[4] == e["wcap_old"]
     assert np.array_equal(ref.part("Shb"), e["shb"])
+
+
+def test_output3_hmin_line(ref):
+    """The reference's own output3 (S:1673-1717) on designed fields of its
+    box, against the restatement's hmin.dat line.  On the plane x = xl - 1
+    an interface whose height varies with y (its lowest column, the neck,
+    is not the first), a second, lower sign change in some columns (the last
+    crossing counts), crossings next to a wall or pout node in other columns
+    (skipped), and a few columns without a crossing (bottom stays 1); the
+    neighbouring planes hold other heights, so reading the wrong plane, the
+    first crossing, or z for z+1 changes the line."""
+    from oracle import hmin as hm
+
+    xl, yl, zl = ref.xl, ref.yl, ref.zl
+    sc = ref.scal()
+    lamda_d = (yl - 2) * 2.0
+    sc[3] = lamda_d
+    rng = np.random.default_rng(11)
+    zz = np.arange(zl, dtype=float) + 1.0
+    for t, tr in ((0, 123.25), (400, 123.25), (1200, 77.5)):
+        phi, wal, pou = ref.field("phi"), ref.field("walls_all"), ref.field("pout")
+        wal[...] = 0
+        pou[...] = 0
+        xp = xl - 2
+        for x in range(xl):
+            height = 80.0 + 25.0 * np.cos(2 * np.pi * (np.arange(yl) - 37.0 - 3 * x) / yl) + rng.uniform(-2, 2, yl)
+            phi[x] = np.tanh(height[:, None] - zz[None, :])
+            if x == xp:
+                neck = 1 + int(np.argmax(height[1:yl - 1]))
+        cols = rng.choice(np.setdiff1d(np.arange(1, yl - 1), [neck]), 40, replace=False)
+        for y in [neck] + list(cols[:12]):  # a lower blue band: an earlier crossing
+            phi[xp, y, 20:24] = -0.5
+        for y in cols[12:20]:  # the crossing's upper node is a wall node
+            zc = int(np.argmax(phi[xp, y] < 0))
+            wal[xp, y, zc] = 10
+        for y in cols[20:28]:  # its lower node is a pout node
+            zc = int(np.argmax(phi[xp, y] < 0))
+            pou[xp, y, zc - 1] = 1
+        for y in cols[28:32]:  # no crossing
+            phi[xp, y] = 0.5
+        h, y0 = hm.hmin(phi.copy(), wal.copy(), pou.copy())
+        assert 2 < y0 < yl - 1 and h < zl
+        sc[5], sc[6] = tr, float(t)
+        ref.call("output3")
+        lines = ref.hmin_lines()
+        assert lines[-1] == hm.line(t, tr, h, y0, RAD_C, lamda_d), (t, lines[-1])
+        if t == 0:
+            assert lines[0].strip() == "VARIABLES=t,t*,h,y0" and len(lines) == 2
