@@ -186,8 +186,43 @@ int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *
 int cgl_get_particle_shb(cgl_t *h, double *shb);
 
 /* Masks of set_walls in force (walls, walls_all, pin, pout; S:326-395),
- * (zl, yl, xl) layout.  Any pointer may be NULL. */
+ * (zl, yl, xl) layout; within a staged step after cgl_set_walls, the new
+ * masks.  Any pointer may be NULL. */
 int cgl_download_masks(cgl_t *h, int32_t *walls, int32_t *walls_all, int32_t *pin, int32_t *pout);
+
+/* A time step with particles, stage by stage: the reference calls these
+ * routines one after another in its time loop (S:218-251), and each entry
+ * point runs exactly the kernels cgl_step runs at that point, so a staged
+ * step equals a cgl_step step bit for bit.  A staged step opens with
+ * cgl_calmacros (calmacros, S:770-831; any output pointer may be NULL) and
+ * continues in this order; an entry point called out of order returns
+ * CGL_ERR_ARG, and while a staged step is open cgl_step, cgl_time_steps and
+ * the calls that recompute moments (calmacros, derivatives, diagnostics,
+ * downloads, checkpoint save) refuse.  cgl_get_particles,
+ * cgl_get_particle_shb, cgl_download_masks and cgl_particle_band read the
+ * state between stages.  Installing a state or particles abandons an open
+ * staged step.  Wall-only handles (npar = 0) go from cgl_calmacros straight
+ * to cgl_collide_stream. */
+int cgl_capillary(cgl_t *h);      /* Lagrange_points + capillary_force2 (S:1799-2028): fcap, tcap, Shb */
+int cgl_momentum(cgl_t *h);       /* momentum (S:1178-1261): force and torque of the links */
+int cgl_repulsive(cgl_t *h);      /* repulsive (S:1594-1671): wall and pair repulsion */
+int cgl_move(cgl_t *h);           /* movement (S:1263-1349): symmetry rules, velocities, centres */
+int cgl_set_walls(cgl_t *h);      /* set_walls (S:326-395): masks of the new centres */
+int cgl_refill(cgl_t *h);         /* refilling (S:1351-1418): nodes the move uncovered */
+/* derivatives (S:452-597), collision (S:599-695), Recolor (S:697-732),
+ * boundary (S:1544-1592), streaming of r and b (S:735-768) and curve
+ * (S:990-1034): one fused pull kernel plus the bounce-back links (the pull
+ * layout streams as it reads, so the reference's collision, Recolor,
+ * boundary and streaming calls have no separate entry points).  Ends the
+ * staged step. */
+int cgl_collide_stream(cgl_t *h);
+
+/* The FP64 macroscopic fields the particle stages read and write in the band
+ * around the particles (rho, rho_R, rho_B, ux, uy, uz in (zl, yl, xl)
+ * layout): calmacros' values there, and refilling's at the nodes it
+ * refilled (S:1364-1391).  Defined at the band's nodes only (within R + 3 of
+ * a centre); any pointer may be NULL. */
+int cgl_particle_band(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *ux, double *uy, double *uz);
 
 /* Global FP64 sums on the device over the live fluid nodes (ghost layers and
  * particle interiors excluded) of the current state's macroscopic fields:

@@ -65,7 +65,12 @@ EXPORTS = (
     "cgl_batch_create", "cgl_box_upload_state", "cgl_box_init_cylinder", "cgl_box_download_state",
     "cgl_box_calmacros", "cgl_diagnostics", "cgl_init_interface", "cgl_phase_field_form",
     "cgl_set_particle_geometry", "cgl_checkpoint_save", "cgl_checkpoint_load", "cgl_diagnostics_noalloc",
+    "cgl_capillary", "cgl_momentum", "cgl_repulsive", "cgl_move", "cgl_set_walls", "cgl_refill",
+    "cgl_collide_stream", "cgl_particle_band",
 )
+
+# The stages of a staged step after cgl_calmacros, in order.
+STAGES = ("capillary", "momentum", "repulsive", "move", "set_walls", "refill", "collide_stream")
 
 
 def lib():
@@ -104,6 +109,9 @@ def lib():
         L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.cgl_phase_field_form.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
         L.cgl_set_particle_geometry.argtypes = [vp] + [vp] * 5 + [ctypes.c_int32]
+        for st in STAGES:
+            getattr(L, "cgl_" + st).argtypes = [vp]
+        L.cgl_particle_band.argtypes = [vp] + [vp] * 6
         L.cgl_checkpoint_save.argtypes = [vp, ctypes.c_char_p]
         L.cgl_checkpoint_load.argtypes = [vp, ctypes.c_char_p]
         L.cgl_probe_copy_bandwidth.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
@@ -250,6 +258,20 @@ class Lattice:
         names = ("walls", "walls_all", "pin", "pout")
         out = {k: np.empty(self.shape, dtype=np.int32) for k in names}
         _check(lib().cgl_download_masks(self.h, *(_ptr(out[k]) for k in names)))
+        return out
+
+    def stage(self, name):
+        """Run one stage of a staged step (``STAGES``; calmacros opens it)."""
+        if name not in STAGES:
+            raise ValueError(name)
+        _check(getattr(lib(), "cgl_" + name)(self.h))
+
+    def particle_band(self):
+        """The FP64 band fields {rho, rho_r, rho_b, ux, uy, uz} the particle
+        stages read and write (defined within R + 3 of the centres)."""
+        names = ("rho", "rho_r", "rho_b", "ux", "uy", "uz")
+        out = {k: np.empty(self.shape) for k in names}
+        _check(lib().cgl_particle_band(self.h, *(_ptr(out[k]) for k in names)))
         return out
 
     def tuned_xchunk(self):

@@ -170,9 +170,20 @@ int clear_range(cgl_handle *h)
     return CGL_OK;
 }
 
+// A staged step between its stages holds moments, masks and particle state
+// of different time levels: only the particle, mask and band read-backs may
+// look at it.
+int no_open_stage(const cgl_handle *h)
+{
+    if (h->stage > STAGE_CALMACROS)
+        return fail(CGL_ERR_ARG, "a staged step is open: finish it with cgl_collide_stream");
+    return CGL_OK;
+}
+
 int state_readable(const cgl_handle *h)
 {
     if (h->scal_stale) return fail(CGL_ERR_HIP, "internal: the state's recolour scalars were not written");
+    if (int rc = no_open_stage(h)) return rc;
     return check_range(h);
 }
 
@@ -448,6 +459,7 @@ int cgl_upload_state(cgl_t *h, const double *ri, const double *bi)
         h->cur = 0; // uploaded states are held in RB form in buffer 0
         h->state_layout = cgl::RB;
         h->primed = 0;
+        h->stage = STAGE_NONE;
         h->phi_ok = 0;
         h->scal_stale = 0;
         (void)clear_range(h);
@@ -499,6 +511,7 @@ int cgl_init_cylinder(cgl_t *h, double xc, double yc, double r0, double kprof, i
     h->cur = 0; // initial states are held in RB form in buffer 0
     h->state_layout = cgl::RB;
     h->primed = 0;
+    h->stage = STAGE_NONE;
     h->phi_ok = 0;
     h->scal_stale = 0;
     (void)clear_range(h);
@@ -521,6 +534,7 @@ int cgl_init_interface(cgl_t *h, double z0, double kprof)
     h->cur = 0; // initial states are held in RB form in buffer 0
     h->state_layout = cgl::RB;
     h->primed = 0;
+    h->stage = STAGE_NONE;
     h->phi_ok = 0;
     h->scal_stale = 0;
     (void)clear_range(h);
@@ -614,6 +628,8 @@ int cgl_step(cgl_t *h, int32_t nsteps)
     if (!h || nsteps < 0) return fail(CGL_ERR_ARG, "bad arguments");
     int rc = set_device(h);
     if (rc) return rc;
+    if ((rc = no_open_stage(h))) return rc;
+    h->stage = STAGE_NONE;
     if ((rc = tune_some(h, &nsteps))) return rc;
     if (h->use_graph && !h->tuning && nsteps >= 2) {
         // steady state: replay two captured steps (no host work per step).  The
@@ -669,11 +685,12 @@ int cgl_calmacros(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *u
     if ((rc = ensure_scratch(h, true, false, false))) return rc;
     dispatch(h, [&](auto impl) {
         if (h->ext)
-            decltype(impl)::moments_ext(h, true);
+            decltype(impl)::st_calmacros(h, true);
         else
             decltype(impl)::moments(h, true);
     });
     if ((rc = check_launch("moments"))) return rc;
+    if (h->ext) h->stage = STAGE_CALMACROS; // the first stage of a staged step
     double *dst[7] = {rho, rho_r, rho_b, ux, uy, uz, phi};
     for (int k = 0; k < 7; ++k)
         if (dst[k]) HIPCHK(hipMemcpyAsync(dst[k], h->mac + k * h->N, h->N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -690,6 +707,7 @@ int cgl_derivatives(cgl_t *h, double *gradx, double *grady, double *gradz, doubl
     if (rc) return rc;
     if ((rc = state_readable(h))) return rc;
     if ((rc = ensure_scratch(h, true, false, true))) return rc;
+    h->stage = STAGE_NONE;
     dispatch(h, [&](auto impl) {
         if (h->ext) {
             decltype(impl)::derivatives_ext(h, h->mac, h->iflag);
@@ -719,6 +737,8 @@ int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moment
     if (!h || nsteps < 1) return fail(CGL_ERR_ARG, "bad arguments");
     int rc = set_device(h);
     if (rc) return rc;
+    if ((rc = no_open_stage(h))) return rc;
+    h->stage = STAGE_NONE;
     HIPCHK(hipStreamSynchronize(h->stream));
     // pass 1: the whole loop between two events (no per-kernel events inside)
     HIPCHK(hipEventRecord(h->ev[0], h->stream));
@@ -847,6 +867,7 @@ int cgl_set_particles(cgl_t *h, const double *center, const double *up, const do
     dispatch(h, [&](auto impl) { decltype(impl)::build_masks(h, 0); });
     if ((rc = check_launch("set_walls"))) return rc;
     h->primed = 0;
+    h->stage = STAGE_NONE;
     h->cold = 1;
     HIPCHK(hipStreamSynchronize(h->stream));
     return CGL_OK;
@@ -895,6 +916,105 @@ int cgl_get_particles(cgl_t *h, double *center, double *up, double *rp, double *
     return CGL_OK;
 }
 
+// ------------------------------------------------------------ staged steps
+// One step with particles driven stage by stage, each stage the kernels
+// cgl_step runs at that point, so that each can be checked against its
+// reference routine.  cgl_calmacros opens a staged step.
+extern "C++" {
+static int stage_enter(cgl_handle *h, int prev, const char *name, bool particles)
+{
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    if (h->nbox || !h->ext)
+        return fail(CGL_ERR_UNSUPPORTED, "%s: staged steps run on single boxes with walls or particles", name);
+    if (particles && h->npar == 0) return fail(CGL_ERR_ARG, "%s: the handle was created with npar = 0", name);
+    if (h->stage != prev)
+        return fail(CGL_ERR_ARG,
+                    "%s: stage out of order (a staged step is cgl_calmacros, cgl_capillary, cgl_momentum, "
+                    "cgl_repulsive, cgl_move, cgl_set_walls, cgl_refill, cgl_collide_stream)",
+                    name);
+    return set_device(h);
+}
+
+template <class F>
+static int run_stage(cgl_handle *h, int prev, int next, const char *name, F &&launch, bool particles = true)
+{
+    int rc = stage_enter(h, prev, name, particles);
+    if (rc) return rc;
+    dispatch(h, launch);
+    if ((rc = check_launch(name))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->stage = next;
+    return CGL_OK;
+}
+} // extern "C++"
+
+int cgl_capillary(cgl_t *h)
+{
+    return run_stage(h, STAGE_CALMACROS, STAGE_CAPILLARY, "cgl_capillary", [&](auto impl) {
+        decltype(impl)::st_lagrange(h);
+        decltype(impl)::st_move(h, cgl::MV_CAP);
+    });
+}
+
+int cgl_momentum(cgl_t *h)
+{
+    return run_stage(h, STAGE_CAPILLARY, STAGE_MOMENTUM, "cgl_momentum", [&](auto impl) {
+        decltype(impl)::st_momentum(h);
+        decltype(impl)::st_move(h, cgl::MV_MOM);
+    });
+}
+
+int cgl_repulsive(cgl_t *h)
+{
+    return run_stage(h, STAGE_MOMENTUM, STAGE_REPULSIVE, "cgl_repulsive",
+                     [&](auto impl) { decltype(impl)::st_move(h, cgl::MV_REP); });
+}
+
+int cgl_move(cgl_t *h)
+{
+    return run_stage(h, STAGE_REPULSIVE, STAGE_MOVE, "cgl_move",
+                     [&](auto impl) { decltype(impl)::st_move(h, cgl::MV_MOVE); });
+}
+
+int cgl_set_walls(cgl_t *h)
+{
+    return run_stage(h, STAGE_MOVE, STAGE_SET_WALLS, "cgl_set_walls",
+                     [&](auto impl) { decltype(impl)::st_set_walls(h); });
+}
+
+int cgl_refill(cgl_t *h)
+{
+    return run_stage(h, STAGE_SET_WALLS, STAGE_REFILL, "cgl_refill",
+                     [&](auto impl) { decltype(impl)::st_refill(h); });
+}
+
+int cgl_collide_stream(cgl_t *h)
+{
+    const int prev = h && h->npar ? STAGE_REFILL : STAGE_CALMACROS;
+    int rc = run_stage(
+        h, prev, STAGE_NONE, "cgl_collide_stream", [&](auto impl) { decltype(impl)::st_collide_stream(h); }, false);
+    if (rc) return rc;
+    h->cur ^= 1;
+    h->phicur ^= 1;
+    h->steps_done++;
+    return CGL_OK;
+}
+
+int cgl_particle_band(cgl_t *h, double *rho, double *rho_r, double *rho_b, double *ux, double *uy, double *uz)
+{
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    if (h->npar == 0) return fail(CGL_ERR_ARG, "the handle was created with npar = 0");
+    int rc = set_device(h);
+    if (rc) return rc;
+    double *dst[6] = {rho, rho_r, rho_b, ux, uy, uz};
+    for (int k = 0; k < 6; ++k)
+        if (dst[k])
+            HIPCHK(hipMemcpyAsync(dst[k], h->E.mac + k * h->N, h->N * sizeof(double), hipMemcpyDeviceToHost,
+                                  h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CGL_OK;
+}
+
 int cgl_get_particle_shb(cgl_t *h, double *shb)
 {
     if (!h || !shb) return fail(CGL_ERR_ARG, "null argument");
@@ -925,7 +1045,9 @@ int cgl_download_masks(cgl_t *h, int32_t *walls, int32_t *walls_all, int32_t *pi
     if (pout) HIPCHK(hipMemcpyAsync(pout, h->pout_i, bytes, hipMemcpyDeviceToHost, h->stream));
     if (walls_all) {
         if ((rc = ensure_scratch(h, false, false, true))) return rc;
-        k_masks_out<<<(unsigned)((h->N + 255) / 256), 256, 0, h->stream>>>(h->ntype[h->ntcur], h->N, h->iflag);
+        // between set_walls and the end of a staged step the new masks are the spare buffer
+        const int nt = h->stage >= STAGE_SET_WALLS ? h->ntcur ^ 1 : h->ntcur;
+        k_masks_out<<<(unsigned)((h->N + 255) / 256), 256, 0, h->stream>>>(h->ntype[nt], h->N, h->iflag);
         if ((rc = check_launch("masks"))) return rc;
         HIPCHK(hipMemcpyAsync(walls_all, h->iflag, bytes, hipMemcpyDeviceToHost, h->stream));
     }
@@ -1391,6 +1513,7 @@ int cgl_checkpoint_load(cgl_t *h, const char *path)
     h->xc = H.xc; h->tuning = H.tuning; h->tune_idx = H.tune_idx; h->tune_best_xc = H.tune_best_xc;
     h->tune_best_ms = H.tune_best_ms; h->steps_done = (int)H.steps_done; h->scal_stale = 0;
     (void)clear_range(h);
+    h->stage = STAGE_NONE;
     if (h->npar) {
         h->P.fg = Pp.fg;
         for (int k = 0; k < 6; ++k) { h->P.rep_plane[k] = Pp.rep_plane[k]; h->P.rep_on[k] = Pp.rep_on[k]; }

@@ -63,7 +63,12 @@ struct cgl_handle {
     int scal_stale;      // the current state's recolour scalars were not written (mid-run only)
     cgl::ScBufs sb;      // fixed-point partial sums
     hipEvent_t *tev;     // timing runs: events around the wall/particle step's moments and collide kernels
+    int stage;           // staged step: the last stage run (STAGE_*), STAGE_NONE between steps
 };
+
+// Stages of a step driven through the per-stage entry points, in order.
+enum { STAGE_NONE = 0, STAGE_CALMACROS, STAGE_CAPILLARY, STAGE_MOMENTUM, STAGE_REPULSIVE, STAGE_MOVE, STAGE_SET_WALLS,
+       STAGE_REFILL };
 
 namespace cgl {
 
@@ -397,6 +402,88 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             k_mirror_masks<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->ntype[buf], h->walls_i, h->pin_i,
                                                                 h->pout_i);
     }
+    // The stages of a step with walls and/or particles (S:216-251).  The
+    // masks of the step's start stay in ntype[ntcur] until the step ends;
+    // set_walls builds the new ones into ntype[ntcur ^ 1].
+    static void st_calmacros(cgl_handle *h, bool full)
+    {
+        if (h->npar && !h->primed) {
+            prime(h);
+            h->primed = 1;
+        }
+        moments_ext(h, full); // calmacros with the masks in force
+    }
+    static void st_lagrange(cgl_handle *h)
+    {
+        const Ext Eo = ext_with(h, h->ntype[h->ntcur]);
+        k_lagrange<P><<<pgrid(h, LAG_BLOCKS), PB, 0, h->stream>>>(h->L, Eo, h->P, (P *)h->phi[h->phicur ^ 1],
+                                                                  h->walls_i);
+    }
+    static void st_momentum(cgl_handle *h)
+    {
+        const Ext Eo = ext_with(h, h->ntype[h->ntcur]);
+        const S *rs = (const S *)h->pops[h->cur][0], *bs = (const S *)h->pops[h->cur][1];
+        k_momentum<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->ntype[h->ntcur], h->walls_i,
+                                                                      h->pout_i, rs, bs, h->cold);
+        h->cold = 0;
+    }
+    static void st_move(cgl_handle *h, int phases)
+    {
+        k_move<<<1, PB, 0, h->stream>>>(h->L, ext_with(h, h->ntype[h->ntcur]), h->P, phases);
+    }
+    static void st_set_walls(cgl_handle *h)
+    {
+        // clear the spare buffer (built two steps ago) and the int masks
+        // (built last step), then the new masks
+        const Ext Eo = ext_with(h, h->ntype[h->ntcur]);
+        const int nb = h->ntcur ^ 1;
+        k_masks_clear<<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->P.center_prev, 4.0, h->ntype[nb],
+                                                               nullptr, nullptr, nullptr);
+        k_masks_clear<<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->P.center_old, 4.0,
+                                                               h->ntype[nb] /*already clear*/, h->walls_i, h->pin_i,
+                                                               h->pout_i);
+        build_masks(h, nb);
+    }
+    static void st_refill(cgl_handle *h)
+    {
+        const int nb = h->ntcur ^ 1;
+        P *phi = (P *)h->phi[h->phicur ^ 1];
+        const Ext En = ext_with(h, h->ntype[nb]);
+        k_refill<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[h->ntcur], h->ntype[nb],
+                                                             h->pout_i, phi);
+        mirror_phi(h, phi);
+        mirror_mac(h);
+    }
+    // ghost phi, wetting, derivatives, collision, Recolor, boundary and
+    // streaming (one fused kernel), then curve; the new masks take force
+    static void st_collide_stream(cgl_handle *h)
+    {
+        const int nb = h->npar ? h->ntcur ^ 1 : h->ntcur;
+        P *phi = (P *)h->phi[h->phicur ^ 1];
+        const Ext En = ext_with(h, h->ntype[nb]);
+        if (h->npar) {
+            k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[nb], h->pin_i, phi);
+            mirror_phi(h, phi);
+            // wetted normals next to the particles, read by the collide kernel
+            k_wet_normals<R, P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, phi);
+        }
+        const void *i0, *i1;
+        void *o0, *o1;
+        state_in(h, i0, i1);
+        state_out(h, o0, o1);
+        const P *phio = (const P *)h->phi[h->phicur];
+        if (h->tev) (void)hipEventRecord(h->tev[2], h->stream);
+        k_collide<RB, RB, S, R, P, SHIFT, true>
+            <<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
+        if (h->tev) (void)hipEventRecord(h->tev[3], h->stream);
+        if (h->npar) {
+            S *ro = (S *)o0, *bo = (S *)o1;
+            k_curve<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(
+                h->L, En, h->P, h->ntype[nb], h->walls_i, h->pout_i, ro, bo, SHIFT ? (const float *)phi : nullptr);
+            if (mirrors(h)) k_mirror_der<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->E.der);
+            h->ntcur = nb;
+        }
+    }
     // one time step with walls and/or particles (S:216-251)
     static void step_ext(cgl_handle *h)
     {
@@ -405,55 +492,16 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             h->primed = 1;
         }
         if (h->tev) (void)hipEventRecord(h->tev[0], h->stream);
-        moments_ext(h, false); // calmacros with the masks in force
+        moments_ext(h, false);
         if (h->tev) (void)hipEventRecord(h->tev[1], h->stream);
-        P *phi = (P *)h->phi[h->phicur ^ 1];
-        const int nb = h->ntcur ^ 1;
         if (h->npar) {
-            const Ext Eo = ext_with(h, h->ntype[h->ntcur]);
-            const S *rs = (const S *)h->pops[h->cur][0], *bs = (const S *)h->pops[h->cur][1];
-            k_lagrange<P><<<pgrid(h, LAG_BLOCKS), PB, 0, h->stream>>>(h->L, Eo, h->P, phi, h->walls_i);
-            k_momentum<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->ntype[h->ntcur],
-                                                                          h->walls_i, h->pout_i, rs, bs, h->cold);
-            h->cold = 0;
-            k_move<<<1, PB, 0, h->stream>>>(h->L, Eo, h->P, 1);
-            // set_walls: clear the spare buffer (built two steps ago) and the
-            // int masks (built last step), then the new masks
-            k_masks_clear<<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->P.center_prev, 4.0,
-                                                                   h->ntype[nb], nullptr, nullptr, nullptr);
-            k_masks_clear<<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->P.center_old, 4.0,
-                                                                   h->ntype[nb] /*already clear*/, h->walls_i,
-                                                                   h->pin_i, h->pout_i);
-            build_masks(h, nb);
-            const Ext En = ext_with(h, h->ntype[nb]);
-            k_refill<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[h->ntcur], h->ntype[nb],
-                                                                 h->pout_i, phi);
-            mirror_phi(h, phi);
-            mirror_mac(h);
-            k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[nb], h->pin_i, phi);
-            mirror_phi(h, phi);
+            st_lagrange(h);
+            st_momentum(h);
+            st_move(h, MV_ALL);
+            st_set_walls(h);
+            st_refill(h);
         }
-        {
-            const void *i0, *i1;
-            void *o0, *o1;
-            state_in(h, i0, i1);
-            state_out(h, o0, o1);
-            const P *phio = (const P *)h->phi[h->phicur];
-            const Ext En = ext_with(h, h->ntype[nb]);
-            if (h->npar) // wetted normals next to the particles, read by the collide kernel
-                k_wet_normals<R, P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, phi);
-            if (h->tev) (void)hipEventRecord(h->tev[2], h->stream);
-            k_collide<RB, RB, S, R, P, SHIFT, true>
-                <<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
-            if (h->tev) (void)hipEventRecord(h->tev[3], h->stream);
-            if (h->npar) {
-                S *ro = (S *)o0, *bo = (S *)o1;
-                k_curve<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(
-                    h->L, En, h->P, h->ntype[nb], h->walls_i, h->pout_i, ro, bo, SHIFT ? (const float *)phi : nullptr);
-                if (mirrors(h)) k_mirror_der<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->E.der);
-            }
-        }
-        if (h->npar) h->ntcur = nb;
+        st_collide_stream(h);
     }
 
     static void init_cylinder(cgl_handle *h, const CylIC &ic)

@@ -653,28 +653,39 @@ __global__ __launch_bounds__(PB) void k_lagrange(Lat L, Ext E, Part Pt, const P 
 }
 
 // ------------------------------------------------- repulsion + Newton-Euler
-// One workgroup: per-particle sums of the partial results in a fixed order,
-// soft wall and pair repulsion (S:1602-1668), explicit update of velocity,
-// angular velocity and centre (S:1337-1346).
-static __global__ __launch_bounds__(PB) void k_move(Lat L, Ext E, Part P, int with_capillary)
+// One workgroup, in phases that a step runs together and the staged entry
+// points one at a time: MV_MOM and MV_CAP finish the momentum exchange and
+// the capillary force (per-particle sums of the partial results in a fixed
+// order), MV_REP the soft wall and pair repulsion (S:1602-1668), MV_MOVE the
+// symmetry-plane rules and the explicit update of velocity, angular velocity
+// and centre (S:1272-1346).
+enum { MV_MOM = 1, MV_CAP = 2, MV_REP = 4, MV_MOVE = 8, MV_ALL = 15 };
+static __global__ __launch_bounds__(PB) void k_move(Lat L, Ext E, Part P, int phases)
 {
     const int np = P.npar;
     const double range = 3.0, order = 0.01;
     const int n3[3] = {L.nx, L.ny, L.nz};
     for (int p = threadIdx.x; p < np; p += PB) {
-        double m[6] = {0, 0, 0, 0, 0, 0}, c[7] = {0, 0, 0, 0, 0, 0, 0};
-        for (int s = 0; s < PSLICES; ++s)
-            for (int k = 0; k < 6; ++k) m[k] += P.part_mom[((size_t)p * PSLICES + s) * 6 + k];
-        if (with_capillary)
+        if (phases & MV_MOM) {
+            double m[6] = {0, 0, 0, 0, 0, 0};
+            for (int s = 0; s < PSLICES; ++s)
+                for (int k = 0; k < 6; ++k) m[k] += P.part_mom[((size_t)p * PSLICES + s) * 6 + k];
+            for (int k = 0; k < 3; ++k) {
+                P.fp[3 * p + k] = m[k];
+                P.tp[3 * p + k] = m[3 + k];
+            }
+        }
+        if (phases & MV_CAP) {
+            double c[7] = {0, 0, 0, 0, 0, 0, 0};
             for (int s = 0; s < LAG_BLOCKS; ++s)
                 for (int k = 0; k < 7; ++k) c[k] += P.part_lag[((size_t)p * LAG_BLOCKS + s) * 7 + k];
-        for (int k = 0; k < 3; ++k) {
-            P.fp[3 * p + k] = m[k];
-            P.tp[3 * p + k] = m[3 + k];
-            P.ten[3 * p + k] = c[k];
-            P.tent[3 * p + k] = c[3 + k];
+            for (int k = 0; k < 3; ++k) {
+                P.ten[3 * p + k] = c[k];
+                P.tent[3 * p + k] = c[3 + k];
+            }
+            P.shb[p] = c[6];
         }
-        P.shb[p] = c[6];
+        if (!(phases & MV_REP)) continue;
         // walls (S:1606-1650): high faces push with +, low faces with -
         const double R = P.rad[p];
         double fr[3] = {0, 0, 0};
@@ -705,6 +716,7 @@ static __global__ __launch_bounds__(PB) void k_move(Lat L, Ext E, Part P, int wi
         }
         for (int a = 0; a < 3; ++a) P.frep[3 * p + a] = fr[a];
     }
+    if (!(phases & MV_MOVE)) return;
     __syncthreads(); // every repulsion read the old centres
     if (P.sym_rules) {
         // the reference's symmetry-plane rules (S:1272-1335): a particle that
