@@ -14,10 +14,20 @@
 ! centre with nmodes cosine modes of wavelength zl (ic = 'cylinder'), or a
 ! flat interface at z = z0 with red fluid below (ic = 'interface').  With
 ! zwalls the z faces are half-way bounce-back walls with ghost layers (the
-! reference's z=1 kind, S:1557-1590).  npar > 0 adds spheres at
-! (px, py, pz) of radius rad_p, density rho_p and contact angle cita_deg,
-! with mass and moment of inertia as S:81-82 and Fg as S:84; their state is
-! appended to PARTICLES.dat at every output step.  ref_faces selects the
+! reference's z=1 kind, S:1557-1590); ic_on_device builds either initial
+! state on the device.  npar > 0 adds spheres, their arrays allocated to
+! npar: centres px, py, pz, radius p_rad, density p_rho and contact angle
+! p_cita (degrees) per sphere, from a second namelist group &particles in
+! the same file (p_rad, p_rho, p_cita default to the scalars rad_p, rho_p,
+! cita_deg of &cgl), or from particle_file, one sphere per line:
+! x y z radius density angle.  Mass and moment of inertia as S:81-82, Fg as
+! S:84 with the first sphere's radius; their state is appended to
+! PARTICLES.dat at every output step.  nbox > 0 runs a batch of independent
+! periodic boxes instead, one per line of box_file:
+! nx ny nz tau_R tau_B R nmodes amp(1..nmodes), each a z-axis cylinder of
+! radius R built on the device; BOXES.dat gets the red and blue mass and the
+! largest speed of every box at every output step, and dump_binary the
+! fields of the first dump_boxes boxes (all when negative).  ref_faces selects the
 ! reference's own box: walls at x = 1 and z = 1, mirrors at x = xl, y = 1,
 ! y = yl and z = zl (S:1443-1592), and for particles its repulsion planes
 ! (S:1607-1612), Lagrange in-domain box (S:1832) and symmetry-plane rules of
@@ -38,13 +48,16 @@ program cgl_driver
   character(len=300) :: ckpt_name
   character(len=256) :: outdir = '.'
   character(len=16) :: ic = 'cylinder'
-  integer, parameter :: MAXPAR = 64
-  integer :: npar = 0
+  integer :: npar = 0, nbox = 0, dump_boxes = -1
   double precision :: z0 = 0.0d0, rad_p = 10.0d0, rho_p = 1.5d0, cita_deg = 90.0d0
-  double precision :: px(MAXPAR) = 0.0d0, py(MAXPAR) = 0.0d0, pz(MAXPAR) = 0.0d0
+  character(len=256) :: particle_file = ' ', box_file = ' '
+  ! per-sphere set-up, allocated to npar: centre, radius, density, contact angle (degrees)
+  double precision, allocatable :: px(:), py(:), pz(:), p_rad(:), p_rho(:), p_cita(:)
   namelist /cgl/ xl, yl, zl, nsteps, out_every, prec, xchunk, sigma, rho0, tau_R, tau_B, beta, Rad_c, &
                  nmodes, amp, ic_on_device, write_tecplot, dump_binary, outdir, layout, ic, z0, zwalls, &
-                 npar, px, py, pz, rad_p, rho_p, cita_deg, ref_faces, ckpt_every, restart, restart_step
+                 npar, rad_p, rho_p, cita_deg, particle_file, ref_faces, ckpt_every, restart, restart_step, &
+                 nbox, box_file, dump_boxes
+  namelist /particles/ px, py, pz, p_rad, p_rho, p_cita
 
   double precision, allocatable :: ri(:,:,:,:), bi(:,:,:,:)
   double precision, allocatable :: rho(:,:,:), rho_R(:,:,:), rho_B(:,:,:), rho_N(:,:,:), ux(:,:,:), uy(:,:,:), uz(:,:,:)
@@ -72,6 +85,11 @@ program cgl_driver
     read(u, nml=cgl)
     close(u)
   end if
+  if (npar < 0) then
+    write(*,*) 'npar must not be negative'
+    error stop 2
+  end if
+  if (npar > 0) call read_particles()
 
   ! relaxation times to viscosities (inverse of S:71-72)
   nu_R = (tau_R - 0.5d0)/3.0d0
@@ -81,10 +99,6 @@ program cgl_driver
   wi(7:18) = 1.0d0/36.0d0
   write(*,*) 'box', xl, yl, zl, ' nu_R=', nu_R, ' nu_B=', nu_B, ' precision', prec
 
-  if (npar < 0 .or. npar > MAXPAR) then
-    write(*,*) 'npar must be in 0 ..', MAXPAR
-    error stop 2
-  end if
   p%nx = xl; p%ny = yl; p%nz = zl
   p%bc = CGL_BC_PERIODIC
   if (zwalls) p%bc(5:6) = CGL_BC_WALL
@@ -96,6 +110,10 @@ program cgl_driver
   p%layout = layout
   p%reserved = 0
   p%sigma = sigma; p%rho0 = rho0; p%nu_r = nu_R; p%nu_b = nu_B; p%beta = beta
+  if (nbox > 0) then
+    call run_batch()
+    stop
+  end if
   call cgl_check(cgl_create(h, p), 'cgl_create')
 
   allocate(rho(zl,yl,xl), rho_R(zl,yl,xl), rho_B(zl,yl,xl), rho_N(zl,yl,xl), ux(zl,yl,xl), uy(zl,yl,xl), uz(zl,yl,xl))
@@ -105,6 +123,8 @@ program cgl_driver
   if (ic_on_device .and. trim(ic) == 'cylinder') then
     call cgl_check(cgl_init_cylinder(h, dble(xl)/2.0d0 + 0.5d0, dble(yl)/2.0d0 + 0.5d0, Rad_c, &
                    6.0d0*beta*0.134d0, int(nmodes, c_int32_t), amp), 'cgl_init_cylinder')
+  else if (ic_on_device .and. trim(ic) == 'interface') then
+    call cgl_check(cgl_init_interface(h, z0, 6.0d0*beta*0.134d0), 'cgl_init_interface')
   else
     allocate(ri(0:18,zl,yl,xl), bi(0:18,zl,yl,xl))
     if (trim(ic) == 'interface') then
@@ -118,13 +138,13 @@ program cgl_driver
   if (npar > 0) then
     allocate(center(npar,3), up(npar,3), rp(npar,3), fp(npar,3), tp(npar,3), fcap(npar,3), tcap(npar,3), &
              frep(npar,3), prad(npar), pmass(npar), pinert(npar), pcita(npar))
-    center(:,1) = px(1:npar); center(:,2) = py(1:npar); center(:,3) = pz(1:npar)
+    center(:,1) = px; center(:,2) = py; center(:,3) = pz
     up = 0.0d0; rp = 0.0d0
-    prad = rad_p
-    pmass = 4.0d0/3.0d0*rho_p*PI*rad_p**3            ! S:81
-    pinert = 2.0d0/5.0d0*pmass*rad_p**2              ! S:82
-    pcita = cita_deg/180.0d0*PI
-    fg = -PI*2.0d0*rad_p*sigma                       ! S:84
+    prad = p_rad
+    pmass = 4.0d0/3.0d0*p_rho*PI*p_rad**3            ! S:81
+    pinert = 2.0d0/5.0d0*pmass*p_rad**2              ! S:82
+    pcita = p_cita/180.0d0*PI
+    fg = -PI*2.0d0*p_rad(1)*sigma                    ! S:84
     call cgl_check(cgl_set_particles(h, center, up, rp, prad, pmass, pinert, pcita, fg), 'cgl_set_particles')
     if (ref_faces) call cgl_check(cgl_set_particle_geometry(h, &
         [1.5d0, dble(xl) - 1.5d0, 1.5d0, dble(yl) - 1.5d0, 1.5d0, dble(zl) - 0.5d0], [1, 1, 1, 1, 1, 1], &
@@ -165,6 +185,97 @@ program cgl_driver
   call cgl_check(cgl_destroy(h), 'cgl_destroy')
 
 contains
+
+  ! the spheres' set-up: particle_file, or the &particles group of the
+  ! namelist file
+  subroutine read_particles()
+    integer :: k, up_, ios_
+    allocate(px(npar), py(npar), pz(npar), p_rad(npar), p_rho(npar), p_cita(npar))
+    px = 0.0d0; py = 0.0d0; pz = 0.0d0
+    p_rad = rad_p; p_rho = rho_p; p_cita = cita_deg
+    if (len_trim(particle_file) > 0) then
+      open(newunit=up_, file=trim(particle_file), status='old', action='read', iostat=ios_)
+      if (ios_ /= 0) then
+        write(*,*) 'cannot open ', trim(particle_file)
+        error stop 2
+      end if
+      do k = 1, npar
+        read(up_, *, iostat=ios_) px(k), py(k), pz(k), p_rad(k), p_rho(k), p_cita(k)
+        if (ios_ /= 0) then
+          write(*,*) trim(particle_file), ': fewer than npar =', npar, 'lines of x y z radius density angle'
+          error stop 2
+        end if
+      end do
+      close(up_)
+    else
+      open(newunit=up_, file=trim(arg), status='old', action='read')
+      read(up_, nml=particles, iostat=ios_)
+      close(up_)
+      if (ios_ /= 0) then
+        write(*,*) 'npar > 0 needs particle_file or a &particles group (px, py, pz, ...)'
+        error stop 2
+      end if
+    end if
+  end subroutine read_particles
+
+  ! a batch of independent periodic boxes in one handle (cgl_batch_create):
+  ! z-axis cylinders built on the device, per-box output lines
+  subroutine run_batch()
+    type(cgl_box), allocatable :: boxes(:)
+    integer, allocatable :: nm(:)
+    double precision, allocatable :: r0(:), am(:,:), tr_(:), tb_(:)
+    double precision, allocatable :: f(:,:,:,:)
+    integer :: b, k, ub, uo, tt, nc
+    character(len=64) :: name
+    allocate(boxes(nbox), nm(nbox), r0(nbox), am(16, nbox), tr_(nbox), tb_(nbox))
+    am = 0.0d0
+    open(newunit=ub, file=trim(box_file), status='old', action='read')
+    do b = 1, nbox
+      read(ub, *) boxes(b)%nx, boxes(b)%ny, boxes(b)%nz, tr_(b), tb_(b), r0(b), nm(b), (am(k, b), k = 1, nm(b))
+      boxes(b)%reserved = 0
+      boxes(b)%nu_r = (tr_(b) - 0.5d0)/3.0d0
+      boxes(b)%nu_b = (tb_(b) - 0.5d0)/3.0d0
+    end do
+    close(ub)
+    p%bc = CGL_BC_PERIODIC
+    p%npar = 0
+    call cgl_check(cgl_batch_create(h, p, int(nbox, c_int32_t), boxes), 'cgl_batch_create')
+    do b = 1, nbox
+      call cgl_check(cgl_box_init_cylinder(h, int(b - 1, c_int32_t), dble(boxes(b)%nx)/2.0d0 + 0.5d0, &
+                     dble(boxes(b)%ny)/2.0d0 + 0.5d0, r0(b), 6.0d0*beta*0.134d0, int(nm(b), c_int32_t), &
+                     am(:, b)), 'cgl_box_init_cylinder')
+    end do
+    open(newunit=uo, file=trim(outdir)//'/BOXES.dat', status='replace')
+    write(uo, '(a)') 'VARIABLES=T,BOX,MASS_R,MASS_B,UMAX'
+    close(uo)
+    tt = 0
+    do
+      if (mod(tt, out_every) == 0 .or. tt == nsteps) then
+        open(newunit=uo, file=trim(outdir)//'/BOXES.dat', status='old', position='append')
+        do b = 1, nbox
+          allocate(f(boxes(b)%nz, boxes(b)%ny, boxes(b)%nx, 7))
+          call cgl_check(cgl_box_calmacros(h, int(b - 1, c_int32_t), f(:,:,:,1), f(:,:,:,2), f(:,:,:,3), &
+                         f(:,:,:,4), f(:,:,:,5), f(:,:,:,6), f(:,:,:,7)), 'cgl_box_calmacros')
+          write(uo, '(i8, 1x, i4, 3(1x, es24.16e3))') tt, b, sum(f(:,:,:,2)), sum(f(:,:,:,3)), &
+                maxval(sqrt(f(:,:,:,4)**2 + f(:,:,:,5)**2 + f(:,:,:,6)**2))
+          if (dump_binary .and. (dump_boxes < 0 .or. b <= dump_boxes)) then
+            write(name, '("/BOX_", i0, "_FIELDS_", i0, ".bin")') b, tt
+            open(newunit=ub, file=trim(outdir)//trim(name), access='stream', form='unformatted', status='replace')
+            write(ub) f
+            close(ub)
+          end if
+          deallocate(f)
+        end do
+        close(uo)
+      end if
+      if (tt >= nsteps) exit
+      nc = min(out_every - mod(tt, out_every), nsteps - tt)
+      call cgl_check(cgl_step(h, int(nc, c_int32_t)), 'cgl_step')
+      tt = tt + nc
+    end do
+    call cgl_check(cgl_sync(h), 'cgl_sync')
+    call cgl_check(cgl_destroy(h), 'cgl_destroy')
+  end subroutine run_batch
 
   ! initialization (S:850-896) for a z-axis cylinder through the box centre
   subroutine initial_state()

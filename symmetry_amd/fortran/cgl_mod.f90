@@ -22,6 +22,13 @@ module cgl
     real(c_double) :: sigma, rho0, nu_r, nu_b, beta
   end type cgl_params
 
+  ! one box of a batch (cgl_batch_create)
+  type, bind(C) :: cgl_box
+    integer(c_int32_t) :: nx, ny, nz
+    integer(c_int32_t) :: reserved = 0
+    real(c_double) :: nu_r, nu_b
+  end type cgl_box
+
   interface
     integer(c_int) function cgl_create(h, p) bind(C, name="cgl_create")
       import :: c_int, c_ptr, cgl_params
@@ -109,6 +116,129 @@ module cgl
     integer(c_int) function cgl_download_masks(h, walls, walls_all, pin, pout) bind(C, name="cgl_download_masks")
       import :: c_int, c_ptr
       type(c_ptr), value :: h, walls, walls_all, pin, pout
+    end function
+    integer(c_int) function cgl_init_interface(h, z0, kprof) bind(C, name="cgl_init_interface")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: z0, kprof
+    end function
+    ! cgl_calmacros with each output by c_loc, or c_null_ptr to skip it: the
+    ! opening stage of a staged step needs no output at all
+    integer(c_int) function cgl_calmacros_ptr(h, rho, rho_r, rho_b, ux, uy, uz, phi) bind(C, name="cgl_calmacros")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, rho, rho_r, rho_b, ux, uy, uz, phi
+    end function
+    integer(c_int) function cgl_derivatives(h, gradx, grady, gradz, n_x, n_y, n_z, flag) &
+        bind(C, name="cgl_derivatives")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: gradx(*), grady(*), gradz(*), n_x(*), n_y(*), n_z(*)
+      integer(c_int32_t), intent(out) :: flag(*)
+    end function
+    integer(c_int) function cgl_tuned_xchunk(h, xchunk, tuning) bind(C, name="cgl_tuned_xchunk")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: xchunk, tuning
+    end function
+    integer(c_int) function cgl_phase_field_form(h, scatter) bind(C, name="cgl_phase_field_form")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: scatter
+    end function
+    ! the stages of a staged step, in order after cgl_calmacros (include/cgl.h)
+    integer(c_int) function cgl_capillary(h) bind(C, name="cgl_capillary")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cgl_momentum(h) bind(C, name="cgl_momentum")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cgl_repulsive(h) bind(C, name="cgl_repulsive")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cgl_move(h) bind(C, name="cgl_move")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cgl_set_walls(h) bind(C, name="cgl_set_walls")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cgl_refill(h) bind(C, name="cgl_refill")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cgl_collide_stream(h) bind(C, name="cgl_collide_stream")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    ! each field by c_loc of a double (zl, yl, xl) array, or c_null_ptr to skip it
+    integer(c_int) function cgl_particle_band(h, rho, rho_r, rho_b, ux, uy, uz) bind(C, name="cgl_particle_band")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, rho, rho_r, rho_b, ux, uy, uz
+    end function
+    ! sums(7); slice_red by c_loc of a double (zl) array, or c_null_ptr
+    integer(c_int) function cgl_diagnostics(h, sums, slice_red) bind(C, name="cgl_diagnostics")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: sums(7)
+      type(c_ptr), value :: slice_red
+    end function
+    integer(c_int) function cgl_diagnostics_noalloc(h, sums, slice_red) bind(C, name="cgl_diagnostics_noalloc")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: sums(7)
+      type(c_ptr), value :: slice_red
+    end function
+    integer(c_int) function cgl_batch_create(h, p, nbox, boxes) bind(C, name="cgl_batch_create")
+      import :: c_int, c_ptr, c_int32_t, cgl_params, cgl_box
+      type(c_ptr), intent(out) :: h
+      type(cgl_params), intent(in) :: p
+      integer(c_int32_t), value :: nbox
+      type(cgl_box), intent(in) :: boxes(*)
+    end function
+    ! box indices are 0-based, as in the C ABI
+    integer(c_int) function cgl_box_upload_state(h, box, ri, bi) bind(C, name="cgl_box_upload_state")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: box
+      real(c_double), intent(in) :: ri(*), bi(*)
+    end function
+    integer(c_int) function cgl_box_init_cylinder(h, box, xc, yc, r0, kprof, nmodes, amp) &
+        bind(C, name="cgl_box_init_cylinder")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: box
+      real(c_double), value :: xc, yc, r0, kprof
+      integer(c_int32_t), value :: nmodes
+      real(c_double), intent(in) :: amp(*)
+    end function
+    integer(c_int) function cgl_box_download_state(h, box, ri, bi) bind(C, name="cgl_box_download_state")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: box
+      real(c_double), intent(out) :: ri(*), bi(*)
+    end function
+    integer(c_int) function cgl_box_calmacros(h, box, rho, rho_r, rho_b, ux, uy, uz, phi) &
+        bind(C, name="cgl_box_calmacros")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: box
+      real(c_double), intent(out) :: rho(*), rho_r(*), rho_b(*), ux(*), uy(*), uz(*), phi(*)
+    end function
+    integer(c_int) function cgl_probe_copy_bandwidth(device, bytes, reps, gbps) bind(C, name="cgl_probe_copy_bandwidth")
+      import :: c_int, c_int32_t, c_int64_t, c_double
+      integer(c_int32_t), value :: device, reps
+      integer(c_int64_t), value :: bytes
+      real(c_double), intent(out) :: gbps
+    end function
+    integer(c_int) function cgl_device_name(h, buf, len) bind(C, name="cgl_device_name")
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_int32_t), value :: len
     end function
     type(c_ptr) function cgl_last_error_c() bind(C, name="cgl_last_error")
       import :: c_ptr

@@ -68,9 +68,9 @@ def test_fortran_driver_particle_at_interface(tmp_path):
     nml = tmp_path / "in.nml"
     nml.write_text(f"&cgl\n xl={n}, yl={n}, zl={n}, nsteps={steps}, out_every={steps}, prec=0,\n"
                    f" sigma=0.005d0, rho0=1.0d0, tau_R=0.8d0, tau_B=0.8d0, beta=0.7d0,\n"
-                   f" ic='interface', z0=16.0d0, zwalls=.true., npar=1, px(1)={c[0]}d0, py(1)={c[1]}d0,\n"
-                   f" pz(1)={c[2]}d0, rad_p={rad}d0, rho_p=1.5d0, cita_deg=60.0d0,\n"
-                   f" dump_binary=.true., write_tecplot=.true., outdir='{tmp_path}'\n/\n")
+                   f" ic='interface', z0=16.0d0, zwalls=.true., npar=1, rad_p={rad}d0, rho_p=1.5d0, cita_deg=60.0d0,\n"
+                   f" dump_binary=.true., write_tecplot=.true., outdir='{tmp_path}'\n/\n"
+                   f"&particles\n px={c[0]}d0, py={c[1]}d0, pz={c[2]}d0\n/\n")
     print("driver particle", n, steps, flush=True)
     r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
@@ -112,9 +112,10 @@ def test_fortran_driver_reference_faces_with_particles(tmp_path):
     nml.write_text(f"&cgl\n xl={n}, yl={n}, zl={n}, nsteps={steps}, out_every={steps}, prec=0,\n"
                    f" sigma=0.005d0, rho0=1.0d0, tau_R=0.8d0, tau_B=0.8d0, beta=0.7d0,\n"
                    f" ic='interface', z0=16.0d0, ref_faces=.true., npar=2,\n"
-                   f" px(1)={cs[0][0]}d0, py(1)={cs[0][1]}d0, pz(1)={cs[0][2]}d0,\n"
-                   f" px(2)={cs[1][0]}d0, py(2)={cs[1][1]}d0, pz(2)={cs[1][2]}d0, rad_p={rad}d0, rho_p=1.5d0,\n"
-                   f" cita_deg=90.0d0, dump_binary=.true., write_tecplot=.false., outdir='{tmp_path}'\n/\n")
+                   f" rad_p={rad}d0, rho_p=1.5d0,\n"
+                   f" cita_deg=90.0d0, dump_binary=.true., write_tecplot=.false., outdir='{tmp_path}'\n/\n"
+                   f"&particles\n px={cs[0][0]}d0, {cs[1][0]}d0, py={cs[0][1]}d0, {cs[1][1]}d0,\n"
+                   f" pz={cs[0][2]}d0, {cs[1][2]}d0\n/\n")
     print("driver reference faces", n, steps, flush=True)
     r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
@@ -183,14 +184,14 @@ def test_fortran_driver_checkpoint_restart(tmp_path):
     n, steps = 24, 10
     common = (f" xl={n}, yl={n}, zl={n}, nsteps={steps}, out_every=5, prec=1,\n"
               f" sigma=0.005d0, rho0=1.0d0, tau_R=0.8d0, tau_B=0.8d0, beta=0.7d0,\n"
-              f" ic='interface', z0=12.0d0, zwalls=.true., npar=1, px(1)=12.3d0, py(1)=11.8d0, pz(1)=12.4d0,\n"
+              f" ic='interface', z0=12.0d0, zwalls=.true., npar=1,\n"
               f" rad_p=4.0d0, rho_p=1.5d0, cita_deg=90.0d0, dump_binary=.true., write_tecplot=.false.,\n")
     runs = {}
     for tag, extra in (("a", " ckpt_every=4,"), ("b", f" restart='{tmp_path}/a/CKPT_4.bin', restart_step=4,")):
         d = tmp_path / tag
         d.mkdir()
         nml = d / "in.nml"
-        nml.write_text("&cgl\n" + common + extra + f" outdir='{d}'\n/\n")
+        nml.write_text("&cgl\n" + common + extra + f" outdir='{d}'\n/\n&particles\n px=12.3d0, py=11.8d0, pz=12.4d0\n/\n")
         print("driver checkpoint run", tag, flush=True)
         r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=120)
         print(r.stdout[-1500:], r.stderr[-1500:])
@@ -202,3 +203,84 @@ def test_fortran_driver_checkpoint_restart(tmp_path):
     assert np.array_equal(fa, fb)
     last = lambda d: [l for l in (d / "PARTICLES.dat").read_text().splitlines()[1:] if int(l.split()[0]) == steps]
     assert last(runs["a"]) == last(runs["b"])
+
+
+def test_fortran_driver_ensemble_batch(tmp_path):
+    """The 64-box ensemble (seed 7) through the driver's batch mode (box_file,
+    FP32 factorised): the first two boxes' dumped fields equal the binding's
+    batch bit for bit, and every box's BOXES.dat line has its largest speed
+    exactly and its masses to summation order."""
+    import symmetry_amd
+    boxes = ic.ensemble_config4()
+    steps = 4
+    bf = tmp_path / "boxes.txt"
+    bf.write_text("".join(f"{b['nx']} {b['ny']} {b['nz']} {b['tau']!r} {b['tau']!r} {b['r0']!r} {len(b['amps'])} "
+                          + " ".join(repr(a) for a in b["amps"]) + "\n" for b in boxes))
+    nml = tmp_path / "in.nml"
+    nml.write_text(f"&cgl\n nbox={len(boxes)}, box_file='{bf}', nsteps={steps}, out_every={steps}, prec=1, layout=1,\n"
+                   f" xchunk=8, sigma=0.01d0, rho0=1.0d0, beta=0.7d0, dump_binary=.true., dump_boxes=2,\n"
+                   f" outdir='{tmp_path}'\n/\n")
+    print("driver batch, 64 boxes", flush=True)
+    r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    bt = symmetry_amd.Batch([(b["nx"], b["ny"], b["nz"], b["nu_r"], b["nu_b"]) for b in boxes], precision="f32",
+                            layout="fs", sigma=0.01, rho0=1.0, beta=0.7, xchunk=8)
+    for k, b in enumerate(boxes):
+        bt.init_cylinder(k, b["nx"] / 2 + 0.5, b["ny"] / 2 + 0.5, b["r0"], 6.0 * 0.7 * 0.134, b["amps"])
+    bt.step(steps)
+    rows = [l.split() for l in (tmp_path / "BOXES.dat").read_text().splitlines()[1:]]
+    last = {int(row[1]): [float(v) for v in row[2:]] for row in rows if int(row[0]) == steps}
+    assert sorted(last) == list(range(1, len(boxes) + 1))
+    names = ("rho", "rho_r", "rho_b", "ux", "uy", "uz", "phi")
+    for k, b in enumerate(boxes):
+        m = bt.calmacros(k)
+        umax = float(np.sqrt(m["ux"] ** 2 + m["uy"] ** 2 + m["uz"] ** 2).max())
+        mr, mb, um = last[k + 1]
+        assert um == umax, (k, um, umax)
+        assert abs(mr / m["rho_r"].sum() - 1.0) < 1e-12 and abs(mb / m["rho_b"].sum() - 1.0) < 1e-12, k
+        if k < 2:
+            raw = np.fromfile(tmp_path / f"BOX_{k + 1}_FIELDS_{steps}.bin", dtype=np.float64)
+            raw = raw.reshape(7, b["nx"], b["ny"], b["nz"])
+            for j, name in enumerate(names):
+                assert np.array_equal(raw[j], m[name]), (k, name)
+
+
+def test_fortran_driver_dense_particles(tmp_path):
+    """The 200-sphere dense workload (512^3, seed 99: per-sphere densities
+    and contact angles) through the driver, spheres from particle_file and
+    the interface built on the device: its PARTICLES.dat rows equal the
+    binding's particle state bit for bit after 0 and 2 steps."""
+    import symmetry_amd
+    c5 = ic.particle_config5()
+    parts = ic.sphere_particles(c5["centers"], c5["rad"], c5["rho_p"], c5["cita_deg"], c5["sigma"])
+    n, steps, npar = c5["nx"], 2, len(c5["centers"])
+    pf = tmp_path / "spheres.txt"
+    pf.write_text("".join(" ".join(repr(float(v)) for v in (*c, c5["rad"], rp, th)) + "\n"
+                          for c, rp, th in zip(c5["centers"], c5["rho_p"], c5["cita_deg"])))
+    tau = 3.0 * c5["nu_r"] + 0.5
+    nml = tmp_path / "in.nml"
+    nml.write_text(f"&cgl\n xl={n}, yl={n}, zl={n}, nsteps={steps}, out_every={steps}, prec=1, xchunk=8,\n"
+                   f" sigma={c5['sigma']!r}d0, rho0=1.0d0, tau_R={tau!r}d0, tau_B={tau!r}d0, beta=0.7d0,\n"
+                   f" ic='interface', ic_on_device=.true., z0={c5['z0']!r}d0, zwalls=.true., npar={npar},\n"
+                   f" particle_file='{pf}', write_tecplot=.false., outdir='{tmp_path}'\n/\n")
+    print("driver 512^3, 200 spheres", flush=True)
+    r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=600)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    lat = symmetry_amd.Lattice(n, n, n, precision="f32", sigma=c5["sigma"], rho0=1.0, nu_r=c5["nu_r"],
+                               nu_b=c5["nu_b"], beta=0.7, bc=c5["bc"], npar=npar, xchunk=8)
+    lat.init_interface(c5["z0"], 6.0 * 0.7 * 0.134)
+    lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"], parts["fg"])
+    rows = [l.split() for l in (tmp_path / "PARTICLES.dat").read_text().splitlines()[1:]]
+    for t in (0, steps):
+        if t:
+            lat.step(t)
+        got = lat.get_particles()
+        mine = [row for row in rows if int(row[0]) == t]
+        assert len(mine) == npar
+        for p, row in enumerate(mine):
+            vals = [float(v) for v in row[2:]]
+            assert vals[0:3] == list(got["center"][p]) and vals[3:6] == list(got["up"][p]), (t, p)
+            assert vals[6:9] == list(got["force"][p]) and vals[9:12] == list(got["fcap"][p]), (t, p)
+    lat.close()
