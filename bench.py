@@ -51,15 +51,16 @@ DTYPE = {"f64": "f64", "f32": "f32", "f16": "f16-storage/f32-compute"}
 
 
 def finish_tuning(lat, max_calls=64):
-    """Run the x-chunk autotuner to its end outside any timed region (it
-    times each candidate on real steps), then four steady steps so that the
-    step graph is captured before the clock starts."""
+    """Run the launch-shape autotuner (x chunk, XCD tile order, tile height)
+    to its end outside any timed region (it times each candidate on real
+    steps), then four steady steps so that the step graph is captured before
+    the clock starts."""
     for _ in range(max_calls):
-        if not lat.tuned_xchunk()[1]:
+        if not lat.tuned_params()["tuning"]:
             break
         lat.step(2)
-    if lat.tuned_xchunk()[1]:
-        raise RuntimeError("x-chunk autotuner still running after the warm-up")
+    if lat.tuned_params()["tuning"]:
+        raise RuntimeError("launch-shape autotuner still running after the warm-up")
     lat.step(4)
     lat.sync()
 
@@ -309,9 +310,9 @@ def main():
           file=sys.stderr, flush=True)
     copy_gbps = symmetry_amd.copy_bandwidth(2 << 30, 10)
     lat.init_cylinder(nx / 2 + 0.5, ny / 2 + 0.5, cfg["r0"], 6.0 * cfg["beta"] * 0.134, cfg["amps"])
-    lat.step(args.warmup)  # with --xchunk 0 the first warm-up steps also autotune the x chunk
+    lat.step(args.warmup)  # with --xchunk 0 the first warm-up steps also autotune the launch shape
     finish_tuning(lat)
-    xchunk, still_tuning = lat.tuned_xchunk()
+    tuned = lat.tuned_params()
     nodes = nx * ny * nz
     # timed region: exactly `steps` steps, synchronised on both sides
     t0 = time.perf_counter()
@@ -375,7 +376,8 @@ def main():
         "config": {"workload": f"Rayleigh-Plateau {nx}x{ny}x{nz} periodic D3Q19 colour-gradient MRT, R=40, "
                                "8 modes seed 1234, tau_r=1.0 tau_b=0.7, sigma=0.01, beta=0.7",
                    "precision": args.precision, "layout": args.layout, "nodes": nodes,
-                   "xchunk": xchunk, "xchunk_autotuned": args.xchunk == 0 and not still_tuning,
+                   "xchunk": tuned["xchunk"], "xcd_order": tuned["xcd_order"], "tile_rows": tuned["tile_rows"],
+                   "launch_autotuned": args.xchunk == 0 and not tuned["tuning"],
                    "phase_field": "scatter" if scatter else "gather"},
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,

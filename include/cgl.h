@@ -57,7 +57,8 @@ typedef struct cgl_params {
     int32_t xchunk;      /* x planes per workgroup of the fused kernel; 0 = autotune on the first steps */
     int32_t device;      /* HIP device ordinal */
     int32_t layout;      /* CGL_LAYOUT_*: device storage of the populations */
-    int32_t reserved;    /* must be 0 */
+    int32_t tile_rows;   /* rows per workgroup tile of the scatter-form step: 4, 8 or 16; 0 = 8, or autotuned with
+                            xchunk = 0 (batches and the other kernels run their fixed tiles) */
     double sigma;        /* CSF surface-tension coefficient (sigma, S:66) */
     double rho0;         /* reference density (rho0, S:67) */
     double nu_r, nu_b;   /* kinematic viscosities of red and blue (S:69-70) */
@@ -113,12 +114,24 @@ int cgl_derivatives(cgl_t *h, double *gradx, double *grady, double *gradz, doubl
  * reference. */
 int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning);
 
+/* The launch shape the steps use, as the autotuner (xchunk = 0 at create)
+ * chose it on the first steps, one parameter after the other with the
+ * others held: the x chunk (x planes per workgroup, which is the number of
+ * nodes each thread steps through along x: 2 .. 64), the tile order over
+ * the XCDs (xcd_order: 0 launch order, 1 a contiguous range of tiles per
+ * XCD, 2 whole z columns of tiles per XCD; fixed when CGL_XCD_REMAP is set)
+ * and, for the scatter-form step, the tile height (tile_rows: 4, 8 or 16
+ * rows of 64 nodes; fixed when cgl_params.tile_rows is).  Every choice
+ * gives the same results bit for bit.  tuning = 1 while candidates are still
+ * being timed.  Any pointer may be NULL.  No counterpart in the reference. */
+int cgl_tuned_params(cgl_t *h, int32_t *xchunk, int32_t *xcd_order, int32_t *tile_rows, int32_t *tuning);
+
 /* 1 when the handle's steps compute the next phase field in scatter form
  * (calmacros' sums S:792-799 accumulated by the writers of the populations),
  * so that a step reads the state once: periodic FP32/FP16 factorised boxes,
- * single or batched, with ny % 8 != 1 and nz % 64 != 1 (every workgroup tile
- * has two rows and two lanes), unless the environment sets CGL_SCATTER=0 at
- * creation.  0: a moments pass per step (gather form).
+ * single or batched, with ny % rows != 1 (rows = tile_rows, 8 when 0) and
+ * nz % 64 != 1 (every workgroup tile has two rows and two lanes), unless the
+ * environment sets CGL_SCATTER=0 at creation.  0: a moments pass per step (gather form).
  * No counterpart in the reference. */
 int cgl_phase_field_form(cgl_t *h, int32_t *scatter);
 

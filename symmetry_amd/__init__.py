@@ -42,7 +42,7 @@ class Params(ctypes.Structure):
         ("bc", ctypes.c_int32 * 6),
         ("precision", ctypes.c_int32), ("npar", ctypes.c_int32),
         ("xchunk", ctypes.c_int32), ("device", ctypes.c_int32),
-        ("layout", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("layout", ctypes.c_int32), ("tile_rows", ctypes.c_int32),
         ("sigma", ctypes.c_double), ("rho0", ctypes.c_double),
         ("nu_r", ctypes.c_double), ("nu_b", ctypes.c_double), ("beta", ctypes.c_double),
     ]
@@ -66,7 +66,7 @@ EXPORTS = (
     "cgl_box_calmacros", "cgl_diagnostics", "cgl_init_interface", "cgl_phase_field_form",
     "cgl_set_particle_geometry", "cgl_checkpoint_save", "cgl_checkpoint_load", "cgl_diagnostics_noalloc",
     "cgl_capillary", "cgl_momentum", "cgl_repulsive", "cgl_move", "cgl_set_walls", "cgl_refill",
-    "cgl_collide_stream", "cgl_particle_band",
+    "cgl_collide_stream", "cgl_particle_band", "cgl_tuned_params",
 )
 
 # The stages of a staged step after cgl_calmacros, in order.
@@ -107,6 +107,7 @@ def lib():
         L.cgl_diagnostics_noalloc.argtypes = [vp, vp, vp]
         L.cgl_init_interface.argtypes = [vp, ctypes.c_double, ctypes.c_double]
         L.cgl_tuned_xchunk.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+        L.cgl_tuned_params.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int32)] * 4
         L.cgl_phase_field_form.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
         L.cgl_set_particle_geometry.argtypes = [vp] + [vp] * 5 + [ctypes.c_int32]
         for st in STAGES:
@@ -141,7 +142,7 @@ class Lattice:
     0-based indices: populations ``[x, y, z, i]``, scalars ``[x, y, z]``."""
 
     def __init__(self, nx, ny, nz, precision="f64", sigma=0.01, rho0=1.0, nu_r=1.0 / 6.0, nu_b=1.0 / 6.0,
-                 beta=0.7, bc=(BC_PERIODIC,) * 6, xchunk=0, device=0, layout="rb", npar=0):
+                 beta=0.7, bc=(BC_PERIODIC,) * 6, xchunk=0, device=0, layout="rb", npar=0, tile_rows=0):
         p = Params()
         p.nx, p.ny, p.nz = nx, ny, nz
         for k, v in enumerate(bc):
@@ -151,6 +152,7 @@ class Lattice:
         p.xchunk = xchunk
         p.device = device
         p.layout = LAYOUT[layout] if isinstance(layout, str) else int(layout)
+        p.tile_rows = tile_rows
         p.sigma, p.rho0, p.nu_r, p.nu_b, p.beta = sigma, rho0, nu_r, nu_b, beta
         self.params = p
         self.shape = (nx, ny, nz)
@@ -279,6 +281,12 @@ class Lattice:
         xc, tu = ctypes.c_int32(), ctypes.c_int32()
         _check(lib().cgl_tuned_xchunk(self.h, ctypes.byref(xc), ctypes.byref(tu)))
         return xc.value, bool(tu.value)
+
+    def tuned_params(self):
+        """{xchunk, xcd_order, tile_rows, tuning}: the launch shape in use."""
+        v = [ctypes.c_int32() for _ in range(4)]
+        _check(lib().cgl_tuned_params(self.h, *(ctypes.byref(x) for x in v)))
+        return dict(xchunk=v[0].value, xcd_order=v[1].value, tile_rows=v[2].value, tuning=bool(v[3].value))
 
     def scatter_form(self):
         """True when the steps sum the next phase field in scatter form (one

@@ -198,7 +198,7 @@ int setup_scatter(cgl_handle *h)
     if (h->nbox) {
         for (int b = 0; b < h->nbox; ++b)
             if (!cgl::sc_shape_ok(h->hboxes[b].L.ny, h->hboxes[b].L.nz)) return CGL_OK;
-    } else if (!cgl::sc_shape_ok(h->p.ny, h->p.nz)) {
+    } else if (!cgl::sc_shape_ok(h->p.ny, h->p.nz, h->ky)) {
         return CGL_OK;
     }
     int2 **bufs[cgl::SC_NBUF] = {&h->sb.own,   &h->sb.fxlo,  &h->sb.fxhi,  &h->sb.fy,    &h->sb.fz,
@@ -292,7 +292,8 @@ int cgl_create(cgl_t **out, const cgl_params *p)
         return fail(CGL_ERR_UNSUPPORTED, "walls, mirrors and particles are implemented for the red/blue layout");
     if (!(p->rho0 > 0) || !(p->nu_r > 0) || !(p->nu_b > 0)) return fail(CGL_ERR_ARG, "rho0, nu_r, nu_b must be > 0");
     if (p->layout != CGL_LAYOUT_RB && p->layout != CGL_LAYOUT_FS) return fail(CGL_ERR_ARG, "unknown layout %d", p->layout);
-    if (p->reserved != 0) return fail(CGL_ERR_ARG, "reserved field must be 0");
+    if (p->tile_rows != 0 && p->tile_rows != 4 && p->tile_rows != 8 && p->tile_rows != 16)
+        return fail(CGL_ERR_ARG, "tile_rows must be 0, 4, 8 or 16");
     int rc = alloc_handle(out, p, (size_t)p->nx * p->ny * p->nz, walls | mirrors);
     if (rc) return rc;
     if ((rc = setup_scatter(*out))) {
@@ -330,7 +331,11 @@ static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls)
         const char *env = getenv("CGL_XCD_REMAP");
         h->L.xcd_remap = env ? atoi(env) : 1;
         h->remap_env = env != nullptr;
+        h->sc_remap = env ? h->L.xcd_remap : 2;
     }
+    h->tile_rows = p->tile_rows;
+    h->ky = p->tile_rows ? p->tile_rows : 8;
+    h->tune_best_ms = -1.f;
     h->esize = p->precision == CGL_PREC_F64 ? 8 : (p->precision == CGL_PREC_F32 ? 4 : 2);
     h->phisize = p->precision == CGL_PREC_F64 ? 8 : 4;
     h->xc = p->xchunk > 0 ? p->xchunk : 8;
@@ -581,12 +586,56 @@ static int capture_two_steps(cgl_handle *h)
     return CGL_OK;
 }
 
-// Autotuner of the fused kernel's x chunk (planes per workgroup).  Results do
-// not depend on the chunk, so the first steps of a handle serve as the timing
-// runs: each candidate advances TUNE_STEPS real steps between two events.
+// Autotuner of the steps' launch shape: the x chunk (planes per workgroup),
+// then the tile order over the XCDs, then the scatter-form step's tile
+// height, each dimension with the others held at their current values.
+// Results do not depend on any of them, so the first steps of a handle serve
+// as the timing runs: each candidate advances TUNE_STEPS real steps between
+// two events.
 static const int TUNE_XC[] = {2, 4, 8, 16, 32, 64};
 static const int TUNE_N = sizeof(TUNE_XC) / sizeof(TUNE_XC[0]);
+static const int TUNE_REMAP[] = {0, 1, 2};
+static const int TUNE_KY[] = {4, 8, 16};
 static const int TUNE_STEPS = 2;
+static const int TUNE_DIMS = 3;
+
+static int tune_count(int dim) { return dim == 0 ? TUNE_N : 3; }
+static int tune_value(int dim, int k) { return dim == 0 ? TUNE_XC[k] : (dim == 1 ? TUNE_REMAP[k] : TUNE_KY[k]); }
+
+// whether candidate k of dimension dim applies to the handle
+static bool tune_valid(const cgl_handle *h, int dim, int k)
+{
+    if (dim == 0) return k == 0 || TUNE_XC[k] < 2 * h->p.nx;
+    if (dim == 1) return !h->remap_env && !h->nbox;
+    return h->sc && !h->nbox && !h->tile_rows && cgl::sc_shape_ok(h->p.ny, h->p.nz, TUNE_KY[k]);
+}
+
+static void tune_set(cgl_handle *h, int dim, int v)
+{
+    if (dim == 0)
+        h->xc = v;
+    else if (dim == 1)
+        (h->sc ? h->sc_remap : h->L.xcd_remap) = v;
+    else
+        h->ky = v;
+}
+
+// Move to the next candidate that applies; at the end of a dimension keep
+// its best value and go on to the next; after the last one stop tuning.
+static void tune_advance(cgl_handle *h)
+{
+    while (h->tuning) {
+        if (h->tune_idx < tune_count(h->tune_dim)) {
+            if (tune_valid(h, h->tune_dim, h->tune_idx)) return;
+            ++h->tune_idx;
+            continue;
+        }
+        if (h->tune_best_ms >= 0.f) tune_set(h, h->tune_dim, h->tune_best_val);
+        h->tune_best_ms = -1.f;
+        h->tune_idx = 0;
+        if (++h->tune_dim == TUNE_DIMS) h->tuning = 0;
+    }
+}
 
 static int tune_some(cgl_handle *h, int32_t *left)
 {
@@ -601,7 +650,10 @@ static int tune_some(cgl_handle *h, int32_t *left)
             --*left;
         }
         if (*left < TUNE_STEPS) break;
-        h->xc = TUNE_XC[h->tune_idx];
+        tune_advance(h);
+        if (!h->tuning) break;
+        const int v = tune_value(h->tune_dim, h->tune_idx);
+        tune_set(h, h->tune_dim, v);
         HIPCHK(hipEventRecord(h->ev[0], h->stream));
         for (int t = 0; t < TUNE_STEPS; ++t) enqueue_step(h);
         HIPCHK(hipEventRecord(h->ev[1], h->stream));
@@ -610,15 +662,13 @@ static int tune_some(cgl_handle *h, int32_t *left)
         HIPCHK(hipEventSynchronize(h->ev[1]));
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
-        if (h->tune_idx == 0 || ms < h->tune_best_ms) {
+        if (h->tune_best_ms < 0.f || ms < h->tune_best_ms) {
             h->tune_best_ms = ms;
-            h->tune_best_xc = h->xc;
+            h->tune_best_val = v;
         }
         *left -= TUNE_STEPS;
-        if (++h->tune_idx == TUNE_N || TUNE_XC[h->tune_idx] >= 2 * h->p.nx) {
-            h->tuning = 0;
-            h->xc = h->tune_best_xc;
-        }
+        ++h->tune_idx;
+        tune_advance(h);
     }
     return CGL_OK;
 }
@@ -664,6 +714,16 @@ int cgl_tuned_xchunk(cgl_t *h, int32_t *xchunk, int32_t *tuning)
 {
     if (!h) return fail(CGL_ERR_ARG, "null handle");
     if (xchunk) *xchunk = h->xc;
+    if (tuning) *tuning = h->tuning;
+    return CGL_OK;
+}
+
+int cgl_tuned_params(cgl_t *h, int32_t *xchunk, int32_t *xcd_order, int32_t *tile_rows, int32_t *tuning)
+{
+    if (!h) return fail(CGL_ERR_ARG, "null handle");
+    if (xchunk) *xchunk = h->xc;
+    if (xcd_order) *xcd_order = h->sc ? h->sc_remap : h->L.xcd_remap;
+    if (tile_rows) *tile_rows = h->sc ? h->ky : (h->p.precision == CGL_PREC_F64 ? cgl::KTY<double> : cgl::KTY<float>);
     if (tuning) *tuning = h->tuning;
     return CGL_OK;
 }
@@ -1158,7 +1218,7 @@ int cgl_batch_create(cgl_t **out, const cgl_params *p, int32_t nbox, const cgl_b
         if (p->bc[f] != CGL_BC_PERIODIC) return fail(CGL_ERR_UNSUPPORTED, "batched boxes are periodic");
     if (p->npar != 0) return fail(CGL_ERR_UNSUPPORTED, "batched boxes carry no particles");
     if (!(p->rho0 > 0)) return fail(CGL_ERR_ARG, "rho0 must be > 0");
-    if (p->reserved != 0) return fail(CGL_ERR_ARG, "reserved field must be 0");
+    if (p->tile_rows != 0 && p->tile_rows != 8) return fail(CGL_ERR_ARG, "batches run tiles of 8 rows (tile_rows 0 or 8)");
     const double eb = addressed_bytes(p);
     size_t total = 0, nmax = 0;
     for (int b = 0; b < nbox; ++b) {
@@ -1333,14 +1393,15 @@ struct CkptHeader {
     int32_t version;
     int32_t nx, ny, nz, precision, layout, npar, nbox;
     int32_t bc[6];
-    int32_t cur, phicur, state_layout, ntcur, phi_ok, primed, cold, xc, tuning, tune_idx, tune_best_xc;
+    int32_t cur, phicur, state_layout, ntcur, phi_ok, primed, cold, xc, tuning, tune_idx, tune_best_val;
+    int32_t tune_dim, ky, sc_remap, xcd_remap;
     float tune_best_ms;
     int64_t N;
     int64_t steps_done;
     double sigma, rho0, nu_r, nu_b, beta;
 };
 static const char CKPT_MAGIC[8] = {'C', 'G', 'L', 'C', 'K', 'P', 'T', '1'};
-static const int CKPT_VERSION = 2;
+static const int CKPT_VERSION = 3;
 
 // a batch's boxes, written after the header: a load must find the same
 // boxes (sizes and viscosities) in the same order
@@ -1406,7 +1467,8 @@ int cgl_checkpoint_save(cgl_t *h, const char *path)
     for (int f = 0; f < 6; ++f) H.bc[f] = h->p.bc[f];
     H.cur = h->cur; H.phicur = h->phicur; H.state_layout = h->state_layout; H.ntcur = h->ntcur;
     H.phi_ok = h->phi_ok; H.primed = h->primed; H.cold = h->cold; H.xc = h->xc; H.tuning = h->tuning;
-    H.tune_idx = h->tune_idx; H.tune_best_xc = h->tune_best_xc; H.tune_best_ms = h->tune_best_ms;
+    H.tune_idx = h->tune_idx; H.tune_best_val = h->tune_best_val; H.tune_best_ms = h->tune_best_ms;
+    H.tune_dim = h->tune_dim; H.ky = h->ky; H.sc_remap = h->sc_remap; H.xcd_remap = h->L.xcd_remap;
     H.N = (int64_t)h->N; H.steps_done = h->steps_done;
     H.sigma = h->p.sigma; H.rho0 = h->p.rho0; H.nu_r = h->p.nu_r; H.nu_b = h->p.nu_b; H.beta = h->p.beta;
     FILE *f = fopen(path, "wb");
@@ -1467,8 +1529,13 @@ int cgl_checkpoint_load(cgl_t *h, const char *path)
     const bool layout_ok = H.state_layout == cgl::RB ? (h->p.layout == CGL_LAYOUT_RB || H.cur == 0)
                                                      : h->p.layout == CGL_LAYOUT_FS;
     // a batch's tile tables were built for its x chunk at creation; a single
-    // box takes any chunk, and the autotuner's state must index its table
-    const bool tune_ok = H.tuning == 0 || (H.tuning == 1 && H.tune_idx >= 0 && H.tune_idx < TUNE_N);
+    // box takes any chunk, and the autotuner's state must index its tables;
+    // the tile height must suit the box
+    const bool tune_ok =
+        (H.tuning == 0 || (H.tuning == 1 && H.tune_dim >= 0 && H.tune_dim < TUNE_DIMS && H.tune_idx >= 0 &&
+                           H.tune_idx <= tune_count(H.tune_dim))) &&
+        (H.ky == 4 || H.ky == 8 || H.ky == 16) && (!h->sc || cgl::sc_shape_ok(h->p.ny, h->p.nz, H.ky)) &&
+        H.sc_remap >= 0 && H.sc_remap <= 2 && H.xcd_remap >= 0 && H.xcd_remap <= 2;
     if (!same || !layout_ok || H.cur < 0 || H.cur > 1 || H.phicur < 0 || H.phicur > 1 || H.ntcur < 0 ||
         H.ntcur > 1 || H.xc < 1 || H.xc > h->p.nx + 64 || (h->nbox && (H.xc != h->xc || H.tuning)) || !tune_ok) {
         fclose(f);
@@ -1510,8 +1577,13 @@ int cgl_checkpoint_load(cgl_t *h, const char *path)
         return fail(CGL_ERR_ARG, "%s is truncated", path);
     }
     h->phicur = H.phicur; h->ntcur = H.ntcur; h->phi_ok = H.phi_ok; h->primed = H.primed; h->cold = H.cold;
-    h->xc = H.xc; h->tuning = H.tuning; h->tune_idx = H.tune_idx; h->tune_best_xc = H.tune_best_xc;
-    h->tune_best_ms = H.tune_best_ms; h->steps_done = (int)H.steps_done; h->scal_stale = 0;
+    h->xc = H.xc; h->tuning = H.tuning; h->tune_idx = H.tune_idx; h->tune_best_val = H.tune_best_val;
+    h->tune_best_ms = H.tune_best_ms; h->tune_dim = H.tune_dim; h->steps_done = (int)H.steps_done; h->scal_stale = 0;
+    if (!h->tile_rows) h->ky = H.ky;
+    if (!h->remap_env) {
+        h->sc_remap = H.sc_remap;
+        h->L.xcd_remap = H.xcd_remap;
+    }
     (void)clear_range(h);
     h->stage = STAGE_NONE;
     if (h->npar) {

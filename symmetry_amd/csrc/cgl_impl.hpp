@@ -28,8 +28,8 @@ struct cgl_handle {
     int xc;           // x planes per workgroup of the fused kernel
     int tuning;       // 1 while the first steps still time candidate x chunks
     int tune_idx;     // next candidate
-    int tune_best_xc;
-    float tune_best_ms;
+    int tune_best_val;   // best value of the dimension being tuned so far
+    float tune_best_ms;  // its time, < 0 before the dimension's first candidate
     double *mac;      // scratch: 7*N doubles for macroscopic downloads (lazy)
     double *aos;      // scratch: 19*N doubles for host-layout conversion (lazy)
     int *iflag;       // scratch: N ints (lazy)
@@ -59,6 +59,10 @@ struct cgl_handle {
     // periodic FP32/FP16 factorised boxes: next phase field in scatter form
     int sc;              // 1: steady steps use k_collide_sc + k_phi_sc
     int remap_env;       // 1: CGL_XCD_REMAP was set, and its value holds for every kernel
+    int sc_remap;        // tile order of the scatter step (tuned; CGL_XCD_REMAP when set)
+    int ky;              // rows per workgroup tile of the scatter step: 4, 8 or 16 (tuned)
+    int tile_rows;       // cgl_params.tile_rows: 0 = tuned, else the fixed ky
+    int tune_dim;        // dimension being tuned: 0 x chunk, 1 tile order, 2 tile rows
     int phi_ok;          // phi[phicur ^ 1] holds the current state's phase field (from the last step)
     int scal_stale;      // the current state's recolour scalars were not written (mid-run only)
     cgl::ScBufs sb;      // fixed-point partial sums
@@ -231,22 +235,36 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             S *fout = (S *)h->fsf[h->cur ^ 1];
             float4 *sout = last ? (float4 *)h->fss[h->cur ^ 1] : nullptr;
             const float *phi = (const float *)h->phi[h->phicur ^ 1];
-            // tile order: whole z columns of tiles on one XCD, consecutive
-            // columns over the XCDs (at 256x256x512 +2 % for FP32 and +1.4 %
-            // for FP16 storage over the XCD-contiguous order), unless
-            // CGL_XCD_REMAP says otherwise; the FP32 batch takes launch order
+            // tile order h->sc_remap (tuned; by default whole z columns of
+            // tiles on one XCD, consecutive columns over the XCDs: at
+            // 256x256x512 +2 % for FP32 and +1.4 % for FP16 storage over the
+            // XCD-contiguous order) and tile height h->ky (tuned, 8 rows by
+            // default); the FP32 batch takes launch order and 8 rows
             Lat Ls = h->L;
-            if (!h->remap_env) Ls.xcd_remap = 2;
-            if (h->nbox)
+            Ls.xcd_remap = h->sc_remap;
+            if (h->nbox) {
                 k_collide_sc_batch<S><<<h->tiles2, k2_block<R>(), 0, h->stream>>>(
                     h->dboxes, h->nbox, sizeof(S) == 4 && !h->remap_env ? 0 : h->L.xcd_remap, h->xc, fin, fout,
                     sout, phi, h->sb);
-            else if ((double)Q * (double)h->N * sizeof(S) < 4294967296.0) // one base for all directions
-                k_collide_sc<S, true><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(
-                    Ls, fin, fout, sout, phi, h->sb, h->xc);
-            else
-                k_collide_sc<S><<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(Ls, fin, fout, sout, phi,
-                                                                                         h->sb, h->xc);
+            } else {
+                const bool one = (double)Q * (double)h->N * sizeof(S) < 4294967296.0; // one base for all directions
+                auto launch = [&](auto rows) {
+                    constexpr int KY = decltype(rows)::value;
+                    const dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + KY - 1) / KY, (h->L.nx + h->xc - 1) / h->xc);
+                    if (one)
+                        k_collide_sc<S, true, KY>
+                            <<<grid, dim3(TZ, KY, 1), 0, h->stream>>>(Ls, fin, fout, sout, phi, h->sb, h->xc);
+                    else
+                        k_collide_sc<S, false, KY>
+                            <<<grid, dim3(TZ, KY, 1), 0, h->stream>>>(Ls, fin, fout, sout, phi, h->sb, h->xc);
+                };
+                if (h->ky == 4)
+                    launch(std::integral_constant<int, 4>{});
+                else if (h->ky == 16)
+                    launch(std::integral_constant<int, 16>{});
+                else
+                    launch(std::integral_constant<int, 8>{});
+            }
             h->scal_stale = !last;
         }
     }
@@ -259,9 +277,10 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                                                                        (float *)h->phi[h->phicur]);
         else if (h->L.nz % 2 == 0) // two nodes per thread
             k_phi_sc2<<<dim3((h->L.nz / 2 + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, h->L.nx), node_block(), 0,
-                        h->stream>>>(h->L, h->sb, h->xc, (float *)h->phi[h->phicur]);
+                        h->stream>>>(h->L, h->sb, h->xc, h->ky, (float *)h->phi[h->phicur]);
         else
-            k_phi_sc<<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, h->sb, h->xc, (float *)h->phi[h->phicur]);
+            k_phi_sc<<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, h->sb, h->xc, h->ky,
+                                                                      (float *)h->phi[h->phicur]);
         h->phi_ok = 1;
     }
     static void step_sc(cgl_handle *h, bool last)

@@ -1169,9 +1169,9 @@ __device__ __forceinline__ void write_node(const Lat &L, void *o0, void *o1, uns
 //   sn[3]    unit normals of planes x-1 .. x+1 with a 1-node halo (the
 //            curvature stencil needs n at the 18 neighbours)
 // Streaming is the pull gather from x - c_i (periodic wrap).
-template <class R, class P> struct K2Smem {
-    P phi[5][KTY<R> + 4][TZ + 4];
-    R n[3][3][KTY<R> + 2][TZ + 2];
+template <class R, class P, int KY = KTY<R>> struct K2Smem {
+    P phi[5][KY + 4][TZ + 4];
+    R n[3][3][KY + 2][TZ + 2];
 };
 
 __device__ __forceinline__ int slot5(int p) { return (p + 10) % 5; }

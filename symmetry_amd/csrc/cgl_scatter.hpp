@@ -100,11 +100,12 @@ __device__ __forceinline__ float q_of_sums(int2 s)
     return b <= r ? -(float(b) / tot) : float(r) / tot;
 }
 
-// Shapes the scatter step takes: every workgroup box has at least two rows
-// and two lanes, so that its low and high faces are different threads.
-__host__ __device__ inline bool sc_shape_ok(int ny, int nz)
+// Shapes the scatter step takes with tiles of ky rows: every workgroup box
+// has at least two rows and two lanes, so that its low and high faces are
+// different threads.
+__host__ __device__ inline bool sc_shape_ok(int ny, int nz, int ky = KTY<float>)
 {
-    return ny >= 2 && nz >= 2 && ny % KTY<float> != 1 && nz % TZ != 1;
+    return ny >= 2 && nz >= 2 && ny % ky != 1 && nz % TZ != 1;
 }
 
 // The box (z-tile bx, y-tile by, x-chunk bz) of one workgroup.  A tile at
@@ -113,14 +114,14 @@ __host__ __device__ inline bool sc_shape_ok(int ny, int nz)
 // ONE: all 19 direction arrays of a state lie within 4 GiB of its first, so
 // every load and store takes the one base and a 32-bit offset (no per-direction
 // 64-bit bases held in scalar registers).
-template <class S, bool ONE = false>
+// KY: rows of the tile (64 x KY threads).
+template <class S, bool ONE = false, int KY = KTY<float>>
 __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restrict__ fin, S *__restrict__ fout,
                                                 float4 *__restrict__ sout, const float *__restrict__ phi,
                                                 const ScBufs &B, int xc, unsigned bx, unsigned by, unsigned bz)
 {
     using R = float;
-    constexpr int KY = KTY<float>;
-    __shared__ K2Smem<R, float> sm;
+    __shared__ K2Smem<R, float, KY> sm;
     __shared__ int2 yx[2][3][KY][TZ]; // terms for the row above [0] / below [1], per x offset
     __shared__ int2 zdx[2][2][KY];    // edge-lane terms for the next row: [above/below][lane 0 / last][row]
     const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
@@ -133,41 +134,44 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     const size_t N = (size_t)L.N;
     const R rho0 = R(L.rho0), qs = SC_ONE / R(L.rho0);
 
-    // phi ring: each thread loads at most two elements of the (KY+4) x (TZ+4)
-    // plane region; their in-plane offsets are the same for every plane
+    // phi ring: each thread loads at most NE elements of the (KY+4) x (TZ+4)
+    // plane region (the first always exists); their in-plane offsets are the
+    // same for every plane
     constexpr int NPHI = (KY + 4) * (TZ + 4);
-    static_assert(NPHI <= 2 * TZ * KY, "two phi elements per thread");
-    int pj0, pj1 = -1;
-    unsigned po0, po1 = 0;
-    {
-        const int e0 = tid, e1 = tid + TZ * KY;
-        const int jy0 = e0 / (TZ + 4), jz0 = e0 - jy0 * (TZ + 4);
-        pj0 = e0;
-        po0 = (unsigned)wrapm(y0 - 2 + jy0, L.ny) * (unsigned)L.nz + (unsigned)wrapm(z0 - 2 + jz0, L.nz);
-        if (e1 < NPHI) {
-            const int jy1 = e1 / (TZ + 4), jz1 = e1 - jy1 * (TZ + 4);
-            pj1 = e1;
-            po1 = (unsigned)wrapm(y0 - 2 + jy1, L.ny) * (unsigned)L.nz + (unsigned)wrapm(z0 - 2 + jz1, L.nz);
-        }
+    constexpr int NE = (NPHI + TZ * KY - 1) / (TZ * KY);
+    int pj[NE];
+    unsigned po[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        const int e = tid + k * TZ * KY;
+        const int jy = e / (TZ + 4), jz = e - jy * (TZ + 4);
+        pj[k] = e < NPHI ? e : -1;
+        po[k] = e < NPHI ? (unsigned)wrapm(y0 - 2 + jy, L.ny) * (unsigned)L.nz + (unsigned)wrapm(z0 - 2 + jz, L.nz)
+                         : 0u;
     }
     auto load_phi = [&](int p) {
         const unsigned gx = (unsigned)wrapm(p, L.nx) * sx;
         float *dst = &sm.phi[slot5(p)][0][0];
-        dst[pj0] = phi_of_q(at(phi, gx + po0));
-        if (pj1 >= 0) dst[pj1] = phi_of_q(at(phi, gx + po1));
+#pragma unroll
+        for (int k = 0; k < NE; ++k)
+            if (k == 0 || pj[k] >= 0) dst[pj[k]] = phi_of_q(at(phi, gx + po[k]));
     };
     // the ring's next plane is fetched one plane ahead into registers, so its
     // load latency overlaps a plane of work
-    float pf0 = 0.f, pf1 = 0.f;
+    float pf[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) pf[k] = 0.f;
     auto prefetch_phi = [&](int p) {
         const unsigned gx = (unsigned)wrapm(p, L.nx) * sx;
-        pf0 = at(phi, gx + po0);
-        if (pj1 >= 0) pf1 = at(phi, gx + po1);
+#pragma unroll
+        for (int k = 0; k < NE; ++k)
+            if (k == 0 || pj[k] >= 0) pf[k] = at(phi, gx + po[k]);
     };
     auto store_phi = [&](int p) {
         float *dst = &sm.phi[slot5(p)][0][0];
-        dst[pj0] = phi_of_q(pf0);
-        if (pj1 >= 0) dst[pj1] = phi_of_q(pf1);
+#pragma unroll
+        for (int k = 0; k < NE; ++k)
+            if (k == 0 || pj[k] >= 0) dst[pj[k]] = phi_of_q(pf[k]);
     };
     // unit normals of plane p on the (KY+2) x (TZ+2) halo region (S:498-532).
     // Each thread computes its own node first (keeping that node's gradient)
@@ -527,14 +531,14 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     }
 }
 
-template <class S, bool ONE = false>
-__global__ __launch_bounds__(TZ *KTY<float>, 4) void k_collide_sc(Lat L, const S *__restrict__ fin, S *__restrict__ fout,
-                                                                float4 *__restrict__ sout,
-                                                                const float *__restrict__ phi, ScBufs B, int xc)
+template <class S, bool ONE = false, int KY = KTY<float>>
+__global__ __launch_bounds__(TZ *KY, 4) void k_collide_sc(Lat L, const S *__restrict__ fin, S *__restrict__ fout,
+                                                          float4 *__restrict__ sout, const float *__restrict__ phi,
+                                                          ScBufs B, int xc)
 {
     unsigned bx, by, bz;
     tile_of_block(L, bx, by, bz);
-    collide_sc_tile<S, ONE>(L, fin, fout, sout, phi, B, xc, bx, by, bz);
+    collide_sc_tile<S, ONE, KY>(L, fin, fout, sout, phi, B, xc, bx, by, bz);
 }
 
 // Partial-sum arrays of box `base` in a packed batch.
@@ -565,16 +569,16 @@ __global__ __launch_bounds__(TZ *KTY<float>, 4) void k_collide_sc_batch(const Bo
 
 // Finishing pass at node (x, y, z): phi(y) = (sR - sB) / (sR + sB) from the
 // node's own-box sums and the partial sums its position in its workgroup box
-// calls for.
-__device__ __forceinline__ void phi_sc_node(const Lat &L, const ScBufs &B, int xc, float *__restrict__ phi, int x,
-                                            int y, int z)
+// (x chunk xc, ky rows) calls for.
+__device__ __forceinline__ void phi_sc_node(const Lat &L, const ScBufs &B, int xc, int ky, float *__restrict__ phi,
+                                            int x, int y, int z)
 {
     const unsigned n = ((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z;
     int2 s = at(B.own, n);
     const int cx0 = (x / xc) * xc;
     const bool xlo = x == cx0, xhi = x == min(cx0 + xc, L.nx) - 1;
-    const int yr = y % KTY<float>, zr = z % TZ;
-    const bool fy = yr == 0 || y == min(y - yr + KTY<float>, L.ny) - 1;
+    const int yr = y % ky, zr = z % TZ;
+    const bool fy = yr == 0 || y == min(y - yr + ky, L.ny) - 1;
     const bool fz = zr == 0 || z == min(z - zr + TZ, L.nz) - 1;
     if (xlo) add2(s, at(B.fxlo, n));
     if (xhi) add2(s, at(B.fxhi, n));
@@ -592,11 +596,11 @@ __device__ __forceinline__ void phi_sc_node(const Lat &L, const ScBufs &B, int x
     at(phi, n) = q_of_sums(s);
 }
 
-static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, float *__restrict__ phi)
+static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, int ky, float *__restrict__ phi)
 {
     const int z = blockIdx.x * TZ + threadIdx.x, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z;
     if (z >= L.nz || y >= L.ny) return;
-    phi_sc_node(L, B, xc, phi, x, y, z);
+    phi_sc_node(L, B, xc, ky, phi, x, y, z);
 }
 
 // The finishing pass two nodes at a time (z and z + 1, z even, nz even): the
@@ -604,7 +608,7 @@ static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, 
 // call for both nodes of the pair or for neither) likewise, the z-face sums
 // per node, phi in one 8-byte store.  Integer sums are exact, so the order of
 // the additions does not change phi.
-static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc, float *__restrict__ phi)
+static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc, int ky, float *__restrict__ phi)
 {
     const int z = (blockIdx.x * TZ + threadIdx.x) * 2, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z;
     if (z >= L.nz || y >= L.ny) return;
@@ -619,8 +623,8 @@ static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc,
     };
     const int cx0 = (x / xc) * xc;
     const bool xlo = x == cx0, xhi = x == min(cx0 + xc, L.nx) - 1;
-    const int yr = y % KTY<float>;
-    const bool fy = yr == 0 || y == min(y - yr + KTY<float>, L.ny) - 1;
+    const int yr = y % ky;
+    const bool fy = yr == 0 || y == min(y - yr + ky, L.ny) - 1;
     if (xlo) add_pair(B.fxlo);
     if (xhi) add_pair(B.fxhi);
     if (fy) add_pair(B.fy);
@@ -655,7 +659,7 @@ static __global__ __launch_bounds__(256) void k_phi_sc_batch(const BoxDev *__res
     const int xb = (int)(lt / (gz * gy)) * xs;
     if (z >= L.nz || y >= L.ny) return;
     const ScBufs Bb = sc_offset(B, base);
-    for (int x = xb; x < min(xb + xs, L.nx); ++x) phi_sc_node(L, Bb, xc, phi + base, x, y, z);
+    for (int x = xb; x < min(xb + xs, L.nx); ++x) phi_sc_node(L, Bb, xc, KTY<float>, phi + base, x, y, z);
 }
 
 } // namespace cgl

@@ -39,6 +39,7 @@ program cgl_driver
   use cgl
   implicit none
   integer :: xl = 32, yl = 32, zl = 128, nsteps = 2000, out_every = 100, prec = 0, xchunk = 0, nmodes = 1, layout = 0
+  integer :: tile_rows = 0
   double precision :: sigma = 0.01d0, rho0 = 1.0d0, tau_R = 1.0d0, tau_B = 1.0d0, beta = 0.7d0, Rad_c = 8.0d0
   double precision :: amp(16) = 0.0d0
   logical :: ic_on_device = .false., write_tecplot = .true., dump_binary = .false., zwalls = .false.
@@ -56,7 +57,7 @@ program cgl_driver
   namelist /cgl/ xl, yl, zl, nsteps, out_every, prec, xchunk, sigma, rho0, tau_R, tau_B, beta, Rad_c, &
                  nmodes, amp, ic_on_device, write_tecplot, dump_binary, outdir, layout, ic, z0, zwalls, &
                  npar, rad_p, rho_p, cita_deg, particle_file, ref_faces, ckpt_every, restart, restart_step, &
-                 nbox, box_file, dump_boxes
+                 nbox, box_file, dump_boxes, tile_rows
   namelist /particles/ px, py, pz, p_rad, p_rho, p_cita
 
   double precision, allocatable :: ri(:,:,:,:), bi(:,:,:,:)
@@ -108,7 +109,7 @@ program cgl_driver
   p%xchunk = xchunk
   p%device = 0
   p%layout = layout
-  p%reserved = 0
+  p%tile_rows = tile_rows
   p%sigma = sigma; p%rho0 = rho0; p%nu_r = nu_R; p%nu_b = nu_B; p%beta = beta
   if (nbox > 0) then
     call run_batch()

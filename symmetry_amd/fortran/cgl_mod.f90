@@ -18,7 +18,7 @@ module cgl
   type, bind(C) :: cgl_params
     integer(c_int32_t) :: nx, ny, nz
     integer(c_int32_t) :: bc(6)
-    integer(c_int32_t) :: precision, npar, xchunk, device, layout, reserved
+    integer(c_int32_t) :: precision, npar, xchunk, device, layout, tile_rows
     real(c_double) :: sigma, rho0, nu_r, nu_b, beta
   end type cgl_params
 
@@ -139,6 +139,11 @@ module cgl
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: h
       integer(c_int32_t), intent(out) :: xchunk, tuning
+    end function
+    integer(c_int) function cgl_tuned_params(h, xchunk, xcd_order, tile_rows, tuning) bind(C, name="cgl_tuned_params")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), intent(out) :: xchunk, xcd_order, tile_rows, tuning
     end function
     integer(c_int) function cgl_phase_field_form(h, scatter) bind(C, name="cgl_phase_field_form")
       import :: c_int, c_ptr, c_int32_t

@@ -184,17 +184,22 @@ def test_fs_device_ic_steps_equal_rb():
 
 @pytest.mark.parametrize("layout", ["rb", "fs"])
 def test_autotuner_picks_a_candidate_and_keeps_results(rp1, layout):
-    """xchunk=0 times candidate x chunks on the first real steps; the state it
-    produces equals a run with a fixed chunk."""
+    """xchunk=0 times candidate launch shapes on the first real steps (x
+    chunks, then XCD tile orders, then for the scatter form tile heights);
+    the state it produces equals a run with a fixed shape."""
     cfg, ri, bi = rp1
     auto = _lattice(cfg, "f32", layout=layout, xchunk=0)
     auto.upload(ri, bi)
-    auto.step(15)
+    steps = 15
+    auto.step(steps)
+    while auto.tuned_xchunk()[1] and steps < 60:  # a candidate is timed over two steps of one call
+        auto.step(2)
+        steps += 2
     xc, tuning = auto.tuned_xchunk()
-    _print(layout, "autotuned xchunk", xc, tuning)
+    _print(layout, "autotuned after", steps, "steps:", auto.tuned_params())
     assert not tuning and xc in (2, 4, 8, 16, 32, 64)
     fixed = _lattice(cfg, "f32", layout=layout, xchunk=8)
     fixed.upload(ri, bi)
-    fixed.step(15)
+    fixed.step(steps)
     a, b = auto.download(), fixed.download()
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

@@ -223,3 +223,64 @@ def test_density_beyond_the_fixed_point_range_is_reported(rp, rho_in):
     lat.sync()
     phi, u = _fields(lat)
     assert np.isfinite(phi).all() and np.isfinite(u).all()
+
+
+def _shaped(cfg, tile_rows=0, remap=None, xchunk=8):
+    saved = os.environ.get("CGL_XCD_REMAP")
+    if remap is not None:
+        os.environ["CGL_XCD_REMAP"] = str(remap)
+    try:
+        return _lattice(cfg, xchunk=xchunk, tile_rows=tile_rows)
+    finally:
+        if saved is None:
+            os.environ.pop("CGL_XCD_REMAP", None)
+        else:
+            os.environ["CGL_XCD_REMAP"] = saved
+
+
+@pytest.mark.parametrize("shape", [(32, 24, 128), (20, 12, 96)])
+def test_launch_shape_does_not_change_the_run(shape):
+    """Tile heights 4, 8 and 16 rows and the three XCD tile orders give the
+    same state bit for bit (the second shape has partial y and z tiles)."""
+    nx, ny, nz = shape
+    cfg, ri, bi = small_rp(nx=nx, ny=ny, nz=nz, r0=5.0, amps=(0.5,), nu_b=0.1)
+    ref = _shaped(cfg)
+    ref.upload(ri, bi)
+    ref.step(9)
+    ra, ba = ref.download()
+    for rows in (4, 8, 16):
+        for remap in (0, 1, 2):
+            lat = _shaped(cfg, tile_rows=rows, remap=remap)
+            assert lat.scatter_form()
+            tp = lat.tuned_params()
+            assert tp["tile_rows"] == rows and tp["xcd_order"] == remap and not tp["tuning"]
+            lat.upload(ri, bi)
+            lat.step(9)
+            r, b = lat.download()
+            _print(shape, "rows", rows, "order", remap, "max |d|", float(np.max(np.abs(r - ra))))
+            assert np.array_equal(r, ra) and np.array_equal(b, ba), (rows, remap)
+            lat.close()
+
+
+def test_autotuner_reports_its_choice_and_keeps_the_run(rp):
+    """xchunk = 0: the first steps time the x chunks, then the tile orders,
+    then the tile heights; afterwards the handle reports its choice, and the
+    run equals one with that shape fixed from the start bit for bit."""
+    cfg, ri, bi = rp
+    lat = _lattice(cfg, xchunk=0)
+    lat.upload(ri, bi)
+    steps = 0
+    while lat.tuned_params()["tuning"]:
+        lat.step(2)
+        steps += 2
+        assert steps <= 60
+    tp = lat.tuned_params()
+    _print("tuned after", steps, "steps:", tp)
+    assert tp["xchunk"] in (2, 4, 8, 16, 32, 64) and tp["xcd_order"] in (0, 1, 2) and tp["tile_rows"] in (4, 8, 16)
+    lat.step(4)
+    ra, ba = lat.download()
+    fixed = _shaped(cfg, tile_rows=tp["tile_rows"], remap=tp["xcd_order"], xchunk=tp["xchunk"])
+    fixed.upload(ri, bi)
+    fixed.step(steps + 4)
+    rb, bb = fixed.download()
+    assert np.array_equal(ra, rb) and np.array_equal(ba, bb)

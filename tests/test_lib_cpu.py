@@ -69,6 +69,7 @@ def _params(**kw):
 
 @pytest.mark.parametrize("kw,code", [({"nx": 0}, -1), ({"precision": 7}, -1), ({"npar": -1}, -1),
                                      ({"npar": 3, "layout": 1}, -3), ({"rho0": 0.0}, -1),
+                                     ({"tile_rows": 3}, -1), ({"tile_rows": 32}, -1),
                                      # largest boxes: above 2^28 nodes in FP32 and 2^27 (512^3) in FP64
                                      # a group of four direction arrays exceeds the 4 GiB per-array
                                      # addressing, and so do the FP64 factorised layout's 48 B of
