@@ -389,7 +389,8 @@ static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls)
         for (int k = 0; k < 5 && e == hipSuccess; ++k) e = dmalloc((void **)v1[k], n1, PZ_PART, h->stream);
         for (int k = 0; k < 5 && e == hipSuccess; ++k) e = hipMemsetAsync(*v1[k], 0, n1, h->stream);
         if (e == hipSuccess) e = dmalloc((void **)&h->P.part_lag, (size_t)h->npar * cgl::LAG_BLOCKS * 7 * sizeof(double), PZ_PART, h->stream);
-        if (e == hipSuccess) e = dmalloc((void **)&h->P.part_mom, (size_t)h->npar * cgl::PSLICES * 6 * sizeof(double), PZ_PART, h->stream);
+        if (e == hipSuccess) e = dmalloc((void **)&h->P.part_mom, (size_t)h->npar * cgl::LSLICES * 6 * sizeof(double), PZ_PART, h->stream);
+        if (e == hipSuccess) e = dmalloc((void **)&h->P.lcount, 2 * (size_t)h->npar * cgl::LSLICES * sizeof(int), PZ_PART, h->stream);
         h->E.pout = h->pout_i;
         h->E.center = h->P.center;
         h->E.cita = h->P.cita;
@@ -426,7 +427,7 @@ int cgl_destroy(cgl_t *h)
         void *ptrs[] = {h->ntype[0], h->ntype[1], h->walls_i, h->pin_i, h->pout_i, h->E.mac, h->E.der,
                         h->P.center, h->P.center_old, h->P.center_prev, h->P.up, h->P.rp, h->P.fp, h->P.tp,
                         h->P.frep, h->P.ten, h->P.tent, h->P.rad, h->P.mass, h->P.inertia, h->P.cita, h->P.shb,
-                        h->P.part_lag, h->P.part_mom, h->P.center0};
+                        h->P.part_lag, h->P.part_mom, h->P.center0, h->P.lout, h->P.lmask, h->P.lin, h->P.lcount};
         for (void *q : ptrs)
             if (q) (void)hipFree(q);
     }
@@ -868,6 +869,27 @@ int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moment
     return check_range(h);
 }
 
+// Link-list storage for particles of radius up to rmax: the box k_lists
+// scans, box_of(centre, R, 1), spans at most 2 ceil(R) + 5 nodes per axis,
+// and each of its LSLICES slices at most a ceil(1 / LSLICES) share of it.
+static int ensure_lists(cgl_handle *h, double rmax)
+{
+    const double w = 2.0 * ceil(rmax) + 5.0;
+    if (!(w * w * w < 2147483647.0)) return fail(CGL_ERR_ARG, "particle radius %g too large", rmax);
+    const int cap = (int)((w * w * w + cgl::LSLICES - 1) / cgl::LSLICES);
+    if (h->P.lout && cap <= h->P.lcap) return CGL_OK;
+    unsigned **v[3] = {&h->P.lout, &h->P.lmask, &h->P.lin};
+    for (unsigned **q : v) {
+        if (*q) HIPCHK(hipFree(*q));
+        *q = nullptr;
+    }
+    h->P.lcap = 0;
+    const size_t bytes = (size_t)h->npar * cgl::LSLICES * (size_t)cap * sizeof(unsigned);
+    for (unsigned **q : v) HIPCHK(dmalloc((void **)q, bytes, PZ_PART, h->stream));
+    h->P.lcap = cap;
+    return CGL_OK;
+}
+
 int cgl_set_particles(cgl_t *h, const double *center, const double *up, const double *rp, const double *rad,
                       const double *mass, const double *inertia, const double *cita, double fg)
 {
@@ -918,6 +940,9 @@ int cgl_set_particles(cgl_t *h, const double *center, const double *up, const do
     }
     h->P.sym_rules = 0;
     if ((rc = drop_graph(h))) return rc;
+    double rmax = 0.0;
+    for (int k = 0; k < h->npar; ++k) rmax = rad[k] > rmax ? rad[k] : rmax;
+    if ((rc = ensure_lists(h, rmax))) return rc;
     // set_walls of the start-up (S:212): fresh masks
     for (int b = 0; b < 2; ++b) HIPCHK(hipMemsetAsync(h->ntype[b], 0, h->N * sizeof(unsigned), h->stream));
     HIPCHK(hipMemsetAsync(h->walls_i, 0, h->N * sizeof(int), h->stream));
@@ -1595,6 +1620,17 @@ int cgl_checkpoint_load(cgl_t *h, const char *path)
             h->P.dom_hi_on[a] = Pp.dom_hi_on[a];
         }
         h->P.sym_rules = Pp.sym_rules;
+        // the link lists are derived from the restored masks and centres
+        std::vector<double> rad((size_t)h->npar);
+        HIPCHK(hipMemcpyAsync(rad.data(), h->P.rad, rad.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        double rmax = 0.0;
+        for (double r : rad) rmax = r > rmax ? r : rmax;
+        if ((rc = drop_graph(h))) return rc;
+        if ((rc = ensure_lists(h, rmax))) return rc;
+        dispatch(h, [&](auto impl) { decltype(impl)::build_lists(h, h->ntcur); });
+        if ((rc = check_launch("link lists"))) return rc;
+        HIPCHK(hipStreamSynchronize(h->stream));
     }
     return drop_graph(h);
 }

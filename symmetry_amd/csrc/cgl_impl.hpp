@@ -366,7 +366,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         moments_ext(h, false);
         P *phi = (P *)h->phi[h->phicur ^ 1];
         const Ext E = ext_with(h, h->ntype[h->ntcur]);
-        k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[h->ntcur], h->pin_i, phi);
+        k_ghost_phi<P><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[h->ntcur], h->pin_i, phi);
         mirror_phi(h, phi);
         k_derivs_ext<P><<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, E, phi, nullptr, nullptr);
     }
@@ -403,7 +403,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         P *phi = (P *)h->phi[h->phicur ^ 1];
         const Ext E = ext_with(h, h->ntype[h->ntcur]);
         if (h->npar) {
-            k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[h->ntcur], h->pin_i, phi);
+            k_ghost_phi<P><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[h->ntcur], h->pin_i, phi);
             mirror_phi(h, phi);
         }
         Ext E2 = E;
@@ -420,6 +420,13 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         if (mirrors(h))
             k_mirror_masks<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->ntype[buf], h->walls_i, h->pin_i,
                                                                 h->pout_i);
+        build_lists(h, buf);
+    }
+    // the particles' link lists from the masks in buffer `buf`
+    static void build_lists(cgl_handle *h, int buf)
+    {
+        k_lists<<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, ext_with(h, h->ntype[buf]), h->P, h->ntype[buf],
+                                                          h->walls_i, h->pin_i, h->pout_i);
     }
     // The stages of a step with walls and/or particles (S:216-251).  The
     // masks of the step's start stay in ntype[ntcur] until the step ends;
@@ -442,7 +449,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     {
         const Ext Eo = ext_with(h, h->ntype[h->ntcur]);
         const S *rs = (const S *)h->pops[h->cur][0], *bs = (const S *)h->pops[h->cur][1];
-        k_momentum<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->ntype[h->ntcur], h->walls_i,
+        k_momentum<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->ntype[h->ntcur], h->walls_i,
                                                                       h->pout_i, rs, bs, h->cold);
         h->cold = 0;
     }
@@ -468,7 +475,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         const int nb = h->ntcur ^ 1;
         P *phi = (P *)h->phi[h->phicur ^ 1];
         const Ext En = ext_with(h, h->ntype[nb]);
-        k_refill<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[h->ntcur], h->ntype[nb],
+        k_refill<P><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[h->ntcur], h->ntype[nb],
                                                              h->pout_i, phi);
         mirror_phi(h, phi);
         mirror_mac(h);
@@ -481,10 +488,10 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         P *phi = (P *)h->phi[h->phicur ^ 1];
         const Ext En = ext_with(h, h->ntype[nb]);
         if (h->npar) {
-            k_ghost_phi<P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[nb], h->pin_i, phi);
+            k_ghost_phi<P><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[nb], h->pin_i, phi);
             mirror_phi(h, phi);
             // wetted normals next to the particles, read by the collide kernel
-            k_wet_normals<R, P><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, En, h->P, phi);
+            k_wet_normals<R, P><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, phi);
         }
         const void *i0, *i1;
         void *o0, *o1;
@@ -497,7 +504,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         if (h->tev) (void)hipEventRecord(h->tev[3], h->stream);
         if (h->npar) {
             S *ro = (S *)o0, *bo = (S *)o1;
-            k_curve<S, SHIFT><<<pgrid(h, PSLICES), PB, 0, h->stream>>>(
+            k_curve<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(
                 h->L, En, h->P, h->ntype[nb], h->walls_i, h->pout_i, ro, bo, SHIFT ? (const float *)phi : nullptr);
             if (mirrors(h)) k_mirror_der<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->E.der);
             h->ntcur = nb;

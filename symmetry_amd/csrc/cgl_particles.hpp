@@ -7,10 +7,14 @@
 // Newton-Euler update (movement, S:1337-1346).  All particle quantities are
 // FP64.
 //
-// Every kernel visits the bounding box of each particle (grid.x = particle,
-// grid.y = slices of the box).  Per-particle sums are reduced in FP64 first
-// within a workgroup and then, in a fixed order, over workgroups, so results
-// are reproducible from run to run.
+// The mask kernels visit the bounding box of each particle (grid.x =
+// particle, grid.y = slices of the box) and k_lists then compacts, per
+// particle and in box order, its live fluid neighbours (with the bit mask of
+// their links into it) and its live solid nodes; the bounce-back, momentum,
+// refill, wetting and ghost-phi kernels walk those lists.  Per-particle sums
+// are reduced in FP64 by wavefront shuffles, then over the workgroup's waves
+// and over workgroups in a fixed order, so results are reproducible from run
+// to run.
 //
 // Positions are the reference's 1-based lattice coordinates.  On periodic
 // axes a box is unwrapped around the particle, so displacements are the
@@ -23,6 +27,7 @@ namespace cgl {
 
 constexpr int PB = 256;        // threads per particle-kernel workgroup
 constexpr int PSLICES = 48;    // workgroups per particle for box kernels
+constexpr int LSLICES = 16;    // slices of the link lists per particle (one workgroup each)
 constexpr int LAG_BLOCKS = 64; // workgroups per particle for the Lagrange points (one point per thread)
 constexpr int LA_M = 180, LA_N = 90, LA_NUM = LA_M * (LA_N + 1);
 
@@ -42,8 +47,26 @@ struct Part {
     double *fp, *tp, *frep, *ten, *tent; // momentum-exchange force/torque, repulsion, capillary force/torque
     double *rad, *mass, *inertia, *cita, *shb;
     double *part_lag;            // [npar][LAG_BLOCKS][7]
-    double *part_mom;            // [npar][PSLICES][6]
+    double *part_mom;            // [npar][LSLICES][6]
+    // link lists (k_lists) in LSLICES slices per particle, each slice the
+    // nodes of one contiguous range of box_of(centre, R, 1), lcap entries per
+    // slice, as box indices k: live fluid nodes next to p (pout = p+1), with
+    // bit i of lmask set when x - c_i is p's solid node, and p's live solid
+    // nodes (pin = p+1).  Slice s of particle p: entries from
+    // ((p LSLICES + s) lcap), lcount[2 (p LSLICES + s)] and [.. + 1] of them.
+    int lcap;
+    int pad3;
+    unsigned *lout, *lmask, *lin;
+    int *lcount;
 };
+
+// Workgroup (p, s) of a list kernel walks slice s of particle p's list
+// (which = 0 live fluid, 1 live solid): j over the slice's entries, at
+// index base + j.
+#define FOR_LIST(Pt, which, j, base)                                                                          \
+    const size_t base = ((size_t)blockIdx.x * LSLICES + blockIdx.y) * (size_t)(Pt).lcap;                     \
+    const int cnt_##j = (Pt).lcount[2 * (blockIdx.x * LSLICES + blockIdx.y) + (which)];                       \
+    for (int j = (int)threadIdx.x; j < cnt_##j; j += PB)
 
 // The box of particle p around centre c with margin m: unwrapped 0-based
 // lower corner and extent per axis.
@@ -311,12 +334,12 @@ __global__ __launch_bounds__(PB) void k_refill(Lat L, Ext E, Part Pt, const unsi
     const int p = blockIdx.x;
     const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
     const size_t N = (size_t)L.N;
-    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+    FOR_LIST(Pt, 0, j, lb)
+    {
         int X[3];
         unsigned n;
-        box_node(L, b, k, X, n);
-        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
-        if (pout[n] != p + 1 || !(nt_old[n] & NT_SOLID)) continue;
+        box_node(L, b, Pt.lout[lb + j], X, n);
+        if (!(nt_old[n] & NT_SOLID)) continue;
         double xr[3], uw[3];
         wall_vel(Pt, p, Pt.center_old + 3 * p, X, xr, uw);
         double sr = 0.0, sb = 0.0, num = 0.0;
@@ -347,12 +370,11 @@ __global__ __launch_bounds__(PB) void k_ghost_phi(Lat L, Ext E, Part Pt, const u
 {
     const int p = blockIdx.x;
     const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
-    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+    FOR_LIST(Pt, 1, j, lb)
+    {
         int X[3];
         unsigned n;
-        box_node(L, b, k, X, n);
-        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
-        if (pin[n] != p + 1) continue;
+        box_node(L, b, Pt.lin[lb + j], X, n);
         double wsum = 0.0, acc = 0.0;
         for (int i = 1; i < Q; ++i) {
             const unsigned m = nb_index(L, X, CX[i], CY[i], CZ[i]);
@@ -362,6 +384,75 @@ __global__ __launch_bounds__(PB) void k_ghost_phi(Lat L, Ext E, Part Pt, const u
             }
         }
         phi[n] = P(acc / wsum);
+    }
+}
+
+// Link lists of particle p, slice s = blockIdx.y: the box nodes from
+// s * per to (s + 1) * per (per = ceil(box / LSLICES)) in box order, each
+// chunk of PB of them taking its place by a block-wide prefix (wave ballots,
+// then the waves in order), so the lists, and every sum over them, are the
+// same from run to run.
+static __global__ __launch_bounds__(PB) void k_lists(Lat L, Ext E, Part P, const unsigned *nt, const int *walls,
+                                                     const int *pin, const int *pout)
+{
+    const int p = blockIdx.x;
+    const Box b = box_of(P.center + 3 * p, P.rad[p], 1.0);
+    const unsigned nbox = box_size(b), per = (nbox + LSLICES - 1) / LSLICES;
+    const unsigned k1 = min(nbox, (blockIdx.y + 1) * per);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ unsigned wsum[2][PB / 64];
+    __shared__ unsigned base[2];
+    if (threadIdx.x == 0) base[0] = base[1] = 0;
+    __syncthreads();
+    const size_t off = ((size_t)p * LSLICES + blockIdx.y) * (size_t)P.lcap;
+    for (unsigned k0 = blockIdx.y * per; k0 < k1; k0 += PB) {
+        const unsigned k = k0 + threadIdx.x;
+        bool isout = false, isin = false;
+        unsigned mask = 0;
+        if (k < k1) {
+            int X[3];
+            unsigned n;
+            box_node(L, b, k, X, n);
+            if (box_in_domain(E, L, X) && interior(E, L, X)) {
+                if (pout[n] == p + 1 && !(nt[n] & NT_SOLID)) {
+                    isout = true;
+                    for (int i = 1; i < Q; ++i)
+                        if (walls[nb_index(L, X, -CX[i], -CY[i], -CZ[i])] == p + 1) mask |= 1u << i;
+                }
+                isin = pin[n] == p + 1;
+            }
+        }
+        const unsigned long long bo = __ballot(isout), bi = __ballot(isin);
+        const unsigned long long below = (1ull << lane) - 1ull;
+        if (lane == 0) {
+            wsum[0][wave] = (unsigned)__popcll(bo);
+            wsum[1][wave] = (unsigned)__popcll(bi);
+        }
+        __syncthreads();
+        unsigned jo = base[0], ji = base[1];
+        for (int w = 0; w < wave; ++w) {
+            jo += wsum[0][w];
+            ji += wsum[1][w];
+        }
+        jo += (unsigned)__popcll(bo & below);
+        ji += (unsigned)__popcll(bi & below);
+        if (isout && jo < (unsigned)P.lcap) {
+            P.lout[off + jo] = k;
+            P.lmask[off + jo] = mask;
+        }
+        if (isin && ji < (unsigned)P.lcap) P.lin[off + ji] = k;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int w = 0; w < PB / 64; ++w) {
+                base[0] += wsum[0][w];
+                base[1] += wsum[1][w];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const size_t c = 2 * ((size_t)p * LSLICES + blockIdx.y);
+        P.lcount[c] = (int)min(base[0], (unsigned)P.lcap);
+        P.lcount[c + 1] = (int)min(base[1], (unsigned)P.lcap);
     }
 }
 
@@ -379,12 +470,11 @@ __global__ __launch_bounds__(PB) void k_wet_normals(Lat L, Ext E, Part Pt, const
     const int p = blockIdx.x;
     const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
     const size_t N = (size_t)L.N;
-    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+    FOR_LIST(Pt, 0, j, lb)
+    {
         int X[3];
         unsigned n;
-        box_node(L, b, k, X, n);
-        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
-        if (!(E.ntype[n] & NT_POUT) || E.pout[n] != p + 1) continue;
+        box_node(L, b, Pt.lout[lb + j], X, n);
         R gx, gy, gz, nx, ny, nz;
         grad19<R>([&](int dx, int dy, int dz) { return phi[nb_index(L, X, dx, dy, dz)]; }, gx, gy, gz);
         if (unit_normal<R>(gx, gy, gz, nx, ny, nz)) {
@@ -414,18 +504,18 @@ __global__ __launch_bounds__(PB) void k_curve(Lat L, Ext E, Part Pt, const unsig
     const int p = blockIdx.x;
     const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
     const size_t N = (size_t)L.N;
-    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+    FOR_LIST(Pt, 0, j, lb)
+    {
         int X[3];
         unsigned n;
-        box_node(L, b, k, X, n);
-        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
-        if (pout[n] != p + 1 || (nt[n] & NT_SOLID)) continue;
+        box_node(L, b, Pt.lout[lb + j], X, n);
+        const unsigned links = Pt.lmask[lb + j];
         double xr[3], uw[3];
         wall_vel(Pt, p, Pt.center + 3 * p, X, xr, uw);
         const double rr = E.mac[N + n], rbv = E.mac[2 * N + n];
         for (int i = 1; i < Q; ++i) {
+            if (!((links >> i) & 1u)) continue;
             const unsigned m = nb_index(L, X, -CX[i], -CY[i], -CZ[i]);
-            if (walls[m] != p + 1) continue;
             unsigned ms;
             int ds;
             read_slot(E, L, X, i, ms, ds);
@@ -449,19 +539,26 @@ __global__ __launch_bounds__(PB) void k_curve(Lat L, Ext E, Part Pt, const unsig
     }
 }
 
-// Fixed-order FP64 reduction of K values per thread across the workgroup.
+// Fixed-order FP64 sum of K values per thread over the workgroup: an xor
+// butterfly of wavefront shuffles (every lane ends with its wave's sum; the
+// pairing is the same every run, and addition commutes, so all lanes agree),
+// then the PB / 64 wave sums in wave order.
 template <int K> __device__ __forceinline__ void block_sum(double v[K], double *out)
 {
-    __shared__ double sh[PB][K];
-    for (int k = 0; k < K; ++k) sh[threadIdx.x][k] = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += __shfl_xor(v[k], off);
+    __shared__ double ws[PB / 64][K];
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < K; ++k) ws[threadIdx.x >> 6][k] = v[k];
     __syncthreads();
-    for (int s = PB / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int k = 0; k < K; ++k) sh[threadIdx.x][k] += sh[threadIdx.x + s][k];
-        __syncthreads();
-    }
     if (threadIdx.x == 0)
-        for (int k = 0; k < K; ++k) out[k] = sh[0][k];
+        for (int k = 0; k < K; ++k) {
+            double t = ws[0][k];
+            for (int w = 1; w < PB / 64; ++w) t += ws[w][k];
+            out[k] = t;
+        }
 }
 
 // ---------------------------------------------------------- momentum exchange
@@ -479,17 +576,16 @@ __global__ __launch_bounds__(PB) void k_momentum(Lat L, Ext E, Part Pt, const un
     const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
     const size_t N = (size_t)L.N;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+    FOR_LIST(Pt, 0, j, lb)
+    {
         int X[3];
         unsigned n;
-        box_node(L, b, k, X, n);
-        if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
-        if (pout[n] != p + 1 || (nt[n] & NT_SOLID)) continue;
+        box_node(L, b, Pt.lout[lb + j], X, n);
+        const unsigned links = Pt.lmask[lb + j];
         double xr[3], uw[3];
         wall_vel(Pt, p, Pt.center + 3 * p, X, xr, uw);
         for (int i = 1; i < Q; ++i) {
-            const unsigned m = nb_index(L, X, -CX[i], -CY[i], -CZ[i]);
-            if (walls[m] != p + 1) continue;
+            if (!((links >> i) & 1u)) continue;
             const int o = OPC[i];
             unsigned mi, no = n;
             int di, dof = o;
@@ -512,7 +608,7 @@ __global__ __launch_bounds__(PB) void k_momentum(Lat L, Ext E, Part Pt, const un
             acc[5] -= xr[0] * fmy - xr[1] * fmx;
         }
     }
-    block_sum<6>(acc, Pt.part_mom + ((size_t)p * PSLICES + blockIdx.y) * 6);
+    block_sum<6>(acc, Pt.part_mom + ((size_t)p * LSLICES + blockIdx.y) * 6);
 }
 
 // --------------------------------------------------- capillary force (FP64)
@@ -668,8 +764,8 @@ static __global__ __launch_bounds__(PB) void k_move(Lat L, Ext E, Part P, int ph
     for (int p = threadIdx.x; p < np; p += PB) {
         if (phases & MV_MOM) {
             double m[6] = {0, 0, 0, 0, 0, 0};
-            for (int s = 0; s < PSLICES; ++s)
-                for (int k = 0; k < 6; ++k) m[k] += P.part_mom[((size_t)p * PSLICES + s) * 6 + k];
+            for (int s = 0; s < LSLICES; ++s)
+                for (int k = 0; k < 6; ++k) m[k] += P.part_mom[((size_t)p * LSLICES + s) * 6 + k];
             for (int k = 0; k < 3; ++k) {
                 P.fp[3 * p + k] = m[k];
                 P.tp[3 * p + k] = m[3 + k];
