@@ -74,6 +74,7 @@ program cgl_driver
   character(len=512) :: arg
   integer :: t, nchunk, u, ios
   double precision :: wcap_old = 0.0d0   ! output4's accumulated capillary work (S:202)
+  integer :: t_breakup = -1              ! first output step with the jet's axis blue (ic = 'cylinder')
 
   amp(1) = 0.8d0
   if (command_argument_count() >= 1) then
@@ -172,6 +173,7 @@ program cgl_driver
       if (npar > 0) call output_particles(t)
       if (npar > 0) call output_result2(t)
       if (ref_faces) call output3(t)
+      if (trim(ic) == 'cylinder') call output_neck(t)
       call output4(t)
       if (write_tecplot) call output_results(t)
       if (dump_binary) call dump_fields(t)
@@ -337,6 +339,45 @@ contains
     end do
     close(uo)
   end subroutine output_particles
+
+  ! The neck of a z-axis jet (ic = 'cylinder'), the quantity output3 follows
+  ! for the reference's y-axis thread (S:1673-1715), restated for a jet
+  ! along z: per z slice R(z) = sqrt(red area / pi), the red area being the
+  ! device sum of (1 + phi)/2 over the slice (cgl_diagnostics).  NECK.dat
+  ! gets t, t*, the neck radius min R, max R, the neck's z (1-based), the
+  ! amplitude (max R - min R)/2 and, at the z where it is lowest, phi on the
+  ! axis (the mean over the 2 x 2 nodes nearest it).  The first output step
+  ! at which that phi is below 0 (the axis has turned blue: the thread has
+  ! broken) is reported once.
+  subroutine output_neck(tt)
+    integer, intent(in) :: tt
+    real(c_double), target :: area(zl)
+    real(c_double) :: sums(7)
+    double precision :: r(zl), ax(zl), tr
+    integer :: z, zn, uo, xa, ya
+    call cgl_check(cgl_diagnostics(h, sums, c_loc(area)), 'cgl_diagnostics')
+    r = sqrt(area/PI)
+    xa = xl/2
+    ya = yl/2
+    do z = 1, zl
+      ax(z) = (rho_N(z,ya,xa) + rho_N(z,ya+1,xa) + rho_N(z,ya,xa+1) + rho_N(z,ya+1,xa+1))/4.0d0
+    end do
+    zn = minloc(r, 1)
+    tr = sqrt(Rad_c**3*rho0/sigma)
+    if (tt == 0) then
+      open(newunit=uo, file=trim(outdir)//'/NECK.dat', status='replace')
+      write(uo, '(a)') 'VARIABLES=T,T*,RMIN,RMAX,ZMIN,AMP,PHI_AXIS_MIN'
+    else
+      open(newunit=uo, file=trim(outdir)//'/NECK.dat', status='unknown', position='append')
+    end if
+    write(uo, '(i8, 1x, es24.16e3, 2(1x, es24.16e3), 1x, i6, 2(1x, es24.16e3))') tt, tt/tr, r(zn), maxval(r), zn, &
+          (maxval(r) - r(zn))/2.0d0, minval(ax)
+    close(uo)
+    if (t_breakup < 0 .and. minval(ax) < 0.0d0) then
+      t_breakup = tt
+      write(*, '(a, i0)') ' breakup at step ', tt
+    end if
+  end subroutine output_neck
 
   ! output_result2 (S:1420-1441): one history file P_<p>.dat per particle,
   ! the reference's columns and format; appended at every output step (the

@@ -284,3 +284,43 @@ def test_fortran_driver_dense_particles(tmp_path):
             assert vals[0:3] == list(got["center"][p]) and vals[3:6] == list(got["up"][p]), (t, p)
             assert vals[6:9] == list(got["force"][p]) and vals[9:12] == list(got["fcap"][p]), (t, p)
     lat.close()
+
+
+def test_fortran_driver_neck_and_breakup(tmp_path):
+    """The 32x32x128 jet (R = 8, 10 % mode, tau = 1, FP32 factorised) through
+    the driver to its breakup: every NECK.dat line (neck radius from the
+    device slice sums, its z, the amplitude, phi on the axis at the neck)
+    equals the ctypes binding's same run bit for bit, and the driver reports
+    the breakup at the same output step."""
+    import symmetry_amd
+    every, last = 100, 20000
+    nml = tmp_path / "in.nml"
+    nml.write_text(f"&cgl\n xl=32, yl=32, zl=128, nsteps={last}, out_every={every}, prec=1, layout=1,\n"
+                   f" sigma=0.01d0, rho0=1.0d0, tau_R=1.0d0, tau_B=1.0d0, beta=0.7d0, ic_on_device=.true.,\n"
+                   f" write_tecplot=.false., outdir='{tmp_path}'\n/\n")
+    print("driver jet to breakup", flush=True)
+    r = subprocess.run([DRIVER, str(nml)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-600:], r.stderr[-600:])
+    assert r.returncode == 0
+    rows = [l.split() for l in (tmp_path / "NECK.dat").read_text().splitlines()[1:]]
+    assert len(rows) == last // every + 1
+    cfg = ic.rp_config1()
+    lat = symmetry_amd.Lattice(32, 32, 128, precision="f32", layout="fs", sigma=0.01, rho0=1.0, nu_r=cfg["nu_r"],
+                               nu_b=cfg["nu_b"], beta=0.7)
+    lat.init_cylinder(16.5, 16.5, 8.0, 6.0 * 0.7 * 0.134, (0.8,))
+    breakup = None
+    for k, row in enumerate(rows):
+        if k:
+            lat.step(every)
+        rz = np.sqrt(lat.diagnostics(slices=True)["slice_red"] / np.pi)
+        phi = lat.calmacros()["phi"]
+        ax = (((phi[15, 15] + phi[15, 16]) + phi[16, 15]) + phi[16, 16]) / 4.0  # [x, y] = (16, 16), (16, 17), ... 1-based
+        zn = int(np.argmin(rz))
+        t = int(row[0])
+        assert t == k * every
+        assert float(row[2]) == rz[zn] and float(row[3]) == rz.max() and int(row[4]) == zn + 1, (t, row)
+        assert float(row[5]) == (rz.max() - rz[zn]) / 2.0 and float(row[6]) == ax.min(), (t, row)
+        if breakup is None and ax.min() < 0.0:
+            breakup = t
+    print("breakup at", breakup, flush=True)
+    assert breakup is not None and f" breakup at step {breakup}" in r.stdout
