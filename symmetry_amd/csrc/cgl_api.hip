@@ -288,8 +288,6 @@ int cgl_create(cgl_t **out, const cgl_params *p)
             return fail(CGL_ERR_ARG, "axis %d: a bounded axis needs two ghost layers and two live nodes", a);
     }
     if (p->npar < 0) return fail(CGL_ERR_ARG, "npar must be >= 0");
-    if ((walls || mirrors || p->npar > 0) && p->layout != CGL_LAYOUT_RB)
-        return fail(CGL_ERR_UNSUPPORTED, "walls, mirrors and particles are implemented for the red/blue layout");
     if (!(p->rho0 > 0) || !(p->nu_r > 0) || !(p->nu_b > 0)) return fail(CGL_ERR_ARG, "rho0, nu_r, nu_b must be > 0");
     if (p->layout != CGL_LAYOUT_RB && p->layout != CGL_LAYOUT_FS) return fail(CGL_ERR_ARG, "unknown layout %d", p->layout);
     if (p->tile_rows != 0 && p->tile_rows != 4 && p->tile_rows != 8 && p->tile_rows != 16)
@@ -359,8 +357,10 @@ static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls)
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     for (int k = 0; k < 4 && e == hipSuccess; ++k) e = hipEventCreate(&h->ev[k]);
     // RB: both population buffers.  FS: one RB buffer (staging of uploaded or
-    // initial states) plus the factorised double buffer.
-    const int nrb = p->layout == CGL_LAYOUT_RB ? 2 : 1;
+    // initial states) plus the factorised double buffer; with walls or
+    // particles both RB buffers, which then hold the red/blue side arrays of
+    // the factorised states (the slots of solid nodes).
+    const int nrb = p->layout == CGL_LAYOUT_RB || h->ext ? 2 : 1;
     for (int b = 0; b < nrb && e == hipSuccess; ++b)
         for (int c = 0; c < 2 && e == hipSuccess; ++c) e = dmalloc(&h->pops[b][c], (size_t)cgl::Q * h->N * h->esize, PZ_POPS, h->stream);
     if (p->layout == CGL_LAYOUT_FS) {
@@ -1453,6 +1453,10 @@ static std::vector<std::pair<void *, size_t>> ckpt_sections(cgl_handle *h)
         const size_t ns = h->p.precision == CGL_PREC_F64 ? 6 * sizeof(double) : 4 * sizeof(float);
         v.push_back({h->fsf[h->cur], pops});
         v.push_back({h->fss[h->cur], N * ns});
+        if (h->ext) { // the side arrays of the solid nodes
+            v.push_back({h->pops[h->cur][0], pops});
+            v.push_back({h->pops[h->cur][1], pops});
+        }
     }
     v.push_back({h->phi[0], N * h->phisize});
     v.push_back({h->phi[1], N * h->phisize});

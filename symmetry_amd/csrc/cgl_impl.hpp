@@ -304,6 +304,10 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         if (h->state_layout == RB)
             k_unpack<S, P, SHIFT><<<node_grid(h->L), node_block(), 0, h->stream>>>(
                 h->L, (const S *)h->pops[h->cur][red ? 0 : 1], aos, (const P *)h->phi[h->phicur], red, h->E);
+        else if (h->ext)
+            k_unpack_fs_ext<S, P, SHIFT><<<node_grid(h->L), node_block(), 0, h->stream>>>(
+                h->L, h->E, (const S *)h->fsf[h->cur], h->fss[h->cur], (const S *)h->pops[h->cur][0],
+                (const S *)h->pops[h->cur][1], (const P *)h->phi[h->phicur], aos, red);
         else
             k_unpack_fs<S, SHIFT><<<node_grid(h->L), node_block(), 0, h->stream>>>(
                 h->L, (const S *)h->fsf[h->cur], h->fss[h->cur], aos, red);
@@ -313,6 +317,11 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     {
         Ext E = h->E;
         E.ntype = h->npar ? nt : nullptr;
+        // factorised layout: the side arrays of the current and the next state
+        E.rbi[0] = h->pops[h->cur][0];
+        E.rbi[1] = h->pops[h->cur][1];
+        E.rbo[0] = h->pops[h->cur ^ 1][0];
+        E.rbo[1] = h->pops[h->cur ^ 1][1];
         return E;
     }
     static dim3 pgrid(const cgl_handle *h, int slices) { return dim3(h->npar, slices, 1); }
@@ -346,12 +355,22 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         const P *phio = (const P *)h->phi[h->phicur];
         P *phi = (P *)h->phi[h->phicur ^ 1];
         const Ext E = ext_with(h, h->ntype[h->ntcur]);
-        if (full)
+        if (h->state_layout == FS) {
+            if (full) {
+                k_moments<FS, S, R, P, SHIFT, true, true>
+                    <<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, h->mac, E);
+            } else {
+                const int xc = K1_XC;
+                const dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, (h->L.nx + xc - 1) / xc);
+                k_moments_fsx_ext<S, R, P, SHIFT><<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, xc, E);
+            }
+        } else if (full) {
             k_moments<RB, S, R, P, SHIFT, true, true>
                 <<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, h->mac, E);
-        else
+        } else {
             k_moments<RB, S, R, P, SHIFT, false, true>
                 <<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, nullptr, E);
+        }
         wall_phi(h, phi);
         if (full) wall_phi(h, h->mac + 6 * h->N); // the FP64 phi output
         mirror_mac(h);
@@ -386,7 +405,10 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         const dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, 1);
         double *out = scr, *area = scr + 8, *part = area + h->L.nz, *spart = part + (size_t)NDIAG * grid.x * grid.y;
         const Ext E = ext_with(h, h->npar ? h->ntype[h->ntcur] : nullptr);
-        if (h->ext)
+        if (h->ext && h->state_layout == FS)
+            k_diag_tiles<FS, S, R, P, SHIFT, true>
+                <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, E, part, spart);
+        else if (h->ext)
             k_diag_tiles<RB, S, R, P, SHIFT, true>
                 <<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, E, part, spart);
         else if (h->state_layout == RB)
@@ -449,8 +471,12 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     {
         const Ext Eo = ext_with(h, h->ntype[h->ntcur]);
         const S *rs = (const S *)h->pops[h->cur][0], *bs = (const S *)h->pops[h->cur][1];
-        k_momentum<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->ntype[h->ntcur], h->walls_i,
-                                                                      h->pout_i, rs, bs, h->cold);
+        if (h->state_layout == FS)
+            k_momentum_fs<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(
+                h->L, Eo, h->P, (const S *)h->fsf[h->cur], h->fss[h->cur], rs, bs, h->cold);
+        else
+            k_momentum<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->ntype[h->ntcur],
+                                                                          h->walls_i, h->pout_i, rs, bs, h->cold);
         h->cold = 0;
     }
     static void st_move(cgl_handle *h, int phases)
@@ -499,13 +525,27 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         state_out(h, o0, o1);
         const P *phio = (const P *)h->phi[h->phicur];
         if (h->tev) (void)hipEventRecord(h->tev[2], h->stream);
-        k_collide<RB, RB, S, R, P, SHIFT, true>
-            <<<k2_grid<R>(h->L, h->xc), k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
+        const dim3 g2 = k2_grid<R>(h->L, h->xc);
+        if (h->state_layout == RB && h->layout == RB)
+            k_collide<RB, RB, S, R, P, SHIFT, true>
+                <<<g2, k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
+        else if (h->state_layout == RB)
+            k_collide<RB, FS, S, R, P, SHIFT, true>
+                <<<g2, k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
+        else
+            k_collide<FS, FS, S, R, P, SHIFT, true>
+                <<<g2, k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
         if (h->tev) (void)hipEventRecord(h->tev[3], h->stream);
+        h->state_layout = h->layout;
         if (h->npar) {
-            S *ro = (S *)o0, *bo = (S *)o1;
-            k_curve<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(
-                h->L, En, h->P, h->ntype[nb], h->walls_i, h->pout_i, ro, bo, SHIFT ? (const float *)phi : nullptr);
+            if (h->layout == FS)
+                k_curve_fs<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(
+                    h->L, En, h->P, (const S *)o0, o1, (S *)En.rbo[0], (S *)En.rbo[1],
+                    SHIFT ? (const float *)phi : nullptr);
+            else
+                k_curve<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[nb], h->walls_i,
+                                                                          h->pout_i, (S *)o0, (S *)o1,
+                                                                          SHIFT ? (const float *)phi : nullptr);
             if (mirrors(h)) k_mirror_der<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->E.der);
             h->ntcur = nb;
         }

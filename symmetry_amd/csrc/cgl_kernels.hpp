@@ -50,6 +50,12 @@ template <class R> constexpr int KTY = sizeof(R) == 8 ? 4 : 8;
 // 1.2x faster than at one workgroup per CU); no bound for FP64, which runs
 // 256-thread workgroups.
 template <class R> constexpr int KWAVES = sizeof(R) == 8 ? 1 : 4;
+// The factorised step with walls and particles (its general gather: bounce-
+// back, mirror and solid-node slots) is compiled for 2 waves per SIMD: bounded
+// to 4 it spilled about 120 registers, and that build gave non-finite values
+// at some wall-adjacent nodes on the GPU (found with a NaN scan of the state,
+// not explained); at 2 it needs no spill.
+constexpr int KWAVES_FSX = 2;
 
 struct Lat {
     int nx, ny, nz;
@@ -76,7 +82,17 @@ struct Ext {
     const double *cita;      // contact angle per particle (S:21)
     double *mac;             // 6 x N: rho, rho_R, rho_B, ux, uy, uz at NEAR nodes
     double *der;             // 6 x N: gradx, grady, gradz, n_x, n_y, n_z at NEAR nodes
+    // factorised layout: red/blue side arrays (19 x N each, RB storage) of
+    // the input and output states, which hold the slots of nodes that were
+    // solid when the state was written
+    const void *rbi[2];
+    void *rbo[2];
 };
+
+// Recolour scalars of a factorised node that was solid when its step wrote
+// the state: the colour fraction slot holds this value (a fraction never
+// exceeds 1), and the node's slots are in the red/blue side arrays.
+constexpr float FS_SOLID = 4.0f;
 
 template <class T> __device__ __forceinline__ const T &at(const T *base, unsigned idx)
 {
@@ -229,6 +245,15 @@ template <class R, class P> __device__ __forceinline__ Shift<R> shift_of(P ph)
     s.hm = R(0.5) * (R(1) - R(ph < P(0) ? -ph : ph));
     return s;
 }
+// Red shift fraction of a node with shift sh minus that of a node with shift
+// t, with the rounding of a difference of two small fractions where both
+// nodes have the same majority colour.
+template <class R> __device__ __forceinline__ R red_shift_diff(const Shift<R> &sh, const Shift<R> &t)
+{
+    if (sh.redmaj == t.redmaj) return sh.redmaj ? t.hm - sh.hm : sh.hm - t.hm;
+    return sh.redmaj ? (R(1) - sh.hm) - t.hm : sh.hm - (R(1) - t.hm);
+}
+
 // Shift fractions in FP64 (exact for an FP32 phase field): host-layout
 // conversions and the particle kernels.
 __device__ __forceinline__ double shift_d(float ph, int red)
@@ -481,6 +506,173 @@ __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void 
     }
 }
 
+// pull_slot, with the source node's position relative to the node (o, each
+// component -1, 0 or +1: an index into the LDS rings) and the axes along
+// which the slot's direction d reverses c_i (flip bit a: the mirrored axes,
+// or all three for the bounce-back).
+template <int I>
+__device__ __forceinline__ void pull_slot_f(const unsigned xo[3], const unsigned yo[3], const unsigned zo[3],
+                                            unsigned bb, unsigned madj, unsigned &m, int &d, unsigned &flip,
+                                            int o[3])
+{
+    pull_slot<I>(xo, yo, zo, bb, madj, m, d);
+    if ((bb >> I) & 1u) {
+        flip = 7u;
+        o[0] = o[1] = o[2] = 0;
+        return;
+    }
+    flip = mirrored_axes<I>(madj);
+    o[0] = (flip & 1u) ? 0 : -CX[I];
+    o[1] = (flip & 2u) ? 0 : -CY[I];
+    o[2] = (flip & 4u) ? 0 : -CZ[I];
+}
+
+// dot_c of the direction d = c_I with the components of the axes in `flip`
+// reversed: the same terms in the same order with exact sign changes, so it
+// rounds as dot_c(d, ...).
+template <int I, class R> __device__ __forceinline__ R dot_cf(unsigned flip, R nx, R ny, R nz)
+{
+    R dot = 0;
+    bool first = true;
+    if (CX[I] != 0) { R t = CX[I] == 1 ? nx : -nx; if (flip & 1u) t = -t; dot = t; first = false; }
+    if (CY[I] != 0) { R t = CY[I] == 1 ? ny : -ny; if (flip & 2u) t = -t; dot = first ? t : dot + t; first = false; }
+    if (CZ[I] != 0) { R t = CZ[I] == 1 ? nz : -nz; if (flip & 4u) t = -t; dot = first ? t : dot + t; }
+    return dot;
+}
+
+// Recolour scalars of node m in the FP64 factorised layout.
+__device__ __forceinline__ FsScal64 fs64_at(const double *sc, unsigned m)
+{
+    const double2 s01 = at((const double2 *)sc, 3 * m);
+    const double2 s23 = at((const double2 *)sc, 3 * m + 1);
+    return FsScal64{s01.x, s01.y, s23.x, s23.y, at(sc, 6 * m + 4)};
+}
+__device__ __forceinline__ bool solid_scal(const FsScal64 &s) { return s.a > 0.5 * double(FS_SOLID); }
+__device__ __forceinline__ bool solid_scal(const float4 &s) { return s.x > 0.5f * FS_SOLID; }
+
+// Pull gather of the factorised layout with walls, mirrors and particles.
+// The slot of direction i (pull_slot: the upstream node, the node's own
+// opposite slot off a wall, or the mirror image's reflected slot) is rebuilt
+// from its node's recolour scalars as gather_fs does, with direction d; where
+// that node was solid when the state was written (FS_SOLID scalars) the slot
+// comes from the red/blue side arrays rs, bs instead: the populations that
+// stream through a solid node (S:622, S:712) and the moving-wall bounce-back
+// values of the curve links (S:1004-1018) are not a colour split of one
+// node.  In shifted storage the side arrays hold deviations from w_i rho0
+// s_red and w_i rho0 s_blue of the reading node, whose phase field of the
+// stored state is ax.  scal(m, o) returns the scalars of source node m at
+// offset o.
+template <class S, class R, bool SHIFT, class Scal>
+__device__ __forceinline__ void gather_fs_gen(const S *__restrict__ fs, const S *__restrict__ rs,
+                                              const S *__restrict__ bs, size_t N, const unsigned xo[3],
+                                              const unsigned yo[3], const unsigned zo[3], unsigned bb,
+                                              unsigned madj, Scal scal, R ax, Node<R, SHIFT> &nd, R rho0)
+{
+    if constexpr (!SHIFT) {
+        static_for<Q>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            unsigned m, flip;
+            int d, o[3];
+            pull_slot_f<i>(xo, yo, zo, bb, madj, m, d, flip, o);
+            const FsScal64 sc = scal(m, o);
+            const unsigned k = grp_off(d, N, m);
+            if (solid_scal(sc)) {
+                nd.r[i] = StoreOf<S>::ld(grp(rs, d, N), k);
+                nd.b[i] = StoreOf<S>::ld(grp(bs, d, N), k);
+            } else {
+                const R f = StoreOf<S>::ld(grp(fs, d, N), k);
+                R r = sc.a * f;
+                if (i > 0) r = r + (sc.g * R(W[i])) * dot_cf<i, R>(flip, sc.nx, sc.ny, sc.nz);
+                nd.r[i] = r;
+                nd.b[i] = f - r;
+            }
+        });
+    } else {
+        const Shift<R> sh = shift_of<R>(ax);
+        const R xr = sh.redmaj ? R(1) - sh.hm : sh.hm, xb = sh.redmaj ? sh.hm : R(1) - sh.hm;
+        R a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0, dva = 0, dvb = 0, g1 = 0, g2 = 0;
+        static_for<Q>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            unsigned m, flip;
+            int d, o[3];
+            pull_slot_f<i>(xo, yo, zo, bb, madj, m, d, flip, o);
+            const float4 g = scal(m, o);
+            const unsigned k = grp_off(d, N, m);
+            R A, B, f, gc = 0;
+            if (solid_scal(g)) {
+                const R rv = StoreOf<S>::ld(grp(rs, d, N), k), bv = StoreOf<S>::ld(grp(bs, d, N), k);
+                A = xr;
+                B = xb;
+                dva += rv;
+                dvb += bv;
+                f = rv + bv;
+            } else {
+                f = StoreOf<S>::ld(grp(fs, d, N), k);
+                decode_frac(g.x, A, B);
+                dva += A * f;
+                dvb += B * f;
+                if (CX[i] != 0) { R t = CX[i] == 1 ? R(g.y) : -R(g.y); gc += (flip & 1u) ? -t : t; }
+                if (CY[i] != 0) { R t = CY[i] == 1 ? R(g.z) : -R(g.z); gc += (flip & 2u) ? -t : t; }
+                if (CZ[i] != 0) { R t = CZ[i] == 1 ? R(g.w) : -R(g.w); gc += (flip & 4u) ? -t : t; }
+            }
+            if (i == 0) { a0 = A; b0 = B; }
+            else if (i <= 6) { a1 += A; b1 += B; g1 += gc; }
+            else { a2 += A; b2 += B; g2 += gc; }
+            nd.ft[i] = f;
+        });
+        const R gw = g1 * R(W[1]) + g2 * R(W[7]);
+        nd.sr = rho0 * (a0 * R(W[0]) + a1 * R(W[1]) + a2 * R(W[7])) + dva + gw;
+        nd.sb = rho0 * (b0 * R(W[0]) + b1 * R(W[1]) + b2 * R(W[7])) + dvb - gw;
+    }
+}
+
+// One slot (m, d) of a factorised state as the red and blue values that the
+// red/blue layout stores for the node reading it: FP64 the populations
+// themselves; shifted storage deviations from w_i rho0 s_red and w_i rho0
+// s_blue of the reader's shift rd, the minority colour formed first (as
+// write_node does).  An FS_SOLID node's slot is in the side arrays, shifted
+// for the reader of the stored state (phase field ax), and is re-shifted.
+template <int I, class S, class R, bool SHIFT, class Sc>
+__device__ __forceinline__ void fs_slot_rb(const S *__restrict__ fs, const S *__restrict__ rs,
+                                           const S *__restrict__ bs, size_t N, unsigned m, int d, unsigned flip,
+                                           const Sc &sc, R ax, R arw, R rho0, R &vr, R &vb)
+{
+    const unsigned k = grp_off(d, N, m);
+    if (solid_scal(sc)) {
+        vr = StoreOf<S>::ld(grp(rs, d, N), k);
+        vb = StoreOf<S>::ld(grp(bs, d, N), k);
+        if constexpr (SHIFT) {
+            const R dsh = R(W[I]) * rho0 * red_shift_diff(shift_of<R>(ax), shift_of<R>(arw));
+            vr += dsh;
+            vb -= dsh;
+        }
+        return;
+    }
+    const R f = StoreOf<S>::ld(grp(fs, d, N), k);
+    if constexpr (!SHIFT) {
+        R r = sc.a * f;
+        if (I > 0) r = r + (sc.g * R(W[I])) * dot_cf<I, R>(flip, sc.nx, sc.ny, sc.nz);
+        vr = r;
+        vb = f - r;
+    } else {
+        const bool redmaj = __builtin_signbitf(sc.x);
+        const Shift<R> me{R(redmaj ? -sc.x : sc.x), redmaj};
+        R gc = 0;
+        if (CX[I] != 0) { R t = CX[I] == 1 ? R(sc.y) : -R(sc.y); gc += (flip & 1u) ? -t : t; }
+        if (CY[I] != 0) { R t = CY[I] == 1 ? R(sc.z) : -R(sc.z); gc += (flip & 2u) ? -t : t; }
+        if (CZ[I] != 0) { R t = CZ[I] == 1 ? R(sc.w) : -R(sc.w); gc += (flip & 4u) ? -t : t; }
+        const R wr = R(W[I]) * rho0, gd = R(W[I]) * gc;
+        const R dsh = red_shift_diff(me, shift_of<R>(arw));
+        if (redmaj) {
+            vb = me.hm * f - wr * dsh - gd;
+            vr = f - vb;
+        } else {
+            vr = me.hm * f + wr * dsh + gd;
+            vb = f - vr;
+        }
+    }
+}
+
 // Moments of the gathered populations.
 //  FP64: calmacros (S:792-799) operation for operation.
 //  shifted: compensated sums of the deviations.
@@ -589,8 +781,21 @@ __device__ __forceinline__ bool node_moments(const Lat &L, const void *__restric
     Node<R, SHIFT> nd;
     nt = 0;
     if constexpr (EXT) {
-        gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd, wall_bb_mask(E, L, x, y, z),
-                                 mirror_adjacent(E, L, x, y, z));
+        const unsigned bb = wall_bb_mask(E, L, x, y, z), madj = mirror_adjacent(E, L, x, y, z);
+        if constexpr (LIN == FS) {
+            if constexpr (SHIFT) ax = R(at(phio, n));
+            gather_fs_gen<S, R, SHIFT>(
+                (const S *)p0, (const S *)E.rbi[0], (const S *)E.rbi[1], (size_t)L.N, xo, yo, zo, bb, madj,
+                [&](unsigned m, const int *) {
+                    if constexpr (SHIFT)
+                        return at((const float4 *)p1, m);
+                    else
+                        return fs64_at((const double *)p1, m);
+                },
+                ax, nd, R(L.rho0));
+        } else {
+            gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd, bb, madj);
+        }
         if (E.ntype) nt = at(E.ntype, n);
     } else {
         gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd);
@@ -604,6 +809,26 @@ __device__ __forceinline__ bool node_moments(const Lat &L, const void *__restric
         }
     }
     return true;
+}
+
+// The macroscopic fields of a node in the particles' band (NEAR), which the
+// particle kernels read (FP64): rho and the velocity everywhere in the band,
+// the colour densities at fluid nodes (calmacros leaves them at solid nodes).
+template <class R, bool SHIFT>
+__device__ __forceinline__ void band_macros(const Lat &L, const Ext &E, unsigned n, unsigned nt,
+                                            const Moments<R, SHIFT> &mo)
+{
+    if (nt & NT_NEAR) {
+        const size_t N = (size_t)L.N;
+        E.mac[n] = mo.rho;
+        if (!(nt & NT_SOLID)) {
+            E.mac[N + n] = mo.sr;
+            E.mac[2 * N + n] = mo.sb;
+        }
+        E.mac[3 * N + n] = mo.ux;
+        E.mac[4 * N + n] = mo.uy;
+        E.mac[5 * N + n] = mo.uz;
+    }
 }
 
 template <int LIN, class S, class R, class P, bool SHIFT, bool FULL, bool EXT>
@@ -629,19 +854,7 @@ __device__ __forceinline__ void moments_node(const Lat &L, const void *__restric
         }
         return;
     }
-    if constexpr (EXT) {
-        if (nt & NT_NEAR) {
-            const size_t N = (size_t)L.N;
-            E.mac[n] = mo.rho;
-            if (!(nt & NT_SOLID)) {
-                E.mac[N + n] = mo.sr;
-                E.mac[2 * N + n] = mo.sb;
-            }
-            E.mac[3 * N + n] = mo.ux;
-            E.mac[4 * N + n] = mo.uy;
-            E.mac[5 * N + n] = mo.uz;
-        }
-    }
+    if constexpr (EXT) band_macros(L, E, n, nt, mo);
     at(phi, n) = P(mo.phi);
     if constexpr (FULL) {
         const size_t N = (size_t)L.N;
@@ -677,11 +890,15 @@ template <int ROWS> struct FsxRing<false, ROWS> {
 };
 
 // ENC: write the signed minority fraction (the scatter form's phase-field
-// buffer) instead of phi.
-template <class S, class R, class P, bool SHIFT, bool ENC = false>
+// buffer) instead of phi.  EXT: walls, mirrors and particles as
+// node_moments handles them (no phi on ghost layers; solid nodes' fields;
+// the band's FP64 fields); nodes next to a ghost layer or in the particles'
+// band take the general gather.
+template <class S, class R, class P, bool SHIFT, bool ENC = false, bool EXT = false>
 __device__ __forceinline__ void moments_fsx_tile(const Lat &L, const void *__restrict__ p0,
                                                  const void *__restrict__ p1, const P *__restrict__ phio,
-                                                 P *__restrict__ phi, int xc, unsigned bx, unsigned by, unsigned bz)
+                                                 P *__restrict__ phi, int xc, unsigned bx, unsigned by, unsigned bz,
+                                                 const Ext &E = Ext{})
 {
     __shared__ FsxRing<SHIFT> sm;
     const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * TZ + tz;
@@ -731,12 +948,44 @@ __device__ __forceinline__ void moments_fsx_tile(const Lat &L, const void *__res
                     return FsScal64{sm.s[sl][0][jy][jz], sm.s[sl][1][jy][jz], sm.s[sl][2][jy][jz],
                                     sm.s[sl][3][jy][jz], sm.s[sl][4][jy][jz]};
             };
-            gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, scal, nd, R(L.rho0));
-            const Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
-            if constexpr (ENC)
-                at(phi, xo[1] + yo[1] + zo[1]) = P(minority_q(mo.sr, mo.sb, fdiv(R(1), mo.rho)));
-            else
-                at(phi, xo[1] + yo[1] + zo[1]) = P(mo.phi);
+            const unsigned n = xo[1] + yo[1] + zo[1];
+            if constexpr (EXT) {
+                const unsigned bb = wall_bb_mask(E, L, x, y, z), madj = mirror_adjacent(E, L, x, y, z);
+                const unsigned nt = E.ntype ? at(E.ntype, n) : 0u;
+                if (!is_ghost(E, L, x, y, z)) {
+                    if (bb | madj | (nt & NT_NEAR)) {
+                        R ax = 0;
+                        if constexpr (SHIFT) ax = R(at(phio, n));
+                        auto scal_o = [&](unsigned, const int o[3]) {
+                            const int sl = (x + o[0] + 9) % 3, jy = ty + 1 + o[1], jz = tz + 1 + o[2];
+                            if constexpr (SHIFT)
+                                return sm.s[sl][jy][jz];
+                            else
+                                return FsScal64{sm.s[sl][0][jy][jz], sm.s[sl][1][jy][jz], sm.s[sl][2][jy][jz],
+                                                sm.s[sl][3][jy][jz], sm.s[sl][4][jy][jz]};
+                        };
+                        gather_fs_gen<S, R, SHIFT>(fs, (const S *)E.rbi[0], (const S *)E.rbi[1], N, xo, yo, zo, bb,
+                                                   madj, scal_o, ax, nd, R(L.rho0));
+                    } else {
+                        gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, scal, nd, R(L.rho0));
+                    }
+                    Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
+                    if (nt & NT_SOLID) { // calmacros at solid nodes (S:784-789)
+                        mo.phi = 0;
+                        mo.rho = R(L.rho0);
+                        mo.ux = mo.uy = mo.uz = 0;
+                    }
+                    band_macros(L, E, n, nt, mo);
+                    at(phi, n) = P(mo.phi);
+                }
+            } else {
+                gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, scal, nd, R(L.rho0));
+                const Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
+                if constexpr (ENC)
+                    at(phi, n) = P(minority_q(mo.sr, mo.sb, fdiv(R(1), mo.rho)));
+                else
+                    at(phi, n) = P(mo.phi);
+            }
         }
         __syncthreads(); // the next load overwrites the oldest slot
     }
@@ -749,6 +998,17 @@ __global__ __launch_bounds__(256) void k_moments_fsx(Lat L, const void *__restri
     unsigned bx, by, bz;
     tile_of_block(L, bx, by, bz);
     moments_fsx_tile<S, R, P, SHIFT, ENC>(L, p0, p1, phio, phi, xc, bx, by, bz);
+}
+
+// K1 of the factorised layout with walls, mirrors and particles.
+template <class S, class R, class P, bool SHIFT>
+__global__ __launch_bounds__(256) void k_moments_fsx_ext(Lat L, const void *__restrict__ p0,
+                                                         const void *__restrict__ p1, const P *__restrict__ phio,
+                                                         P *__restrict__ phi, int xc, Ext E)
+{
+    unsigned bx, by, bz;
+    tile_of_block(L, bx, by, bz);
+    moments_fsx_tile<S, R, P, SHIFT, false, true>(L, p0, p1, phio, phi, xc, bx, by, bz, E);
 }
 
 // ----------------------------------------------------- gradient of phi (b)
@@ -1067,15 +1327,6 @@ template <class R> __device__ __forceinline__ float encode_frac(const Recol<R> &
     return c.redmaj ? -float(c.M) : float(c.M);
 }
 
-// Red shift fraction of a node with shift sh minus that of a node with shift
-// t, with the rounding of a difference of two small fractions where both
-// nodes have the same majority colour.
-template <class R> __device__ __forceinline__ R red_shift_diff(const Shift<R> &sh, const Shift<R> &t)
-{
-    if (sh.redmaj == t.redmaj) return sh.redmaj ? t.hm - sh.hm : sh.hm - t.hm;
-    return sh.redmaj ? (R(1) - sh.hm) - t.hm : sh.hm - (R(1) - t.hm);
-}
-
 // Output writers.  RB: the recoloured red and blue populations (S:713-726);
 // shifted RB stores deviations from w_i rho0 s_red and w_i rho0 s_blue of the
 // node that reads direction i next (its phase field phw_i), the node's
@@ -1389,29 +1640,34 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                 }
             }
         }
-        // normal derivatives for the curvature (S:635-660)
+        // normal derivatives for the curvature (S:635-660), from the normals
+        // ring; with walls and particles formed after the gather, so that they
+        // are not live across it
         R d[3][3];
-        if (force_on) {
+        auto curv_d = [&]() {
+            if (force_on) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
+                for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
 #pragma unroll
-            for (int i = 1; i < Q; ++i) {
+                for (int i = 1; i < Q; ++i) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    R tmp = R(W[i]) * sm.n[k][slot3(x + CX[i])][ty + 1 + CY[i]][tz + 1 + CZ[i]];
-                    if (CX[i] == 1) d[k][0] = d[k][0] + tmp;
-                    if (CX[i] == -1) d[k][0] = d[k][0] - tmp;
-                    if (CY[i] == 1) d[k][1] = d[k][1] + tmp;
-                    if (CY[i] == -1) d[k][1] = d[k][1] - tmp;
-                    if (CZ[i] == 1) d[k][2] = d[k][2] + tmp;
-                    if (CZ[i] == -1) d[k][2] = d[k][2] - tmp;
+                    for (int k = 0; k < 3; ++k) {
+                        R tmp = R(W[i]) * sm.n[k][slot3(x + CX[i])][ty + 1 + CY[i]][tz + 1 + CZ[i]];
+                        if (CX[i] == 1) d[k][0] = d[k][0] + tmp;
+                        if (CX[i] == -1) d[k][0] = d[k][0] - tmp;
+                        if (CY[i] == 1) d[k][1] = d[k][1] + tmp;
+                        if (CY[i] == -1) d[k][1] = d[k][1] - tmp;
+                        if (CZ[i] == 1) d[k][2] = d[k][2] + tmp;
+                        if (CZ[i] == -1) d[k][2] = d[k][2] - tmp;
+                    }
                 }
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
             }
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
-        }
+        };
+        if constexpr (!EXT) curv_d();
 
         R ax = 0, aw[Q]; // phase fields fixing the shift fractions: this node's of the stored
                          // state, and aw_i of the node that reads direction i next
@@ -1451,6 +1707,65 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                     StoreOf<S>::st(grp(ro, i, N), kd, rv);
                     StoreOf<S>::st(grp(bo, i, N), kd, bv);
                 });
+                continue;
+            }
+        }
+        if constexpr (EXT && !(LIN == RB && LOUT == RB)) {
+            if (nt & NT_SOLID) {
+                // solid nodes are not collided (S:622, S:712, S:743): the
+                // slots they pull (bounce-back off walls, mirror images as
+                // live nodes pull) go to the red/blue side arrays of the
+                // factorised output, and FS_SOLID scalars mark the node
+                const size_t N = (size_t)L.N;
+                const R rho0 = R(L.rho0);
+                const S *rs = (const S *)(LIN == RB ? i0 : E.rbi[0]), *bs = (const S *)(LIN == RB ? i1 : E.rbi[1]);
+                S *ro = (S *)E.rbo[0], *bo = (S *)E.rbo[1];
+                R axs = 0, aself = 0;
+                if constexpr (SHIFT) {
+                    axs = R(at(phio, n));
+                    aself = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
+                }
+                static_for<Q>([&](auto ic) {
+                    constexpr int i = decltype(ic)::value;
+                    unsigned m, flip;
+                    int d, o[3];
+                    pull_slot_f<i>(xo, yo, zo, bb, madj, m, d, flip, o);
+                    // phase field of the node that reads slot i next (x itself
+                    // where the slot bounces back off a wall)
+                    R arw = 0;
+                    if constexpr (SHIFT)
+                        arw = ((bb >> OPC[i]) & 1u) ? aself
+                                                    : R(sm.phi[slot5(x + CX[i])][ty + 2 + CY[i]][tz + 2 + CZ[i]]);
+                    R vr, vb;
+                    if constexpr (LIN == RB) {
+                        const unsigned ks = grp_off(d, N, m);
+                        vr = StoreOf<S>::ld(grp(rs, d, N), ks);
+                        vb = StoreOf<S>::ld(grp(bs, d, N), ks);
+                        if constexpr (SHIFT) {
+                            const R dsh = R(W[i]) * rho0 * red_shift_diff(shift_of<R>(axs), shift_of<R>(arw));
+                            vr += dsh;
+                            vb -= dsh;
+                        }
+                    } else if constexpr (RING) {
+                        fs_slot_rb<i, S, R, SHIFT>((const S *)i0, rs, bs, N, m, d, flip,
+                                                   ring.s[(x + o[0] + 9) % 3][ty + 1 + o[1]][tz + 1 + o[2]], axs, arw,
+                                                   rho0, vr, vb);
+                    } else {
+                        fs_slot_rb<i, S, R, SHIFT>((const S *)i0, rs, bs, N, m, d, flip,
+                                                   fs64_at((const double *)i1, m), axs, arw, rho0, vr, vb);
+                    }
+                    const unsigned kd = grp_off(i, N, n);
+                    StoreOf<S>::st(grp(ro, i, N), kd, vr);
+                    StoreOf<S>::st(grp(bo, i, N), kd, vb);
+                });
+                if constexpr (SHIFT) {
+                    at((float4 *)o1, n) = make_float4(FS_SOLID, 0.f, 0.f, 0.f);
+                } else {
+                    double2 *sc = (double2 *)o1;
+                    at(sc, 3 * n) = make_double2(double(FS_SOLID), 0.0);
+                    at(sc, 3 * n + 1) = make_double2(0.0, 0.0);
+                    at(sc, 3 * n + 2) = make_double2(0.0, 0.0);
+                }
                 continue;
             }
         }
@@ -1500,13 +1815,34 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
             mo.rho = R(E.mac[n]);
             mo.phi = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
         } else {
-            if constexpr (RING)
-                gather_fs<S, R, SHIFT>(
-                    (const S *)i0, (size_t)L.N, xo, yo, zo,
-                    [&](int i, unsigned) { return ring.s[(x - CX[i] + 9) % 3][ty + 1 - CY[i]][tz + 1 - CZ[i]]; },
-                    nd, R(L.rho0));
-            else
+            const size_t N = (size_t)L.N;
+            if constexpr (RING) {
+                auto rscal = [&](int i, unsigned) {
+                    return ring.s[(x - CX[i] + 9) % 3][ty + 1 - CY[i]][tz + 1 - CZ[i]];
+                };
+                if constexpr (EXT) {
+                    // next to a ghost layer or in the particles' band: the
+                    // general gather (bounce-back, mirrors, solid side arrays)
+                    if (bb | madj | (nt & NT_NEAR)) {
+                        gather_fs_gen<S, R, SHIFT>(
+                            (const S *)i0, (const S *)E.rbi[0], (const S *)E.rbi[1], N, xo, yo, zo, bb, madj,
+                            [&](unsigned, const int o[3]) {
+                                return ring.s[(x + o[0] + 9) % 3][ty + 1 + o[1]][tz + 1 + o[2]];
+                            },
+                            R(at(phio, n)), nd, R(L.rho0));
+                    } else {
+                        gather_fs<S, R, SHIFT>((const S *)i0, N, xo, yo, zo, rscal, nd, R(L.rho0));
+                    }
+                } else {
+                    gather_fs<S, R, SHIFT>((const S *)i0, N, xo, yo, zo, rscal, nd, R(L.rho0));
+                }
+            } else if constexpr (EXT && LIN == FS) {
+                gather_fs_gen<S, R, SHIFT>(
+                    (const S *)i0, (const S *)E.rbi[0], (const S *)E.rbi[1], N, xo, yo, zo, bb, madj,
+                    [&](unsigned m, const int *) { return fs64_at((const double *)i1, m); }, ax, nd, R(L.rho0));
+            } else {
                 gather<LIN, S, R, SHIFT>(L, i0, i1, xo, yo, zo, ax, nd, bb, madj);
+            }
             mo = moments<R, SHIFT>(L, nd);
         }
         const R phin = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
@@ -1518,6 +1854,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
 #pragma unroll
             for (int i = 0; i < Q; ++i) f[i] = nd.ft[i];
         }
+        if constexpr (EXT) curv_d();
         collide<R, SHIFT>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, force_on, d);
         const Recol<R> rc = recol<R, SHIFT>(L, mo, flag);
         write_node<LOUT, S, R, SHIFT>(L, o0, o1, n, f, rc, nx, ny, nz, aw);
@@ -1525,7 +1862,7 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
 }
 
 template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT = false>
-__global__ __launch_bounds__(TZ * KTY<R>, KWAVES<R>) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
+__global__ __launch_bounds__(TZ * KTY<R>, (EXT && LIN == FS) ? KWAVES_FSX : KWAVES<R>) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
                                                  void *__restrict__ o0, void *__restrict__ o1,
                                                  const P *__restrict__ phi, const P *__restrict__ phio, int xc,
                                                  Ext E = Ext{})
@@ -1925,6 +2262,76 @@ __global__ __launch_bounds__(256) void k_unpack_fs(Lat L, const S *__restrict__ 
         }
         aos[(size_t)n * Q + i] = red ? r : f - r;
     }
+}
+
+// Unpack the factorised state with walls, mirrors and particles: the slot
+// each live node gathers (pull_slot), rebuilt from the recolour scalars of
+// its node, or taken from the side arrays rs/bs where that node is FS_SOLID;
+// zero on ghost layers (as k_unpack).
+template <class S, class P, bool SHIFT>
+__global__ __launch_bounds__(256) void k_unpack_fs_ext(Lat L, Ext E, const S *__restrict__ fs,
+                                                       const void *__restrict__ scv, const S *__restrict__ rs,
+                                                       const S *__restrict__ bs, const P *__restrict__ phio,
+                                                       double *__restrict__ aos, int red)
+{
+    const int z = blockIdx.x * TZ + threadIdx.x;
+    const int y = blockIdx.y * TY + threadIdx.y;
+    const int x = blockIdx.z;
+    if (z >= L.nz || y >= L.ny) return;
+    unsigned xo[3], yo[3], zo[3];
+    node_offsets(L, x, y, z, xo, yo, zo);
+    const unsigned n = xo[1] + yo[1] + zo[1];
+    const size_t N = (size_t)L.N;
+    if (is_ghost(E, L, x, y, z)) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) aos[(size_t)n * Q + i] = 0.0;
+        return;
+    }
+    const unsigned bb = wall_bb_mask(E, L, x, y, z);
+    const unsigned madj = mirror_adjacent(E, L, x, y, z);
+    const S *cs = red ? rs : bs;
+    static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        unsigned m, flip;
+        int d, o[3];
+        pull_slot_f<i>(xo, yo, zo, bb, madj, m, d, flip, o);
+        const size_t k = (size_t)d * N + m;
+        double v;
+        if constexpr (!SHIFT) {
+            const FsScal64 sc = fs64_at((const double *)scv, m);
+            if (solid_scal(sc)) {
+                v = double(StoreOf<S>::ld(cs + k));
+            } else {
+                const double f = double(StoreOf<S>::ld(fs + k));
+                double r = sc.a * f;
+                if (i > 0) r = r + (sc.g * W[i]) * dot_cf<i, double>(flip, sc.nx, sc.ny, sc.nz);
+                v = red ? r : f - r;
+            }
+        } else {
+            const float4 g = ((const float4 *)scv)[m];
+            if (solid_scal(g)) {
+                v = double(StoreOf<S>::ld(cs + k)) + W[i] * (L.rho0 * shift_d(float(phio[n]), red));
+            } else {
+                double A, B;
+                decode_frac(g.x, A, B); // exact in FP64
+                const double f = double(StoreOf<S>::ld(fs + k)) + W[i] * L.rho0;
+                double gc = 0.0;
+                if (CX[i] != 0) { double t = CX[i] * double(g.y); gc += (flip & 1u) ? -t : t; }
+                if (CY[i] != 0) { double t = CY[i] * double(g.z); gc += (flip & 2u) ? -t : t; }
+                if (CZ[i] != 0) { double t = CZ[i] * double(g.w); gc += (flip & 4u) ? -t : t; }
+                const double gd = W[i] * gc;
+                double r;
+                if (__builtin_signbitf(g.x)) { // red majority: the blue part first
+                    const double b = B * f - gd;
+                    r = f - b;
+                } else {
+                    r = A * f + gd;
+                }
+                v = red ? r : f - r;
+            }
+        }
+        aos[(size_t)n * Q + i] = v;
+    });
 }
 
 // ------------------------------------------------- device initial condition

@@ -539,6 +539,60 @@ __global__ __launch_bounds__(PB) void k_curve(Lat L, Ext E, Part Pt, const unsig
     }
 }
 
+// k_curve for the factorised layout: the node's own post-collision slot opp
+// is rebuilt from its recolour scalars (fo, so: the step's output), and the
+// bounce-back value goes, as red and blue, into the side arrays ro, bo of
+// the solid slot the next gather reads (K2 marked that node FS_SOLID).
+// Shifted storage: deviations from w_i rho0 s_red and w_i rho0 s_blue of x,
+// formed in FP64 from the colour fractions of x.
+template <class S, bool SHIFT>
+__global__ __launch_bounds__(PB) void k_curve_fs(Lat L, Ext E, Part Pt, const S *fo, const void *so, S *ro, S *bo,
+                                                 const float *phis)
+{
+    const int p = blockIdx.x;
+    const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
+    const size_t N = (size_t)L.N;
+    FOR_LIST(Pt, 0, j, lb)
+    {
+        int X[3];
+        unsigned n;
+        box_node(L, b, Pt.lout[lb + j], X, n);
+        const unsigned links = Pt.lmask[lb + j];
+        double xr[3], uw[3];
+        wall_vel(Pt, p, Pt.center + 3 * p, X, xr, uw);
+        const double rr = E.mac[N + n], rbv = E.mac[2 * N + n];
+        for (int i = 1; i < Q; ++i) {
+            if (!((links >> i) & 1u)) continue;
+            unsigned ms;
+            int ds;
+            read_slot(E, L, X, i, ms, ds);
+            const double star = uw[0] * CX[i] + uw[1] * CY[i] + uw[2] * CZ[i];
+            const int o = OPC[i];
+            const double f = double(StoreOf<S>::ld(fo + (size_t)o * N, n));
+            double vr, vb;
+            if constexpr (SHIFT) {
+                const float4 g = ((const float4 *)so)[n];
+                double A, B;
+                decode_frac(g.x, A, B);
+                const double gc = W[o] * (CX[o] * double(g.y) + CY[o] * double(g.z) + CZ[o] * double(g.w));
+                const double wr = W[o] * L.rho0;
+                vr = A * f + wr * (A - shift_d(phis[n], 1)) + gc;
+                vb = B * f + wr * (B - shift_d(phis[n], 0)) - gc;
+                vr = vr + 6.0 * W[i] * rr * star;
+                vb = vb + 6.0 * W[i] * rbv * star;
+            } else {
+                const FsScal64 sc = fs64_at((const double *)so, n);
+                double r = sc.a * f;
+                r = r + (sc.g * W[o]) * dot_c<double>(o, sc.nx, sc.ny, sc.nz);
+                vr = r + 6.0 * W[i] * rr * star;
+                vb = (f - r) + 6.0 * W[i] * rbv * star;
+            }
+            StoreOf<S>::st(ro + (size_t)ds * N, ms, (decltype(StoreOf<S>::ld(ro)))vr);
+            StoreOf<S>::st(bo + (size_t)ds * N, ms, (decltype(StoreOf<S>::ld(bo)))vb);
+        }
+    }
+}
+
 // Fixed-order FP64 sum of K values per thread over the workgroup: an xor
 // butterfly of wavefront shuffles (every lane ends with its wave's sum; the
 // pairing is the same every run, and addition commutes, so all lanes agree),
@@ -593,6 +647,67 @@ __global__ __launch_bounds__(PB) void k_momentum(Lat L, Ext E, Part Pt, const un
             if (cold) read_slot(E, L, X, o, no, dof);
             double fi = double(StoreOf<S>::ld(rs + (size_t)di * N, mi)) + double(StoreOf<S>::ld(bs + (size_t)di * N, mi));
             double fo = double(StoreOf<S>::ld(rs + (size_t)dof * N, no)) + double(StoreOf<S>::ld(bs + (size_t)dof * N, no));
+            if constexpr (SHIFT) {
+                fi += W[i] * L.rho0;
+                fo += W[o] * L.rho0;
+            }
+            const double fmx = ((double)CX[i] - uw[0]) * fi - ((double)CX[o] - uw[0]) * fo;
+            const double fmy = ((double)CY[i] - uw[1]) * fi - ((double)CY[o] - uw[1]) * fo;
+            const double fmz = ((double)CZ[i] - uw[2]) * fi - ((double)CZ[o] - uw[2]) * fo;
+            acc[0] -= fmx;
+            acc[1] -= fmy;
+            acc[2] -= fmz;
+            acc[3] -= xr[1] * fmz - xr[2] * fmy;
+            acc[4] -= xr[2] * fmx - xr[0] * fmz;
+            acc[5] -= xr[0] * fmy - xr[1] * fmx;
+        }
+    }
+    block_sum<6>(acc, Pt.part_mom + ((size_t)p * LSLICES + blockIdx.y) * 6);
+}
+
+// k_momentum for the factorised layout: the bounced-back f_i(x) is the red
+// plus blue value in the side arrays rs, bs of the solid slot, ffi_opp(x)
+// the node's own stored post-collision total fs.  Cold start: the slot that x
+// gathers for opp, from the side arrays where its node is FS_SOLID.
+template <class S, bool SHIFT>
+__global__ __launch_bounds__(PB) void k_momentum_fs(Lat L, Ext E, Part Pt, const S *fs, const void *sc, const S *rs,
+                                                    const S *bs, int cold)
+{
+    const int p = blockIdx.x;
+    const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
+    const size_t N = (size_t)L.N;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    FOR_LIST(Pt, 0, j, lb)
+    {
+        int X[3];
+        unsigned n;
+        box_node(L, b, Pt.lout[lb + j], X, n);
+        const unsigned links = Pt.lmask[lb + j];
+        double xr[3], uw[3];
+        wall_vel(Pt, p, Pt.center + 3 * p, X, xr, uw);
+        for (int i = 1; i < Q; ++i) {
+            if (!((links >> i) & 1u)) continue;
+            const int o = OPC[i];
+            unsigned mi;
+            int di;
+            read_slot(E, L, X, i, mi, di);
+            double fi = double(StoreOf<S>::ld(rs + (size_t)di * N, mi)) + double(StoreOf<S>::ld(bs + (size_t)di * N, mi));
+            double fo;
+            if (cold) {
+                unsigned no;
+                int dof;
+                read_slot(E, L, X, o, no, dof);
+                bool solid;
+                if constexpr (SHIFT)
+                    solid = solid_scal(((const float4 *)sc)[no]);
+                else
+                    solid = solid_scal(fs64_at((const double *)sc, no));
+                fo = solid ? double(StoreOf<S>::ld(rs + (size_t)dof * N, no)) +
+                                 double(StoreOf<S>::ld(bs + (size_t)dof * N, no))
+                           : double(StoreOf<S>::ld(fs + (size_t)dof * N, no));
+            } else {
+                fo = double(StoreOf<S>::ld(fs + (size_t)o * N, n));
+            }
             if constexpr (SHIFT) {
                 fi += W[i] * L.rho0;
                 fo += W[o] * L.rho0;

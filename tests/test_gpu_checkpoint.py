@@ -52,16 +52,18 @@ def test_fluid_restart_bit_exact(tmp_path, precision, layout):
     assert all(np.array_equal(ma[k], mb[k]) for k in ma)
 
 
-def test_particles_restart_bit_exact(tmp_path):
+@pytest.mark.parametrize("precision,layout", [("f64", "rb"), ("f64", "fs"), ("f32", "fs")])
+def test_particles_restart_bit_exact(tmp_path, precision, layout):
     cfg, parts, ri, bi = _particle_setup()
-    a = _lattice_particles(cfg, parts, ri, bi)
+    _print("particle restart", precision, layout)
+    a = _lattice_particles(cfg, parts, ri, bi, precision, layout=layout)
     a.step(1)  # past the cold start
     path = tmp_path / "p.bin"
     a.save(path)
     a.step(8)
-    b = symmetry_amd.Lattice(cfg["nx"], cfg["ny"], cfg["nz"], precision="f64", sigma=cfg["sigma"], rho0=cfg["rho0"],
-                             nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], bc=cfg["bc"],
-                             npar=len(parts["rad"]))
+    b = symmetry_amd.Lattice(cfg["nx"], cfg["ny"], cfg["nz"], precision=precision, sigma=cfg["sigma"],
+                             rho0=cfg["rho0"], nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], bc=cfg["bc"],
+                             npar=len(parts["rad"]), layout=layout)
     b.load(path)
     b.step(8)
     assert _same(a.download(), b.download())

@@ -1,7 +1,9 @@
 """GPU parity with wall faces and moving particles, against the FP64 oracle.
 
 * Wall faces only: the FP64 library reproduces the oracle bit for bit
-  (half-way bounce-back is a copy, S:1544-1592).
+  (half-way bounce-back is a copy, S:1544-1592), in the red/blue layout and
+  in the factorised layout (whose solid nodes keep their slots in red/blue
+  side arrays).
 * Moving particles: masks (walls, walls_all, pin, pout; set_walls S:326-395)
   must match exactly; fields and particle trajectories match within a
   tolerance, because the FP64 force/torque sums are block reductions on the
@@ -79,22 +81,23 @@ def test_wall_roundtrip(case):
     _print("wall roundtrip", case)
     ri, bi, _ = ic.flat_interface_state(cfg["nx"], cfg["ny"], cfg["nz"], z0=-1e3, rho0=1.0, drop=drop,
                                         sigma=cfg["sigma"])
-    lat = _lattice(cfg)
+    lat = _lattice(cfg, layout="fs")
     lat.upload(ri, bi)
     r2, b2 = lat.download()
     live = _live(ri.shape[:3], cfg["bc"])
     assert np.array_equal(r2[live], ri[live]) and np.array_equal(b2[live], bi[live])
 
 
+@pytest.mark.parametrize("layout", ["rb", "fs"])
 @pytest.mark.parametrize("nsteps", [1, 15])
 @pytest.mark.parametrize("case", sorted(WALL_CASES))
-def test_walls_f64_bit_exact(case, nsteps):
+def test_walls_f64_bit_exact(case, nsteps, layout):
     """A red drop touching the wall(s), relaxing for nsteps steps."""
     cfg, drop = WALL_CASES[case]
-    _print("walls f64", case, nsteps)
+    _print("walls f64", case, nsteps, layout)
     ri, bi, _ = ic.flat_interface_state(cfg["nx"], cfg["ny"], cfg["nz"], z0=-1e3, rho0=1.0, drop=drop,
                                         sigma=cfg["sigma"])
-    lat = _lattice(cfg)
+    lat = _lattice(cfg, layout=layout)
     lat.upload(ri, bi)
     lat.step(nsteps)
     r2, b2 = lat.download()
@@ -113,16 +116,17 @@ def test_walls_f64_bit_exact(case, nsteps):
     assert np.array_equal(mac["phi"], st.phi)
 
 
+@pytest.mark.parametrize("layout", ["rb", "fs"])
 @pytest.mark.parametrize("case", ["ref", "mirror", "z"])
-def test_wall_derivatives_match_oracle(case):
+def test_wall_derivatives_match_oracle(case, layout):
     """Gradient, normal and flag of the derivatives stage (S:452-597) at every
     node, ghost layers included: zero at wall ghosts, the reflected copy of
     the image at mirror ghosts (S:589-594)."""
     cfg, drop = WALL_CASES[case]
-    _print("wall derivatives", case)
+    _print("wall derivatives", case, layout)
     ri, bi, _ = ic.flat_interface_state(cfg["nx"], cfg["ny"], cfg["nz"], z0=-1e3, rho0=1.0, drop=drop,
                                         sigma=cfg["sigma"])
-    lat = _lattice(cfg)
+    lat = _lattice(cfg, layout=layout)
     lat.upload(ri, bi)
     lat.step(5)
     st = _oracle(cfg, ri, bi)
@@ -135,7 +139,8 @@ def test_wall_derivatives_match_oracle(case):
     assert np.array_equal(der["flag"], st.flag)
 
 
-def test_walls_f32_close_to_f64():
+@pytest.mark.parametrize("layout", ["rb", "fs"])
+def test_walls_f32_close_to_f64(layout):
     """A perturbed flat interface between z walls relaxing as a capillary
     wave, 200 steps: FP32 phi within 1e-5 (L2-relative) of the FP64 path and
     velocity within 3e-5.  The wave's velocities are ~1e-3, so the FP32
@@ -147,11 +152,11 @@ def test_walls_f32_close_to_f64():
     legitimately part; where a colour density falls below the evaporation
     limit 1e-8 (S:716-725) the recolouring switches discontinuously."""
     cfg = _wall_cfg(32, 16, 24, (0, 0, 0, 0, 1, 1))
-    _print("walls f32 vs f64")
+    _print("walls f32 vs f64", layout)
     ri, bi, _ = ic.flat_interface_state(cfg["nx"], cfg["ny"], cfg["nz"], z0=12.5, amp=2.0, rho0=1.0)
     out = {}
     for prec in ("f64", "f32"):
-        lat = _lattice(cfg, prec)
+        lat = _lattice(cfg, prec, layout="rb" if prec == "f64" else layout)
         lat.upload(ri, bi)
         lat.step(200)
         out[prec] = lat.calmacros()
@@ -200,8 +205,8 @@ def _oracle_particles(cfg, parts, ri, bi):
     return st
 
 
-def _lattice_particles(cfg, parts, ri, bi, precision="f64"):
-    lat = _lattice(cfg, precision, npar=len(parts["rad"]))
+def _lattice_particles(cfg, parts, ri, bi, precision="f64", layout="rb"):
+    lat = _lattice(cfg, precision, npar=len(parts["rad"]), layout=layout)
     lat.upload(ri, bi)
     lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"], parts["fg"],
                       up=parts["up"], rp=parts["rp"])
@@ -224,14 +229,15 @@ def test_particle_masks_at_start():
     assert np.count_nonzero(m["pout"]) > 0 and np.count_nonzero(m["pin"]) > 0
 
 
+@pytest.mark.parametrize("layout", ["rb", "fs"])
 @pytest.mark.parametrize("nsteps", [1, 12])
-def test_particle_steps_f64(nsteps):
+def test_particle_steps_f64(nsteps, layout):
     """Moving sphere (pushed so that nodes are covered and uncovered) at a
     flat interface between z walls: masks exact, fields and trajectory to
     rounding of the force sums."""
     cfg, parts, ri, bi = _particle_setup()
-    _print("particle f64 steps", nsteps)
-    lat = _lattice_particles(cfg, parts, ri, bi)
+    _print("particle f64 steps", nsteps, layout)
+    lat = _lattice_particles(cfg, parts, ri, bi, layout=layout)
     st = _oracle_particles(cfg, parts, ri, bi)
     lat.step(nsteps)
     st.step(nsteps, particles=True)
@@ -276,12 +282,13 @@ def test_particle_refill_happens():
     assert refilled > 0
 
 
-def test_particle_f32_close_to_f64():
+@pytest.mark.parametrize("layout", ["rb", "fs"])
+def test_particle_f32_close_to_f64(layout):
     cfg, parts, ri, bi = _particle_setup()
-    _print("particle f32 vs f64")
+    _print("particle f32 vs f64", layout)
     res = {}
     for prec in ("f64", "f32"):
-        lat = _lattice_particles(cfg, parts, ri, bi, prec)
+        lat = _lattice_particles(cfg, parts, ri, bi, prec, layout="rb" if prec == "f64" else layout)
         lat.step(30)
         res[prec] = (lat.calmacros(), lat.get_particles(), lat.masks())
     assert np.array_equal(res["f32"][2]["walls_all"], res["f64"][2]["walls_all"])
@@ -294,17 +301,18 @@ def test_particle_f32_close_to_f64():
     assert dc < 1e-5
 
 
-def test_particle_trajectory_vs_oracle_and_f32():
+@pytest.mark.parametrize("layout", ["rb", "fs"])
+def test_particle_trajectory_vs_oracle_and_f32(layout):
     """A sphere released at rest at the interface (offset from the lattice so
     no node sits exactly on its surface): FP64 centre and velocity against the
     oracle after 50 steps, FP32 against FP64 after 300 steps."""
     cfg, parts, ri, bi = _particle_setup(up=(0.0, 0.0, 0.0), rp=(0.0, 0.0, 0.0))
-    _print("particle trajectory")
+    _print("particle trajectory", layout)
     st = _oracle_particles(cfg, parts, ri, bi)
     st.step(50, particles=True)
     traj = {}
     for prec in ("f64", "f32"):
-        lat = _lattice_particles(cfg, parts, ri, bi, prec)
+        lat = _lattice_particles(cfg, parts, ri, bi, prec, layout=layout)
         lat.step(50)
         traj[prec, 50] = lat.get_particles()
         lat.step(250)

@@ -76,10 +76,10 @@ def _oracle(cfg, parts, ri, bi, geo):
     return st
 
 
-def _lattice(cfg, parts, ri, bi, geo, precision="f64"):
+def _lattice(cfg, parts, ri, bi, geo, precision="f64", layout="rb"):
     lat = symmetry_amd.Lattice(cfg["nx"], cfg["ny"], cfg["nz"], precision=precision, sigma=cfg["sigma"],
                                rho0=cfg["rho0"], nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"],
-                               bc=cfg["bc"], npar=len(parts["rad"]))
+                               bc=cfg["bc"], npar=len(parts["rad"]), layout=layout)
     lat.upload(ri, bi)
     lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"], parts["fg"],
                       up=parts["up"], rp=parts["rp"])
@@ -103,11 +103,12 @@ def test_mirror_masks_at_start():
     assert np.count_nonzero(m["walls"][-1, :, :] == 1) > 0 and np.count_nonzero(m["walls"][:, 0, :] == 2) > 0
 
 
+@pytest.mark.parametrize("layout", ["rb", "fs"])
 @pytest.mark.parametrize("nsteps", [1, 12])
-def test_mirror_particle_steps_f64(nsteps):
+def test_mirror_particle_steps_f64(nsteps, layout):
     cfg, parts, ri, bi, geo = _setup()
-    _print("mirror particle f64 steps", nsteps)
-    lat = _lattice(cfg, parts, ri, bi, geo)
+    _print("mirror particle f64 steps", nsteps, layout)
+    lat = _lattice(cfg, parts, ri, bi, geo, layout=layout)
     st = _oracle(cfg, parts, ri, bi, geo)
     lat.step(nsteps)
     st.step(nsteps, particles=True)
@@ -150,12 +151,13 @@ def test_mirror_particles_refill_happens():
     assert refilled > 0
 
 
-def test_mirror_particles_f32_close_to_f64():
+@pytest.mark.parametrize("layout", ["rb", "fs"])
+def test_mirror_particles_f32_close_to_f64(layout):
     cfg, parts, ri, bi, geo = _setup()
-    _print("mirror particle f32 vs f64")
+    _print("mirror particle f32 vs f64", layout)
     res = {}
     for prec in ("f64", "f32"):
-        lat = _lattice(cfg, parts, ri, bi, geo, prec)
+        lat = _lattice(cfg, parts, ri, bi, geo, prec, layout="rb" if prec == "f64" else layout)
         lat.step(30)
         res[prec] = (lat.calmacros(), lat.get_particles(), lat.masks())
     assert np.array_equal(res["f32"][2]["walls_all"], res["f64"][2]["walls_all"])
