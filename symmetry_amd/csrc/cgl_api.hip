@@ -360,9 +360,16 @@ static int alloc_handle(cgl_t **out, const cgl_params *p, size_t N, int walls)
     // initial states) plus the factorised double buffer; with walls or
     // particles both RB buffers, which then hold the red/blue side arrays of
     // the factorised states (the slots of solid nodes).
+    // Shifted storage keeps one FP32 colour fraction per slot there (the red
+    // array, at least 4 B per value); FP64 red and blue values.
     const int nrb = p->layout == CGL_LAYOUT_RB || h->ext ? 2 : 1;
+    const bool fsx = p->layout == CGL_LAYOUT_FS && h->ext, fsx_frac = fsx && p->precision != CGL_PREC_F64;
     for (int b = 0; b < nrb && e == hipSuccess; ++b)
-        for (int c = 0; c < 2 && e == hipSuccess; ++c) e = dmalloc(&h->pops[b][c], (size_t)cgl::Q * h->N * h->esize, PZ_POPS, h->stream);
+        for (int c = 0; c < 2 && e == hipSuccess; ++c) {
+            if (fsx_frac && b == 1 && c == 1) continue; // no blue side array
+            const size_t es = fsx_frac && c == 0 && h->esize < 4 ? 4 : h->esize;
+            e = dmalloc(&h->pops[b][c], (size_t)cgl::Q * h->N * es, PZ_POPS, h->stream);
+        }
     if (p->layout == CGL_LAYOUT_FS) {
         const size_t ns = p->precision == CGL_PREC_F64 ? 6 * sizeof(double) : 4 * sizeof(float);
         for (int b = 0; b < 2 && e == hipSuccess; ++b) e = dmalloc(&h->fsf[b], (size_t)cgl::Q * h->N * h->esize, PZ_FS, h->stream);
@@ -1454,8 +1461,12 @@ static std::vector<std::pair<void *, size_t>> ckpt_sections(cgl_handle *h)
         v.push_back({h->fsf[h->cur], pops});
         v.push_back({h->fss[h->cur], N * ns});
         if (h->ext) { // the side arrays of the solid nodes
-            v.push_back({h->pops[h->cur][0], pops});
-            v.push_back({h->pops[h->cur][1], pops});
+            if (h->p.precision == CGL_PREC_F64) {
+                v.push_back({h->pops[h->cur][0], pops});
+                v.push_back({h->pops[h->cur][1], pops});
+            } else {
+                v.push_back({h->pops[h->cur][0], (size_t)cgl::Q * N * sizeof(float)});
+            }
         }
     }
     v.push_back({h->phi[0], N * h->phisize});

@@ -306,8 +306,8 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                 h->L, (const S *)h->pops[h->cur][red ? 0 : 1], aos, (const P *)h->phi[h->phicur], red, h->E);
         else if (h->ext)
             k_unpack_fs_ext<S, P, SHIFT><<<node_grid(h->L), node_block(), 0, h->stream>>>(
-                h->L, h->E, (const S *)h->fsf[h->cur], h->fss[h->cur], (const S *)h->pops[h->cur][0],
-                (const S *)h->pops[h->cur][1], (const P *)h->phi[h->phicur], aos, red);
+                h->L, h->E, (const S *)h->fsf[h->cur], h->fss[h->cur], h->pops[h->cur][0], h->pops[h->cur][1], aos,
+                red);
         else
             k_unpack_fs<S, SHIFT><<<node_grid(h->L), node_block(), 0, h->stream>>>(
                 h->L, (const S *)h->fsf[h->cur], h->fss[h->cur], aos, red);
@@ -325,6 +325,12 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         return E;
     }
     static dim3 pgrid(const cgl_handle *h, int slices) { return dim3(h->npar, slices, 1); }
+    static bool ghost_faces(const cgl_handle *h)
+    {
+        for (int a = 0; a < 3; ++a)
+            if (h->E.wall[a][0] || h->E.wall[a][1] || h->E.mirror[a][0] || h->E.mirror[a][1]) return true;
+        return false;
+    }
     static bool mirrors(const cgl_handle *h)
     {
         for (int a = 0; a < 3; ++a)
@@ -363,6 +369,12 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                 const int xc = K1_XC;
                 const dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, (h->L.nx + xc - 1) / xc);
                 k_moments_fsx_ext<S, R, P, SHIFT><<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, xc, E);
+                if (ghost_faces(h))
+                    k_moments_special<S, R, P, SHIFT, false>
+                        <<<face_grid(h), 256, 0, h->stream>>>(h->L, i0, i1, phio, phi, E, h->P);
+                if (h->npar)
+                    k_moments_special<S, R, P, SHIFT, true>
+                        <<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, i0, i1, phio, phi, E, h->P);
             }
         } else if (full) {
             k_moments<RB, S, R, P, SHIFT, true, true>
@@ -472,8 +484,8 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         const Ext Eo = ext_with(h, h->ntype[h->ntcur]);
         const S *rs = (const S *)h->pops[h->cur][0], *bs = (const S *)h->pops[h->cur][1];
         if (h->state_layout == FS)
-            k_momentum_fs<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(
-                h->L, Eo, h->P, (const S *)h->fsf[h->cur], h->fss[h->cur], rs, bs, h->cold);
+            k_momentum_fs<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P,
+                                                                          (const S *)h->fsf[h->cur], h->cold);
         else
             k_momentum<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, Eo, h->P, h->ntype[h->ntcur],
                                                                           h->walls_i, h->pout_i, rs, bs, h->cold);
@@ -532,16 +544,25 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         else if (h->state_layout == RB)
             k_collide<RB, FS, S, R, P, SHIFT, true>
                 <<<g2, k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
-        else
-            k_collide<FS, FS, S, R, P, SHIFT, true>
+        else {
+            // the tile kernel for all but the special lanes, then those
+            k_collide<FS, FS, S, R, P, SHIFT, false, true>
                 <<<g2, k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
+            if (ghost_faces(h))
+                k_collide_special<S, R, P, SHIFT, false>
+                    <<<face_grid(h), 256, 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, En, h->P);
+            if (h->npar)
+                k_collide_special<S, R, P, SHIFT, true>
+                    <<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, En, h->P);
+        }
         if (h->tev) (void)hipEventRecord(h->tev[3], h->stream);
         h->state_layout = h->layout;
+        if (h->layout == FS && ghost_faces(h))
+            k_fill_ghosts<S, SHIFT><<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, (S *)o0, En.rbo[0], En.rbo[1]);
         if (h->npar) {
             if (h->layout == FS)
-                k_curve_fs<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(
-                    h->L, En, h->P, (const S *)o0, o1, (S *)En.rbo[0], (S *)En.rbo[1],
-                    SHIFT ? (const float *)phi : nullptr);
+                k_curve_fs<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, (S *)o0, o1,
+                                                                              En.rbo[0], En.rbo[1]);
             else
                 k_curve<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[nb], h->walls_i,
                                                                           h->pout_i, (S *)o0, (S *)o1,

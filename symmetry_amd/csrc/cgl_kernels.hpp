@@ -50,12 +50,6 @@ template <class R> constexpr int KTY = sizeof(R) == 8 ? 4 : 8;
 // 1.2x faster than at one workgroup per CU); no bound for FP64, which runs
 // 256-thread workgroups.
 template <class R> constexpr int KWAVES = sizeof(R) == 8 ? 1 : 4;
-// The factorised step with walls and particles (its general gather: bounce-
-// back, mirror and solid-node slots) is compiled for 2 waves per SIMD: bounded
-// to 4 it spilled about 120 registers, and that build gave non-finite values
-// at some wall-adjacent nodes on the GPU (found with a NaN scan of the state,
-// not explained); at 2 it needs no spill.
-constexpr int KWAVES_FSX = 2;
 
 struct Lat {
     int nx, ny, nz;
@@ -416,6 +410,17 @@ __device__ __forceinline__ unsigned wall_bb_mask(const Ext &E, const Lat &L, int
     return m;
 }
 
+// Lanes that the tile kernels of the factorised layout with walls, mirrors or
+// particles leave to the per-node kernels (k_moments_special,
+// k_collide_special): ghost layers, nodes next to a ghost
+// layer, and the particles' band (NEAR; it contains every solid and refilled
+// node and every node next to one).
+__device__ __forceinline__ bool special_lane(const Ext &E, const Lat &L, int x, int y, int z, unsigned n)
+{
+    return is_ghost(E, L, x, y, z) || wall_bb_mask(E, L, x, y, z) || mirror_adjacent(E, L, x, y, z) ||
+           (E.ntype && (at(E.ntype, n) & NT_NEAR));
+}
+
 // Source slot of direction i at a live node in the pull layout: the upstream
 // node x - c_i, or with a wall ghost there the node's own opposite slot
 // (bounce-back, bb), or with mirror ghosts there (madj) the slot of the
@@ -506,16 +511,18 @@ __device__ __forceinline__ void gather(const Lat &L, const void *p0, const void 
     }
 }
 
-// pull_slot, with the source node's position relative to the node (o, each
-// component -1, 0 or +1: an index into the LDS rings) and the axes along
-// which the slot's direction d reverses c_i (flip bit a: the mirrored axes,
-// or all three for the bounce-back).
-template <int I>
-__device__ __forceinline__ void pull_slot_f(const unsigned xo[3], const unsigned yo[3], const unsigned zo[3],
-                                            unsigned bb, unsigned madj, unsigned &m, int &d, unsigned &flip,
-                                            int o[3])
+// The factorised layout with walls, mirrors and particles keeps every pull
+// direction-uniform: slot i of node x is read from node x - c_i, as in the
+// periodic gather.  After each step k_fill_ghosts copies into the wall and
+// mirror ghost slots the totals that the reference's boundary fills put
+// there (S:1443-1592): the reader's own opposite slot off a wall, the mirror
+// image's reflected slot across mirror faces.  The reader splits such a slot
+// with the recolour scalars of the node it came from, its own or the mirror
+// image's, in that slot's direction: c_i reversed on all axes or on the
+// mirrored ones.  fs_src gives that node's offset o from x (-1, 0, +1 per
+// axis, an index into the LDS rings) and the reversed axes (flip).
+template <int I> __device__ __forceinline__ void fs_src(unsigned bb, unsigned madj, int o[3], unsigned &flip)
 {
-    pull_slot<I>(xo, yo, zo, bb, madj, m, d);
     if ((bb >> I) & 1u) {
         flip = 7u;
         o[0] = o[1] = o[2] = 0;
@@ -527,9 +534,9 @@ __device__ __forceinline__ void pull_slot_f(const unsigned xo[3], const unsigned
     o[2] = (flip & 4u) ? 0 : -CZ[I];
 }
 
-// dot_c of the direction d = c_I with the components of the axes in `flip`
+// dot_c of the direction c_I with the components of the axes in `flip`
 // reversed: the same terms in the same order with exact sign changes, so it
-// rounds as dot_c(d, ...).
+// rounds as dot_c of the reflected direction.
 template <int I, class R> __device__ __forceinline__ R dot_cf(unsigned flip, R nx, R ny, R nz)
 {
     R dot = 0;
@@ -538,6 +545,15 @@ template <int I, class R> __device__ __forceinline__ R dot_cf(unsigned flip, R n
     if (CY[I] != 0) { R t = CY[I] == 1 ? ny : -ny; if (flip & 2u) t = -t; dot = first ? t : dot + t; first = false; }
     if (CZ[I] != 0) { R t = CZ[I] == 1 ? nz : -nz; if (flip & 4u) t = -t; dot = first ? t : dot + t; }
     return dot;
+}
+// G . c_I with the components of the axes in `flip` reversed (shifted storage).
+template <int I, class R> __device__ __forceinline__ R gdot_f(unsigned flip, float gx, float gy, float gz)
+{
+    R gc = 0;
+    if (CX[I] != 0) { R t = CX[I] == 1 ? R(gx) : -R(gx); gc += (flip & 1u) ? -t : t; }
+    if (CY[I] != 0) { R t = CY[I] == 1 ? R(gy) : -R(gy); gc += (flip & 2u) ? -t : t; }
+    if (CZ[I] != 0) { R t = CZ[I] == 1 ? R(gz) : -R(gz); gc += (flip & 4u) ? -t : t; }
+    return gc;
 }
 
 // Recolour scalars of node m in the FP64 factorised layout.
@@ -550,37 +566,60 @@ __device__ __forceinline__ FsScal64 fs64_at(const double *sc, unsigned m)
 __device__ __forceinline__ bool solid_scal(const FsScal64 &s) { return s.a > 0.5 * double(FS_SOLID); }
 __device__ __forceinline__ bool solid_scal(const float4 &s) { return s.x > 0.5f * FS_SOLID; }
 
-// Pull gather of the factorised layout with walls, mirrors and particles.
-// The slot of direction i (pull_slot: the upstream node, the node's own
-// opposite slot off a wall, or the mirror image's reflected slot) is rebuilt
-// from its node's recolour scalars as gather_fs does, with direction d; where
-// that node was solid when the state was written (FS_SOLID scalars) the slot
-// comes from the red/blue side arrays rs, bs instead: the populations that
-// stream through a solid node (S:622, S:712) and the moving-wall bounce-back
-// values of the curve links (S:1004-1018) are not a colour split of one
-// node.  In shifted storage the side arrays hold deviations from w_i rho0
-// s_red and w_i rho0 s_blue of the reading node, whose phase field of the
-// stored state is ax.  scal(m, o) returns the scalars of source node m at
-// offset o.
-template <class S, class R, bool SHIFT, class Scal>
-__device__ __forceinline__ void gather_fs_gen(const S *__restrict__ fs, const S *__restrict__ rs,
-                                              const S *__restrict__ bs, size_t N, const unsigned xo[3],
+// Slots of a node that was solid when its step wrote the state (FS_SOLID
+// scalars) are no colour split of one node: the populations streaming through
+// a solid node (S:622, S:712) and the moving-wall bounce-back values of the
+// curve links (S:1004-1018).  The total is in the population array as for any
+// node; the colours are in side arrays: FP64 the red and blue values (r, b),
+// shifted storage the slot's own minority fraction (FP32), with which the
+// reader splits the total as it splits a factorised slot with no gradient
+// term.  A slot's minority population can be negative (recolouring leaves
+// w_i beta rho_R rho_B / rho (n . c_i) on it), so the fraction keeps its
+// sign, and the lowest mantissa bit says which colour it belongs to (1: blue,
+// red being the majority).
+template <class R> __device__ __forceinline__ float enc_slot(R r, R b)
+{
+    const bool redmaj = b <= r;
+    const float q = float((redmaj ? b : r) / (r + b));
+    return __uint_as_float((__float_as_uint(q) & ~1u) | (redmaj ? 1u : 0u));
+}
+template <class R> __device__ __forceinline__ void dec_slot(float v, R &A, R &B)
+{
+    const unsigned u = __float_as_uint(v);
+    const R q = R(__uint_as_float(u & ~1u));
+    if (u & 1u) {
+        B = q;
+        A = R(1) - q;
+    } else {
+        A = q;
+        B = R(1) - q;
+    }
+}
+
+// Pull gather of the factorised layout with walls, mirrors and particles:
+// f_i from x - c_i, split with the scalars that scal(o) returns for the node
+// at offset o (fs_src).  SOLIDS (nodes in the particles' band): where those
+// scalars are FS_SOLID, the colours come from the side arrays instead (a
+// node outside the band has no solid node within reach).
+template <bool SOLIDS, class S, class R, bool SHIFT, class Scal>
+__device__ __forceinline__ void gather_fs_ext(const S *__restrict__ fs, const void *__restrict__ side0,
+                                              const void *__restrict__ side1, size_t N, const unsigned xo[3],
                                               const unsigned yo[3], const unsigned zo[3], unsigned bb,
-                                              unsigned madj, Scal scal, R ax, Node<R, SHIFT> &nd, R rho0)
+                                              unsigned madj, Scal scal, Node<R, SHIFT> &nd, R rho0)
 {
     if constexpr (!SHIFT) {
         static_for<Q>([&](auto ic) {
             constexpr int i = decltype(ic)::value;
-            unsigned m, flip;
-            int d, o[3];
-            pull_slot_f<i>(xo, yo, zo, bb, madj, m, d, flip, o);
-            const FsScal64 sc = scal(m, o);
-            const unsigned k = grp_off(d, N, m);
-            if (solid_scal(sc)) {
-                nd.r[i] = StoreOf<S>::ld(grp(rs, d, N), k);
-                nd.b[i] = StoreOf<S>::ld(grp(bs, d, N), k);
+            const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]], k = grp_off(i, N, m);
+            int o[3];
+            unsigned flip;
+            fs_src<i>(bb, madj, o, flip);
+            const FsScal64 sc = scal(o);
+            if (SOLIDS && solid_scal(sc)) {
+                nd.r[i] = StoreOf<S>::ld(grp((const S *)side0, i, N), k);
+                nd.b[i] = StoreOf<S>::ld(grp((const S *)side1, i, N), k);
             } else {
-                const R f = StoreOf<S>::ld(grp(fs, d, N), k);
+                const R f = StoreOf<S>::ld(grp(fs, i, N), k);
                 R r = sc.a * f;
                 if (i > 0) r = r + (sc.g * R(W[i])) * dot_cf<i, R>(flip, sc.nx, sc.ny, sc.nz);
                 nd.r[i] = r;
@@ -588,33 +627,25 @@ __device__ __forceinline__ void gather_fs_gen(const S *__restrict__ fs, const S 
             }
         });
     } else {
-        const Shift<R> sh = shift_of<R>(ax);
-        const R xr = sh.redmaj ? R(1) - sh.hm : sh.hm, xb = sh.redmaj ? sh.hm : R(1) - sh.hm;
         R a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0, dva = 0, dvb = 0, g1 = 0, g2 = 0;
         static_for<Q>([&](auto ic) {
             constexpr int i = decltype(ic)::value;
-            unsigned m, flip;
-            int d, o[3];
-            pull_slot_f<i>(xo, yo, zo, bb, madj, m, d, flip, o);
-            const float4 g = scal(m, o);
-            const unsigned k = grp_off(d, N, m);
-            R A, B, f, gc = 0;
-            if (solid_scal(g)) {
-                const R rv = StoreOf<S>::ld(grp(rs, d, N), k), bv = StoreOf<S>::ld(grp(bs, d, N), k);
-                A = xr;
-                B = xb;
-                dva += rv;
-                dvb += bv;
-                f = rv + bv;
+            const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]], k = grp_off(i, N, m);
+            int o[3];
+            unsigned flip;
+            fs_src<i>(bb, madj, o, flip);
+            const R f = StoreOf<S>::ld(grp(fs, i, N), k);
+            float4 g = scal(o);
+            R A, B;
+            if (SOLIDS && solid_scal(g)) {
+                dec_slot(at(grp((const float *)side0, i, N), k), A, B);
+                g.y = g.z = g.w = 0.f;
             } else {
-                f = StoreOf<S>::ld(grp(fs, d, N), k);
                 decode_frac(g.x, A, B);
-                dva += A * f;
-                dvb += B * f;
-                if (CX[i] != 0) { R t = CX[i] == 1 ? R(g.y) : -R(g.y); gc += (flip & 1u) ? -t : t; }
-                if (CY[i] != 0) { R t = CY[i] == 1 ? R(g.z) : -R(g.z); gc += (flip & 2u) ? -t : t; }
-                if (CZ[i] != 0) { R t = CZ[i] == 1 ? R(g.w) : -R(g.w); gc += (flip & 4u) ? -t : t; }
             }
+            dva += A * f;
+            dvb += B * f;
+            const R gc = gdot_f<i, R>(flip, g.y, g.z, g.w);
             if (i == 0) { a0 = A; b0 = B; }
             else if (i <= 6) { a1 += A; b1 += B; g1 += gc; }
             else { a2 += A; b2 += B; g2 += gc; }
@@ -626,49 +657,47 @@ __device__ __forceinline__ void gather_fs_gen(const S *__restrict__ fs, const S 
     }
 }
 
-// One slot (m, d) of a factorised state as the red and blue values that the
-// red/blue layout stores for the node reading it: FP64 the populations
-// themselves; shifted storage deviations from w_i rho0 s_red and w_i rho0
-// s_blue of the reader's shift rd, the minority colour formed first (as
-// write_node does).  An FS_SOLID node's slot is in the side arrays, shifted
-// for the reader of the stored state (phase field ax), and is re-shifted.
+// Colours of one slot (total f; deviation f~ in shifted storage) split with
+// the scalars sc of its node in direction c_I reversed on the axes in flip:
+// FP64 the red and blue values with the reference's rounding, shifted
+// storage the slot's encoded minority fraction (the minority formed first).
+// FS_SOLID scalars: the side arrays' colours of slot k.
 template <int I, class S, class R, bool SHIFT, class Sc>
-__device__ __forceinline__ void fs_slot_rb(const S *__restrict__ fs, const S *__restrict__ rs,
-                                           const S *__restrict__ bs, size_t N, unsigned m, int d, unsigned flip,
-                                           const Sc &sc, R ax, R arw, R rho0, R &vr, R &vb)
+__device__ __forceinline__ void slot_side(const void *__restrict__ side0, const void *__restrict__ side1, size_t N,
+                                          unsigned k, R f, const Sc &sc, unsigned flip, R rho0, R &vr, R &vb,
+                                          float &q)
 {
-    const unsigned k = grp_off(d, N, m);
-    if (solid_scal(sc)) {
-        vr = StoreOf<S>::ld(grp(rs, d, N), k);
-        vb = StoreOf<S>::ld(grp(bs, d, N), k);
-        if constexpr (SHIFT) {
-            const R dsh = R(W[I]) * rho0 * red_shift_diff(shift_of<R>(ax), shift_of<R>(arw));
-            vr += dsh;
-            vb -= dsh;
-        }
-        return;
-    }
-    const R f = StoreOf<S>::ld(grp(fs, d, N), k);
     if constexpr (!SHIFT) {
-        R r = sc.a * f;
-        if (I > 0) r = r + (sc.g * R(W[I])) * dot_cf<I, R>(flip, sc.nx, sc.ny, sc.nz);
-        vr = r;
-        vb = f - r;
-    } else {
-        const bool redmaj = __builtin_signbitf(sc.x);
-        const Shift<R> me{R(redmaj ? -sc.x : sc.x), redmaj};
-        R gc = 0;
-        if (CX[I] != 0) { R t = CX[I] == 1 ? R(sc.y) : -R(sc.y); gc += (flip & 1u) ? -t : t; }
-        if (CY[I] != 0) { R t = CY[I] == 1 ? R(sc.z) : -R(sc.z); gc += (flip & 2u) ? -t : t; }
-        if (CZ[I] != 0) { R t = CZ[I] == 1 ? R(sc.w) : -R(sc.w); gc += (flip & 4u) ? -t : t; }
-        const R wr = R(W[I]) * rho0, gd = R(W[I]) * gc;
-        const R dsh = red_shift_diff(me, shift_of<R>(arw));
-        if (redmaj) {
-            vb = me.hm * f - wr * dsh - gd;
-            vr = f - vb;
+        (void)rho0;
+        (void)q;
+        if (solid_scal(sc)) {
+            vr = StoreOf<S>::ld(grp((const S *)side0, I, N), k);
+            vb = StoreOf<S>::ld(grp((const S *)side1, I, N), k);
         } else {
-            vr = me.hm * f + wr * dsh + gd;
-            vb = f - vr;
+            R r = sc.a * f;
+            if (I > 0) r = r + (sc.g * R(W[I])) * dot_cf<I, R>(flip, sc.nx, sc.ny, sc.nz);
+            vr = r;
+            vb = f - r;
+        }
+    } else {
+        (void)side1;
+        (void)vr;
+        (void)vb;
+        if (solid_scal(sc)) {
+            q = at(grp((const float *)side0, I, N), k);
+        } else {
+            const R fa = R(W[I]) * rho0 + f, gd = R(W[I]) * gdot_f<I, R>(flip, sc.y, sc.z, sc.w);
+            R A, B;
+            decode_frac(sc.x, A, B);
+            R r, b;
+            if (__builtin_signbitf(sc.x)) {
+                b = B * fa - gd;
+                r = fa - b;
+            } else {
+                r = A * fa + gd;
+                b = fa - r;
+            }
+            q = enc_slot(r, b);
         }
     }
 }
@@ -783,16 +812,16 @@ __device__ __forceinline__ bool node_moments(const Lat &L, const void *__restric
     if constexpr (EXT) {
         const unsigned bb = wall_bb_mask(E, L, x, y, z), madj = mirror_adjacent(E, L, x, y, z);
         if constexpr (LIN == FS) {
-            if constexpr (SHIFT) ax = R(at(phio, n));
-            gather_fs_gen<S, R, SHIFT>(
-                (const S *)p0, (const S *)E.rbi[0], (const S *)E.rbi[1], (size_t)L.N, xo, yo, zo, bb, madj,
-                [&](unsigned m, const int *) {
+            gather_fs_ext<true, S, R, SHIFT>(
+                (const S *)p0, E.rbi[0], E.rbi[1], (size_t)L.N, xo, yo, zo, bb, madj,
+                [&](const int o[3]) {
+                    const unsigned m = xo[1 + o[0]] + yo[1 + o[1]] + zo[1 + o[2]];
                     if constexpr (SHIFT)
                         return at((const float4 *)p1, m);
                     else
                         return fs64_at((const double *)p1, m);
                 },
-                ax, nd, R(L.rho0));
+                nd, R(L.rho0));
         } else {
             gather<LIN, S, R, SHIFT>(L, p0, p1, xo, yo, zo, ax, nd, bb, madj);
         }
@@ -890,10 +919,9 @@ template <int ROWS> struct FsxRing<false, ROWS> {
 };
 
 // ENC: write the signed minority fraction (the scatter form's phase-field
-// buffer) instead of phi.  EXT: walls, mirrors and particles as
-// node_moments handles them (no phi on ghost layers; solid nodes' fields;
-// the band's FP64 fields); nodes next to a ghost layer or in the particles'
-// band take the general gather.
+// buffer) instead of phi.  EXT (factorised layout with walls, mirrors or
+// particles): the plain moments of every lane but the special ones
+// (special_lane), which k_moments_special forms.
 template <class S, class R, class P, bool SHIFT, bool ENC = false, bool EXT = false>
 __device__ __forceinline__ void moments_fsx_tile(const Lat &L, const void *__restrict__ p0,
                                                  const void *__restrict__ p1, const P *__restrict__ phio,
@@ -950,33 +978,10 @@ __device__ __forceinline__ void moments_fsx_tile(const Lat &L, const void *__res
             };
             const unsigned n = xo[1] + yo[1] + zo[1];
             if constexpr (EXT) {
-                const unsigned bb = wall_bb_mask(E, L, x, y, z), madj = mirror_adjacent(E, L, x, y, z);
-                const unsigned nt = E.ntype ? at(E.ntype, n) : 0u;
-                if (!is_ghost(E, L, x, y, z)) {
-                    if (bb | madj | (nt & NT_NEAR)) {
-                        R ax = 0;
-                        if constexpr (SHIFT) ax = R(at(phio, n));
-                        auto scal_o = [&](unsigned, const int o[3]) {
-                            const int sl = (x + o[0] + 9) % 3, jy = ty + 1 + o[1], jz = tz + 1 + o[2];
-                            if constexpr (SHIFT)
-                                return sm.s[sl][jy][jz];
-                            else
-                                return FsScal64{sm.s[sl][0][jy][jz], sm.s[sl][1][jy][jz], sm.s[sl][2][jy][jz],
-                                                sm.s[sl][3][jy][jz], sm.s[sl][4][jy][jz]};
-                        };
-                        gather_fs_gen<S, R, SHIFT>(fs, (const S *)E.rbi[0], (const S *)E.rbi[1], N, xo, yo, zo, bb,
-                                                   madj, scal_o, ax, nd, R(L.rho0));
-                    } else {
-                        gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, scal, nd, R(L.rho0));
-                    }
-                    Moments<R, SHIFT> mo = moments<R, SHIFT>(L, nd);
-                    if (nt & NT_SOLID) { // calmacros at solid nodes (S:784-789)
-                        mo.phi = 0;
-                        mo.rho = R(L.rho0);
-                        mo.ux = mo.uy = mo.uz = 0;
-                    }
-                    band_macros(L, E, n, nt, mo);
-                    at(phi, n) = P(mo.phi);
+                // special lanes (special_lane) are left to k_moments_special
+                if (!special_lane(E, L, x, y, z, n)) {
+                    gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, scal, nd, R(L.rho0));
+                    at(phi, n) = P(moments<R, SHIFT>(L, nd).phi);
                 }
             } else {
                 gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, scal, nd, R(L.rho0));
@@ -1000,9 +1005,10 @@ __global__ __launch_bounds__(256) void k_moments_fsx(Lat L, const void *__restri
     moments_fsx_tile<S, R, P, SHIFT, ENC>(L, p0, p1, phio, phi, xc, bx, by, bz);
 }
 
-// K1 of the factorised layout with walls, mirrors and particles.
+// K1 of the factorised layout with walls, mirrors and particles (all but the
+// special lanes).
 template <class S, class R, class P, bool SHIFT>
-__global__ __launch_bounds__(256) void k_moments_fsx_ext(Lat L, const void *__restrict__ p0,
+__global__ __launch_bounds__(256, 4) void k_moments_fsx_ext(Lat L, const void *__restrict__ p0,
                                                          const void *__restrict__ p1, const P *__restrict__ phio,
                                                          P *__restrict__ phi, int xc, Ext E)
 {
@@ -1472,8 +1478,281 @@ __device__ __forceinline__ bool away_from_walls(const Ext &E, const Lat &L, int 
              (E.wall[1][1] && y > L.ny - 3) || (E.wall[2][0] && z < 2) || (E.wall[2][1] && z > L.nz - 3));
 }
 
-// The (z-tile bx, y-tile by, x-chunk bz) column of one workgroup.
-template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT>
+// The update of one live node (x, y, z) of the fused step (derivatives,
+// collision, recolouring, streaming as the pull gather), given the phase
+// field phi_at(dx, dy, dz) and the unit normals n_at(k, dx, dy, dz) around it
+// and, for a factorised input, the recolour scalars scal_at(o) of the node at
+// offset o: from the tile's LDS rings in k_collide, from memory in
+// k_collide_special.
+template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT, class PhiAt, class NAt, class ScalAt>
+__device__ __forceinline__ void lane_step(const Lat &L, const void *__restrict__ i0, const void *__restrict__ i1,
+                                          void *__restrict__ o0, void *__restrict__ o1, const P *__restrict__ phio,
+                                          const Ext &E, int x, int y, int z, PhiAt phi_at, NAt n_at, ScalAt scal_at)
+{
+    unsigned nt = 0, bb = 0, madj = 0;
+    if constexpr (EXT) {
+        if (is_ghost(E, L, x, y, z)) return; // ghost layers are not collided
+        bb = wall_bb_mask(E, L, x, y, z);
+        madj = mirror_adjacent(E, L, x, y, z);
+    }
+
+    // gradient, normal and flag at the node (S:498-532)
+    R gx, gy, gz;
+    grad19<R>([&](int dx, int dy, int dz) { return phi_at(dx, dy, dz); }, gx, gy,
+              gz);
+    const R g2 = gx * gx + gy * gy + gz * gz;
+    bool flag = sizeof(R) == 8 ? fsqrt(g2) > R(EVAP_LIM) : g2 > R(EVAP_LIM * EVAP_LIM);
+    const R nx = n_at(0, 0, 0, 0);
+    const R ny = n_at(1, 0, 0, 0);
+    const R nz = n_at(2, 0, 0, 0);
+
+    unsigned xo[3], yo[3], zo[3];
+    node_offsets(L, x, y, z, xo, yo, zo);
+    const unsigned n = xo[1] + yo[1] + zo[1];
+    bool force_on = flag;
+    if constexpr (EXT) {
+        if (E.ntype) nt = at(E.ntype, n);
+        if (nt & NT_SOLID) flag = false;
+        if (flag && (nt & NT_POUT)) { // wetting rescales the gradient too (S:569-581)
+            const R fnorm = fsqrt(g2);
+            gx = fnorm * nx;
+            gy = fnorm * ny;
+            gz = fnorm * nz;
+        }
+        force_on = flag && !(nt & NT_POUT) && away_from_walls(E, L, x, y, z);
+        if (nt & NT_NEAR) {
+            const size_t N = (size_t)L.N;
+            const bool fl = flag;
+            E.der[n] = fl ? double(gx) : 0.0;
+            E.der[N + n] = fl ? double(gy) : 0.0;
+            E.der[2 * N + n] = fl ? double(gz) : 0.0;
+            E.der[3 * N + n] = double(nx);
+            E.der[4 * N + n] = double(ny);
+            E.der[5 * N + n] = double(nz);
+            if (!(nt & NT_SOLID) && !fl) { // non-interface fluid: gradient kept, normal zero (S:519-530)
+                E.der[n] = double(gx);
+                E.der[N + n] = double(gy);
+                E.der[2 * N + n] = double(gz);
+            }
+        }
+    }
+    // normal derivatives for the curvature (S:635-660), from the normals
+    // ring; with walls and particles formed after the gather, so that they
+    // are not live across it
+    R d[3][3];
+    auto curv_d = [&]() {
+        if (force_on) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    R tmp = R(W[i]) * n_at(k, CX[i], CY[i], CZ[i]);
+                    if (CX[i] == 1) d[k][0] = d[k][0] + tmp;
+                    if (CX[i] == -1) d[k][0] = d[k][0] - tmp;
+                    if (CY[i] == 1) d[k][1] = d[k][1] + tmp;
+                    if (CY[i] == -1) d[k][1] = d[k][1] - tmp;
+                    if (CZ[i] == 1) d[k][2] = d[k][2] + tmp;
+                    if (CZ[i] == -1) d[k][2] = d[k][2] - tmp;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
+        }
+    };
+    if constexpr (!EXT) curv_d();
+
+    R ax = 0, aw[Q]; // phase fields fixing the shift fractions: this node's of the stored
+                     // state, and aw_i of the node that reads direction i next
+    if constexpr (SHIFT && LIN == RB) ax = R(at(phio, n));
+    if constexpr (SHIFT && LOUT == RB) {
+        // shift of the slot read by x + c_i at the next step: that node's phi
+        // now; a slot that bounces back off a wall is read by x itself
+        const R aself = R(phi_at(0, 0, 0));
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            aw[i] = R(phi_at(CX[i], CY[i], CZ[i]));
+            if constexpr (EXT)
+                if ((bb >> OPC[i]) & 1u) aw[i] = aself;
+        }
+    }
+    if constexpr (EXT && LIN == RB && LOUT == RB) {
+        if (nt & NT_SOLID) {
+            // solid nodes are not collided: their streamed populations pass
+            // through to the next streaming (S:622, S:712, S:743)
+            // (pulled as the live nodes pull: bounce-back off walls, the
+            // mirror image across mirror faces)
+            const size_t N = (size_t)L.N;
+            const S *rs = (const S *)i0, *bs = (const S *)i1;
+            S *ro = (S *)o0, *bo = (S *)o1;
+            static_for<Q>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                unsigned m;
+                int dsrc;
+                pull_slot<i>(xo, yo, zo, bb, madj, m, dsrc);
+                const unsigned ks = grp_off(dsrc, N, m), kd = grp_off(i, N, n);
+                R rv = StoreOf<S>::ld(grp(rs, dsrc, N), ks), bv = StoreOf<S>::ld(grp(bs, dsrc, N), ks);
+                if constexpr (SHIFT) {
+                    const R dsh = R(W[i]) * R(L.rho0) * red_shift_diff(shift_of<R>(ax), shift_of<R>(aw[i]));
+                    rv += dsh;
+                    bv -= dsh;
+                }
+                StoreOf<S>::st(grp(ro, i, N), kd, rv);
+                StoreOf<S>::st(grp(bo, i, N), kd, bv);
+            });
+            return;
+        }
+    }
+    if constexpr (EXT && !(LIN == RB && LOUT == RB)) {
+        if (nt & NT_SOLID) {
+            // solid nodes are not collided (S:622, S:712, S:743): each slot
+            // takes the total it pulls, and that slot's colours go to the
+            // side arrays of the output; FS_SOLID scalars mark the node
+            const size_t N = (size_t)L.N;
+            const R rho0 = R(L.rho0);
+            S *fo = (S *)o0;
+            static_for<Q>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                const unsigned kd = grp_off(i, N, n);
+                R f, vr = 0, vb = 0;
+                float q = 0.f;
+                if constexpr (LIN == RB) {
+                    // the red/blue state of the first step, pulled as the
+                    // live nodes pull it (bounce-back, mirror images)
+                    unsigned m;
+                    int d;
+                    pull_slot<i>(xo, yo, zo, bb, madj, m, d);
+                    const unsigned ks = grp_off(d, N, m);
+                    const R rv = StoreOf<S>::ld(grp((const S *)i0, d, N), ks);
+                    const R bv = StoreOf<S>::ld(grp((const S *)i1, d, N), ks);
+                    f = rv + bv;
+                    if constexpr (SHIFT) {
+                        // deviations from this node's shifts: the absolute
+                        // colours, the minority's from its own small terms
+                        const Shift<R> sh = shift_of<R>(R(at(phio, n)));
+                        const R wr = R(W[i]) * rho0, sred = sh.redmaj ? R(1) - sh.hm : sh.hm;
+                        const R sblue = sh.redmaj ? sh.hm : R(1) - sh.hm;
+                        q = enc_slot(rv + wr * sred, bv + wr * sblue);
+                    } else {
+                        vr = rv;
+                        vb = bv;
+                    }
+                } else {
+                    const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]], ks = grp_off(i, N, m);
+                    int o[3];
+                    unsigned flip;
+                    fs_src<i>(bb, madj, o, flip);
+                    f = StoreOf<S>::ld(grp((const S *)i0, i, N), ks);
+                    slot_side<i, S, R, SHIFT>(E.rbi[0], E.rbi[1], N, ks, f, scal_at(o), flip, rho0, vr, vb, q);
+                }
+                StoreOf<S>::st(grp(fo, i, N), kd, f);
+                if constexpr (SHIFT) {
+                    at(grp((float *)E.rbo[0], i, N), kd) = q;
+                } else {
+                    StoreOf<S>::st(grp((S *)E.rbo[0], i, N), kd, vr);
+                    StoreOf<S>::st(grp((S *)E.rbo[1], i, N), kd, vb);
+                }
+            });
+            if constexpr (SHIFT) {
+                at((float4 *)o1, n) = make_float4(FS_SOLID, 0.f, 0.f, 0.f);
+            } else {
+                double2 *sc = (double2 *)o1;
+                at(sc, 3 * n) = make_double2(double(FS_SOLID), 0.0);
+                at(sc, 3 * n + 1) = make_double2(0.0, 0.0);
+                at(sc, 3 * n + 2) = make_double2(0.0, 0.0);
+            }
+            return;
+        }
+    }
+    Node<R, SHIFT> nd;
+    Moments<R, SHIFT> mo;
+    bool refilled = false;
+    if constexpr (EXT) refilled = (nt & NT_REFILL) != 0;
+    if (refilled) {
+        // refilled node (S:1364-1391): SRT equilibria of the refill densities
+        // and the rigid-body velocity; macros as refilling set them
+        const size_t N = (size_t)L.N;
+        if constexpr (!SHIFT) {
+            const double rr = E.mac[N + n], rbv = E.mac[2 * N + n];
+            const double ux = E.mac[3 * N + n], uy = E.mac[4 * N + n], uz = E.mac[5 * N + n];
+            const double usqr = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const double uc = ux * CX[i] + uy * CY[i] + uz * CZ[i];
+                const double poly = (1.0 + 3.0 * uc + 4.5 * uc * uc - 1.5 * usqr);
+                nd.r[i] = rr * W[i] * poly;
+                nd.b[i] = rbv * W[i] * poly;
+            }
+            mo.ux = R(ux);
+            mo.uy = R(uy);
+            mo.uz = R(uz);
+        } else {
+            // shifted storage: the deviations w_i (rho poly_i - rho0), in
+            // the working precision
+            const R rt = R(E.mac[N + n] + E.mac[2 * N + n]);
+            const R ux = R(E.mac[3 * N + n]), uy = R(E.mac[4 * N + n]), uz = R(E.mac[5 * N + n]);
+            const R usqr = ux * ux + uy * uy + uz * uz;
+            R s = 0;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const R uc = ux * R(CX[i]) + uy * R(CY[i]) + uz * R(CZ[i]);
+                const R poly = R(1) + R(3) * uc + R(4.5) * uc * uc - R(1.5) * usqr;
+                nd.ft[i] = R(W[i]) * (rt * poly - R(L.rho0));
+                s += nd.ft[i];
+            }
+            mo.drho = s;
+            mo.ux = ux;
+            mo.uy = uy;
+            mo.uz = uz;
+        }
+        mo.sr = R(E.mac[N + n]);
+        mo.sb = R(E.mac[2 * N + n]);
+        mo.rho = R(E.mac[n]);
+        mo.phi = R(phi_at(0, 0, 0));
+    } else {
+        const size_t N = (size_t)L.N;
+        if constexpr (EXT && LIN == FS) {
+            // slots split by the scalars of their node, of the node itself off
+            // a wall or of a mirror image, or taken from the side arrays of
+            // solid nodes
+            gather_fs_ext<true, S, R, SHIFT>((const S *)i0, E.rbi[0], E.rbi[1], N, xo, yo, zo, bb, madj, scal_at, nd,
+                                             R(L.rho0));
+        } else if constexpr (LIN == FS && SHIFT) {
+            gather_fs<S, R, SHIFT>(
+                (const S *)i0, N, xo, yo, zo,
+                [&](int i, unsigned) {
+                    const int o[3] = {-CX[i], -CY[i], -CZ[i]};
+                    return scal_at(o);
+                },
+                nd, R(L.rho0));
+        } else {
+            gather<LIN, S, R, SHIFT>(L, i0, i1, xo, yo, zo, ax, nd, bb, madj);
+        }
+        mo = moments<R, SHIFT>(L, nd);
+    }
+    const R phin = R(phi_at(0, 0, 0));
+    R f[Q];
+    if constexpr (!SHIFT) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = nd.r[i] + nd.b[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = nd.ft[i];
+    }
+    if constexpr (EXT) curv_d();
+    collide<R, SHIFT>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, force_on, d);
+    const Recol<R> rc = recol<R, SHIFT>(L, mo, flag);
+    write_node<LOUT, S, R, SHIFT>(L, o0, o1, n, f, rc, nx, ny, nz, aw);
+}
+
+// The (z-tile bx, y-tile by, x-chunk bz) column of one workgroup.  SKIP
+// (factorised layout with walls, mirrors or particles): the rings and the
+// plain update of every lane but the special ones (special_lane), which
+// k_collide_special updates.
+template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT, bool SKIP = false>
 __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restrict__ i0, const void *__restrict__ i1,
                                              void *__restrict__ o0, void *__restrict__ o1,
                                              const P *__restrict__ phi, const P *__restrict__ phio, int xc,
@@ -1593,283 +1872,34 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
         normals(x + 1, x - 2, x + 2);
         __syncthreads();
         if (!active) continue;
-        unsigned nt = 0, bb = 0, madj = 0;
-        if constexpr (EXT) {
-            if (is_ghost(E, L, x, y, z)) continue; // ghost layers are not collided
-            bb = wall_bb_mask(E, L, x, y, z);
-            madj = mirror_adjacent(E, L, x, y, z);
-        }
-
-        // gradient, normal and flag at the node (S:498-532)
-        R gx, gy, gz;
-        grad19<R>([&](int dx, int dy, int dz) { return sm.phi[slot5(x + dx)][ty + 2 + dy][tz + 2 + dz]; }, gx, gy,
-                  gz);
-        const R g2 = gx * gx + gy * gy + gz * gz;
-        bool flag = sizeof(R) == 8 ? fsqrt(g2) > R(EVAP_LIM) : g2 > R(EVAP_LIM * EVAP_LIM);
-        const R nx = sm.n[0][slot3(x)][ty + 1][tz + 1];
-        const R ny = sm.n[1][slot3(x)][ty + 1][tz + 1];
-        const R nz = sm.n[2][slot3(x)][ty + 1][tz + 1];
-
-        unsigned xo[3], yo[3], zo[3];
-        node_offsets(L, x, y, z, xo, yo, zo);
-        const unsigned n = xo[1] + yo[1] + zo[1];
-        bool force_on = flag;
-        if constexpr (EXT) {
-            if (E.ntype) nt = at(E.ntype, n);
-            if (nt & NT_SOLID) flag = false;
-            if (flag && (nt & NT_POUT)) { // wetting rescales the gradient too (S:569-581)
-                const R fnorm = fsqrt(g2);
-                gx = fnorm * nx;
-                gy = fnorm * ny;
-                gz = fnorm * nz;
-            }
-            force_on = flag && !(nt & NT_POUT) && away_from_walls(E, L, x, y, z);
-            if (nt & NT_NEAR) {
-                const size_t N = (size_t)L.N;
-                const bool fl = flag;
-                E.der[n] = fl ? double(gx) : 0.0;
-                E.der[N + n] = fl ? double(gy) : 0.0;
-                E.der[2 * N + n] = fl ? double(gz) : 0.0;
-                E.der[3 * N + n] = double(nx);
-                E.der[4 * N + n] = double(ny);
-                E.der[5 * N + n] = double(nz);
-                if (!(nt & NT_SOLID) && !fl) { // non-interface fluid: gradient kept, normal zero (S:519-530)
-                    E.der[n] = double(gx);
-                    E.der[N + n] = double(gy);
-                    E.der[2 * N + n] = double(gz);
-                }
-            }
-        }
-        // normal derivatives for the curvature (S:635-660), from the normals
-        // ring; with walls and particles formed after the gather, so that they
-        // are not live across it
-        R d[3][3];
-        auto curv_d = [&]() {
-            if (force_on) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
-#pragma unroll
-                for (int i = 1; i < Q; ++i) {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        R tmp = R(W[i]) * sm.n[k][slot3(x + CX[i])][ty + 1 + CY[i]][tz + 1 + CZ[i]];
-                        if (CX[i] == 1) d[k][0] = d[k][0] + tmp;
-                        if (CX[i] == -1) d[k][0] = d[k][0] - tmp;
-                        if (CY[i] == 1) d[k][1] = d[k][1] + tmp;
-                        if (CY[i] == -1) d[k][1] = d[k][1] - tmp;
-                        if (CZ[i] == 1) d[k][2] = d[k][2] + tmp;
-                        if (CZ[i] == -1) d[k][2] = d[k][2] - tmp;
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
-            }
-        };
-        if constexpr (!EXT) curv_d();
-
-        R ax = 0, aw[Q]; // phase fields fixing the shift fractions: this node's of the stored
-                         // state, and aw_i of the node that reads direction i next
-        if constexpr (SHIFT && LIN == RB) ax = R(at(phio, n));
-        if constexpr (SHIFT && LOUT == RB) {
-            // shift of the slot read by x + c_i at the next step: that node's phi
-            // now; a slot that bounces back off a wall is read by x itself
-            const R aself = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
-#pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                aw[i] = R(sm.phi[slot5(x + CX[i])][ty + 2 + CY[i]][tz + 2 + CZ[i]]);
-                if constexpr (EXT)
-                    if ((bb >> OPC[i]) & 1u) aw[i] = aself;
-            }
-        }
-        if constexpr (EXT && LIN == RB && LOUT == RB) {
-            if (nt & NT_SOLID) {
-                // solid nodes are not collided: their streamed populations pass
-                // through to the next streaming (S:622, S:712, S:743)
-                // (pulled as the live nodes pull: bounce-back off walls, the
-                // mirror image across mirror faces)
-                const size_t N = (size_t)L.N;
-                const S *rs = (const S *)i0, *bs = (const S *)i1;
-                S *ro = (S *)o0, *bo = (S *)o1;
-                static_for<Q>([&](auto ic) {
-                    constexpr int i = decltype(ic)::value;
-                    unsigned m;
-                    int dsrc;
-                    pull_slot<i>(xo, yo, zo, bb, madj, m, dsrc);
-                    const unsigned ks = grp_off(dsrc, N, m), kd = grp_off(i, N, n);
-                    R rv = StoreOf<S>::ld(grp(rs, dsrc, N), ks), bv = StoreOf<S>::ld(grp(bs, dsrc, N), ks);
-                    if constexpr (SHIFT) {
-                        const R dsh = R(W[i]) * R(L.rho0) * red_shift_diff(shift_of<R>(ax), shift_of<R>(aw[i]));
-                        rv += dsh;
-                        bv -= dsh;
-                    }
-                    StoreOf<S>::st(grp(ro, i, N), kd, rv);
-                    StoreOf<S>::st(grp(bo, i, N), kd, bv);
-                });
-                continue;
-            }
-        }
-        if constexpr (EXT && !(LIN == RB && LOUT == RB)) {
-            if (nt & NT_SOLID) {
-                // solid nodes are not collided (S:622, S:712, S:743): the
-                // slots they pull (bounce-back off walls, mirror images as
-                // live nodes pull) go to the red/blue side arrays of the
-                // factorised output, and FS_SOLID scalars mark the node
-                const size_t N = (size_t)L.N;
-                const R rho0 = R(L.rho0);
-                const S *rs = (const S *)(LIN == RB ? i0 : E.rbi[0]), *bs = (const S *)(LIN == RB ? i1 : E.rbi[1]);
-                S *ro = (S *)E.rbo[0], *bo = (S *)E.rbo[1];
-                R axs = 0, aself = 0;
-                if constexpr (SHIFT) {
-                    axs = R(at(phio, n));
-                    aself = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
-                }
-                static_for<Q>([&](auto ic) {
-                    constexpr int i = decltype(ic)::value;
-                    unsigned m, flip;
-                    int d, o[3];
-                    pull_slot_f<i>(xo, yo, zo, bb, madj, m, d, flip, o);
-                    // phase field of the node that reads slot i next (x itself
-                    // where the slot bounces back off a wall)
-                    R arw = 0;
-                    if constexpr (SHIFT)
-                        arw = ((bb >> OPC[i]) & 1u) ? aself
-                                                    : R(sm.phi[slot5(x + CX[i])][ty + 2 + CY[i]][tz + 2 + CZ[i]]);
-                    R vr, vb;
-                    if constexpr (LIN == RB) {
-                        const unsigned ks = grp_off(d, N, m);
-                        vr = StoreOf<S>::ld(grp(rs, d, N), ks);
-                        vb = StoreOf<S>::ld(grp(bs, d, N), ks);
-                        if constexpr (SHIFT) {
-                            const R dsh = R(W[i]) * rho0 * red_shift_diff(shift_of<R>(axs), shift_of<R>(arw));
-                            vr += dsh;
-                            vb -= dsh;
-                        }
-                    } else if constexpr (RING) {
-                        fs_slot_rb<i, S, R, SHIFT>((const S *)i0, rs, bs, N, m, d, flip,
-                                                   ring.s[(x + o[0] + 9) % 3][ty + 1 + o[1]][tz + 1 + o[2]], axs, arw,
-                                                   rho0, vr, vb);
-                    } else {
-                        fs_slot_rb<i, S, R, SHIFT>((const S *)i0, rs, bs, N, m, d, flip,
-                                                   fs64_at((const double *)i1, m), axs, arw, rho0, vr, vb);
-                    }
-                    const unsigned kd = grp_off(i, N, n);
-                    StoreOf<S>::st(grp(ro, i, N), kd, vr);
-                    StoreOf<S>::st(grp(bo, i, N), kd, vb);
-                });
-                if constexpr (SHIFT) {
-                    at((float4 *)o1, n) = make_float4(FS_SOLID, 0.f, 0.f, 0.f);
-                } else {
-                    double2 *sc = (double2 *)o1;
-                    at(sc, 3 * n) = make_double2(double(FS_SOLID), 0.0);
-                    at(sc, 3 * n + 1) = make_double2(0.0, 0.0);
-                    at(sc, 3 * n + 2) = make_double2(0.0, 0.0);
-                }
-                continue;
-            }
-        }
-        Node<R, SHIFT> nd;
-        Moments<R, SHIFT> mo;
-        bool refilled = false;
-        if constexpr (EXT) refilled = (nt & NT_REFILL) != 0;
-        if (refilled) {
-            // refilled node (S:1364-1391): SRT equilibria of the refill densities
-            // and the rigid-body velocity; macros as refilling set them
-            const size_t N = (size_t)L.N;
-            if constexpr (!SHIFT) {
-                const double rr = E.mac[N + n], rbv = E.mac[2 * N + n];
-                const double ux = E.mac[3 * N + n], uy = E.mac[4 * N + n], uz = E.mac[5 * N + n];
-                const double usqr = ux * ux + uy * uy + uz * uz;
-#pragma unroll
-                for (int i = 0; i < Q; ++i) {
-                    const double uc = ux * CX[i] + uy * CY[i] + uz * CZ[i];
-                    const double poly = (1.0 + 3.0 * uc + 4.5 * uc * uc - 1.5 * usqr);
-                    nd.r[i] = rr * W[i] * poly;
-                    nd.b[i] = rbv * W[i] * poly;
-                }
-                mo.ux = R(ux);
-                mo.uy = R(uy);
-                mo.uz = R(uz);
-            } else {
-                // shifted storage: the deviations w_i (rho poly_i - rho0), in
-                // the working precision
-                const R rt = R(E.mac[N + n] + E.mac[2 * N + n]);
-                const R ux = R(E.mac[3 * N + n]), uy = R(E.mac[4 * N + n]), uz = R(E.mac[5 * N + n]);
-                const R usqr = ux * ux + uy * uy + uz * uz;
-                R s = 0;
-#pragma unroll
-                for (int i = 0; i < Q; ++i) {
-                    const R uc = ux * R(CX[i]) + uy * R(CY[i]) + uz * R(CZ[i]);
-                    const R poly = R(1) + R(3) * uc + R(4.5) * uc * uc - R(1.5) * usqr;
-                    nd.ft[i] = R(W[i]) * (rt * poly - R(L.rho0));
-                    s += nd.ft[i];
-                }
-                mo.drho = s;
-                mo.ux = ux;
-                mo.uy = uy;
-                mo.uz = uz;
-            }
-            mo.sr = R(E.mac[N + n]);
-            mo.sb = R(E.mac[2 * N + n]);
-            mo.rho = R(E.mac[n]);
-            mo.phi = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
-        } else {
-            const size_t N = (size_t)L.N;
-            if constexpr (RING) {
-                auto rscal = [&](int i, unsigned) {
-                    return ring.s[(x - CX[i] + 9) % 3][ty + 1 - CY[i]][tz + 1 - CZ[i]];
-                };
-                if constexpr (EXT) {
-                    // next to a ghost layer or in the particles' band: the
-                    // general gather (bounce-back, mirrors, solid side arrays)
-                    if (bb | madj | (nt & NT_NEAR)) {
-                        gather_fs_gen<S, R, SHIFT>(
-                            (const S *)i0, (const S *)E.rbi[0], (const S *)E.rbi[1], N, xo, yo, zo, bb, madj,
-                            [&](unsigned, const int o[3]) {
-                                return ring.s[(x + o[0] + 9) % 3][ty + 1 + o[1]][tz + 1 + o[2]];
-                            },
-                            R(at(phio, n)), nd, R(L.rho0));
-                    } else {
-                        gather_fs<S, R, SHIFT>((const S *)i0, N, xo, yo, zo, rscal, nd, R(L.rho0));
-                    }
-                } else {
-                    gather_fs<S, R, SHIFT>((const S *)i0, N, xo, yo, zo, rscal, nd, R(L.rho0));
-                }
-            } else if constexpr (EXT && LIN == FS) {
-                gather_fs_gen<S, R, SHIFT>(
-                    (const S *)i0, (const S *)E.rbi[0], (const S *)E.rbi[1], N, xo, yo, zo, bb, madj,
-                    [&](unsigned m, const int *) { return fs64_at((const double *)i1, m); }, ax, nd, R(L.rho0));
-            } else {
-                gather<LIN, S, R, SHIFT>(L, i0, i1, xo, yo, zo, ax, nd, bb, madj);
-            }
-            mo = moments<R, SHIFT>(L, nd);
-        }
-        const R phin = R(sm.phi[slot5(x)][ty + 2][tz + 2]);
-        R f[Q];
-        if constexpr (!SHIFT) {
-#pragma unroll
-            for (int i = 0; i < Q; ++i) f[i] = nd.r[i] + nd.b[i];
-        } else {
-#pragma unroll
-            for (int i = 0; i < Q; ++i) f[i] = nd.ft[i];
-        }
-        if constexpr (EXT) curv_d();
-        collide<R, SHIFT>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, force_on, d);
-        const Recol<R> rc = recol<R, SHIFT>(L, mo, flag);
-        write_node<LOUT, S, R, SHIFT>(L, o0, o1, n, f, rc, nx, ny, nz, aw);
+        if constexpr (SKIP)
+            if (special_lane(E, L, x, y, z, ((unsigned)x * L.ny + y) * L.nz + z)) continue;
+        lane_step<LIN, LOUT, S, R, P, SHIFT, EXT>(
+            L, i0, i1, o0, o1, phio, E, x, y, z,
+            [&](int dx, int dy, int dz) { return sm.phi[slot5(x + dx)][ty + 2 + dy][tz + 2 + dz]; },
+            [&](int k, int dx, int dy, int dz) { return sm.n[k][slot3(x + dx)][ty + 1 + dy][tz + 1 + dz]; },
+            [&](const int o[3]) {
+                if constexpr (RING)
+                    return ring.s[(x + o[0] + 9) % 3][ty + 1 + o[1]][tz + 1 + o[2]];
+                else if constexpr (LIN == FS && !SHIFT)
+                    return fs64_at((const double *)i1,
+                                   ((unsigned)wrapm(x + o[0], L.nx) * L.ny + wrapm(y + o[1], L.ny)) * L.nz +
+                                       wrapm(z + o[2], L.nz));
+                else
+                    return 0;
+            });
     }
 }
 
-template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT = false>
-__global__ __launch_bounds__(TZ * KTY<R>, (EXT && LIN == FS) ? KWAVES_FSX : KWAVES<R>) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
+template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT = false, bool SKIP = false>
+__global__ __launch_bounds__(TZ * KTY<R>, KWAVES<R>) void k_collide(Lat L, const void *__restrict__ i0, const void *__restrict__ i1,
                                                  void *__restrict__ o0, void *__restrict__ o1,
                                                  const P *__restrict__ phi, const P *__restrict__ phio, int xc,
                                                  Ext E = Ext{})
 {
     unsigned bx, by, bz;
     tile_of_block(L, bx, by, bz);
-    collide_tile<LIN, LOUT, S, R, P, SHIFT, EXT>(L, i0, i1, o0, o1, phi, phio, xc, E, bx, by, bz);
+    collide_tile<LIN, LOUT, S, R, P, SHIFT, EXT, SKIP>(L, i0, i1, o0, o1, phi, phio, xc, E, bx, by, bz);
 }
 
 // ------------------------------------------------------------- diagnostics
@@ -2264,15 +2294,15 @@ __global__ __launch_bounds__(256) void k_unpack_fs(Lat L, const S *__restrict__ 
     }
 }
 
-// Unpack the factorised state with walls, mirrors and particles: the slot
-// each live node gathers (pull_slot), rebuilt from the recolour scalars of
-// its node, or taken from the side arrays rs/bs where that node is FS_SOLID;
-// zero on ghost layers (as k_unpack).
+// Unpack the factorised state with walls, mirrors and particles: slot i of
+// a live node from x - c_i, split with the scalars fs_src names, or with the
+// side arrays (side0, side1) where those are FS_SOLID; zero on ghost layers
+// (as k_unpack).
 template <class S, class P, bool SHIFT>
 __global__ __launch_bounds__(256) void k_unpack_fs_ext(Lat L, Ext E, const S *__restrict__ fs,
-                                                       const void *__restrict__ scv, const S *__restrict__ rs,
-                                                       const S *__restrict__ bs, const P *__restrict__ phio,
-                                                       double *__restrict__ aos, int red)
+                                                       const void *__restrict__ scv, const void *__restrict__ side0,
+                                                       const void *__restrict__ side1, double *__restrict__ aos,
+                                                       int red)
 {
     const int z = blockIdx.x * TZ + threadIdx.x;
     const int y = blockIdx.y * TY + threadIdx.y;
@@ -2289,18 +2319,19 @@ __global__ __launch_bounds__(256) void k_unpack_fs_ext(Lat L, Ext E, const S *__
     }
     const unsigned bb = wall_bb_mask(E, L, x, y, z);
     const unsigned madj = mirror_adjacent(E, L, x, y, z);
-    const S *cs = red ? rs : bs;
     static_for<Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        unsigned m, flip;
-        int d, o[3];
-        pull_slot_f<i>(xo, yo, zo, bb, madj, m, d, flip, o);
-        const size_t k = (size_t)d * N + m;
+        const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+        int o[3];
+        unsigned flip;
+        fs_src<i>(bb, madj, o, flip);
+        const unsigned ms = xo[1 + o[0]] + yo[1 + o[1]] + zo[1 + o[2]];
+        const size_t k = (size_t)i * N + m;
         double v;
         if constexpr (!SHIFT) {
-            const FsScal64 sc = fs64_at((const double *)scv, m);
+            const FsScal64 sc = fs64_at((const double *)scv, ms);
             if (solid_scal(sc)) {
-                v = double(StoreOf<S>::ld(cs + k));
+                v = double(StoreOf<S>::ld((const S *)(red ? side0 : side1) + k));
             } else {
                 const double f = double(StoreOf<S>::ld(fs + k));
                 double r = sc.a * f;
@@ -2308,29 +2339,70 @@ __global__ __launch_bounds__(256) void k_unpack_fs_ext(Lat L, Ext E, const S *__
                 v = red ? r : f - r;
             }
         } else {
-            const float4 g = ((const float4 *)scv)[m];
+            const float4 g = ((const float4 *)scv)[ms];
+            const double f = double(StoreOf<S>::ld(fs + k)) + W[i] * L.rho0;
+            double A, B, r;
             if (solid_scal(g)) {
-                v = double(StoreOf<S>::ld(cs + k)) + W[i] * (L.rho0 * shift_d(float(phio[n]), red));
+                dec_slot(((const float *)side0)[k], A, B); // exact in FP64
+                r = A * f;
             } else {
-                double A, B;
                 decode_frac(g.x, A, B); // exact in FP64
-                const double f = double(StoreOf<S>::ld(fs + k)) + W[i] * L.rho0;
-                double gc = 0.0;
-                if (CX[i] != 0) { double t = CX[i] * double(g.y); gc += (flip & 1u) ? -t : t; }
-                if (CY[i] != 0) { double t = CY[i] * double(g.z); gc += (flip & 2u) ? -t : t; }
-                if (CZ[i] != 0) { double t = CZ[i] * double(g.w); gc += (flip & 4u) ? -t : t; }
-                const double gd = W[i] * gc;
-                double r;
+                const double gd = W[i] * gdot_f<i, double>(flip, g.y, g.z, g.w);
                 if (__builtin_signbitf(g.x)) { // red majority: the blue part first
                     const double b = B * f - gd;
                     r = f - b;
                 } else {
                     r = A * f + gd;
                 }
-                v = red ? r : f - r;
             }
+            v = red ? r : f - r;
         }
         aos[(size_t)n * Q + i] = v;
+    });
+}
+
+// Wall and mirror ghost nodes of the factorised layout with walls, mirrors
+// and particles: slot i of ghost node g, read by the live node x = g + c_i,
+// takes the total (and the side arrays' colours) of the slot that the
+// reference's boundary fills put there (S:1443-1592; pull_slot): x's own
+// opposite slot off a wall (walls first), the mirror image's reflected slot
+// across mirror faces.  One thread per node of face blockIdx.y (axis y / 2,
+// side y % 2); edge and corner ghosts are written once per face with equal
+// values.
+template <class S, bool SHIFT>
+__global__ __launch_bounds__(256) void k_fill_ghosts(Lat L, Ext E, S *__restrict__ fs, void *__restrict__ side0,
+                                                     void *__restrict__ side1)
+{
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int axis = blockIdx.y / 2, side = blockIdx.y % 2;
+    if (!E.wall[axis][side] && !E.mirror[axis][side]) return;
+    const int n3[3] = {L.nx, L.ny, L.nz};
+    const int u = (axis + 1) % 3, v = (axis + 2) % 3;
+    if (t >= (unsigned)(n3[u] * n3[v])) return;
+    int g[3];
+    g[u] = (int)(t / n3[v]);
+    g[v] = (int)(t % n3[v]);
+    g[axis] = side ? n3[axis] - 1 : 0;
+    const size_t N = (size_t)L.N;
+    const unsigned ng = ((unsigned)g[0] * L.ny + g[1]) * L.nz + g[2];
+    static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        if (i == 0) return;
+        const int x = wrapm(g[0] + CX[i], L.nx), y = wrapm(g[1] + CY[i], L.ny), z = wrapm(g[2] + CZ[i], L.nz);
+        if (is_ghost(E, L, x, y, z)) return;
+        unsigned xo[3], yo[3], zo[3];
+        node_offsets(L, x, y, z, xo, yo, zo);
+        unsigned m;
+        int d;
+        pull_slot<i>(xo, yo, zo, wall_bb_mask(E, L, x, y, z), mirror_adjacent(E, L, x, y, z), m, d);
+        const size_t ks = (size_t)d * N + m, kd = (size_t)i * N + ng;
+        fs[kd] = fs[ks];
+        if constexpr (SHIFT) {
+            ((float *)side0)[kd] = ((const float *)side0)[ks];
+        } else {
+            ((S *)side0)[kd] = ((const S *)side0)[ks];
+            ((S *)side1)[kd] = ((const S *)side1)[ks];
+        }
     });
 }
 

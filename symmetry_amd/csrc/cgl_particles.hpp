@@ -540,14 +540,17 @@ __global__ __launch_bounds__(PB) void k_curve(Lat L, Ext E, Part Pt, const unsig
 }
 
 // k_curve for the factorised layout: the node's own post-collision slot opp
-// is rebuilt from its recolour scalars (fo, so: the step's output), and the
-// bounce-back value goes, as red and blue, into the side arrays ro, bo of
-// the solid slot the next gather reads (K2 marked that node FS_SOLID).
-// Shifted storage: deviations from w_i rho0 s_red and w_i rho0 s_blue of x,
-// formed in FP64 from the colour fractions of x.
+// is rebuilt from its recolour scalars (fo, so: the step's output, FP64
+// arithmetic), the bounce-back terms are added per colour, and the slot that
+// x gathers for direction i -- slot i of node x - c_i, the solid node or, for
+// a particle across a mirror face, the ghost node that the gather reads --
+// takes the total and the colours (side arrays: FP64 red and blue, shifted
+// storage the slot's minority fraction).  Runs after k_fill_ghosts, whose
+// value of that ghost slot it replaces, as curve follows the boundary fills
+// (S:247-251).
 template <class S, bool SHIFT>
-__global__ __launch_bounds__(PB) void k_curve_fs(Lat L, Ext E, Part Pt, const S *fo, const void *so, S *ro, S *bo,
-                                                 const float *phis)
+__global__ __launch_bounds__(PB) void k_curve_fs(Lat L, Ext E, Part Pt, S *fo, const void *so, void *side0,
+                                                 void *side1)
 {
     const int p = blockIdx.x;
     const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
@@ -563,34 +566,186 @@ __global__ __launch_bounds__(PB) void k_curve_fs(Lat L, Ext E, Part Pt, const S 
         const double rr = E.mac[N + n], rbv = E.mac[2 * N + n];
         for (int i = 1; i < Q; ++i) {
             if (!((links >> i) & 1u)) continue;
-            unsigned ms;
-            int ds;
-            read_slot(E, L, X, i, ms, ds);
+            const unsigned m = nb_index(L, X, -CX[i], -CY[i], -CZ[i]);
             const double star = uw[0] * CX[i] + uw[1] * CY[i] + uw[2] * CZ[i];
             const int o = OPC[i];
-            const double f = double(StoreOf<S>::ld(fo + (size_t)o * N, n));
+            double f = double(StoreOf<S>::ld(fo + (size_t)o * N, n));
             double vr, vb;
             if constexpr (SHIFT) {
                 const float4 g = ((const float4 *)so)[n];
                 double A, B;
                 decode_frac(g.x, A, B);
                 const double gc = W[o] * (CX[o] * double(g.y) + CY[o] * double(g.z) + CZ[o] * double(g.w));
-                const double wr = W[o] * L.rho0;
-                vr = A * f + wr * (A - shift_d(phis[n], 1)) + gc;
-                vb = B * f + wr * (B - shift_d(phis[n], 0)) - gc;
-                vr = vr + 6.0 * W[i] * rr * star;
-                vb = vb + 6.0 * W[i] * rbv * star;
+                f = f + W[o] * L.rho0;
+                if (__builtin_signbitf(g.x)) {
+                    vb = B * f - gc;
+                    vr = f - vb;
+                } else {
+                    vr = A * f + gc;
+                    vb = f - vr;
+                }
             } else {
                 const FsScal64 sc = fs64_at((const double *)so, n);
-                double r = sc.a * f;
-                r = r + (sc.g * W[o]) * dot_c<double>(o, sc.nx, sc.ny, sc.nz);
-                vr = r + 6.0 * W[i] * rr * star;
-                vb = (f - r) + 6.0 * W[i] * rbv * star;
+                vr = sc.a * f;
+                vr = vr + (sc.g * W[o]) * dot_c<double>(o, sc.nx, sc.ny, sc.nz);
+                vb = f - vr;
             }
-            StoreOf<S>::st(ro + (size_t)ds * N, ms, (decltype(StoreOf<S>::ld(ro)))vr);
-            StoreOf<S>::st(bo + (size_t)ds * N, ms, (decltype(StoreOf<S>::ld(bo)))vb);
+            vr = vr + 6.0 * W[i] * rr * star;
+            vb = vb + 6.0 * W[i] * rbv * star;
+            const size_t k = (size_t)i * N + m;
+            if constexpr (SHIFT) {
+                fo[k] = (S)(float)(vr + vb - W[i] * L.rho0);
+                ((float *)side0)[k] = enc_slot(vr, vb);
+            } else {
+                fo[k] = vr + vb;
+                ((S *)side0)[k] = vr;
+                ((S *)side1)[k] = vb;
+            }
         }
     }
+}
+
+// ------------------------------------- special nodes of the factorised layout
+// With walls, mirrors or particles in the factorised layout the tile kernels
+// leave the special lanes (special_lane: ghost layers, nodes next to a ghost
+// layer, the particles' band) to these per-node kernels, which read the phase
+// field and the recolour scalars from memory instead of the LDS rings and
+// take the general gather (bounce-back and mirror slots, side arrays of
+// solid nodes).  BAND = false: one thread per node of the live layer next to
+// face blockIdx.y (axis y / 2, side y % 2) where that face is a wall or a
+// mirror; BAND = true: workgroup (p, s) walks slice s of particle p's box
+// around its band (R + 3).  A node reached twice (an edge, a band next to a
+// face, overlapping bands) is computed twice with equal results.
+template <bool BAND, class F>
+__device__ __forceinline__ void for_special_nodes(const Lat &L, const Ext &E, const Part &Pt, F f)
+{
+    const int n3[3] = {L.nx, L.ny, L.nz};
+    if constexpr (!BAND) {
+        const int axis = blockIdx.y / 2, side = blockIdx.y % 2;
+        if (!E.wall[axis][side] && !E.mirror[axis][side]) return;
+        const int u = (axis + 1) % 3, v = (axis + 2) % 3;
+        const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+        if (t >= (unsigned)(n3[u] * n3[v])) return;
+        int c[3];
+        c[u] = (int)(t / n3[v]);
+        c[v] = (int)(t % n3[v]);
+        c[axis] = side ? n3[axis] - 2 : 1;
+        f(c[0], c[1], c[2]);
+    } else {
+        const int p = blockIdx.x;
+        const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 3.0);
+        for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
+            int X[3];
+            unsigned n;
+            box_node(L, b, k, X, n);
+            if (!box_in_domain(E, L, X)) continue;
+            f(wrapm(X[0], L.nx), wrapm(X[1], L.ny), wrapm(X[2], L.nz));
+        }
+    }
+}
+
+// The unit normal at node (px, py, pz) as the tile kernel's normals pass
+// forms it with walls and particles (S:498-532, S:536-594): zero at wall
+// ghosts and solid nodes, the image's reflected at mirror ghosts, the wetted
+// normal stored by k_wet_normals at live fluid nodes next to a particle.
+template <class R, class P>
+__device__ __forceinline__ void ext_normal(const Lat &L, const Ext &E, const P *__restrict__ phi, int px, int py,
+                                           int pz, R nv[3])
+{
+    const int gxw = wrapm(px, L.nx), gyw = wrapm(py, L.ny), gzw = wrapm(pz, L.nz);
+    R nx = 0, ny = 0, nz = 0;
+    bool skip = is_wall_ghost(E, L, gxw, gyw, gzw);
+    unsigned nt = 0, A = 0;
+    int ox = 0, oy = 0, oz = 0;
+    if (!skip) {
+        A = mirror_axes(E, L, gxw, gyw, gzw);
+        ox = (A & 1u) ? (gxw == 0 ? 1 : -1) : 0;
+        oy = (A & 2u) ? (gyw == 0 ? 1 : -1) : 0;
+        oz = (A & 4u) ? (gzw == 0 ? 1 : -1) : 0;
+    }
+    const unsigned sx = (unsigned)L.ny * (unsigned)L.nz;
+    if (!skip && E.ntype) {
+        nt = at(E.ntype, (unsigned)gxw * sx + (unsigned)gyw * L.nz + gzw);
+        skip = (nt & NT_SOLID) != 0;
+    }
+    if (!skip) {
+        R gx, gy, gz;
+        grad19<R>(
+            [&](int dx, int dy, int dz) {
+                return phi[(unsigned)wrapm(gxw + ox + dx, L.nx) * sx + (unsigned)wrapm(gyw + oy + dy, L.ny) * L.nz +
+                           wrapm(gzw + oz + dz, L.nz)];
+            },
+            gx, gy, gz);
+        const bool fl = unit_normal<R>(gx, gy, gz, nx, ny, nz);
+        if (fl && (nt & NT_POUT)) {
+            const size_t N = (size_t)L.N;
+            const unsigned im = (unsigned)(gxw + ox) * sx + (unsigned)(gyw + oy) * L.nz + (gzw + oz);
+            nx = R(E.der[3 * N + im]);
+            ny = R(E.der[4 * N + im]);
+            nz = R(E.der[5 * N + im]);
+        }
+        if (A & 1u) nx = -nx;
+        if (A & 2u) ny = -ny;
+        if (A & 4u) nz = -nz;
+    }
+    nv[0] = nx;
+    nv[1] = ny;
+    nv[2] = nz;
+}
+
+// K1 of the special nodes: calmacros as node_moments forms it (general
+// gather), the band's FP64 fields and phi.
+template <class S, class R, class P, bool SHIFT, bool BAND>
+__global__ __launch_bounds__(256) void k_moments_special(Lat L, const void *__restrict__ p0,
+                                                         const void *__restrict__ p1, const P *__restrict__ phio,
+                                                         P *__restrict__ phi, Ext E, Part Pt)
+{
+    for_special_nodes<BAND>(L, E, Pt, [&](int x, int y, int z) {
+        const unsigned n0 = ((unsigned)x * L.ny + y) * L.nz + z;
+        if (!special_lane(E, L, x, y, z, n0)) return;
+        unsigned n, nt;
+        Moments<R, SHIFT> mo;
+        if (!node_moments<FS, S, R, P, SHIFT, true>(L, p0, p1, phio, E, x, y, z, n, mo, nt)) return;
+        band_macros(L, E, n, nt, mo);
+        at(phi, n) = P(mo.phi);
+    });
+}
+
+// K2 of the special nodes: lane_step with the phase field, the normals of
+// the node and its 18 neighbours (ext_normal) and the recolour scalars read
+// from memory.
+template <class S, class R, class P, bool SHIFT, bool BAND>
+__global__ __launch_bounds__(256) void k_collide_special(Lat L, const void *__restrict__ i0,
+                                                         const void *__restrict__ i1, void *__restrict__ o0,
+                                                         void *__restrict__ o1, const P *__restrict__ phi,
+                                                         const P *__restrict__ phio, Ext E, Part Pt)
+{
+    for_special_nodes<BAND>(L, E, Pt, [&](int x, int y, int z) {
+        const unsigned sx = (unsigned)L.ny * (unsigned)L.nz;
+        if (!special_lane(E, L, x, y, z, (unsigned)x * sx + (unsigned)y * L.nz + z)) return;
+        // normals by offset key (dx + 1) 9 + (dy + 1) 3 + dz + 1
+        R nn[27][3];
+        static_for<Q>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            ext_normal<R, P>(L, E, phi, x + CX[i], y + CY[i], z + CZ[i],
+                             nn[(CX[i] + 1) * 9 + (CY[i] + 1) * 3 + CZ[i] + 1]);
+        });
+        lane_step<FS, FS, S, R, P, SHIFT, true>(
+            L, i0, i1, o0, o1, phio, E, x, y, z,
+            [&](int dx, int dy, int dz) {
+                return phi[(unsigned)wrapm(x + dx, L.nx) * sx + (unsigned)wrapm(y + dy, L.ny) * L.nz +
+                           wrapm(z + dz, L.nz)];
+            },
+            [&](int k, int dx, int dy, int dz) { return nn[(dx + 1) * 9 + (dy + 1) * 3 + dz + 1][k]; },
+            [&](const int o[3]) {
+                const unsigned m = (unsigned)wrapm(x + o[0], L.nx) * sx + (unsigned)wrapm(y + o[1], L.ny) * L.nz +
+                                   wrapm(z + o[2], L.nz);
+                if constexpr (SHIFT)
+                    return at((const float4 *)i1, m);
+                else
+                    return fs64_at((const double *)i1, m);
+            });
+    });
 }
 
 // Fixed-order FP64 sum of K values per thread over the workgroup: an xor
@@ -665,13 +820,13 @@ __global__ __launch_bounds__(PB) void k_momentum(Lat L, Ext E, Part Pt, const un
     block_sum<6>(acc, Pt.part_mom + ((size_t)p * LSLICES + blockIdx.y) * 6);
 }
 
-// k_momentum for the factorised layout: the bounced-back f_i(x) is the red
-// plus blue value in the side arrays rs, bs of the solid slot, ffi_opp(x)
-// the node's own stored post-collision total fs.  Cold start: the slot that x
-// gathers for opp, from the side arrays where its node is FS_SOLID.
+// k_momentum for the factorised layout: every node's slots hold their totals
+// (solid nodes' and ghost nodes' too), so f_i(x) is slot i of x - c_i, the
+// bounced-back value, and ffi_opp(x) slot opp of x, the node's own
+// post-collision population; at the cold start ffi is the slot x gathers
+// for opp, slot opp of x - c_opp.
 template <class S, bool SHIFT>
-__global__ __launch_bounds__(PB) void k_momentum_fs(Lat L, Ext E, Part Pt, const S *fs, const void *sc, const S *rs,
-                                                    const S *bs, int cold)
+__global__ __launch_bounds__(PB) void k_momentum_fs(Lat L, Ext E, Part Pt, const S *fs, int cold)
 {
     const int p = blockIdx.x;
     const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
@@ -688,26 +843,10 @@ __global__ __launch_bounds__(PB) void k_momentum_fs(Lat L, Ext E, Part Pt, const
         for (int i = 1; i < Q; ++i) {
             if (!((links >> i) & 1u)) continue;
             const int o = OPC[i];
-            unsigned mi;
-            int di;
-            read_slot(E, L, X, i, mi, di);
-            double fi = double(StoreOf<S>::ld(rs + (size_t)di * N, mi)) + double(StoreOf<S>::ld(bs + (size_t)di * N, mi));
-            double fo;
-            if (cold) {
-                unsigned no;
-                int dof;
-                read_slot(E, L, X, o, no, dof);
-                bool solid;
-                if constexpr (SHIFT)
-                    solid = solid_scal(((const float4 *)sc)[no]);
-                else
-                    solid = solid_scal(fs64_at((const double *)sc, no));
-                fo = solid ? double(StoreOf<S>::ld(rs + (size_t)dof * N, no)) +
-                                 double(StoreOf<S>::ld(bs + (size_t)dof * N, no))
-                           : double(StoreOf<S>::ld(fs + (size_t)dof * N, no));
-            } else {
-                fo = double(StoreOf<S>::ld(fs + (size_t)o * N, n));
-            }
+            const unsigned mi = nb_index(L, X, -CX[i], -CY[i], -CZ[i]);
+            const unsigned no = cold ? nb_index(L, X, -CX[o], -CY[o], -CZ[o]) : n;
+            double fi = double(StoreOf<S>::ld(fs + (size_t)i * N + mi));
+            double fo = double(StoreOf<S>::ld(fs + (size_t)o * N + no));
             if constexpr (SHIFT) {
                 fi += W[i] * L.rho0;
                 fo += W[o] * L.rho0;
