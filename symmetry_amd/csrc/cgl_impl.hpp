@@ -369,7 +369,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                 const int xc = K1_XC;
                 const dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, (h->L.nx + xc - 1) / xc);
                 k_moments_fsx_ext<S, R, P, SHIFT><<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, xc, E);
-                if (ghost_faces(h))
+                if (mirrors(h))
                     k_moments_special<S, R, P, SHIFT, false>
                         <<<face_grid(h), 256, 0, h->stream>>>(h->L, i0, i1, phio, phi, E, h->P);
                 if (h->npar)
@@ -548,7 +548,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             // the tile kernel for all but the special lanes, then those
             k_collide<FS, FS, S, R, P, SHIFT, false, true>
                 <<<g2, k2_block<R>(), 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, h->xc, En);
-            if (ghost_faces(h))
+            if (mirrors(h))
                 k_collide_special<S, R, P, SHIFT, false>
                     <<<face_grid(h), 256, 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, En, h->P);
             if (h->npar)

@@ -412,13 +412,13 @@ __device__ __forceinline__ unsigned wall_bb_mask(const Ext &E, const Lat &L, int
 
 // Lanes that the tile kernels of the factorised layout with walls, mirrors or
 // particles leave to the per-node kernels (k_moments_special,
-// k_collide_special): ghost layers, nodes next to a ghost
-// layer, and the particles' band (NEAR; it contains every solid and refilled
-// node and every node next to one).
+// k_collide_special): ghost layers, nodes next to a mirror ghost layer, and
+// the particles' band (NEAR; it contains every solid and refilled node and
+// every node next to one).  Nodes next to a wall stay in the tile kernels
+// (rev_scal).
 __device__ __forceinline__ bool special_lane(const Ext &E, const Lat &L, int x, int y, int z, unsigned n)
 {
-    return is_ghost(E, L, x, y, z) || wall_bb_mask(E, L, x, y, z) || mirror_adjacent(E, L, x, y, z) ||
-           (E.ntype && (at(E.ntype, n) & NT_NEAR));
+    return is_ghost(E, L, x, y, z) || mirror_adjacent(E, L, x, y, z) || (E.ntype && (at(E.ntype, n) & NT_NEAR));
 }
 
 // Source slot of direction i at a live node in the pull layout: the upstream
@@ -555,6 +555,14 @@ template <int I, class R> __device__ __forceinline__ R gdot_f(unsigned flip, flo
     if (CZ[I] != 0) { R t = CZ[I] == 1 ? R(gz) : -R(gz); gc += (flip & 4u) ? -t : t; }
     return gc;
 }
+
+// Nodes next to a wall in the tile kernels of the factorised layout: a slot
+// read from a wall ghost (bit i of bb) holds the node's own opposite slot
+// (k_fill_ghosts) and is split with the node's own scalars in the reversed
+// direction, i.e. with its gradient term negated (rev_scal): the same
+// rounding as dot_c of the opposite direction.
+__device__ __forceinline__ float4 rev_scal(const float4 &g) { return make_float4(g.x, -g.y, -g.z, -g.w); }
+__device__ __forceinline__ FsScal64 rev_scal(const FsScal64 &g) { return FsScal64{g.a, g.g, -g.nx, -g.ny, -g.nz}; }
 
 // Recolour scalars of node m in the FP64 factorised layout.
 __device__ __forceinline__ FsScal64 fs64_at(const double *sc, unsigned m)
@@ -980,7 +988,11 @@ __device__ __forceinline__ void moments_fsx_tile(const Lat &L, const void *__res
             if constexpr (EXT) {
                 // special lanes (special_lane) are left to k_moments_special
                 if (!special_lane(E, L, x, y, z, n)) {
-                    gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, scal, nd, R(L.rho0));
+                    const unsigned bb = wall_bb_mask(E, L, x, y, z);
+                    const auto own = rev_scal(scal(0, 0u));
+                    gather_fs<S, R, SHIFT>(
+                        fs, N, xo, yo, zo, [&](int i, unsigned) { return ((bb >> i) & 1u) ? own : scal(i, 0u); }, nd,
+                        R(L.rho0));
                     at(phi, n) = P(moments<R, SHIFT>(L, nd).phi);
                 }
             } else {
@@ -1483,11 +1495,15 @@ __device__ __forceinline__ bool away_from_walls(const Ext &E, const Lat &L, int 
 // field phi_at(dx, dy, dz) and the unit normals n_at(k, dx, dy, dz) around it
 // and, for a factorised input, the recolour scalars scal_at(o) of the node at
 // offset o: from the tile's LDS rings in k_collide, from memory in
-// k_collide_special.
-template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT, class PhiAt, class NAt, class ScalAt>
+// k_collide_special.  WALLS (factorised tile kernel with walls, mirrors or
+// particles): wbb holds the node's wall bounce-back directions (rev_scal);
+// such a node is next to a wall, where no curvature force acts (S:624).
+template <int LIN, int LOUT, class S, class R, class P, bool SHIFT, bool EXT, bool WALLS = false, class PhiAt,
+          class NAt, class ScalAt>
 __device__ __forceinline__ void lane_step(const Lat &L, const void *__restrict__ i0, const void *__restrict__ i1,
                                           void *__restrict__ o0, void *__restrict__ o1, const P *__restrict__ phio,
-                                          const Ext &E, int x, int y, int z, PhiAt phi_at, NAt n_at, ScalAt scal_at)
+                                          const Ext &E, int x, int y, int z, PhiAt phi_at, NAt n_at, ScalAt scal_at,
+                                          unsigned wbb = 0)
 {
     unsigned nt = 0, bb = 0, madj = 0;
     if constexpr (EXT) {
@@ -1509,7 +1525,7 @@ __device__ __forceinline__ void lane_step(const Lat &L, const void *__restrict__
     unsigned xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
     const unsigned n = xo[1] + yo[1] + zo[1];
-    bool force_on = flag;
+    bool force_on = flag && !wbb;
     if constexpr (EXT) {
         if (E.ntype) nt = at(E.ntype, n);
         if (nt & NT_SOLID) flag = false;
@@ -1720,12 +1736,16 @@ __device__ __forceinline__ void lane_step(const Lat &L, const void *__restrict__
             // solid nodes
             gather_fs_ext<true, S, R, SHIFT>((const S *)i0, E.rbi[0], E.rbi[1], N, xo, yo, zo, bb, madj, scal_at, nd,
                                              R(L.rho0));
-        } else if constexpr (LIN == FS && SHIFT) {
+        } else if constexpr (LIN == FS && (SHIFT || WALLS)) {
+            // the periodic gather; next to a wall (wbb) the bounce-back slots
+            // are split with the node's own scalars (rev_scal)
+            const int oc[3] = {0, 0, 0};
+            const auto own = rev_scal(scal_at(oc));
             gather_fs<S, R, SHIFT>(
                 (const S *)i0, N, xo, yo, zo,
                 [&](int i, unsigned) {
                     const int o[3] = {-CX[i], -CY[i], -CZ[i]};
-                    return scal_at(o);
+                    return ((wbb >> i) & 1u) ? own : scal_at(o);
                 },
                 nd, R(L.rho0));
         } else {
@@ -1872,9 +1892,12 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
         normals(x + 1, x - 2, x + 2);
         __syncthreads();
         if (!active) continue;
-        if constexpr (SKIP)
+        unsigned wbb = 0;
+        if constexpr (SKIP) {
             if (special_lane(E, L, x, y, z, ((unsigned)x * L.ny + y) * L.nz + z)) continue;
-        lane_step<LIN, LOUT, S, R, P, SHIFT, EXT>(
+            wbb = wall_bb_mask(E, L, x, y, z);
+        }
+        lane_step<LIN, LOUT, S, R, P, SHIFT, EXT, SKIP>(
             L, i0, i1, o0, o1, phio, E, x, y, z,
             [&](int dx, int dy, int dz) { return sm.phi[slot5(x + dx)][ty + 2 + dy][tz + 2 + dz]; },
             [&](int k, int dx, int dy, int dz) { return sm.n[k][slot3(x + dx)][ty + 1 + dy][tz + 1 + dz]; },
@@ -1887,7 +1910,8 @@ __device__ __forceinline__ void collide_tile(const Lat &L, const void *__restric
                                        wrapm(z + o[2], L.nz));
                 else
                     return 0;
-            });
+            },
+            wbb);
     }
 }
 

@@ -612,8 +612,8 @@ __global__ __launch_bounds__(PB) void k_curve_fs(Lat L, Ext E, Part Pt, S *fo, c
 // field and the recolour scalars from memory instead of the LDS rings and
 // take the general gather (bounce-back and mirror slots, side arrays of
 // solid nodes).  BAND = false: one thread per node of the live layer next to
-// face blockIdx.y (axis y / 2, side y % 2) where that face is a wall or a
-// mirror; BAND = true: workgroup (p, s) walks slice s of particle p's box
+// face blockIdx.y (axis y / 2, side y % 2) where that face is a mirror;
+// BAND = true: workgroup (p, s) walks slice s of particle p's box
 // around its band (R + 3).  A node reached twice (an edge, a band next to a
 // face, overlapping bands) is computed twice with equal results.
 template <bool BAND, class F>
@@ -622,7 +622,7 @@ __device__ __forceinline__ void for_special_nodes(const Lat &L, const Ext &E, co
     const int n3[3] = {L.nx, L.ny, L.nz};
     if constexpr (!BAND) {
         const int axis = blockIdx.y / 2, side = blockIdx.y % 2;
-        if (!E.wall[axis][side] && !E.mirror[axis][side]) return;
+        if (!E.mirror[axis][side]) return;
         const int u = (axis + 1) % 3, v = (axis + 2) % 3;
         const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
         if (t >= (unsigned)(n3[u] * n3[v])) return;
