@@ -883,6 +883,10 @@ static int ensure_lists(cgl_handle *h, double rmax)
 {
     const double w = 2.0 * ceil(rmax) + 5.0;
     if (!(w * w * w < 2147483647.0)) return fail(CGL_ERR_ARG, "particle radius %g too large", rmax);
+    // the band box of the per-node kernels, box_of(centre, R, 3): at most
+    // 2 ceil(R) + 9 nodes per axis, one thread per node
+    const double wb = w + 4.0;
+    h->band_blocks = (unsigned)((wb * wb * wb + cgl::PB - 1) / cgl::PB);
     const int cap = (int)((w * w * w + cgl::LSLICES - 1) / cgl::LSLICES);
     if (h->P.lout && cap <= h->P.lcap) return CGL_OK;
     unsigned **v[3] = {&h->P.lout, &h->P.lmask, &h->P.lin};

@@ -68,6 +68,7 @@ struct cgl_handle {
     cgl::ScBufs sb;      // fixed-point partial sums
     hipEvent_t *tev;     // timing runs: events around the wall/particle step's moments and collide kernels
     int stage;           // staged step: the last stage run (STAGE_*), STAGE_NONE between steps
+    unsigned band_blocks; // workgroups per particle of the band kernels (one thread per node of its band box)
 };
 
 // Stages of a step driven through the per-stage entry points, in order.
@@ -373,8 +374,8 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                     k_moments_special<S, R, P, SHIFT, false>
                         <<<face_grid(h), 256, 0, h->stream>>>(h->L, i0, i1, phio, phi, E, h->P);
                 if (h->npar)
-                    k_moments_special<S, R, P, SHIFT, true>
-                        <<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, i0, i1, phio, phi, E, h->P);
+                    k_moments_special<S, R, P, SHIFT, true><<<dim3(h->band_blocks, h->npar), PB, 0, h->stream>>>(
+                        h->L, i0, i1, phio, phi, E, h->P);
             }
         } else if (full) {
             k_moments<RB, S, R, P, SHIFT, true, true>
@@ -552,8 +553,8 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                 k_collide_special<S, R, P, SHIFT, false>
                     <<<face_grid(h), 256, 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, En, h->P);
             if (h->npar)
-                k_collide_special<S, R, P, SHIFT, true>
-                    <<<pgrid(h, PSLICES), PB, 0, h->stream>>>(h->L, i0, i1, o0, o1, phi, phio, En, h->P);
+                k_collide_special<S, R, P, SHIFT, true><<<dim3(h->band_blocks, h->npar), PB, 0, h->stream>>>(
+                    h->L, i0, i1, o0, o1, phi, phio, En, h->P);
         }
         if (h->tev) (void)hipEventRecord(h->tev[3], h->stream);
         h->state_layout = h->layout;

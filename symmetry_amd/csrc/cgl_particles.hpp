@@ -613,8 +613,9 @@ __global__ __launch_bounds__(PB) void k_curve_fs(Lat L, Ext E, Part Pt, S *fo, c
 // take the general gather (bounce-back and mirror slots, side arrays of
 // solid nodes).  BAND = false: one thread per node of the live layer next to
 // face blockIdx.y (axis y / 2, side y % 2) where that face is a mirror;
-// BAND = true: workgroup (p, s) walks slice s of particle p's box
-// around its band (R + 3).  A node reached twice (an edge, a band next to a
+// BAND = true: one thread per node of particle blockIdx.y's box around its
+// band (R + 3), node blockIdx.x PB + threadIdx.x in box order (band_blocks
+// workgroups per particle).  A node reached twice (an edge, a band next to a
 // face, overlapping bands) is computed twice with equal results.
 template <bool BAND, class F>
 __device__ __forceinline__ void for_special_nodes(const Lat &L, const Ext &E, const Part &Pt, F f)
@@ -632,15 +633,15 @@ __device__ __forceinline__ void for_special_nodes(const Lat &L, const Ext &E, co
         c[axis] = side ? n3[axis] - 2 : 1;
         f(c[0], c[1], c[2]);
     } else {
-        const int p = blockIdx.x;
+        const int p = blockIdx.y;
         const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 3.0);
-        for (unsigned k = blockIdx.y * PB + threadIdx.x; k < box_size(b); k += gridDim.y * PB) {
-            int X[3];
-            unsigned n;
-            box_node(L, b, k, X, n);
-            if (!box_in_domain(E, L, X)) continue;
-            f(wrapm(X[0], L.nx), wrapm(X[1], L.ny), wrapm(X[2], L.nz));
-        }
+        const unsigned k = blockIdx.x * PB + threadIdx.x;
+        if (k >= box_size(b)) return;
+        int X[3];
+        unsigned n;
+        box_node(L, b, k, X, n);
+        if (!box_in_domain(E, L, X)) return;
+        f(wrapm(X[0], L.nx), wrapm(X[1], L.ny), wrapm(X[2], L.nz));
     }
 }
 
@@ -723,20 +724,26 @@ __global__ __launch_bounds__(256) void k_collide_special(Lat L, const void *__re
     for_special_nodes<BAND>(L, E, Pt, [&](int x, int y, int z) {
         const unsigned sx = (unsigned)L.ny * (unsigned)L.nz;
         if (!special_lane(E, L, x, y, z, (unsigned)x * sx + (unsigned)y * L.nz + z)) return;
-        // normals by offset key (dx + 1) 9 + (dy + 1) 3 + dz + 1
+        // normals by offset key (dx + 1) 9 + (dy + 1) 3 + dz + 1, each formed
+        // at its first use: the node's own always, the neighbours' only for
+        // the curvature of an interface node (the keys are compile-time
+        // constants in lane_step, so the flags fold away)
         R nn[27][3];
-        static_for<Q>([&](auto ic) {
-            constexpr int i = decltype(ic)::value;
-            ext_normal<R, P>(L, E, phi, x + CX[i], y + CY[i], z + CZ[i],
-                             nn[(CX[i] + 1) * 9 + (CY[i] + 1) * 3 + CZ[i] + 1]);
-        });
+        bool have[27] = {};
         lane_step<FS, FS, S, R, P, SHIFT, true>(
             L, i0, i1, o0, o1, phio, E, x, y, z,
             [&](int dx, int dy, int dz) {
                 return phi[(unsigned)wrapm(x + dx, L.nx) * sx + (unsigned)wrapm(y + dy, L.ny) * L.nz +
                            wrapm(z + dz, L.nz)];
             },
-            [&](int k, int dx, int dy, int dz) { return nn[(dx + 1) * 9 + (dy + 1) * 3 + dz + 1][k]; },
+            [&](int k, int dx, int dy, int dz) {
+                const int key = (dx + 1) * 9 + (dy + 1) * 3 + dz + 1;
+                if (!have[key]) {
+                    ext_normal<R, P>(L, E, phi, x + dx, y + dy, z + dz, nn[key]);
+                    have[key] = true;
+                }
+                return nn[key][k];
+            },
             [&](const int o[3]) {
                 const unsigned m = (unsigned)wrapm(x + o[0], L.nx) * sx + (unsigned)wrapm(y + o[1], L.ny) * L.nz +
                                    wrapm(z + o[2], L.nz);
