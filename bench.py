@@ -47,6 +47,9 @@ B_ALG_COLLIDE_SC = {"f32": 2 * 19 * 4 + 4 + 8, "f16": 2 * 19 * 2 + 4 + 8}
 # Moments kernel of the red/blue layout: 38 populations read and phi written;
 # shifted storage also reads the previous phi (the shift reference).
 B_ALG_MOMENTS_RB = {"f64": 2 * 19 * 8 + 8, "f32": 2 * 19 * 4 + 2 * 4, "f16": 2 * 19 * 2 + 2 * 4}
+# Moments kernel of the factorised layout: 19 totals and the recolour scalars
+# read, phi written.
+B_ALG_MOMENTS_FS = {"f64": 19 * 8 + 48 + 8, "f32": 19 * 4 + 16 + 4, "f16": 19 * 2 + 16 + 4}
 DTYPE = {"f64": "f64", "f32": "f32", "f16": "f16-storage/f32-compute"}
 
 
@@ -208,15 +211,18 @@ def run_ensemble(steps, seq_steps):
 
 
 def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512")):
-    """The particle workloads, red/blue layout (walls and particles run on it):
-    one sphere (R=10, density 1.5, 60 degrees) at a flat interface in a 96^3
-    box with z walls, FP64 and FP32; and 200 spheres (R=8, seed 99, densities
-    in [0.8, 2.0], contact angles in [30, 150] degrees) within +-24 of the
-    interface in a 512^3 box with z walls, FP32 fields with FP64 particle
-    dynamics, and FP64.  MLUPS over all nodes, particle kernels included;
-    per workload the step against its counted bytes (312 B/node FP32, 624
-    FP64), and the fused collide kernel and the moments kernel (HIP events on
-    the library's stream) against the HBM roofline at their algorithmic bytes."""
+    """The particle workloads: one sphere (R=10, density 1.5, 60 degrees) at a
+    flat interface in a 96^3 box with z walls, FP64 (red/blue layout) and FP32
+    (factorised and red/blue layouts); and 200 spheres (R=8, seed 99,
+    densities in [0.8, 2.0], contact angles in [30, 150] degrees) within +-24
+    of the interface in a 512^3 box with z walls, FP32 fields with FP64
+    particle dynamics (factorised and red/blue), and FP64 (red/blue).  MLUPS
+    over all nodes, particle kernels included; per workload the step against
+    its counted bytes (312 B/node FP32, 624 FP64), and the step's collide
+    kernels and moments kernels (HIP events on the library's stream: in the
+    factorised layout the tile kernel plus the per-node kernels of the
+    particles' band) against the HBM roofline at the tile kernels'
+    algorithmic bytes."""
     import symmetry_amd
     from symmetry_amd import ic
 
@@ -225,17 +231,18 @@ def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512")):
     p2 = ic.sphere_particles([c2["center"]], c2["rad"], c2["rho_p"], c2["cita_deg"], c2["sigma"])
     c5 = ic.particle_config5()
     p5 = ic.sphere_particles(c5["centers"], c5["rad"], c5["rho_p"], c5["cita_deg"], c5["sigma"])
-    runs = [("sphere_96", c2, p2, "f64", steps), ("sphere_96", c2, p2, "f32", steps),
-            ("dense_512", c5, p5, "f32", steps)]
+    runs = [("sphere_96", c2, p2, "f64", "rb", steps), ("sphere_96", c2, p2, "f32", "fs", steps),
+            ("sphere_96", c2, p2, "f32", "rb", steps), ("dense_512", c5, p5, "f32", "fs", steps),
+            ("dense_512", c5, p5, "f32", "rb", steps)]
     if f64_dense_steps > 0:
-        runs.append(("dense_512", c5, p5, "f64", f64_dense_steps))
+        runs.append(("dense_512", c5, p5, "f64", "rb", f64_dense_steps))
     runs = [r for r in runs if r[0] in which]
     roof = {}
-    for name, cfg, parts, prec, nsteps in runs:
+    for name, cfg, parts, prec, layout, nsteps in runs:
         n = cfg["nx"]
         lat = symmetry_amd.Lattice(n, n, n, precision=prec, sigma=cfg["sigma"], rho0=cfg["rho0"],
                                    nu_r=cfg["nu_r"], nu_b=cfg["nu_b"], beta=cfg["beta"], bc=cfg["bc"],
-                                   npar=len(parts["rad"]))
+                                   npar=len(parts["rad"]), layout=layout)
         lat.init_interface(cfg["z0"], 6.0 * cfg["beta"] * 0.134)
         lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"],
                           parts["fg"])
@@ -247,18 +254,23 @@ def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512")):
         dt = time.perf_counter() - t0
         nodes = n ** 3
         mlups = nodes * nsteps / dt / 1e6
-        out[f"{name}_{prec}_mlups"] = round(mlups, 1)
+        out[f"{name}_{prec}_{layout}_mlups"] = round(mlups, 1)
         k = 10
         _, ms_mom, ms_col = lat.time_steps(k)
         lat.close()
-        b_col, b_mom = B_ALG_COLLIDE[(prec, "rb")], B_ALG_MOMENTS_RB[prec]
+        b_col = B_ALG_COLLIDE[(prec, layout)]
+        b_mom = B_ALG_MOMENTS_RB[prec] if layout == "rb" else B_ALG_MOMENTS_FS[prec]
         col, mom = ms_col / k * 1e-3, ms_mom / k * 1e-3
-        roof[f"{name}_{prec}"] = {
+        roof[f"{name}_{prec}_{layout}"] = {
             "steps": nsteps, "step_ms": round(dt / nsteps * 1e3, 4),
             "step_frac": round(mlups * 1e6 * B_ALG_STEP[prec] / HBM_PEAK, 4),
-            "collide": {"kernel": "k_collide (walls/particles)", "alg_bytes_per_node": b_col,
+            "collide": {"kernel": ("k_collide (walls/particles)" if layout == "rb" else
+                                   "k_collide (tile kernel) + k_collide_special (band)"),
+                        "alg_bytes_per_node": b_col,
                         "avg_launch_ms": round(col * 1e3, 4), "frac": round(b_col * nodes / col / HBM_PEAK, 4)},
-            "moments": {"kernel": "k_moments (walls/particles)", "alg_bytes_per_node": b_mom,
+            "moments": {"kernel": ("k_moments (walls/particles)" if layout == "rb" else
+                                   "k_moments_fsx_ext (tile kernel) + k_moments_special (band)"),
+                        "alg_bytes_per_node": b_mom,
                         "avg_launch_ms": round(mom * 1e3, 4), "frac": round(b_mom * nodes / mom / HBM_PEAK, 4)},
             "particle_kernels_ms": round(dt / nsteps * 1e3 - (col + mom) * 1e3, 4)}
     out["roofline"] = roof
