@@ -210,7 +210,7 @@ def run_ensemble(steps, seq_steps):
             "sequential_f64_mlups": round(seq_mlups, 1), "ratio": round(batch_mlups / seq_mlups, 2)}
 
 
-def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512")):
+def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512"), layouts=("fs", "rb")):
     """The particle workloads: one sphere (R=10, density 1.5, 60 degrees) at a
     flat interface in a 96^3 box with z walls, FP64 (red/blue layout) and FP32
     (factorised and red/blue layouts); and 200 spheres (R=8, seed 99,
@@ -236,7 +236,7 @@ def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512")):
             ("dense_512", c5, p5, "f32", "rb", steps)]
     if f64_dense_steps > 0:
         runs.append(("dense_512", c5, p5, "f64", "rb", f64_dense_steps))
-    runs = [r for r in runs if r[0] in which]
+    runs = [r for r in runs if r[0] in which and r[4] in layouts]
     roof = {}
     for name, cfg, parts, prec, layout, nsteps in runs:
         n = cfg["nx"]
@@ -304,6 +304,8 @@ def main():
                     help="timed steps of the 512^3 200-sphere workload in FP64 (0 = skip)")
     ap.add_argument("--particle-workloads", default="sphere_96,dense_512",
                     help="comma-separated particle workloads to run (sphere_96, dense_512)")
+    ap.add_argument("--particle-layouts", default="fs,rb",
+                    help="comma-separated layouts of the particle workloads (fs, rb)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this implementation runs on one GPU only; sharding over GPUs is designed, "
@@ -369,7 +371,8 @@ def main():
 
     accuracy = run_accuracy(nx, ny, nz, cfg, args.accuracy_steps) if args.accuracy_steps > 0 else None
     ensemble = run_ensemble(args.ensemble_steps, args.ensemble_seq_steps) if args.ensemble_steps > 0 else None
-    particles = (run_particles(args.particle_steps, args.particle_f64_dense_steps, args.particle_workloads.split(","))
+    particles = (run_particles(args.particle_steps, args.particle_f64_dense_steps, args.particle_workloads.split(","),
+                               args.particle_layouts.split(","))
                  if args.particle_steps > 0 else None)
 
     out = {

@@ -43,7 +43,9 @@ extern "C" {
 
 #define CGL_LAYOUT_RB 0 /* red and blue populations, 2 x 19 structure-of-arrays */
 #define CGL_LAYOUT_FS 1 /* factorised: total populations (19 SoA) + per-node recolour scalars,
-                           r_i = A f_i + w_i G.c_i (the form Recolor produces, S:713-726) */
+                           r_i = A f_i + w_i G.c_i (the form Recolor produces, S:713-726); with
+                           walls, mirrors or particles, colour side arrays for the slots of solid
+                           and ghost nodes */
 
 typedef struct cgl_handle cgl_t;
 
@@ -69,7 +71,7 @@ typedef struct cgl_params {
  * reference's main (S:144-179).  Faces may be periodic (in pairs),
  * half-way bounce-back walls (S:1557-1590) or specular mirrors
  * (S:1443-1542), the last two with a ghost layer; npar > 0 adds moving
- * particles.  Walls, mirrors and particles run on the red/blue layout. */
+ * particles.  Walls, mirrors and particles run on both layouts. */
 int cgl_create(cgl_t **h, const cgl_params *p);
 int cgl_destroy(cgl_t *h);
 
