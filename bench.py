@@ -44,6 +44,10 @@ B_ALG_COLLIDE = {
 # and written, phi read once, the node's own fixed-point sums (2 x int32)
 # written once; the recolour scalars are written on a run's last step only.
 B_ALG_COLLIDE_SC = {"f32": 2 * 19 * 4 + 4 + 8, "f16": 2 * 19 * 2 + 4 + 8}
+# With particles the scatter step's tile kernel also reads each node's type
+# (4 B); its finishing pass reads the node's own sums and writes phi.
+B_ALG_COLLIDE_SC_EXT = {k: v + 4 for k, v in B_ALG_COLLIDE_SC.items()}
+B_ALG_PHI_SC = 8 + 4
 # Moments kernel of the red/blue layout: 38 populations read and phi written;
 # shifted storage also reads the previous phi (the shift reference).
 B_ALG_MOMENTS_RB = {"f64": 2 * 19 * 8 + 8, "f32": 2 * 19 * 4 + 2 * 4, "f16": 2 * 19 * 2 + 2 * 4}
@@ -220,9 +224,11 @@ def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512"), layo
     over all nodes, particle kernels included; per workload the step against
     its counted bytes (312 B/node FP32, 624 FP64), and the step's collide
     kernels and moments kernels (HIP events on the library's stream: in the
-    factorised layout the tile kernel plus the per-node kernels of the
-    particles' band) against the HBM roofline at the tile kernels'
-    algorithmic bytes."""
+    factorised layout the scatter step's tile kernel plus the per-node kernel
+    of the particles' band, and its finishing pass, which forms the next phase
+    field) against the HBM roofline at the tile kernels' algorithmic bytes;
+    `other_kernels_ms` is the rest of the step (particle, band-field and wall
+    kernels)."""
     import symmetry_amd
     from symmetry_amd import ic
 
@@ -257,22 +263,31 @@ def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512"), layo
         out[f"{name}_{prec}_{layout}_mlups"] = round(mlups, 1)
         k = 10
         _, ms_mom, ms_col = lat.time_steps(k)
+        sc = lat.scatter_form()
         lat.close()
-        b_col = B_ALG_COLLIDE[(prec, layout)]
-        b_mom = B_ALG_MOMENTS_RB[prec] if layout == "rb" else B_ALG_MOMENTS_FS[prec]
+        if sc:
+            b_col, b_mom = B_ALG_COLLIDE_SC_EXT[prec], B_ALG_PHI_SC
+            k_col = "k_collide_sc_ext (tile kernel) + k_collide_sc_band (band)"
+            k_mom = "k_phi_sc2 (finishing pass of the scatter form)"
+        elif layout == "rb":
+            b_col, b_mom = B_ALG_COLLIDE[(prec, layout)], B_ALG_MOMENTS_RB[prec]
+            k_col, k_mom = "k_collide (walls/particles)", "k_moments (walls/particles)"
+        else:
+            b_col, b_mom = B_ALG_COLLIDE[(prec, layout)], B_ALG_MOMENTS_FS[prec]
+            k_col = "k_collide (tile kernel) + k_collide_special (band)"
+            k_mom = "k_moments_fsx_ext (tile kernel) + k_moments_special (band)"
         col, mom = ms_col / k * 1e-3, ms_mom / k * 1e-3
         roof[f"{name}_{prec}_{layout}"] = {
             "steps": nsteps, "step_ms": round(dt / nsteps * 1e3, 4),
             "step_frac": round(mlups * 1e6 * B_ALG_STEP[prec] / HBM_PEAK, 4),
-            "collide": {"kernel": ("k_collide (walls/particles)" if layout == "rb" else
-                                   "k_collide (tile kernel) + k_collide_special (band)"),
+            "phase_field": "scatter form" if sc else "moments pass",
+            "collide": {"kernel": k_col,
                         "alg_bytes_per_node": b_col,
                         "avg_launch_ms": round(col * 1e3, 4), "frac": round(b_col * nodes / col / HBM_PEAK, 4)},
-            "moments": {"kernel": ("k_moments (walls/particles)" if layout == "rb" else
-                                   "k_moments_fsx_ext (tile kernel) + k_moments_special (band)"),
+            "moments": {"kernel": k_mom,
                         "alg_bytes_per_node": b_mom,
                         "avg_launch_ms": round(mom * 1e3, 4), "frac": round(b_mom * nodes / mom / HBM_PEAK, 4)},
-            "particle_kernels_ms": round(dt / nsteps * 1e3 - (col + mom) * 1e3, 4)}
+            "other_kernels_ms": round(dt / nsteps * 1e3 - (col + mom) * 1e3, 4)}
     out["roofline"] = roof
     out["workloads"] = {"sphere_96": "96^3, z walls, one sphere R=10 at the interface",
                         "dense_512": "512^3, z walls, 200 spheres R=8 within +-24 of the interface, seed 99"}

@@ -100,14 +100,18 @@ int set_device(cgl_handle *h)
 }
 
 // One time step, then swap buffers.  Gather form: K1 then K2.  Scatter form
-// (periodic FP32/FP16 factorised boxes, once the state is factorised): the
-// step kernel and the phase-field pass; `last` = the run's last step, which
-// writes the recolour scalars so that the state can be read back.
+// (FP32/FP16 factorised boxes, once the state is factorised; with particles
+// once their start-up step is done): the step kernels and the phase-field
+// pass; `last` = the run's last step, which writes the recolour scalars so
+// that the state can be read back.
 void enqueue_step(cgl_handle *h, bool last = true)
 {
     dispatch(h, [&](auto impl) {
-        if (h->sc && h->state_layout == cgl::FS) {
-            decltype(impl)::step_sc(h, last);
+        if (h->sc && h->state_layout == cgl::FS && !(h->npar && (!h->primed || h->cold))) {
+            if (h->ext)
+                decltype(impl)::step_sc_ext(h, last);
+            else
+                decltype(impl)::step_sc(h, last);
         } else if (h->nbox) {
             decltype(impl)::batch_moments(h);
             decltype(impl)::batch_collide(h);
@@ -187,14 +191,21 @@ int state_readable(const cgl_handle *h)
     return check_range(h);
 }
 
-// Fixed-point partial sums of the scatter step, for periodic FP32/FP16
-// factorised boxes (single, or every box of a batch) whose workgroup boxes
-// have at least two rows and two lanes.
+// Fixed-point partial sums of the scatter step, for FP32/FP16 factorised
+// boxes (single, or every box of a batch) whose workgroup boxes have at
+// least two rows and two lanes; with wall faces and particles for single
+// boxes without mirror faces, at the standard tile height.
 int setup_scatter(cgl_handle *h)
 {
     const char *env = getenv("CGL_SCATTER"); // 0: gather form (moments pass) on every step
     const bool want = env ? atoi(env) != 0 : true;
-    if (!want || h->ext || h->p.precision == CGL_PREC_F64 || h->p.layout != CGL_LAYOUT_FS) return CGL_OK;
+    if (!want || h->p.precision == CGL_PREC_F64 || h->p.layout != CGL_LAYOUT_FS) return CGL_OK;
+    if (h->ext) {
+        for (int a = 0; a < 3; ++a)
+            if (h->E.mirror[a][0] || h->E.mirror[a][1]) return CGL_OK;
+        if (h->nbox || (h->tile_rows && h->tile_rows != cgl::KTY<float>)) return CGL_OK;
+        h->ky = cgl::KTY<float>;
+    }
     if (h->nbox) {
         for (int b = 0; b < h->nbox; ++b)
             if (!cgl::sc_shape_ok(h->hboxes[b].L.ny, h->hboxes[b].L.nz)) return CGL_OK;
@@ -615,7 +626,7 @@ static bool tune_valid(const cgl_handle *h, int dim, int k)
 {
     if (dim == 0) return k == 0 || TUNE_XC[k] < 2 * h->p.nx;
     if (dim == 1) return !h->remap_env && !h->nbox;
-    return h->sc && !h->nbox && !h->tile_rows && cgl::sc_shape_ok(h->p.ny, h->p.nz, TUNE_KY[k]);
+    return h->sc && !h->nbox && !h->ext && !h->tile_rows && cgl::sc_shape_ok(h->p.ny, h->p.nz, TUNE_KY[k]);
 }
 
 static void tune_set(cgl_handle *h, int dim, int v)
@@ -826,7 +837,7 @@ int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moment
         h->tev = h->ev;
         enqueue_step(h);
         h->tev = nullptr;
-        HIPCHK(hipEventSynchronize(h->ev[3]));
+        HIPCHK(hipStreamSynchronize(h->stream)); // (the scatter form records the moments pair last)
         HIPCHK(hipEventElapsedTime(&a, h->ev[0], h->ev[1]));
         HIPCHK(hipEventElapsedTime(&b, h->ev[2], h->ev[3]));
         km += a;

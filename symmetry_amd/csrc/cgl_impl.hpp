@@ -355,28 +355,38 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     {
         if (h->npar && mirrors(h)) k_mirror_mac<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->E.mac);
     }
+    // calmacros with walls and particles into the step's phi buffer, whose
+    // entries are PP (P, or QPhi for the scatter form's buffer)
     static void moments_ext(cgl_handle *h, bool full)
+    {
+        moments_ext_t<P>(h, full);
+        h->phi_ok = 0; // the buffer no longer holds a scatter step's phase field
+    }
+    template <class PP> static void moments_ext_t(cgl_handle *h, bool full)
     {
         const void *i0, *i1;
         state_in(h, i0, i1);
-        const P *phio = (const P *)h->phi[h->phicur];
-        P *phi = (P *)h->phi[h->phicur ^ 1];
+        const PP *phio = (const PP *)h->phi[h->phicur];
+        PP *phi = (PP *)h->phi[h->phicur ^ 1];
         const Ext E = ext_with(h, h->ntype[h->ntcur]);
         if (h->state_layout == FS) {
             if (full) {
-                k_moments<FS, S, R, P, SHIFT, true, true>
-                    <<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, h->mac, E);
+                if constexpr (std::is_same_v<PP, P>)
+                    k_moments<FS, S, R, P, SHIFT, true, true>
+                        <<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, h->mac, E);
             } else {
                 const int xc = K1_XC;
                 const dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, (h->L.nx + xc - 1) / xc);
-                k_moments_fsx_ext<S, R, P, SHIFT><<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, xc, E);
+                k_moments_fsx_ext<S, R, PP, SHIFT><<<grid, node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, xc, E);
                 if (mirrors(h))
-                    k_moments_special<S, R, P, SHIFT, false>
+                    k_moments_special<S, R, PP, SHIFT, false>
                         <<<face_grid(h), 256, 0, h->stream>>>(h->L, i0, i1, phio, phi, E, h->P);
                 if (h->npar)
-                    k_moments_special<S, R, P, SHIFT, true><<<dim3(h->band_blocks, h->npar), PB, 0, h->stream>>>(
+                    k_moments_special<S, R, PP, SHIFT, true><<<dim3(h->band_blocks, h->npar), PB, 0, h->stream>>>(
                         h->L, i0, i1, phio, phi, E, h->P);
             }
+        } else if constexpr (!std::is_same_v<PP, P>) {
+            return; // the scatter form starts from a factorised state
         } else if (full) {
             k_moments<RB, S, R, P, SHIFT, true, true>
                 <<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, i0, i1, phio, phi, h->mac, E);
@@ -570,6 +580,72 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                                                                           SHIFT ? (const float *)phi : nullptr);
             if (mirrors(h)) k_mirror_der<<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, h->E.der);
             h->ntcur = nb;
+        }
+    }
+    // ---- scatter-form step with wall faces and particles (FP32/FP16
+    // factorised storage, no mirror faces).  The tile step kernel and the
+    // band kernel add the next phase field's fixed-point terms (the band's by
+    // atomic addition), the links add theirs (k_curve_fs), and the finishing
+    // pass forms the field; the wall, particle and band kernels read and
+    // write that buffer as phi through QPhi.  The stages are those of
+    // step_ext, with calmacros of the band from the scatter step's field.
+    static void step_sc_ext(cgl_handle *h, bool last)
+    {
+        if constexpr (SHIFT && std::is_same_v<P, float>) {
+            QPhi *phi = (QPhi *)h->phi[h->phicur ^ 1];
+            if (!h->phi_ok) {
+                // the current state's phase field by the gather
+                moments_ext_t<QPhi>(h, false);
+            } else {
+                if (h->npar)
+                    k_band_sc<S><<<dim3(h->band_blocks, h->npar), PB, 0, h->stream>>>(
+                        h->L, ext_with(h, h->ntype[h->ntcur]), h->P, (const S *)h->fsf[h->cur], (float *)phi);
+                wall_phi(h, phi);
+            }
+            int nb = h->ntcur;
+            if (h->npar) {
+                const Ext Eo = ext_with(h, h->ntype[h->ntcur]);
+                k_lagrange<QPhi><<<pgrid(h, LAG_BLOCKS), PB, 0, h->stream>>>(h->L, Eo, h->P, phi, h->walls_i);
+                st_momentum(h);
+                st_move(h, MV_ALL);
+                st_set_walls(h);
+                nb = h->ntcur ^ 1;
+                const Ext En = ext_with(h, h->ntype[nb]);
+                k_refill<QPhi><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[h->ntcur],
+                                                                       h->ntype[nb], h->pout_i, phi);
+                k_ghost_phi<QPhi><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, h->ntype[nb], h->pin_i, phi);
+                k_wet_normals<R, QPhi><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, phi);
+            }
+            const Ext En = ext_with(h, h->ntype[nb]);
+            const S *fin = (const S *)h->fsf[h->cur];
+            S *fout = (S *)h->fsf[h->cur ^ 1];
+            float4 *sall = (float4 *)h->fss[h->cur ^ 1];
+            Lat Ls = h->L;
+            Ls.xcd_remap = h->sc_remap;
+            if (h->tev) (void)hipEventRecord(h->tev[2], h->stream);
+            const dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + KTY<float> - 1) / KTY<float>,
+                            (h->L.nx + h->xc - 1) / h->xc);
+            // (per-direction bases at every size: the one-base form spills registers here)
+            k_collide_sc_ext<S, false><<<grid, dim3(TZ, KTY<float>, 1), 0, h->stream>>>(
+                Ls, fin, fout, last ? sall : nullptr, (const float *)phi, h->sb, h->xc, En);
+            if (h->npar)
+                k_collide_sc_band<S><<<dim3(h->band_blocks, h->npar), PB, 0, h->stream>>>(
+                    h->L, En, h->P, fin, fout, sall, (const float *)phi, h->sb.own);
+            if (h->tev) (void)hipEventRecord(h->tev[3], h->stream);
+            h->state_layout = FS;
+            if (ghost_faces(h))
+                k_fill_ghosts<S, SHIFT, false><<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, fout, nullptr, nullptr);
+            if (h->npar) {
+                k_curve_fs<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, fout, sall, En.rbo[0],
+                                                                              En.rbo[1], h->sb.own);
+                h->ntcur = nb;
+            }
+            h->scal_stale = !last;
+            // timing runs: the finishing pass counts as the step's moments
+            // pass, the tile and band kernels as its collide kernels
+            if (h->tev) (void)hipEventRecord(h->tev[0], h->stream);
+            sc_fin(h);
+            if (h->tev) (void)hipEventRecord(h->tev[1], h->stream);
         }
     }
     // one time step with walls and/or particles (S:216-251)

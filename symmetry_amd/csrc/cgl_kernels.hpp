@@ -275,6 +275,36 @@ template <class R> __device__ __forceinline__ float minority_q(R sr, R sb, R irh
 }
 __device__ __forceinline__ float phi_of_q(float q) { return __builtin_signbitf(q) ? 1.0f + 2.0f * q : 2.0f * q - 1.0f; }
 
+// Fixed-point units of the scatter form's phase-field sums: 2^29 per rho0.
+constexpr float SC_ONE = 536870912.0f;
+
+// floor(v + 0.5) as an int in one instruction; its result equals the exact
+// floor(v + 0.5) on gfx950 for every value tried (1M floats: |v| < 2^30,
+// exact halves, integers above 2^23 where v + 0.5 is not a float).
+__device__ __forceinline__ int round_half_up(float v)
+{
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+// An entry of that buffer seen as phi: it converts from phi on writes and to
+// phi on reads, so that the wall, particle and band kernels (templates on the
+// phase-field type) run on the scatter form's buffer.  raw() stores q itself.
+struct QPhi {
+    float q;
+    QPhi() = default;
+    __device__ QPhi(float phi) : q(phi >= 0.0f ? -0.5f * (1.0f - phi) : 0.5f * (1.0f + phi)) {}
+    __device__ operator float() const { return phi_of_q(q); }
+    __device__ static QPhi raw(float v)
+    {
+        QPhi e;
+        e.q = v;
+        return e;
+    }
+};
+static_assert(sizeof(QPhi) == sizeof(float), "QPhi is a float in memory");
+
 template <class R> __device__ __forceinline__ void decode_frac(float x, R &A, R &B)
 {
     if (__builtin_signbitf(x)) {
@@ -796,6 +826,17 @@ __device__ __forceinline__ void node_offsets(const Lat &L, int x, int y, int z, 
     }
 }
 
+// The phase-field buffer entry of a node from its moments: phi, or for the
+// scatter form's buffer (QPhi) the signed minority fraction from the colour
+// densities, to their relative precision.
+template <class P, class R, bool SHIFT> __device__ __forceinline__ P phi_entry(const Moments<R, SHIFT> &mo)
+{
+    if constexpr (std::is_same_v<P, QPhi>)
+        return QPhi::raw(minority_q(mo.sr, mo.sb, fdiv(R(1), mo.rho)));
+    else
+        return P(mo.phi);
+}
+
 // ---------------------------------------------------------------- K1 moments
 // Writes phi of the step.  With FULL it also writes the macroscopic fields
 // rho, rho_R, rho_B, ux, uy, uz, phi (bare velocity, as calmacros) as FP64.
@@ -993,7 +1034,7 @@ __device__ __forceinline__ void moments_fsx_tile(const Lat &L, const void *__res
                     gather_fs<S, R, SHIFT>(
                         fs, N, xo, yo, zo, [&](int i, unsigned) { return ((bb >> i) & 1u) ? own : scal(i, 0u); }, nd,
                         R(L.rho0));
-                    at(phi, n) = P(moments<R, SHIFT>(L, nd).phi);
+                    at(phi, n) = phi_entry<P>(moments<R, SHIFT>(L, nd));
                 }
             } else {
                 gather_fs<S, R, SHIFT>(fs, N, xo, yo, zo, scal, nd, R(L.rho0));
@@ -2393,7 +2434,10 @@ __global__ __launch_bounds__(256) void k_unpack_fs_ext(Lat L, Ext E, const S *__
 // across mirror faces.  One thread per node of face blockIdx.y (axis y / 2,
 // side y % 2); edge and corner ghosts are written once per face with equal
 // values.
-template <class S, bool SHIFT>
+// SIDE = false (the scatter form, wall faces only): the totals only; the
+// colours of a wall ghost slot are never read (a live node splits its
+// bounce-back slots with its own scalars).
+template <class S, bool SHIFT, bool SIDE = true>
 __global__ __launch_bounds__(256) void k_fill_ghosts(Lat L, Ext E, S *__restrict__ fs, void *__restrict__ side0,
                                                      void *__restrict__ side1)
 {
@@ -2421,7 +2465,8 @@ __global__ __launch_bounds__(256) void k_fill_ghosts(Lat L, Ext E, S *__restrict
         pull_slot<i>(xo, yo, zo, wall_bb_mask(E, L, x, y, z), mirror_adjacent(E, L, x, y, z), m, d);
         const size_t ks = (size_t)d * N + m, kd = (size_t)i * N + ng;
         fs[kd] = fs[ks];
-        if constexpr (SHIFT) {
+        if constexpr (!SIDE) {
+        } else if constexpr (SHIFT) {
             ((float *)side0)[kd] = ((const float *)side0)[ks];
         } else {
             ((S *)side0)[kd] = ((const S *)side0)[ks];

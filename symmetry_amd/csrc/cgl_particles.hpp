@@ -548,9 +548,13 @@ __global__ __launch_bounds__(PB) void k_curve(Lat L, Ext E, Part Pt, const unsig
 // storage the slot's minority fraction).  Runs after k_fill_ghosts, whose
 // value of that ghost slot it replaces, as curve follows the boundary fills
 // (S:247-251).
+// own (the scatter form, shifted storage): the slot's colours are also added
+// to x's fixed-point phase-field sums (total from the stored value, minority
+// colour formed first); a slot whose upstream node is a wall ghost is left
+// to the wall's bounce-back, whose terms x added itself.
 template <class S, bool SHIFT>
 __global__ __launch_bounds__(PB) void k_curve_fs(Lat L, Ext E, Part Pt, S *fo, const void *so, void *side0,
-                                                 void *side1)
+                                                 void *side1, int2 *own = nullptr)
 {
     const int p = blockIdx.x;
     const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 1.0);
@@ -596,6 +600,15 @@ __global__ __launch_bounds__(PB) void k_curve_fs(Lat L, Ext E, Part Pt, S *fo, c
             if constexpr (SHIFT) {
                 fo[k] = (S)(float)(vr + vb - W[i] * L.rho0);
                 ((float *)side0)[k] = enc_slot(vr, vb);
+                if (own && !is_wall_ghost(E, L, wrapm(X[0] - CX[i], L.nx), wrapm(X[1] - CY[i], L.ny),
+                                          wrapm(X[2] - CZ[i], L.nz))) {
+                    const float qs = SC_ONE / float(L.rho0);
+                    const bool redmaj = vb <= vr;
+                    const int qf = round_half_up(__builtin_fmaf(float(fo[k]), qs, float(W[i] * double(SC_ONE))));
+                    const int qm = round_half_up(float(redmaj ? vb : vr) * qs);
+                    atomicAdd((unsigned *)&own[n].x, (unsigned)(redmaj ? qf - qm : qm));
+                    atomicAdd((unsigned *)&own[n].y, (unsigned)(redmaj ? qm : qf - qm));
+                }
             } else {
                 fo[k] = vr + vb;
                 ((S *)side0)[k] = vr;
@@ -708,7 +721,7 @@ __global__ __launch_bounds__(256) void k_moments_special(Lat L, const void *__re
         Moments<R, SHIFT> mo;
         if (!node_moments<FS, S, R, P, SHIFT, true>(L, p0, p1, phio, E, x, y, z, n, mo, nt)) return;
         band_macros(L, E, n, nt, mo);
-        at(phi, n) = P(mo.phi);
+        at(phi, n) = (nt & NT_SOLID) ? P(0.0f) : phi_entry<P>(mo);
     });
 }
 

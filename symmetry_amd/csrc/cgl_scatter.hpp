@@ -35,10 +35,10 @@
 #pragma once
 
 #include "cgl_kernels.hpp"
+#include "cgl_particles.hpp"
 
 namespace cgl {
 
-constexpr float SC_ONE = 536870912.0f; // 2^29 fixed-point units per rho0
 
 // Arrays of N entries.  The first five kinds are indexed by node; the sums
 // across a z face (fz, exzlo, exzhi, eyz) exist only for the first and last
@@ -63,16 +63,6 @@ __device__ __forceinline__ unsigned zcol_len(const Lat &L) { return 2u * (unsign
 __device__ __forceinline__ unsigned zslot(const Lat &L, unsigned x, unsigned y, unsigned tile, unsigned side)
 {
     return ((x * (unsigned)L.ny + y) * zcol_len(L)) + 2u * tile + side;
-}
-
-// floor(v + 0.5) as an int in one instruction; its result equals the exact
-// floor(v + 0.5) on gfx950 for every value tried (1M floats: |v| < 2^30,
-// exact halves, integers above 2^23 where v + 0.5 is not a float).
-__device__ __forceinline__ int round_half_up(float v)
-{
-    int r;
-    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
-    return r;
 }
 
 // Sums wrap modulo 2^32 (unsigned arithmetic: defined behaviour).  A colour
@@ -115,10 +105,17 @@ __host__ __device__ inline bool sc_shape_ok(int ny, int nz, int ky = KTY<float>)
 // every load and store takes the one base and a 32-bit offset (no per-direction
 // 64-bit bases held in scalar registers).
 // KY: rows of the tile (64 x KY threads).
-template <class S, bool ONE = false, int KY = KTY<float>>
+// EXT (wall faces and particles, no mirror faces): the ghost layers and the
+// particles' band (NEAR) are left to k_collide_sc_band (their terms are zero
+// here); the normals are zero at wall ghosts and solid nodes and the wetted
+// ones at the particles' link nodes; a node next to a wall adds the terms
+// that leave through it to its own sums (half-way bounce-back returns them
+// to the node, S:1557-1590) and takes no curvature force (S:624).
+template <class S, bool ONE = false, int KY = KTY<float>, bool EXT = false>
 __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restrict__ fin, S *__restrict__ fout,
                                                 float4 *__restrict__ sout, const float *__restrict__ phi,
-                                                const ScBufs &B, int xc, unsigned bx, unsigned by, unsigned bz)
+                                                const ScBufs &B, int xc, unsigned bx, unsigned by, unsigned bz,
+                                                const Ext &E = Ext{})
 {
     using R = float;
     __shared__ K2Smem<R, float, KY> sm;
@@ -186,7 +183,34 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
         else if (h < 2 * (TZ + 2) + KY) { hy = h - 2 * (TZ + 2) + 1; hz = 0; }
         else { hy = h - 2 * (TZ + 2) - KY + 1; hz = TZ + 1; }
     }
-    auto normal_at = [&](int p, int jy, int jz, R &gx, R &gy, R &gz) {
+    // EXT: in-plane coordinates and offsets of this thread's node and of its
+    // halo element, and their node types for the next plane of normals
+    // (loaded a plane early, so that no load holds up a plane's start)
+    // (bits of eflags: 0 and 1 whether the node in planes x and x + 1 is in
+    // the band, 2 and 3 whether this node's / the halo element's column
+    // lies in a y or z wall ghost layer)
+    const unsigned oin = (unsigned)wrapm(y, L.ny) * (unsigned)L.nz + (unsigned)wrapm(z, L.nz);
+    const unsigned hin = (unsigned)wrapm(y0 - 1 + hy, L.ny) * (unsigned)L.nz + (unsigned)wrapm(z0 - 1 + hz, L.nz);
+    unsigned eflags = 0;
+    if constexpr (EXT) {
+        auto wall_yz = [&](int py, int pz) {
+            return (E.wall[1][0] && py == 0) || (E.wall[1][1] && py == L.ny - 1) || (E.wall[2][0] && pz == 0) ||
+                   (E.wall[2][1] && pz == L.nz - 1);
+        };
+        if (wall_yz(wrapm(y, L.ny), wrapm(z, L.nz))) eflags |= 4u;
+        if (wall_yz(wrapm(y0 - 1 + hy, L.ny), wrapm(z0 - 1 + hz, L.nz))) eflags |= 8u;
+    }
+    unsigned nto = 0, nth = 0;
+    auto load_nt = [&](int p) {
+        if constexpr (EXT) {
+            if (E.ntype) {
+                const unsigned gxp = (unsigned)wrapm(p, L.nx) * sx;
+                nto = at(E.ntype, gxp + oin);
+                if (hy >= 0) nth = at(E.ntype, gxp + hin);
+            }
+        }
+    };
+    auto normal_at = [&](int p, int jy, int jz, R &gx, R &gy, R &gz, bool wyz, unsigned pin, unsigned nt) {
         const int s = slot3(p);
         grad19<R>([&](int dx, int dy, int dz) { return sm.phi[slot5(p + dx)][jy + 1 + dy][jz + 1 + dz]; }, gx, gy,
                   gz);
@@ -198,15 +222,29 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             ny = gy * inv;
             nz = gz * inv;
         }
+        if constexpr (EXT) {
+            // (S:498-532, S:536-594) zero at wall ghosts and solid nodes, the
+            // wetted normal that k_wet_normals stored at link nodes
+            const int px = wrapm(p, L.nx);
+            const bool zero =
+                wyz || (E.wall[0][0] && px == 0) || (E.wall[0][1] && px == L.nx - 1) || (nt & NT_SOLID);
+            if (!zero && (nt & NT_POUT) && g2 > R(EVAP_LIM * EVAP_LIM)) {
+                const unsigned m = (unsigned)px * sx + pin;
+                nx = R(at(E.der, 3 * N + m));
+                ny = R(at(E.der, 4 * N + m));
+                nz = R(at(E.der, 5 * N + m));
+            }
+            if (zero) nx = ny = nz = 0;
+        }
         sm.n[0][s][jy][jz] = nx;
         sm.n[1][s][jy][jz] = ny;
         sm.n[2][s][jy][jz] = nz;
     };
     auto normals = [&](int p, R &gx, R &gy, R &gz) {
-        normal_at(p, ty + 1, tz + 1, gx, gy, gz);
+        normal_at(p, ty + 1, tz + 1, gx, gy, gz, (eflags & 4u) != 0, oin, nto);
         if (hy >= 0) {
             R a, b, c;
-            normal_at(p, hy, hz, a, b, c);
+            normal_at(p, hy, hz, a, b, c, (eflags & 8u) != 0, hin, nth);
         }
     };
 
@@ -239,11 +277,15 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     for (int p = x0 - 2; p <= x0 + 2; ++p) load_phi(p);
     __syncthreads();
     R gx, gy, gz; // gradient of phi at the node, plane x
+    load_nt(x0 - 1);
     {
         R a, b, c;
         normals(x0 - 1, a, b, c);
     }
+    load_nt(x0);
     normals(x0, gx, gy, gz);
+    if (nto & NT_NEAR) eflags |= 1u;
+    load_nt(x0 + 1);
 
     for (int x = x0; x < x1; ++x) {
         __syncthreads(); // readers of the slots refilled below are done
@@ -252,6 +294,8 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
         __syncthreads();
         R ngx, ngy, ngz;
         normals(x + 1, ngx, ngy, ngz);
+        if constexpr (EXT)
+            if (nto & NT_NEAR) eflags |= 2u;
         __syncthreads();
 
         // terms (rho_R, rho_B) for the downstream nodes, one direction per
@@ -259,16 +303,26 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
         // gyp/gym (dx,+-1,0), dpp/dpm/dmp/dmm (0,+-1,+-1); zero outside the box
         int2 g00[3] = {Z2, Z2, Z2}, gzp[3] = {Z2, Z2, Z2}, gzm[3] = {Z2, Z2, Z2}, gyp[3] = {Z2, Z2, Z2},
              gym[3] = {Z2, Z2, Z2}, dpp = Z2, dpm = Z2, dmp = Z2, dmm = Z2;
-        if (active) {
+        bool live = active;
+        unsigned wbb = 0; // wall bounce-back directions of the node
+        if constexpr (EXT) {
+            if (live)
+                live = !(eflags & 5u) && !(E.wall[0][0] && x == 0) && !(E.wall[0][1] && x == L.nx - 1);
+            if (live) wbb = wall_bb_mask(E, L, x, y, z);
+            eflags = (eflags & ~3u) | ((eflags >> 1) & 1u);
+        }
+        if (live) {
             // gradient, normal and flag at the node (S:498-532)
             const R g2 = gx * gx + gy * gy + gz * gz;
             const bool flag = g2 > R(EVAP_LIM * EVAP_LIM);
+            bool force_on = flag;
+            if constexpr (EXT) force_on = flag && away_from_walls(E, L, x, y, z);
             const R nx = sm.n[0][slot3(x)][ty + 1][tz + 1];
             const R ny = sm.n[1][slot3(x)][ty + 1][tz + 1];
             const R nz = sm.n[2][slot3(x)][ty + 1][tz + 1];
             // normal derivatives for the curvature (S:635-660)
             R d[3][3];
-            if (flag) {
+            if (force_on) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
 #pragma unroll
@@ -330,7 +384,7 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             mo.uy = jy * irho;
             mo.uz = jz * irho;
             mo.phi = phin;
-            collide<R, true>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, flag, d);
+            collide<R, true>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, force_on, d);
             // the colour split (the collision needs only rho, u and phi): the
             // minority colour's fraction hm from the phase-field buffer (its
             // signed minority fraction), so that a nearly pure node keeps its
@@ -396,7 +450,14 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
                 if (i > 0 && i <= 6) cq = __builtin_fmaf(Mf, fq, dot_c<R>(i, G1x, G1y, G1z));
                 if (i > 6) cq = __builtin_fmaf(Mf, fq, dot_c<R>(i, G7x, G7y, G7z));
                 const int qf = round_half_up(fq), qm = round_half_up(cq);
-                const int2 c = redmaj ? make_int2(qf - qm, qm) : make_int2(qm, qf - qm);
+                int2 c = redmaj ? make_int2(qf - qm, qm) : make_int2(qm, qf - qm);
+                if constexpr (EXT) {
+                    // a term into a wall ghost returns to the node itself
+                    if (i > 0 && ((wbb >> OPC[i]) & 1u)) {
+                        add2(g00[1], c);
+                        c = Z2;
+                    }
+                }
                 const int k = CX[i] + 1;
                 if (CY[i] == 0 && CZ[i] == 0) g00[k] = c;
                 else if (CY[i] == 0 && CZ[i] == 1) gzp[k] = c;
@@ -503,6 +564,7 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             if (yedge) at(B.exyhi, pm + nrow) = ym;
             if (zedge) at(B.exzhi, (unsigned)wrapm(x - 1, L.nx) * csx + cself) = zm;
         }
+        load_nt(x + 2);
         gx = ngx;
         gy = ngy;
         gz = ngz;
@@ -539,6 +601,19 @@ __global__ __launch_bounds__(TZ *KY, 4) void k_collide_sc(Lat L, const S *__rest
     unsigned bx, by, bz;
     tile_of_block(L, bx, by, bz);
     collide_sc_tile<S, ONE, KY>(L, fin, fout, sout, phi, B, xc, bx, by, bz);
+}
+
+// The scatter step with wall faces and particles: every lane but the ghost
+// layers and the particles' band.
+template <class S, bool ONE>
+__global__ __launch_bounds__(TZ *KTY<float>, 4) void k_collide_sc_ext(Lat L, const S *__restrict__ fin,
+                                                                    S *__restrict__ fout, float4 *__restrict__ sout,
+                                                                    const float *__restrict__ phi, ScBufs B, int xc,
+                                                                    Ext E)
+{
+    unsigned bx, by, bz;
+    tile_of_block(L, bx, by, bz);
+    collide_sc_tile<S, ONE, KTY<float>, true>(L, fin, fout, sout, phi, B, xc, bx, by, bz, E);
 }
 
 // Partial-sum arrays of box `base` in a packed batch.
@@ -660,6 +735,334 @@ static __global__ __launch_bounds__(256) void k_phi_sc_batch(const BoxDev *__res
     if (z >= L.nz || y >= L.ny) return;
     const ScBufs Bb = sc_offset(B, base);
     for (int x = xb; x < min(xb + xs, L.nx); ++x) phi_sc_node(L, Bb, xc, KTY<float>, phi + base, x, y, z);
+}
+
+// ------------------------------------------------- the particles' band
+// With particles, the scatter step's tile kernel leaves the band (NEAR: the
+// solid nodes and every node within R + 3 of a centre) to the kernels below:
+// one thread per node of particle blockIdx.y's band box (R + 3), node
+// blockIdx.x PB + threadIdx.x in box order.
+
+// Whether box node (x, y, z) of particle blockIdx.y lies in no lower
+// particle's band box, so that a node in several boxes adds its terms once.
+// Every thread of the workgroup takes part (barriers): the lower particles'
+// boxes pass through LDS, PB at a time, their periodic corners wrapped into
+// the domain; only boxes that meet this particle's box are tested per node.
+__device__ __forceinline__ bool first_band(const Lat &L, const Ext &E, const Part &Pt, bool valid, int x, int y,
+                                           int z)
+{
+    __shared__ int bl[PB][6];
+    __shared__ int meets[PB];
+    const int p = blockIdx.y;
+    const int g[3] = {x, y, z}, n3[3] = {L.nx, L.ny, L.nz};
+    const Box bp = box_of(Pt.center + 3 * p, Pt.rad[p], 3.0);
+    bool first = valid;
+    for (int q0 = 0; q0 < p; q0 += PB) {
+        __syncthreads();
+        const int q = q0 + (int)threadIdx.x;
+        int m = 0;
+        if (q < p) {
+            const Box b = box_of(Pt.center + 3 * q, Pt.rad[q], 3.0);
+            m = 1;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const bool per = !bounded(E, a);
+                const int lo = per ? wrapm(b.lo[a], n3[a]) : b.lo[a];
+                bl[threadIdx.x][a] = lo;
+                bl[threadIdx.x][3 + a] = b.w[a];
+                // the two boxes meet along the axis (periodic: some image does)
+                const int d = per ? wrapm(lo - bp.lo[a], n3[a]) : lo - bp.lo[a];
+                const bool meet = per ? (d < bp.w[a] || d + b.w[a] > n3[a]) : (d < bp.w[a] && d + b.w[a] > 0);
+                if (!meet) m = 0;
+            }
+        }
+        meets[threadIdx.x] = m;
+        if (!__syncthreads_or(m)) continue;
+        const int cnt = min(PB, p - q0);
+        for (int j = 0; j < cnt && first; ++j) {
+            if (!meets[j]) continue;
+            bool in = true;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                int d = g[a] - bl[j][a];
+                if (!bounded(E, a) && d < 0) d += n3[a]; // g and the wrapped corner lie in [0, n)
+                in = in && d >= 0 && d < bl[j][3 + a];
+            }
+            if (in) first = false;
+        }
+    }
+    return first;
+}
+
+// Node (x, y, z) of the band (index n, mask nt) in the scatter step, as the
+// tile kernel updates its lanes, with the phase field and the normals read
+// from memory (ext_normal: zero at wall ghosts and solid nodes, wetted at
+// link nodes, S:536-594), refilled nodes from their equilibria (S:1364-1391)
+// and the band's gradients and normals stored for the particle kernels.  Its
+// terms go to the fixed-point sums of their destinations by atomic addition
+// (exact and order-free): into a wall ghost they return to the node, into a
+// solid node they are dropped (the link's bounce-back value replaces them,
+// k_curve_fs).  Solid nodes pass their pulled totals through (S:622, S:712,
+// S:743) and add no terms.
+template <class S>
+__device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S *__restrict__ fin,
+                                             S *__restrict__ fout, float4 *__restrict__ sout,
+                                             const float *__restrict__ phi, int2 *__restrict__ own, int x, int y,
+                                             int z, unsigned n, unsigned nt)
+{
+    using R = float;
+    const size_t N = (size_t)L.N;
+    const unsigned sx = (unsigned)L.ny * (unsigned)L.nz;
+    const R rho0 = R(L.rho0), qs = SC_ONE / R(L.rho0);
+    auto idx = [&](int dx, int dy, int dz) {
+        return (unsigned)wrapm(x + dx, L.nx) * sx + (unsigned)wrapm(y + dy, L.ny) * (unsigned)L.nz +
+               (unsigned)wrapm(z + dz, L.nz);
+    };
+    R gx, gy, gz;
+    grad19<R>([&](int dx, int dy, int dz) { return phi_of_q(at(phi, idx(dx, dy, dz))); }, gx, gy, gz);
+    const R g2 = gx * gx + gy * gy + gz * gz;
+    bool flag = g2 > R(EVAP_LIM * EVAP_LIM);
+    // normals by offset key (dx + 1) 9 + (dy + 1) 3 + dz + 1, each formed at
+    // its first use
+    R nn[27][3];
+    bool have[27] = {};
+    auto n_at = [&](int k, int dx, int dy, int dz) {
+        const int key = (dx + 1) * 9 + (dy + 1) * 3 + dz + 1;
+        if (!have[key]) {
+            ext_normal<R, QPhi>(L, E, (const QPhi *)phi, x + dx, y + dy, z + dz, nn[key]);
+            have[key] = true;
+        }
+        return nn[key][k];
+    };
+    const R nx = n_at(0, 0, 0, 0), ny = n_at(1, 0, 0, 0), nz = n_at(2, 0, 0, 0);
+    if (nt & NT_SOLID) flag = false;
+    if (flag && (nt & NT_POUT)) { // wetting rescales the gradient too (S:569-581)
+        const R fnorm = fsqrt(g2);
+        gx = fnorm * nx;
+        gy = fnorm * ny;
+        gz = fnorm * nz;
+    }
+    const bool force_on = flag && !(nt & NT_POUT) && away_from_walls(E, L, x, y, z);
+    {
+        // the band's gradient (zero at solid nodes) and normal
+        const bool fluid = !(nt & NT_SOLID);
+        E.der[n] = fluid ? double(gx) : 0.0;
+        E.der[N + n] = fluid ? double(gy) : 0.0;
+        E.der[2 * N + n] = fluid ? double(gz) : 0.0;
+        E.der[3 * N + n] = double(nx);
+        E.der[4 * N + n] = double(ny);
+        E.der[5 * N + n] = double(nz);
+    }
+    unsigned xo[3], yo[3], zo[3];
+    node_offsets(L, x, y, z, xo, yo, zo);
+    if (nt & NT_SOLID) {
+        // each slot takes the total it pulls; its colours (side array) split
+        // with the source node's colour fraction
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
+            const R f = StoreOf<S>::ld(fin + (size_t)i * N, m);
+            StoreOf<S>::st(fout + (size_t)i * N, n, f);
+            const float qm = at(phi, m);
+            const R fa = f + R(W[i]) * rho0, hm = __builtin_fabsf(qm) * fa;
+            at((float *)E.rbo[0] + (size_t)i * N, n) =
+                __builtin_signbitf(qm) ? enc_slot(fa - hm, hm) : enc_slot(hm, fa - hm);
+        }
+        at(sout, n) = make_float4(FS_SOLID, 0.f, 0.f, 0.f);
+        return;
+    }
+    R f[Q];
+    Moments<R, true> mo;
+    R hm;
+    bool redmaj;
+    if (nt & NT_REFILL) {
+        // SRT equilibria of the refill densities and the rigid-body velocity
+        const double rr = E.mac[N + n], rb = E.mac[2 * N + n];
+        const R rt = R(rr + rb);
+        const R ux = R(E.mac[3 * N + n]), uy = R(E.mac[4 * N + n]), uz = R(E.mac[5 * N + n]);
+        const R usqr = ux * ux + uy * uy + uz * uz;
+        R s = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const R uc = ux * R(CX[i]) + uy * R(CY[i]) + uz * R(CZ[i]);
+            const R poly = R(1) + R(3) * uc + R(4.5) * uc * uc - R(1.5) * usqr;
+            f[i] = R(W[i]) * (rt * poly - rho0);
+            s += f[i];
+        }
+        mo.drho = s;
+        mo.rho = R(E.mac[n]);
+        mo.ux = ux;
+        mo.uy = uy;
+        mo.uz = uz;
+        mo.sr = R(rr);
+        mo.sb = R(rb);
+        redmaj = rb <= rr;
+        hm = R((redmaj ? rb : rr) / (rr + rb));
+    } else {
+        R df = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            f[i] = StoreOf<S>::ld(fin + (size_t)i * N, xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]]);
+            df += f[i];
+            if (CX[i] == 1) jx += f[i];
+            if (CX[i] == -1) jx -= f[i];
+            if (CY[i] == 1) jy += f[i];
+            if (CY[i] == -1) jy -= f[i];
+            if (CZ[i] == 1) jz += f[i];
+            if (CZ[i] == -1) jz -= f[i];
+        }
+        mo.rho = rho0 + df;
+        mo.drho = df;
+        const R irho = fdiv(R(1), mo.rho);
+        mo.ux = jx * irho;
+        mo.uy = jy * irho;
+        mo.uz = jz * irho;
+        const float qn = at(phi, n);
+        redmaj = __builtin_signbitf(qn);
+        hm = __builtin_fabsf(qn);
+        const R smin = mo.rho * hm;
+        mo.sr = redmaj ? mo.rho - smin : smin;
+        mo.sb = redmaj ? smin : mo.rho - smin;
+    }
+    const R phin = phi_of_q(at(phi, n));
+    mo.phi = phin;
+    // normal derivatives for the curvature (S:635-660)
+    R d[3][3];
+    if (force_on) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
+#pragma unroll
+        for (int i = 1; i < Q; ++i) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                R tmp = R(W[i]) * n_at(k, CX[i], CY[i], CZ[i]);
+                if (CX[i] == 1) d[k][0] = d[k][0] + tmp;
+                if (CX[i] == -1) d[k][0] = d[k][0] - tmp;
+                if (CY[i] == 1) d[k][1] = d[k][1] + tmp;
+                if (CY[i] == -1) d[k][1] = d[k][1] - tmp;
+                if (CZ[i] == 1) d[k][2] = d[k][2] + tmp;
+                if (CZ[i] == -1) d[k][2] = d[k][2] - tmp;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
+    }
+    collide<R, true>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, force_on, d);
+    const Recol<R> rc = recol<R, true>(L, mo, flag);
+    R g = rc.gam;
+    R Mf = hm;
+    if (rc.evr) { g = 0; Mf = redmaj ? R(1) : R(0); }
+    if (rc.evb) { g = 0; Mf = redmaj ? R(0) : R(1); }
+    const R Gx = g * nx, Gy = g * ny, Gz = g * nz;
+    const R Mg = redmaj ? -g : g;
+    // the next state's totals and recolour scalars (k_curve_fs reads those
+    // of the link nodes)
+#pragma unroll
+    for (int i = 0; i < Q; ++i) StoreOf<S>::st(fout + (size_t)i * N, n, f[i]);
+    {
+        Recol<R> ce = rc;
+        ce.M = hm;
+        ce.redmaj = redmaj;
+        at(sout, n) = make_float4(encode_frac(ce), float(Gx), float(Gy), float(Gz));
+    }
+    // the terms, formed as the tile kernel forms them
+    const R s1 = R(W[1]) * qs, s7 = R(W[7]) * qs;
+    const R Mgx = Mg * nx, Mgy = Mg * ny, Mgz = Mg * nz;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        R v = f[i];
+        if constexpr (sizeof(S) != 4) v = R(S(v)); // the value the next step reads
+        const R fq = __builtin_fmaf(v, qs, R(W[i] * double(SC_ONE)));
+        R cq = Mf * fq;
+        if (i > 0 && i <= 6) cq = __builtin_fmaf(Mf, fq, dot_c<R>(i, Mgx * s1, Mgy * s1, Mgz * s1));
+        if (i > 6) cq = __builtin_fmaf(Mf, fq, dot_c<R>(i, Mgx * s7, Mgy * s7, Mgz * s7));
+        const int qf = round_half_up(fq), qm = round_half_up(cq);
+        const int2 c = redmaj ? make_int2(qf - qm, qm) : make_int2(qm, qf - qm);
+        unsigned dst = n;
+        if (i > 0 && !is_wall_ghost(E, L, wrapm(x + CX[i], L.nx), wrapm(y + CY[i], L.ny), wrapm(z + CZ[i], L.nz))) {
+            dst = idx(CX[i], CY[i], CZ[i]);
+            if (at(E.ntype, dst) & NT_SOLID) continue;
+        }
+        atomicAdd((unsigned *)&own[dst].x, (unsigned)c.x);
+        atomicAdd((unsigned *)&own[dst].y, (unsigned)c.y);
+    }
+}
+
+// The band's step in the scatter form (after the tile kernel, which has
+// written every node's own-box sums).
+template <class S>
+__global__ __launch_bounds__(PB) void k_collide_sc_band(Lat L, Ext E, Part Pt, const S *__restrict__ fin,
+                                                        S *__restrict__ fout, float4 *__restrict__ sout,
+                                                        const float *__restrict__ phi, int2 *__restrict__ own)
+{
+    const int p = blockIdx.y;
+    const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 3.0);
+    const unsigned k = blockIdx.x * PB + threadIdx.x;
+    int X[3] = {0, 0, 0};
+    unsigned n = 0;
+    bool valid = k < box_size(b);
+    if (valid) {
+        box_node(L, b, k, X, n);
+        valid = box_in_domain(E, L, X);
+    }
+    const int x = wrapm(X[0], L.nx), y = wrapm(X[1], L.ny), z = wrapm(X[2], L.nz);
+    if (!first_band(L, E, Pt, valid, x, y, z)) return;
+    if (is_wall_ghost(E, L, x, y, z)) return;
+    const unsigned nt = at(E.ntype, n);
+    if (!(nt & NT_NEAR)) return;
+    sc_band_node<S>(L, E, fin, fout, sout, phi, own, x, y, z, n, nt);
+}
+
+// The band's fields at the start of a scatter-form step (band_macros): rho
+// and the velocity from the gathered totals, the colour densities from the
+// node's signed minority fraction; at solid nodes rho0, zero velocity and
+// phi = 0 (calmacros, S:784-789).
+template <class S>
+__global__ __launch_bounds__(PB) void k_band_sc(Lat L, Ext E, Part Pt, const S *__restrict__ fs, float *__restrict__ phi)
+{
+    for_special_nodes<true>(L, E, Pt, [&](int x, int y, int z) {
+        using R = float;
+        if (is_wall_ghost(E, L, x, y, z)) return;
+        unsigned xo[3], yo[3], zo[3];
+        node_offsets(L, x, y, z, xo, yo, zo);
+        const unsigned n = xo[1] + yo[1] + zo[1];
+        const unsigned nt = at(E.ntype, n);
+        if (!(nt & NT_NEAR)) return;
+        const size_t N = (size_t)L.N;
+        const R rho0 = R(L.rho0);
+        Moments<R, true> mo;
+        if (nt & NT_SOLID) {
+            at((QPhi *)phi, n) = QPhi(0.0f);
+            mo.rho = rho0;
+            mo.ux = mo.uy = mo.uz = 0;
+            mo.sr = mo.sb = 0;
+        } else {
+            R df = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const R f = StoreOf<S>::ld(fs + (size_t)i * N, xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]]);
+                df += f;
+                if (CX[i] == 1) jx += f;
+                if (CX[i] == -1) jx -= f;
+                if (CY[i] == 1) jy += f;
+                if (CY[i] == -1) jy -= f;
+                if (CZ[i] == 1) jz += f;
+                if (CZ[i] == -1) jz -= f;
+            }
+            mo.rho = rho0 + df;
+            const R irho = fdiv(R(1), mo.rho);
+            mo.ux = jx * irho;
+            mo.uy = jy * irho;
+            mo.uz = jz * irho;
+            const float qn = at(phi, n);
+            const R smin = mo.rho * __builtin_fabsf(qn);
+            mo.sr = __builtin_signbitf(qn) ? mo.rho - smin : smin;
+            mo.sb = __builtin_signbitf(qn) ? smin : mo.rho - smin;
+        }
+        band_macros(L, E, n, nt, mo);
+    });
 }
 
 } // namespace cgl
