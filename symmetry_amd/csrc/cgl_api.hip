@@ -216,6 +216,7 @@ int setup_scatter(cgl_handle *h)
                                  &h->sb.exylo, &h->sb.exyhi, &h->sb.exzlo, &h->sb.exzhi, &h->sb.eyz};
     for (int k = 0; k < cgl::SC_NBUF; ++k) HIPCHK(dmalloc((void **)bufs[k], h->N * sizeof(int2), PZ_SC, h->stream));
     HIPCHK(dmalloc((void **)&h->sb.range, 256, PZ_SC, h->stream));
+    if (h->npar) HIPCHK(dmalloc((void **)&h->bnrm, 3 * h->N * sizeof(float), PZ_SC, h->stream));
     h->sc = 1;
     return clear_range(h);
 }
@@ -459,6 +460,7 @@ int cgl_destroy(cgl_t *h)
     for (int2 *q : sbufs)
         if (q) (void)hipFree(q);
     if (h->sb.range) (void)hipFree(h->sb.range);
+    if (h->bnrm) (void)hipFree(h->bnrm);
     for (int k = 0; k < 4; ++k)
         if (h->ev[k]) (void)hipEventDestroy(h->ev[k]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -898,6 +900,9 @@ static int ensure_lists(cgl_handle *h, double rmax)
     // 2 ceil(R) + 9 nodes per axis, one thread per node
     const double wb = w + 4.0;
     h->band_blocks = (unsigned)((wb * wb * wb + cgl::PB - 1) / cgl::PB);
+    // and the normals' box, box_of(centre, R, 4)
+    const double wn = w + 6.0;
+    h->nrm_blocks = (unsigned)((wn * wn * wn + cgl::PB - 1) / cgl::PB);
     const int cap = (int)((w * w * w + cgl::LSLICES - 1) / cgl::LSLICES);
     if (h->P.lout && cap <= h->P.lcap) return CGL_OK;
     unsigned **v[3] = {&h->P.lout, &h->P.lmask, &h->P.lin};

@@ -69,6 +69,8 @@ struct cgl_handle {
     hipEvent_t *tev;     // timing runs: events around the wall/particle step's moments and collide kernels
     int stage;           // staged step: the last stage run (STAGE_*), STAGE_NONE between steps
     unsigned band_blocks; // workgroups per particle of the band kernels (one thread per node of its band box)
+    unsigned nrm_blocks;  // workgroups per particle of k_band_normals (its box with margin R + 4)
+    float *bnrm;          // scatter form with particles: the band boxes' unit normals (3 x N)
 };
 
 // Stages of a step driven through the per-stage entry points, in order.
@@ -278,10 +280,11 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                                                                        (float *)h->phi[h->phicur]);
         else if (h->L.nz % 2 == 0) // two nodes per thread
             k_phi_sc2<<<dim3((h->L.nz / 2 + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, h->L.nx), node_block(), 0,
-                        h->stream>>>(h->L, h->sb, h->xc, h->ky, (float *)h->phi[h->phicur]);
+                        h->stream>>>(h->L, h->sb, h->xc, h->ky, (float *)h->phi[h->phicur], h->ext ? SC_BIAS : 0);
         else
             k_phi_sc<<<node_grid(h->L), node_block(), 0, h->stream>>>(h->L, h->sb, h->xc, h->ky,
-                                                                      (float *)h->phi[h->phicur]);
+                                                                      (float *)h->phi[h->phicur],
+                                                                      h->ext ? SC_BIAS : 0);
         h->phi_ok = 1;
     }
     static void step_sc(cgl_handle *h, bool last)
@@ -622,6 +625,9 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
             float4 *sall = (float4 *)h->fss[h->cur ^ 1];
             Lat Ls = h->L;
             Ls.xcd_remap = h->sc_remap;
+            if (h->npar)
+                k_band_normals<<<dim3(h->nrm_blocks, h->npar), PB, 0, h->stream>>>(h->L, En, h->P,
+                                                                                   (const float *)phi, h->bnrm);
             if (h->tev) (void)hipEventRecord(h->tev[2], h->stream);
             const dim3 grid((h->L.nz + TZ - 1) / TZ, (h->L.ny + KTY<float> - 1) / KTY<float>,
                             (h->L.nx + h->xc - 1) / h->xc);
@@ -630,7 +636,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                 Ls, fin, fout, last ? sall : nullptr, (const float *)phi, h->sb, h->xc, En);
             if (h->npar)
                 k_collide_sc_band<S><<<dim3(h->band_blocks, h->npar), PB, 0, h->stream>>>(
-                    h->L, En, h->P, fin, fout, sall, (const float *)phi, h->sb.own);
+                    h->L, En, h->P, fin, fout, sall, (const float *)phi, h->bnrm, h->sb.own);
             if (h->tev) (void)hipEventRecord(h->tev[3], h->stream);
             h->state_layout = FS;
             if (ghost_faces(h))

@@ -606,8 +606,11 @@ __global__ __launch_bounds__(PB) void k_curve_fs(Lat L, Ext E, Part Pt, S *fo, c
                     const bool redmaj = vb <= vr;
                     const int qf = round_half_up(__builtin_fmaf(float(fo[k]), qs, float(W[i] * double(SC_ONE))));
                     const int qm = round_half_up(float(redmaj ? vb : vr) * qs);
-                    atomicAdd((unsigned *)&own[n].x, (unsigned)(redmaj ? qf - qm : qm));
-                    atomicAdd((unsigned *)&own[n].y, (unsigned)(redmaj ? qm : qf - qm));
+                    // one 64-bit addition, exact while the red half stays in
+                    // [0, 2^32) (the own red sums carry a bias of 2^30)
+                    const int cr = redmaj ? qf - qm : qm, cb = redmaj ? qm : qf - qm;
+                    atomicAdd((unsigned long long *)&own[n],
+                              ((unsigned long long)(unsigned)cb << 32) + (unsigned long long)(long long)cr);
                 }
             } else {
                 fo[k] = vr + vb;

@@ -75,6 +75,20 @@ __device__ __forceinline__ void add2(int2 &a, int2 b)
     a.y = (int)((unsigned)a.y + (unsigned)b.y);
 }
 __device__ __forceinline__ bool sums_wrapped(int2 s) { return min(s.x, s.y) < -(1 << 28); }
+
+// With particles the band kernel and the links add their terms to the
+// destinations' own sums by one 64-bit atomic addition each, of
+// (sB << 32) + sR with sR sign-extended.  That is exact for both halves when
+// the red half of the final 64-bit value lies in [0, 2^32): the tile kernel
+// writes every own red sum biased by SC_BIAS = 2^30 (a red partial or total
+// lies in [-2^28, 2^31)), and the finishing pass removes the bias.
+constexpr int SC_BIAS = 1 << 30;
+__device__ __forceinline__ void atomic_add2(int2 *a, int2 c)
+{
+    const unsigned long long v =
+        ((unsigned long long)(unsigned)c.y << 32) + (unsigned long long)(long long)c.x;
+    atomicAdd((unsigned long long *)a, v);
+}
 __device__ __forceinline__ int2 shfl2(int2 v, int src) { return make_int2(__shfl(v.x, src), __shfl(v.y, src)); }
 
 // The phase-field buffer entry (signed minority fraction, minority_q) of a
@@ -556,7 +570,7 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
         const unsigned pm = (unsigned)wrapm(x - 1, L.nx) * sx;
         if (!active) {
         } else if (x > x0) {
-            at(B.own, pm + nself) = am;
+            at(B.own, pm + nself) = EXT ? make_int2((int)((unsigned)am.x + (unsigned)SC_BIAS), am.y) : am;
             if (yedge) at(B.fy, pm + nrow) = ym;
             if (zedge) at(B.fz, (unsigned)wrapm(x - 1, L.nx) * csx + cself) = zm;
         } else { // plane x0 - 1 belongs to the box on the low x side
@@ -581,7 +595,7 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     // plane x1 - 1 (own box) and plane x1 (the box on the high x side)
     if (!active) return;
     const unsigned pl = (unsigned)wrapm(x1 - 1, L.nx) * sx, pn = (unsigned)wrapm(x1, L.nx) * sx;
-    at(B.own, pl + nself) = am;
+    at(B.own, pl + nself) = EXT ? make_int2((int)((unsigned)am.x + (unsigned)SC_BIAS), am.y) : am;
     at(B.fxlo, pn + nself) = ac;
     if (yedge) {
         at(B.fy, pl + nrow) = ym;
@@ -646,10 +660,11 @@ __global__ __launch_bounds__(TZ *KTY<float>, 4) void k_collide_sc_batch(const Bo
 // node's own-box sums and the partial sums its position in its workgroup box
 // (x chunk xc, ky rows) calls for.
 __device__ __forceinline__ void phi_sc_node(const Lat &L, const ScBufs &B, int xc, int ky, float *__restrict__ phi,
-                                            int x, int y, int z)
+                                            int x, int y, int z, int bias = 0)
 {
     const unsigned n = ((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z;
     int2 s = at(B.own, n);
+    s.x = (int)((unsigned)s.x - (unsigned)bias);
     const int cx0 = (x / xc) * xc;
     const bool xlo = x == cx0, xhi = x == min(cx0 + xc, L.nx) - 1;
     const int yr = y % ky, zr = z % TZ;
@@ -671,11 +686,13 @@ __device__ __forceinline__ void phi_sc_node(const Lat &L, const ScBufs &B, int x
     at(phi, n) = q_of_sums(s);
 }
 
-static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, int ky, float *__restrict__ phi)
+// bias: SC_BIAS where the own red sums carry it (walls and particles), else 0.
+static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, int ky, float *__restrict__ phi,
+                                                       int bias)
 {
     const int z = blockIdx.x * TZ + threadIdx.x, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z;
     if (z >= L.nz || y >= L.ny) return;
-    phi_sc_node(L, B, xc, ky, phi, x, y, z);
+    phi_sc_node(L, B, xc, ky, phi, x, y, z, bias);
 }
 
 // The finishing pass two nodes at a time (z and z + 1, z even, nz even): the
@@ -683,14 +700,16 @@ static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, 
 // call for both nodes of the pair or for neither) likewise, the z-face sums
 // per node, phi in one 8-byte store.  Integer sums are exact, so the order of
 // the additions does not change phi.
-static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc, int ky, float *__restrict__ phi)
+static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc, int ky, float *__restrict__ phi,
+                                                        int bias)
 {
     const int z = (blockIdx.x * TZ + threadIdx.x) * 2, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z;
     if (z >= L.nz || y >= L.ny) return;
     const unsigned n = ((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z;
     const unsigned h = n >> 1; // pair index: n is even
     const int4 o = at((const int4 *)B.own, h);
-    int2 s[2] = {make_int2(o.x, o.y), make_int2(o.z, o.w)};
+    int2 s[2] = {make_int2((int)((unsigned)o.x - (unsigned)bias), o.y),
+                 make_int2((int)((unsigned)o.z - (unsigned)bias), o.w)};
     auto add_pair = [&](const int2 *a) {
         const int4 v = at((const int4 *)a, h);
         add2(s[0], make_int2(v.x, v.y));
@@ -807,8 +826,8 @@ __device__ __forceinline__ bool first_band(const Lat &L, const Ext &E, const Par
 template <class S>
 __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S *__restrict__ fin,
                                              S *__restrict__ fout, float4 *__restrict__ sout,
-                                             const float *__restrict__ phi, int2 *__restrict__ own, int x, int y,
-                                             int z, unsigned n, unsigned nt)
+                                             const float *__restrict__ phi, const float *__restrict__ nrm,
+                                             int2 *__restrict__ own, int x, int y, int z, unsigned n, unsigned nt)
 {
     using R = float;
     const size_t N = (size_t)L.N;
@@ -822,18 +841,8 @@ __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S
     grad19<R>([&](int dx, int dy, int dz) { return phi_of_q(at(phi, idx(dx, dy, dz))); }, gx, gy, gz);
     const R g2 = gx * gx + gy * gy + gz * gz;
     bool flag = g2 > R(EVAP_LIM * EVAP_LIM);
-    // normals by offset key (dx + 1) 9 + (dy + 1) 3 + dz + 1, each formed at
-    // its first use
-    R nn[27][3];
-    bool have[27] = {};
-    auto n_at = [&](int k, int dx, int dy, int dz) {
-        const int key = (dx + 1) * 9 + (dy + 1) * 3 + dz + 1;
-        if (!have[key]) {
-            ext_normal<R, QPhi>(L, E, (const QPhi *)phi, x + dx, y + dy, z + dz, nn[key]);
-            have[key] = true;
-        }
-        return nn[key][k];
-    };
+    // the normals of the node and its neighbours (k_band_normals)
+    auto n_at = [&](int k, int dx, int dy, int dz) { return at(nrm + (size_t)k * N, idx(dx, dy, dz)); };
     const R nx = n_at(0, 0, 0, 0), ny = n_at(1, 0, 0, 0), nz = n_at(2, 0, 0, 0);
     if (nt & NT_SOLID) flag = false;
     if (flag && (nt & NT_POUT)) { // wetting rescales the gradient too (S:569-581)
@@ -985,8 +994,7 @@ __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S
             dst = idx(CX[i], CY[i], CZ[i]);
             if (at(E.ntype, dst) & NT_SOLID) continue;
         }
-        atomicAdd((unsigned *)&own[dst].x, (unsigned)c.x);
-        atomicAdd((unsigned *)&own[dst].y, (unsigned)c.y);
+        atomic_add2(own + dst, c);
     }
 }
 
@@ -995,7 +1003,8 @@ __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S
 template <class S>
 __global__ __launch_bounds__(PB) void k_collide_sc_band(Lat L, Ext E, Part Pt, const S *__restrict__ fin,
                                                         S *__restrict__ fout, float4 *__restrict__ sout,
-                                                        const float *__restrict__ phi, int2 *__restrict__ own)
+                                                        const float *__restrict__ phi, const float *__restrict__ nrm,
+                                                        int2 *__restrict__ own)
 {
     const int p = blockIdx.y;
     const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 3.0);
@@ -1012,7 +1021,30 @@ __global__ __launch_bounds__(PB) void k_collide_sc_band(Lat L, Ext E, Part Pt, c
     if (is_wall_ghost(E, L, x, y, z)) return;
     const unsigned nt = at(E.ntype, n);
     if (!(nt & NT_NEAR)) return;
-    sc_band_node<S>(L, E, fin, fout, sout, phi, own, x, y, z, n, nt);
+    sc_band_node<S>(L, E, fin, fout, sout, phi, nrm, own, x, y, z, n, nt);
+}
+
+// The unit normals (ext_normal) of every node of each particle's box with
+// margin R + 4, which holds every neighbour of its band, into nrm (3 x N,
+// component-major), for the band kernel's curvature.  Workgroups (block,
+// particle); a node in several boxes is written with equal values.
+static __global__ __launch_bounds__(PB) void k_band_normals(Lat L, Ext E, Part Pt, const float *__restrict__ phi,
+                                                            float *__restrict__ nrm)
+{
+    const int p = blockIdx.y;
+    const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 4.0);
+    const unsigned k = blockIdx.x * PB + threadIdx.x;
+    if (k >= box_size(b)) return;
+    int X[3];
+    unsigned n;
+    box_node(L, b, k, X, n);
+    if (!box_in_domain(E, L, X)) return;
+    float nv[3];
+    ext_normal<float, QPhi>(L, E, (const QPhi *)phi, wrapm(X[0], L.nx), wrapm(X[1], L.ny), wrapm(X[2], L.nz), nv);
+    const size_t N = (size_t)L.N;
+    at(nrm, n) = nv[0];
+    at(nrm + N, n) = nv[1];
+    at(nrm + 2 * N, n) = nv[2];
 }
 
 // The band's fields at the start of a scatter-form step (band_macros): rho
