@@ -425,6 +425,19 @@ template <int I> __device__ __forceinline__ unsigned mirrored_axes(unsigned madj
     if ((CZ[I] == 1 && (madj & 16u)) || (CZ[I] == -1 && (madj & 32u))) A |= 4u;
     return A;
 }
+// The directions of wall_bb_mask that come from the walls of one axis a
+// (coordinate c of n along it): a march along x holds those of y and z.
+__device__ __forceinline__ unsigned wall_bb_axis(const Ext &E, int a, int c, int n)
+{
+    const bool lo = E.wall[a][0] && c == 1, hi = E.wall[a][1] && c == n - 2;
+    unsigned m = 0;
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+        const int ci = a == 0 ? CX[i] : (a == 1 ? CY[i] : CZ[i]);
+        if ((ci == 1 && lo) || (ci == -1 && hi)) m |= 1u << i;
+    }
+    return m;
+}
 __device__ __forceinline__ unsigned wall_bb_mask(const Ext &E, const Lat &L, int x, int y, int z)
 {
     const bool lo[3] = {E.wall[0][0] && x == 1, E.wall[1][0] && y == 1, E.wall[2][0] && z == 1};

@@ -202,7 +202,9 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     // (loaded a plane early, so that no load holds up a plane's start)
     // (bits of eflags: 0 and 1 whether the node in planes x and x + 1 is in
     // the band, 2 and 3 whether this node's / the halo element's column
-    // lies in a y or z wall ghost layer)
+    // lies in a y or z wall ghost layer, 4 whether the column is clear of
+    // the y and z walls' force guard, 13-31 the node's bounce-back
+    // directions off y and z walls (bits 0-18 of wall_bb_mask))
     const unsigned oin = (unsigned)wrapm(y, L.ny) * (unsigned)L.nz + (unsigned)wrapm(z, L.nz);
     const unsigned hin = (unsigned)wrapm(y0 - 1 + hy, L.ny) * (unsigned)L.nz + (unsigned)wrapm(z0 - 1 + hz, L.nz);
     unsigned eflags = 0;
@@ -211,8 +213,13 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             return (E.wall[1][0] && py == 0) || (E.wall[1][1] && py == L.ny - 1) || (E.wall[2][0] && pz == 0) ||
                    (E.wall[2][1] && pz == L.nz - 1);
         };
-        if (wall_yz(wrapm(y, L.ny), wrapm(z, L.nz))) eflags |= 4u;
+        const int yw0 = wrapm(y, L.ny), zw0 = wrapm(z, L.nz);
+        if (wall_yz(yw0, zw0)) eflags |= 4u;
         if (wall_yz(wrapm(y0 - 1 + hy, L.ny), wrapm(z0 - 1 + hz, L.nz))) eflags |= 8u;
+        if (!((E.wall[1][0] && yw0 < 2) || (E.wall[1][1] && yw0 > L.ny - 3) || (E.wall[2][0] && zw0 < 2) ||
+              (E.wall[2][1] && zw0 > L.nz - 3)))
+            eflags |= 16u;
+        eflags |= (wall_bb_axis(E, 1, yw0, L.ny) | wall_bb_axis(E, 2, zw0, L.nz)) << 13;
     }
     unsigned nto = 0, nth = 0;
     auto load_nt = [&](int p) {
@@ -320,9 +327,10 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
         bool live = active;
         unsigned wbb = 0; // wall bounce-back directions of the node
         if constexpr (EXT) {
+            // (the x walls' parts are the same for the whole plane)
             if (live)
                 live = !(eflags & 5u) && !(E.wall[0][0] && x == 0) && !(E.wall[0][1] && x == L.nx - 1);
-            if (live) wbb = wall_bb_mask(E, L, x, y, z);
+            if (live) wbb = (eflags >> 13) | wall_bb_axis(E, 0, x, L.nx);
             eflags = (eflags & ~3u) | ((eflags >> 1) & 1u);
         }
         if (live) {
@@ -330,7 +338,8 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             const R g2 = gx * gx + gy * gy + gz * gz;
             const bool flag = g2 > R(EVAP_LIM * EVAP_LIM);
             bool force_on = flag;
-            if constexpr (EXT) force_on = flag && away_from_walls(E, L, x, y, z);
+            if constexpr (EXT)
+                force_on = flag && (eflags & 16u) && !((E.wall[0][0] && x < 2) || (E.wall[0][1] && x > L.nx - 3));
             const R nx = sm.n[0][slot3(x)][ty + 1][tz + 1];
             const R ny = sm.n[1][slot3(x)][ty + 1][tz + 1];
             const R nz = sm.n[2][slot3(x)][ty + 1][tz + 1];
@@ -464,14 +473,7 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
                 if (i > 0 && i <= 6) cq = __builtin_fmaf(Mf, fq, dot_c<R>(i, G1x, G1y, G1z));
                 if (i > 6) cq = __builtin_fmaf(Mf, fq, dot_c<R>(i, G7x, G7y, G7z));
                 const int qf = round_half_up(fq), qm = round_half_up(cq);
-                int2 c = redmaj ? make_int2(qf - qm, qm) : make_int2(qm, qf - qm);
-                if constexpr (EXT) {
-                    // a term into a wall ghost returns to the node itself
-                    if (i > 0 && ((wbb >> OPC[i]) & 1u)) {
-                        add2(g00[1], c);
-                        c = Z2;
-                    }
-                }
+                const int2 c = redmaj ? make_int2(qf - qm, qm) : make_int2(qm, qf - qm);
                 const int k = CX[i] + 1;
                 if (CY[i] == 0 && CZ[i] == 0) g00[k] = c;
                 else if (CY[i] == 0 && CZ[i] == 1) gzp[k] = c;
@@ -482,6 +484,27 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
                 else if (CY[i] == 1) dpm = c;
                 else if (CZ[i] == 1) dmp = c;
                 else dmm = c;
+            }
+            if constexpr (EXT) {
+                if (wbb) {
+                    // terms into a wall ghost return to the node itself
+#pragma unroll
+                    for (int i = 1; i < Q; ++i) {
+                        if (!((wbb >> OPC[i]) & 1u)) continue;
+                        const int k = CX[i] + 1;
+                        int2 &t = (CY[i] == 0 && CZ[i] == 0) ? g00[k]
+                                  : (CY[i] == 0 && CZ[i] == 1) ? gzp[k]
+                                  : (CY[i] == 0)               ? gzm[k]
+                                  : (CZ[i] == 0 && CY[i] == 1) ? gyp[k]
+                                  : (CZ[i] == 0)               ? gym[k]
+                                  : (CY[i] == 1 && CZ[i] == 1) ? dpp
+                                  : (CY[i] == 1)               ? dpm
+                                  : (CZ[i] == 1)               ? dmp
+                                                               : dmm;
+                        add2(g00[1], t);
+                        t = Z2;
+                    }
+                }
             }
         }
 
