@@ -220,7 +220,8 @@ def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512"), layo
     (factorised and red/blue layouts); and 200 spheres (R=8, seed 99,
     densities in [0.8, 2.0], contact angles in [30, 150] degrees) within +-24
     of the interface in a 512^3 box with z walls, FP32 fields with FP64
-    particle dynamics (factorised and red/blue), and FP64 (red/blue).  MLUPS
+    particle dynamics (factorised and red/blue), FP16 storage (factorised)
+    and FP64 (red/blue).  MLUPS
     over all nodes, particle kernels included; per workload the step against
     its counted bytes (312 B/node FP32, 624 FP64), and the step's collide
     kernels and moments kernels (HIP events on the library's stream: in the
@@ -238,8 +239,9 @@ def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512"), layo
     c5 = ic.particle_config5()
     p5 = ic.sphere_particles(c5["centers"], c5["rad"], c5["rho_p"], c5["cita_deg"], c5["sigma"])
     runs = [("sphere_96", c2, p2, "f64", "rb", steps), ("sphere_96", c2, p2, "f32", "fs", steps),
-            ("sphere_96", c2, p2, "f32", "rb", steps), ("dense_512", c5, p5, "f32", "fs", steps),
-            ("dense_512", c5, p5, "f32", "rb", steps)]
+            ("sphere_96", c2, p2, "f32", "rb", steps), ("sphere_96", c2, p2, "f16", "fs", steps),
+            ("dense_512", c5, p5, "f32", "fs", steps), ("dense_512", c5, p5, "f32", "rb", steps),
+            ("dense_512", c5, p5, "f16", "fs", steps)]
     if f64_dense_steps > 0:
         runs.append(("dense_512", c5, p5, "f64", "rb", f64_dense_steps))
     runs = [r for r in runs if r[0] in which and r[4] in layouts]

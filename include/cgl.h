@@ -130,9 +130,11 @@ int cgl_tuned_params(cgl_t *h, int32_t *xchunk, int32_t *xcd_order, int32_t *til
 
 /* 1 when the handle's steps compute the next phase field in scatter form
  * (calmacros' sums S:792-799 accumulated by the writers of the populations),
- * so that a step reads the state once: periodic FP32/FP16 factorised boxes,
- * single or batched, with ny % rows != 1 (rows = tile_rows, 8 when 0) and
- * nz % 64 != 1 (every workgroup tile has two rows and two lanes), unless the
+ * so that a step reads the state once: FP32/FP16 factorised boxes, single
+ * or batched, with ny % rows != 1 (rows = tile_rows, 8 when 0) and
+ * nz % 64 != 1 (every workgroup tile has two rows and two lanes); single
+ * boxes also with wall faces and particles (not mirror faces; rows 8, and
+ * with particles from the second step after cgl_set_particles on), unless the
  * environment sets CGL_SCATTER=0 at creation.  0: a moments pass per step (gather form).
  * No counterpart in the reference. */
 int cgl_phase_field_form(cgl_t *h, int32_t *scatter);
@@ -150,7 +152,7 @@ int cgl_checkpoint_save(cgl_t *h, const char *path);
 int cgl_checkpoint_load(cgl_t *h, const char *path);
 
 /* Block until all queued device work of the handle has finished.
- * Periodic FP32/FP16 factorised boxes sum the next step's colour densities in
+ * FP32/FP16 factorised boxes in the scatter form sum the next step's colour densities in
  * 32-bit fixed point at 2^29 units per rho0 (the scatter form): a colour
  * density of 4 rho0 or more does not fit.  Such a step raises a flag that
  * cgl_sync, cgl_time_steps and every call that reads the state (calmacros,
@@ -291,8 +293,9 @@ int cgl_box_calmacros(cgl_t *h, int32_t box, double *rho, double *rho_r, double 
 /* Time nsteps steps on the device with HIP events recorded on the handle's
  * stream: total elapsed ms, and ms summed over the moments (scatter form: the
  * phase-field pass) and fused-collide kernels; with walls or particles, the
- * moments and collide kernels alone (the particle kernels between them are in
- * neither).  No counterpart in the reference (it has no timers). */
+ * moments and collide kernels alone (in the scatter form the finishing pass,
+ * and the tile and band step kernels; the particle kernels between them are
+ * in neither).  No counterpart in the reference (it has no timers). */
 int cgl_time_steps(cgl_t *h, int32_t nsteps, double *ms_total, double *ms_moments, double *ms_collide);
 
 /* Achievable HBM bandwidth of a device: a streaming copy of `bytes` bytes,
