@@ -99,7 +99,7 @@ __device__ __forceinline__ bool box_node(const Lat &L, const Box &b, unsigned k,
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         X[a] = b.lo[a] + c[a];
-        g[a] = wrapm(X[a], n3[a]);
+        g[a] = wrapn(X[a], n3[a]);
     }
     n = ((unsigned)g[0] * L.ny + g[1]) * L.nz + g[2];
     return true;
@@ -129,7 +129,7 @@ __device__ __forceinline__ bool interior(const Ext &E, const Lat &L, const int X
 }
 __device__ __forceinline__ unsigned nb_index(const Lat &L, const int X[3], int dx, int dy, int dz)
 {
-    const int gx = wrapm(X[0] + dx, L.nx), gy = wrapm(X[1] + dy, L.ny), gz = wrapm(X[2] + dz, L.nz);
+    const int gx = wrapn(X[0] + dx, L.nx), gy = wrapn(X[1] + dy, L.ny), gz = wrapn(X[2] + dz, L.nz);
     return ((unsigned)gx * L.ny + gy) * L.nz + gz;
 }
 __device__ __forceinline__ unsigned box_size(const Box &b) { return (unsigned)b.w[0] * b.w[1] * b.w[2]; }
@@ -151,7 +151,7 @@ __device__ __forceinline__ void read_slot(const Ext &E, const Lat &L, const int 
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         src[a] = X[a] - c[a];
-        const int g = wrapm(src[a], n3[a]);
+        const int g = wrapn(src[a], n3[a]);
         if ((E.wall[a][0] && g == 0) || (E.wall[a][1] && g == n3[a] - 1)) wall = true;
         if ((E.mirror[a][0] && g == 0) || (E.mirror[a][1] && g == n3[a] - 1)) A |= 1u << a;
     }
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(PB) void k_wet_normals(Lat L, Ext E, Part Pt, const
         if (unit_normal<R>(gx, gy, gz, nx, ny, nz)) {
             // the node's position in the box (wetting takes the minimum image)
             double nv[3] = {double(nx), double(ny), double(nz)};
-            wetting(L, E, wrapm(X[0], L.nx) + 1.0, wrapm(X[1], L.ny) + 1.0, wrapm(X[2], L.nz) + 1.0,
+            wetting(L, E, wrapn(X[0], L.nx) + 1.0, wrapn(X[1], L.ny) + 1.0, wrapn(X[2], L.nz) + 1.0,
                     Pt.center + 3 * p, Pt.cita[p], nv);
             nx = R(nv[0]);
             ny = R(nv[1]);
@@ -600,8 +600,8 @@ __global__ __launch_bounds__(PB) void k_curve_fs(Lat L, Ext E, Part Pt, S *fo, c
             if constexpr (SHIFT) {
                 fo[k] = (S)(float)(vr + vb - W[i] * L.rho0);
                 ((float *)side0)[k] = enc_slot(vr, vb);
-                if (own && !is_wall_ghost(E, L, wrapm(X[0] - CX[i], L.nx), wrapm(X[1] - CY[i], L.ny),
-                                          wrapm(X[2] - CZ[i], L.nz))) {
+                if (own && !is_wall_ghost(E, L, wrapn(X[0] - CX[i], L.nx), wrapn(X[1] - CY[i], L.ny),
+                                          wrapn(X[2] - CZ[i], L.nz))) {
                     const float qs = SC_ONE / float(L.rho0);
                     const bool redmaj = vb <= vr;
                     const int qf = round_half_up(__builtin_fmaf(float(fo[k]), qs, float(W[i] * double(SC_ONE))));
@@ -657,7 +657,7 @@ __device__ __forceinline__ void for_special_nodes(const Lat &L, const Ext &E, co
         unsigned n;
         box_node(L, b, k, X, n);
         if (!box_in_domain(E, L, X)) return;
-        f(wrapm(X[0], L.nx), wrapm(X[1], L.ny), wrapm(X[2], L.nz));
+        f(wrapn(X[0], L.nx), wrapn(X[1], L.ny), wrapn(X[2], L.nz));
     }
 }
 
@@ -669,7 +669,7 @@ template <class R, class P>
 __device__ __forceinline__ void ext_normal(const Lat &L, const Ext &E, const P *__restrict__ phi, int px, int py,
                                            int pz, R nv[3])
 {
-    const int gxw = wrapm(px, L.nx), gyw = wrapm(py, L.ny), gzw = wrapm(pz, L.nz);
+    const int gxw = wrapn(px, L.nx), gyw = wrapn(py, L.ny), gzw = wrapn(pz, L.nz);
     R nx = 0, ny = 0, nz = 0;
     bool skip = is_wall_ghost(E, L, gxw, gyw, gzw);
     unsigned nt = 0, A = 0;
@@ -689,8 +689,8 @@ __device__ __forceinline__ void ext_normal(const Lat &L, const Ext &E, const P *
         R gx, gy, gz;
         grad19<R>(
             [&](int dx, int dy, int dz) {
-                return phi[(unsigned)wrapm(gxw + ox + dx, L.nx) * sx + (unsigned)wrapm(gyw + oy + dy, L.ny) * L.nz +
-                           wrapm(gzw + oz + dz, L.nz)];
+                return phi[(unsigned)wrapn(gxw + ox + dx, L.nx) * sx + (unsigned)wrapn(gyw + oy + dy, L.ny) * L.nz +
+                           wrapn(gzw + oz + dz, L.nz)];
             },
             gx, gy, gz);
         const bool fl = unit_normal<R>(gx, gy, gz, nx, ny, nz);
@@ -749,8 +749,8 @@ __global__ __launch_bounds__(256) void k_collide_special(Lat L, const void *__re
         lane_step<FS, FS, S, R, P, SHIFT, true>(
             L, i0, i1, o0, o1, phio, E, x, y, z,
             [&](int dx, int dy, int dz) {
-                return phi[(unsigned)wrapm(x + dx, L.nx) * sx + (unsigned)wrapm(y + dy, L.ny) * L.nz +
-                           wrapm(z + dz, L.nz)];
+                return phi[(unsigned)wrapn(x + dx, L.nx) * sx + (unsigned)wrapn(y + dy, L.ny) * L.nz +
+                           wrapn(z + dz, L.nz)];
             },
             [&](int k, int dx, int dy, int dz) {
                 const int key = (dx + 1) * 9 + (dy + 1) * 3 + dz + 1;
@@ -761,8 +761,8 @@ __global__ __launch_bounds__(256) void k_collide_special(Lat L, const void *__re
                 return nn[key][k];
             },
             [&](const int o[3]) {
-                const unsigned m = (unsigned)wrapm(x + o[0], L.nx) * sx + (unsigned)wrapm(y + o[1], L.ny) * L.nz +
-                                   wrapm(z + o[2], L.nz);
+                const unsigned m = (unsigned)wrapn(x + o[0], L.nx) * sx + (unsigned)wrapn(y + o[1], L.ny) * L.nz +
+                                   wrapn(z + o[2], L.nz);
                 if constexpr (SHIFT)
                     return at((const float4 *)i1, m);
                 else

@@ -345,27 +345,9 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             const R nz = sm.n[2][slot3(x)][ty + 1][tz + 1];
             // normal derivatives for the curvature (S:635-660)
             R d[3][3];
-            if (force_on) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
-#pragma unroll
-                for (int i = 1; i < Q; ++i) {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        R tmp = R(W[i]) * sm.n[k][slot3(x + CX[i])][ty + 1 + CY[i]][tz + 1 + CZ[i]];
-                        if (CX[i] == 1) d[k][0] = d[k][0] + tmp;
-                        if (CX[i] == -1) d[k][0] = d[k][0] - tmp;
-                        if (CY[i] == 1) d[k][1] = d[k][1] + tmp;
-                        if (CY[i] == -1) d[k][1] = d[k][1] - tmp;
-                        if (CZ[i] == 1) d[k][2] = d[k][2] + tmp;
-                        if (CZ[i] == -1) d[k][2] = d[k][2] - tmp;
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 3; ++k)
-#pragma unroll
-                    for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
-            }
+            if (force_on)
+                curv19<R>([&](int k, int dx, int dy, int dz) { return sm.n[k][slot3(x + dx)][ty + 1 + dy][tz + 1 + dz]; },
+                          d);
 
             // streaming: pull gather of the total populations (deviations from w rho0)
             unsigned xo[3], yo[3], zo[3];
@@ -809,11 +791,11 @@ __device__ __forceinline__ bool first_band(const Lat &L, const Ext &E, const Par
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const bool per = !bounded(E, a);
-                const int lo = per ? wrapm(b.lo[a], n3[a]) : b.lo[a];
+                const int lo = per ? wrapn(b.lo[a], n3[a]) : b.lo[a];
                 bl[threadIdx.x][a] = lo;
                 bl[threadIdx.x][3 + a] = b.w[a];
                 // the two boxes meet along the axis (periodic: some image does)
-                const int d = per ? wrapm(lo - bp.lo[a], n3[a]) : lo - bp.lo[a];
+                const int d = per ? wrapn(lo - bp.lo[a], n3[a]) : lo - bp.lo[a];
                 const bool meet = per ? (d < bp.w[a] || d + b.w[a] > n3[a]) : (d < bp.w[a] && d + b.w[a] > 0);
                 if (!meet) m = 0;
             }
@@ -857,8 +839,8 @@ __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S
     const unsigned sx = (unsigned)L.ny * (unsigned)L.nz;
     const R rho0 = R(L.rho0), qs = SC_ONE / R(L.rho0);
     auto idx = [&](int dx, int dy, int dz) {
-        return (unsigned)wrapm(x + dx, L.nx) * sx + (unsigned)wrapm(y + dy, L.ny) * (unsigned)L.nz +
-               (unsigned)wrapm(z + dz, L.nz);
+        return (unsigned)wrapn(x + dx, L.nx) * sx + (unsigned)wrapn(y + dy, L.ny) * (unsigned)L.nz +
+               (unsigned)wrapn(z + dz, L.nz);
     };
     R gx, gy, gz;
     grad19<R>([&](int dx, int dy, int dz) { return phi_of_q(at(phi, idx(dx, dy, dz))); }, gx, gy, gz);
@@ -960,27 +942,7 @@ __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S
     mo.phi = phin;
     // normal derivatives for the curvature (S:635-660)
     R d[3][3];
-    if (force_on) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) d[k][0] = d[k][1] = d[k][2] = 0;
-#pragma unroll
-        for (int i = 1; i < Q; ++i) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                R tmp = R(W[i]) * n_at(k, CX[i], CY[i], CZ[i]);
-                if (CX[i] == 1) d[k][0] = d[k][0] + tmp;
-                if (CX[i] == -1) d[k][0] = d[k][0] - tmp;
-                if (CY[i] == 1) d[k][1] = d[k][1] + tmp;
-                if (CY[i] == -1) d[k][1] = d[k][1] - tmp;
-                if (CZ[i] == 1) d[k][2] = d[k][2] + tmp;
-                if (CZ[i] == -1) d[k][2] = d[k][2] - tmp;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int l = 0; l < 3; ++l) d[k][l] = R(3) * d[k][l];
-    }
+    if (force_on) curv19<R>(n_at, d);
     collide<R, true>(L, f, mo, phin, gx, gy, gz, nx, ny, nz, force_on, d);
     const Recol<R> rc = recol<R, true>(L, mo, flag);
     R g = rc.gam;
@@ -1013,7 +975,7 @@ __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S
         const int qf = round_half_up(fq), qm = round_half_up(cq);
         const int2 c = redmaj ? make_int2(qf - qm, qm) : make_int2(qm, qf - qm);
         unsigned dst = n;
-        if (i > 0 && !is_wall_ghost(E, L, wrapm(x + CX[i], L.nx), wrapm(y + CY[i], L.ny), wrapm(z + CZ[i], L.nz))) {
+        if (i > 0 && !is_wall_ghost(E, L, wrapn(x + CX[i], L.nx), wrapn(y + CY[i], L.ny), wrapn(z + CZ[i], L.nz))) {
             dst = idx(CX[i], CY[i], CZ[i]);
             if (at(E.ntype, dst) & NT_SOLID) continue;
         }
@@ -1039,7 +1001,7 @@ __global__ __launch_bounds__(PB) void k_collide_sc_band(Lat L, Ext E, Part Pt, c
         box_node(L, b, k, X, n);
         valid = box_in_domain(E, L, X);
     }
-    const int x = wrapm(X[0], L.nx), y = wrapm(X[1], L.ny), z = wrapm(X[2], L.nz);
+    const int x = wrapn(X[0], L.nx), y = wrapn(X[1], L.ny), z = wrapn(X[2], L.nz);
     if (!first_band(L, E, Pt, valid, x, y, z)) return;
     if (is_wall_ghost(E, L, x, y, z)) return;
     const unsigned nt = at(E.ntype, n);
@@ -1063,7 +1025,7 @@ static __global__ __launch_bounds__(PB) void k_band_normals(Lat L, Ext E, Part P
     box_node(L, b, k, X, n);
     if (!box_in_domain(E, L, X)) return;
     float nv[3];
-    ext_normal<float, QPhi>(L, E, (const QPhi *)phi, wrapm(X[0], L.nx), wrapm(X[1], L.ny), wrapm(X[2], L.nz), nv);
+    ext_normal<float, QPhi>(L, E, (const QPhi *)phi, wrapn(X[0], L.nx), wrapn(X[1], L.ny), wrapn(X[2], L.nz), nv);
     const size_t N = (size_t)L.N;
     at(nrm, n) = nv[0];
     at(nrm + N, n) = nv[1];
