@@ -507,7 +507,9 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     }
     static void st_move(cgl_handle *h, int phases)
     {
-        k_move<<<1, PB, 0, h->stream>>>(h->L, ext_with(h, h->ntype[h->ntcur]), h->P, phases);
+        if (phases & (MV_MOM | MV_CAP)) k_part_sums<<<h->npar, PB, 0, h->stream>>>(h->P, phases);
+        if (phases & (MV_REP | MV_MOVE))
+            k_move<<<1, PB, 0, h->stream>>>(h->L, ext_with(h, h->ntype[h->ntcur]), h->P, phases);
     }
     static void st_set_walls(cgl_handle *h)
     {

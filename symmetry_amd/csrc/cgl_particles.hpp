@@ -1025,39 +1025,58 @@ __global__ __launch_bounds__(PB) void k_lagrange(Lat L, Ext E, Part Pt, const P 
     block_sum<7>(acc, Pt.part_lag + ((size_t)p * LAG_BLOCKS + blockIdx.y) * 7);
 }
 
+// Phases of the particles' movement step (k_part_sums, k_move).
+enum { MV_MOM = 1, MV_CAP = 2, MV_REP = 4, MV_MOVE = 8, MV_ALL = 15 };
+
+// ------------------------------------------------- per-particle sums
+// Momentum-exchange (MV_MOM) and capillary (MV_CAP) force and torque of
+// particle blockIdx.x: its slices' partial results staged in LDS by the whole
+// workgroup, then each component summed over the slices in slice order by
+// one thread (the order a serial sum takes, so every run gives the same
+// bits).
+static __global__ __launch_bounds__(PB) void k_part_sums(Part P, int phases)
+{
+    __shared__ double mom[LSLICES * 6], cap[LAG_BLOCKS * 7];
+    const int p = blockIdx.x;
+    if (phases & MV_MOM)
+        for (int t = threadIdx.x; t < LSLICES * 6; t += PB) mom[t] = P.part_mom[(size_t)p * LSLICES * 6 + t];
+    if (phases & MV_CAP)
+        for (int t = threadIdx.x; t < LAG_BLOCKS * 7; t += PB) cap[t] = P.part_lag[(size_t)p * LAG_BLOCKS * 7 + t];
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k < 6 && (phases & MV_MOM)) {
+        double m = 0;
+        for (int s = 0; s < LSLICES; ++s) m += mom[s * 6 + k];
+        if (k < 3)
+            P.fp[3 * p + k] = m;
+        else
+            P.tp[3 * p + k - 3] = m;
+    } else if (k >= 64 && k < 64 + 7 && (phases & MV_CAP)) { // a wave of its own
+        const int kc = k - 64;
+        double c = 0;
+        for (int s = 0; s < LAG_BLOCKS; ++s) c += cap[s * 7 + kc];
+        if (kc < 3)
+            P.ten[3 * p + kc] = c;
+        else if (kc < 6)
+            P.tent[3 * p + kc - 3] = c;
+        else
+            P.shb[p] = c;
+    }
+}
+
 // ------------------------------------------------- repulsion + Newton-Euler
 // One workgroup, in phases that a step runs together and the staged entry
 // points one at a time: MV_MOM and MV_CAP finish the momentum exchange and
-// the capillary force (per-particle sums of the partial results in a fixed
-// order), MV_REP the soft wall and pair repulsion (S:1602-1668), MV_MOVE the
-// symmetry-plane rules and the explicit update of velocity, angular velocity
-// and centre (S:1272-1346).
-enum { MV_MOM = 1, MV_CAP = 2, MV_REP = 4, MV_MOVE = 8, MV_ALL = 15 };
+// the capillary force (k_part_sums, launched before this kernel), MV_REP the
+// soft wall and pair repulsion (S:1602-1668), MV_MOVE the symmetry-plane
+// rules and the explicit update of velocity, angular velocity and centre
+// (S:1272-1346).
 static __global__ __launch_bounds__(PB) void k_move(Lat L, Ext E, Part P, int phases)
 {
     const int np = P.npar;
     const double range = 3.0, order = 0.01;
     const int n3[3] = {L.nx, L.ny, L.nz};
     for (int p = threadIdx.x; p < np; p += PB) {
-        if (phases & MV_MOM) {
-            double m[6] = {0, 0, 0, 0, 0, 0};
-            for (int s = 0; s < LSLICES; ++s)
-                for (int k = 0; k < 6; ++k) m[k] += P.part_mom[((size_t)p * LSLICES + s) * 6 + k];
-            for (int k = 0; k < 3; ++k) {
-                P.fp[3 * p + k] = m[k];
-                P.tp[3 * p + k] = m[3 + k];
-            }
-        }
-        if (phases & MV_CAP) {
-            double c[7] = {0, 0, 0, 0, 0, 0, 0};
-            for (int s = 0; s < LAG_BLOCKS; ++s)
-                for (int k = 0; k < 7; ++k) c[k] += P.part_lag[((size_t)p * LAG_BLOCKS + s) * 7 + k];
-            for (int k = 0; k < 3; ++k) {
-                P.ten[3 * p + k] = c[k];
-                P.tent[3 * p + k] = c[3 + k];
-            }
-            P.shb[p] = c[6];
-        }
         if (!(phases & MV_REP)) continue;
         // walls (S:1606-1650): high faces push with +, low faces with -
         const double R = P.rad[p];
