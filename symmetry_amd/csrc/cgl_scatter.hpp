@@ -91,6 +91,31 @@ __device__ __forceinline__ void atomic_add2(int2 *a, int2 c)
 }
 __device__ __forceinline__ int2 shfl2(int2 v, int src) { return make_int2(__shfl(v.x, src), __shfl(v.y, src)); }
 
+// ONE (all 19 direction arrays of a state within 4 GiB): the state as a raw
+// buffer resource, so that a load or store takes the node's in-plane offset
+// in a vector register and the plane and direction in a scalar one (one
+// vector add per distinct in-plane offset instead of one per direction).
+// 0x00020000: the gfx9 family's raw-buffer resource word.
+template <class S> __device__ __forceinline__ __amdgpu_buffer_rsrc_t state_rsrc(const S *p, size_t n)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, 0, (int)(unsigned)(n * sizeof(S)), 0x00020000);
+}
+template <class S> __device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+{
+    if constexpr (sizeof(S) == 4)
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    else
+        return float(__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0)));
+}
+// FP32 values streamed past the caches (non-temporal, the `nt` policy bit)
+template <class S> __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so, float v)
+{
+    if constexpr (sizeof(S) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 2);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)v), r, vo, so, 0);
+}
+
 // The phase-field buffer entry (signed minority fraction, minority_q) of a
 // node from its sums: the minority sum over the total, IEEE division (a
 // one-sided rounding of the colour split is amplified by the dynamics).  Pure
@@ -144,6 +169,9 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
     const unsigned sx = (unsigned)L.ny * (unsigned)L.nz;
     const size_t N = (size_t)L.N;
     const R rho0 = R(L.rho0), qs = SC_ONE / R(L.rho0);
+    // (used with ONE only)
+    const __amdgpu_buffer_rsrc_t rin = state_rsrc(fin, (size_t)Q * N);
+    const __amdgpu_buffer_rsrc_t rout = state_rsrc((const S *)fout, (size_t)Q * N);
 
     // phi ring: each thread loads at most NE elements of the (KY+4) x (TZ+4)
     // plane region (the first always exists); their in-plane offsets are the
@@ -361,11 +389,14 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             if constexpr (sizeof(S) == 4) __builtin_amdgcn_s_setprio(2);
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
-                if constexpr (ONE)
-                    f[i] = StoreOf<S>::ld(fin, m + (unsigned)i * (unsigned)N);
-                else
+                if constexpr (ONE) {
+                    const unsigned vo = (yo[1 - CY[i]] + zo[1 - CZ[i]]) * (unsigned)sizeof(S);
+                    const unsigned so = (xo[1 - CX[i]] + (unsigned)i * (unsigned)N) * (unsigned)sizeof(S);
+                    f[i] = buf_ld<S>(rin, vo, so);
+                } else {
+                    const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
                     f[i] = StoreOf<S>::ld(grp(fin, i, N), grp_off(i, N, m));
+                }
             }
             if constexpr (sizeof(S) == 4) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
@@ -416,12 +447,17 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
             // a run's last step; streamed past the caches
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                S *fp = ONE ? fout : grp(fout, i, N);
-                const unsigned k = ONE ? n + (unsigned)i * (unsigned)N : grp_off(i, N, n);
-                if constexpr (sizeof(S) == 4)
-                    __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, k));
-                else
-                    StoreOf<S>::st(fp, k, f[i]);
+                if constexpr (ONE) {
+                    buf_st<S>(rout, (yo[1] + zo[1]) * (unsigned)sizeof(S),
+                              (xo[1] + (unsigned)i * (unsigned)N) * (unsigned)sizeof(S), float(f[i]));
+                } else {
+                    S *fp = grp(fout, i, N);
+                    const unsigned k = grp_off(i, N, n);
+                    if constexpr (sizeof(S) == 4)
+                        __builtin_nontemporal_store(float(f[i]), (float *)&at(fp, k));
+                    else
+                        StoreOf<S>::st(fp, k, f[i]);
+                }
             }
             if (sout) {
                 typedef float f4v __attribute__((ext_vector_type(4)));
