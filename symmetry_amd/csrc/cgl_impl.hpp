@@ -641,8 +641,7 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
                     h->L, En, h->P, fin, fout, sall, (const float *)phi, h->bnrm, h->sb.own);
             if (h->tev) (void)hipEventRecord(h->tev[3], h->stream);
             h->state_layout = FS;
-            if (ghost_faces(h))
-                k_fill_ghosts<S, SHIFT, false><<<face_grid(h), 256, 0, h->stream>>>(h->L, h->E, fout, nullptr, nullptr);
+            // (the step kernels have written the wall ghost slots)
             if (h->npar) {
                 k_curve_fs<S, SHIFT><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, En, h->P, fout, sall, En.rbo[0],
                                                                               En.rbo[1], h->sb.own);

@@ -2506,10 +2506,7 @@ __global__ __launch_bounds__(256) void k_unpack_fs_ext(Lat L, Ext E, const S *__
 // across mirror faces.  One thread per node of face blockIdx.y (axis y / 2,
 // side y % 2); edge and corner ghosts are written once per face with equal
 // values.
-// SIDE = false (the scatter form, wall faces only): the totals only; the
-// colours of a wall ghost slot are never read (a live node splits its
-// bounce-back slots with its own scalars).
-template <class S, bool SHIFT, bool SIDE = true>
+template <class S, bool SHIFT>
 __global__ __launch_bounds__(256) void k_fill_ghosts(Lat L, Ext E, S *__restrict__ fs, void *__restrict__ side0,
                                                      void *__restrict__ side1)
 {
@@ -2537,8 +2534,7 @@ __global__ __launch_bounds__(256) void k_fill_ghosts(Lat L, Ext E, S *__restrict
         pull_slot<i>(xo, yo, zo, wall_bb_mask(E, L, x, y, z), mirror_adjacent(E, L, x, y, z), m, d);
         const size_t ks = (size_t)d * N + m, kd = (size_t)i * N + ng;
         fs[kd] = fs[ks];
-        if constexpr (!SIDE) {
-        } else if constexpr (SHIFT) {
+        if constexpr (SHIFT) {
             ((float *)side0)[kd] = ((const float *)side0)[ks];
         } else {
             ((S *)side0)[kd] = ((const S *)side0)[ks];

@@ -459,6 +459,25 @@ __device__ __forceinline__ void collide_sc_tile(const Lat &L, const S *__restric
                         StoreOf<S>::st(fp, k, f[i]);
                 }
             }
+            if constexpr (EXT) {
+                // the wall ghost slots this node reads at the next step: slot i of
+                // the ghost x - c_i holds the node's own opposite population (the
+                // half-way bounce-back, S:1557-1590), which the gather reads there
+                if (wbb) {
+                    // (offsets formed again here, behind an empty asm that keeps
+                    // the compiler from reusing, and so holding, the gather's)
+                    int yy = y, zz = z;
+                    asm volatile("" : "+v"(yy), "+v"(zz));
+                    unsigned gxo[3], gyo[3], gzo[3];
+                    node_offsets(L, x, yy, zz, gxo, gyo, gzo);
+#pragma unroll
+                    for (int i = 1; i < Q; ++i) {
+                        if (!((wbb >> i) & 1u)) continue;
+                        const unsigned m = gxo[1 - CX[i]] + gyo[1 - CY[i]] + gzo[1 - CZ[i]];
+                        StoreOf<S>::st(grp(fout, i, N), grp_off(i, N, m), f[OPC[i]]);
+                    }
+                }
+            }
             if (sout) {
                 typedef float f4v __attribute__((ext_vector_type(4)));
                 Recol<R> ce = rc;
@@ -905,6 +924,13 @@ __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S
     }
     unsigned xo[3], yo[3], zo[3];
     node_offsets(L, x, y, z, xo, yo, zo);
+    // the wall ghost slots the node reads at the next step (as the tile kernel
+    // writes them): slot i of the ghost x - c_i takes the node's population opp i
+    const unsigned wbb = wall_bb_mask(E, L, x, y, z);
+    auto ghost_slot = [&](int i, R v) {
+        if ((wbb >> i) & 1u)
+            StoreOf<S>::st(fout + (size_t)i * N, xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]], v);
+    };
     if (nt & NT_SOLID) {
         // each slot takes the total it pulls; its colours (side array) split
         // with the source node's colour fraction
@@ -913,6 +939,7 @@ __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S
             const unsigned m = xo[1 - CX[i]] + yo[1 - CY[i]] + zo[1 - CZ[i]];
             const R f = StoreOf<S>::ld(fin + (size_t)i * N, m);
             StoreOf<S>::st(fout + (size_t)i * N, n, f);
+            if (i > 0) ghost_slot(OPC[i], f);
             const float qm = at(phi, m);
             const R fa = f + R(W[i]) * rho0, hm = __builtin_fabsf(qm) * fa;
             at((float *)E.rbo[0] + (size_t)i * N, n) =
@@ -991,6 +1018,17 @@ __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S
     // of the link nodes)
 #pragma unroll
     for (int i = 0; i < Q; ++i) StoreOf<S>::st(fout + (size_t)i * N, n, f[i]);
+    if (wbb) {
+        // (offsets formed again, as in the tile kernel)
+        int yy = y, zz = z;
+        asm volatile("" : "+v"(yy), "+v"(zz));
+        unsigned gxo[3], gyo[3], gzo[3];
+        node_offsets(L, x, yy, zz, gxo, gyo, gzo);
+#pragma unroll
+        for (int i = 1; i < Q; ++i)
+            if ((wbb >> i) & 1u)
+                StoreOf<S>::st(fout + (size_t)i * N, gxo[1 - CX[i]] + gyo[1 - CY[i]] + gzo[1 - CZ[i]], f[OPC[i]]);
+    }
     {
         Recol<R> ce = rc;
         ce.M = hm;
