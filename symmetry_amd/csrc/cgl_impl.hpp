@@ -278,6 +278,9 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
         if (h->nbox)
             k_phi_sc_batch<<<h->tiles1, node_block(), 0, h->stream>>>(h->dboxes, h->nbox, K1_XC, h->xc, h->sb,
                                                                        (float *)h->phi[h->phicur]);
+        else if (h->L.nz % 2 == 0 && h->L.nx % 2 == 0) // four nodes per thread
+            k_phi_sc4<<<dim3((h->L.nz / 2 + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, h->L.nx / 2), node_block(), 0,
+                        h->stream>>>(h->L, h->sb, h->xc, h->ky, (float *)h->phi[h->phicur], h->ext ? SC_BIAS : 0);
         else if (h->L.nz % 2 == 0) // two nodes per thread
             k_phi_sc2<<<dim3((h->L.nz / 2 + TZ - 1) / TZ, (h->L.ny + TY - 1) / TY, h->L.nx), node_block(), 0,
                         h->stream>>>(h->L, h->sb, h->xc, h->ky, (float *)h->phi[h->phicur], h->ext ? SC_BIAS : 0);

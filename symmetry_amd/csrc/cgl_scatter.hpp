@@ -755,19 +755,15 @@ static __global__ __launch_bounds__(256) void k_phi_sc(Lat L, ScBufs B, int xc, 
     phi_sc_node(L, B, xc, ky, phi, x, y, z, bias);
 }
 
-// The finishing pass two nodes at a time (z and z + 1, z even, nz even): the
-// pair's own sums in one 16-byte load, the x-face and y-face sums (which
-// call for both nodes of the pair or for neither) likewise, the z-face sums
-// per node, phi in one 8-byte store.  Integer sums are exact, so the order of
-// the additions does not change phi.
-static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc, int ky, float *__restrict__ phi,
-                                                        int bias)
+// The finishing pass two nodes at a time, the pair (x, y, z), (x, y, z + 1)
+// (z even, nz even): the pair's own sums in one 16-byte load (o, loaded by
+// the caller), the x-face and y-face sums (which call for both nodes of the
+// pair or for neither) likewise, the z-face sums per node, phi in one 8-byte
+// store.  Integer sums are exact, so the order of the additions does not
+// change phi.
+__device__ __forceinline__ void phi_sc_pair(const Lat &L, const ScBufs &B, int xc, int ky, float *__restrict__ phi,
+                                            int bias, int x, int y, int z, unsigned h, int4 o)
 {
-    const int z = (blockIdx.x * TZ + threadIdx.x) * 2, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z;
-    if (z >= L.nz || y >= L.ny) return;
-    const unsigned n = ((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z;
-    const unsigned h = n >> 1; // pair index: n is even
-    const int4 o = at((const int4 *)B.own, h);
     int2 s[2] = {make_int2((int)((unsigned)o.x - (unsigned)bias), o.y),
                  make_int2((int)((unsigned)o.z - (unsigned)bias), o.w)};
     auto add_pair = [&](const int2 *a) {
@@ -797,6 +793,30 @@ static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc,
     }
     if (sums_wrapped(s[0]) || sums_wrapped(s[1])) *B.range = 1;
     at((float2 *)phi, h) = make_float2(q_of_sums(s[0]), q_of_sums(s[1]));
+}
+
+static __global__ __launch_bounds__(256) void k_phi_sc2(Lat L, ScBufs B, int xc, int ky, float *__restrict__ phi,
+                                                        int bias)
+{
+    const int z = (blockIdx.x * TZ + threadIdx.x) * 2, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z;
+    if (z >= L.nz || y >= L.ny) return;
+    const unsigned h = (((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z) >> 1;
+    phi_sc_pair(L, B, xc, ky, phi, bias, x, y, z, h, at((const int4 *)B.own, h));
+}
+
+// The same, two x planes per thread (nx even): both planes' own sums are
+// loaded before either pair is finished, two 16-byte loads in flight per
+// thread.
+static __global__ __launch_bounds__(256) void k_phi_sc4(Lat L, ScBufs B, int xc, int ky, float *__restrict__ phi,
+                                                        int bias)
+{
+    const int z = (blockIdx.x * TZ + threadIdx.x) * 2, y = blockIdx.y * TY + threadIdx.y, x = blockIdx.z * 2;
+    if (z >= L.nz || y >= L.ny) return;
+    const unsigned h = (((unsigned)x * (unsigned)L.ny + (unsigned)y) * (unsigned)L.nz + (unsigned)z) >> 1;
+    const unsigned h2 = h + (((unsigned)L.ny * (unsigned)L.nz) >> 1);
+    const int4 o = at((const int4 *)B.own, h), o2 = at((const int4 *)B.own, h2);
+    phi_sc_pair(L, B, xc, ky, phi, bias, x, y, z, h, o);
+    phi_sc_pair(L, B, xc, ky, phi, bias, x + 1, y, z, h2, o2);
 }
 
 // The batch's finishing pass over the moments launch's tiles (z tile, y tile,
