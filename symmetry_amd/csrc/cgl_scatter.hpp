@@ -845,41 +845,47 @@ static __global__ __launch_bounds__(256) void k_phi_sc_batch(const BoxDev *__res
 // Whether box node (x, y, z) of particle blockIdx.y lies in no lower
 // particle's band box, so that a node in several boxes adds its terms once.
 // Every thread of the workgroup takes part (barriers): the lower particles'
-// boxes pass through LDS, PB at a time, their periodic corners wrapped into
-// the domain; only boxes that meet this particle's box are tested per node.
+// boxes that meet this particle's own pass through LDS, PB candidates at a
+// time, their periodic corners wrapped into the domain, and only those are
+// tested per node (the answer does not depend on their order in the list).
 __device__ __forceinline__ bool first_band(const Lat &L, const Ext &E, const Part &Pt, bool valid, int x, int y,
                                            int z)
 {
     __shared__ int bl[PB][6];
-    __shared__ int meets[PB];
+    __shared__ int nm;
     const int p = blockIdx.y;
     const int g[3] = {x, y, z}, n3[3] = {L.nx, L.ny, L.nz};
     const Box bp = box_of(Pt.center + 3 * p, Pt.rad[p], 3.0);
     bool first = valid;
     for (int q0 = 0; q0 < p; q0 += PB) {
         __syncthreads();
+        if (threadIdx.x == 0) nm = 0;
+        __syncthreads();
         const int q = q0 + (int)threadIdx.x;
-        int m = 0;
         if (q < p) {
             const Box b = box_of(Pt.center + 3 * q, Pt.rad[q], 3.0);
-            m = 1;
+            int lo[3];
+            bool meet = true;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const bool per = !bounded(E, a);
-                const int lo = per ? wrapn(b.lo[a], n3[a]) : b.lo[a];
-                bl[threadIdx.x][a] = lo;
-                bl[threadIdx.x][3 + a] = b.w[a];
+                lo[a] = per ? wrapn(b.lo[a], n3[a]) : b.lo[a];
                 // the two boxes meet along the axis (periodic: some image does)
-                const int d = per ? wrapn(lo - bp.lo[a], n3[a]) : lo - bp.lo[a];
-                const bool meet = per ? (d < bp.w[a] || d + b.w[a] > n3[a]) : (d < bp.w[a] && d + b.w[a] > 0);
-                if (!meet) m = 0;
+                const int d = per ? wrapn(lo[a] - bp.lo[a], n3[a]) : lo[a] - bp.lo[a];
+                meet = meet && (per ? (d < bp.w[a] || d + b.w[a] > n3[a]) : (d < bp.w[a] && d + b.w[a] > 0));
+            }
+            if (meet) {
+                const int j = atomicAdd(&nm, 1);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    bl[j][a] = lo[a];
+                    bl[j][3 + a] = b.w[a];
+                }
             }
         }
-        meets[threadIdx.x] = m;
-        if (!__syncthreads_or(m)) continue;
-        const int cnt = min(PB, p - q0);
+        __syncthreads();
+        const int cnt = nm;
         for (int j = 0; j < cnt && first; ++j) {
-            if (!meets[j]) continue;
             bool in = true;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
@@ -1096,10 +1102,13 @@ __global__ __launch_bounds__(PB) void k_collide_sc_band(Lat L, Ext E, Part Pt, c
         valid = box_in_domain(E, L, X);
     }
     const int x = wrapn(X[0], L.nx), y = wrapn(X[1], L.ny), z = wrapn(X[2], L.nz);
+    // band nodes only (not the box's corners, not the wall ghost layers)
+    unsigned nt = 0;
+    if (valid) {
+        nt = at(E.ntype, n);
+        valid = (nt & NT_NEAR) && !is_wall_ghost(E, L, x, y, z);
+    }
     if (!first_band(L, E, Pt, valid, x, y, z)) return;
-    if (is_wall_ghost(E, L, x, y, z)) return;
-    const unsigned nt = at(E.ntype, n);
-    if (!(nt & NT_NEAR)) return;
     sc_band_node<S>(L, E, fin, fout, sout, phi, nrm, own, x, y, z, n, nt);
 }
 
