@@ -166,3 +166,31 @@ def test_overlapping_bands_scatter_close_to_f64():
     _print("two spheres: phi", l2, "mass", mass, "centre", dc)
     assert l2 < 1e-5 and dc < 1e-5
     assert abs(mass[0] - mass[1]) < 1e-6 * mass[1]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+@pytest.mark.parametrize("case", ["x", "xyz"])
+def test_wall_drop_scatter_close_to_f64(case, prec):
+    """A drop touching walls in x only, and in a closed box (walls on all six
+    faces, so edges and corners where two or three walls return a term), 100
+    steps: phi of the scatter form against FP64 and no farther from it than
+    the gather form of the same precision, with the margins of
+    test_walls_scatter_close_to_f64."""
+    from tests.test_gpu_ext import WALL_CASES
+    cfg, drop = WALL_CASES[case]
+    ri, bi, _ = ic.flat_interface_state(cfg["nx"], cfg["ny"], cfg["nz"], z0=-1e3, rho0=1.0, drop=drop,
+                                        sigma=cfg["sigma"])
+    live = _live(ri.shape[:3], cfg["bc"])
+    out = {}
+    for key, p, layout, sc in (("f64", "f64", "rb", False), ("gather", prec, "fs", False), ("scatter", prec, "fs", True)):
+        lat = _with_scatter(sc, lambda: _lattice(cfg, p, layout=layout))
+        assert lat.scatter_form() == sc
+        lat.upload(ri, bi)
+        lat.step(100)
+        out[key] = lat.calmacros()
+        lat.close()
+    err = {k: rel_err(out[k]["phi"][live], out["f64"]["phi"][live])[0] for k in ("gather", "scatter")}
+    _print(case, prec, "phi rel err: gather", err["gather"], "scatter", err["scatter"])
+    assert err["scatter"] < (1e-5 if prec == "f32" else 2e-5)
+    margin = 2 if prec == "f32" else 4
+    assert err["scatter"] < margin * err["gather"] + 1e-7
