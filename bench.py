@@ -266,11 +266,12 @@ def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512"), layo
         k = 10
         _, ms_mom, ms_col = lat.time_steps(k)
         sc = lat.scatter_form()
+        tuned = {key: v for key, v in lat.tuned_params().items() if key != "tuning"}
         lat.close()
         if sc:
             b_col, b_mom = B_ALG_COLLIDE_SC_EXT[prec], B_ALG_PHI_SC
             k_col = "k_collide_sc_ext (tile kernel) + k_collide_sc_band (band)"
-            k_mom = "k_phi_sc2 (finishing pass of the scatter form)"
+            k_mom = "k_phi_sc4 (finishing pass of the scatter form)"
         elif layout == "rb":
             b_col, b_mom = B_ALG_COLLIDE[(prec, layout)], B_ALG_MOMENTS_RB[prec]
             k_col, k_mom = "k_collide (walls/particles)", "k_moments (walls/particles)"
@@ -282,7 +283,7 @@ def run_particles(steps, f64_dense_steps, which=("sphere_96", "dense_512"), layo
         roof[f"{name}_{prec}_{layout}"] = {
             "steps": nsteps, "step_ms": round(dt / nsteps * 1e3, 4),
             "step_frac": round(mlups * 1e6 * B_ALG_STEP[prec] / HBM_PEAK, 4),
-            "phase_field": "scatter form" if sc else "moments pass",
+            "phase_field": "scatter form" if sc else "moments pass", "launch": tuned,
             "collide": {"kernel": k_col,
                         "alg_bytes_per_node": b_col,
                         "avg_launch_ms": round(col * 1e3, 4), "frac": round(b_col * nodes / col / HBM_PEAK, 4)},
