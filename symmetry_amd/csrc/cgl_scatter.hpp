@@ -1108,8 +1108,29 @@ __global__ __launch_bounds__(PB) void k_collide_sc_band(Lat L, Ext E, Part Pt, c
         nt = at(E.ntype, n);
         valid = (nt & NT_NEAR) && !is_wall_ghost(E, L, x, y, z);
     }
-    if (!first_band(L, E, Pt, valid, x, y, z)) return;
-    sc_band_node<S>(L, E, fin, fout, sout, phi, nrm, own, x, y, z, n, nt);
+    const bool mine = first_band(L, E, Pt, valid, x, y, z);
+    // The band is a shell of the box: pack its nodes, in box order, onto
+    // the workgroup's first lanes, so that whole waves update nodes and the
+    // rest return (one node per lane as before; the sums are integers, so
+    // the lane that adds a term changes no result).
+    __shared__ int4 slot[PB];
+    __shared__ int wcount[PB / 64];
+    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const unsigned long long bal = __ballot(mine);
+    if (lane == 0) wcount[wv] = __popcll(bal);
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < PB / 64; ++i) {
+        base += i < (int)wv ? wcount[i] : 0;
+        total += wcount[i];
+    }
+    if (mine) slot[base + __popcll(bal & ((1ull << lane) - 1ull))] = make_int4(x, y, z, (int)nt);
+    __syncthreads();
+    if ((int)threadIdx.x >= total) return;
+    const int4 s = slot[threadIdx.x];
+    const unsigned m = ((unsigned)s.x * L.ny + s.y) * L.nz + s.z;
+    sc_band_node<S>(L, E, fin, fout, sout, phi, nrm, own, s.x, s.y, s.z, m, (unsigned)s.w);
 }
 
 // The unit normals (ext_normal) of every node of each particle's box with
