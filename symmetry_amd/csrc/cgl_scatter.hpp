@@ -1083,6 +1083,32 @@ __device__ __forceinline__ void sc_band_node(const Lat &L, const Ext &E, const S
     }
 }
 
+// The band is a shell of its box: pack the lanes that hold a band node
+// (mine), in box order, onto the workgroup's first lanes, so that whole
+// waves work and the rest return.  Every lane of the workgroup must call it;
+// lane threadIdx.x receives the packed value in out and returns true if it
+// has one.
+__device__ __forceinline__ bool pack_lanes(bool mine, int4 v, int4 &out)
+{
+    __shared__ int4 slot[PB];
+    __shared__ int wcount[PB / 64];
+    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const unsigned long long bal = __ballot(mine);
+    if (lane == 0) wcount[wv] = __popcll(bal);
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int i = 0; i < PB / 64; ++i) {
+        base += i < (int)wv ? wcount[i] : 0;
+        total += wcount[i];
+    }
+    if (mine) slot[base + __popcll(bal & ((1ull << lane) - 1ull))] = v;
+    __syncthreads();
+    if ((int)threadIdx.x >= total) return false;
+    out = slot[threadIdx.x];
+    return true;
+}
+
 // The band's step in the scatter form (after the tile kernel, which has
 // written every node's own-box sums).
 template <class S>
@@ -1109,26 +1135,9 @@ __global__ __launch_bounds__(PB) void k_collide_sc_band(Lat L, Ext E, Part Pt, c
         valid = (nt & NT_NEAR) && !is_wall_ghost(E, L, x, y, z);
     }
     const bool mine = first_band(L, E, Pt, valid, x, y, z);
-    // The band is a shell of the box: pack its nodes, in box order, onto
-    // the workgroup's first lanes, so that whole waves update nodes and the
-    // rest return (one node per lane as before; the sums are integers, so
-    // the lane that adds a term changes no result).
-    __shared__ int4 slot[PB];
-    __shared__ int wcount[PB / 64];
-    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const unsigned long long bal = __ballot(mine);
-    if (lane == 0) wcount[wv] = __popcll(bal);
-    __syncthreads();
-    int base = 0, total = 0;
-#pragma unroll
-    for (int i = 0; i < PB / 64; ++i) {
-        base += i < (int)wv ? wcount[i] : 0;
-        total += wcount[i];
-    }
-    if (mine) slot[base + __popcll(bal & ((1ull << lane) - 1ull))] = make_int4(x, y, z, (int)nt);
-    __syncthreads();
-    if ((int)threadIdx.x >= total) return;
-    const int4 s = slot[threadIdx.x];
+    // the sums are integers, so the lane that adds a term changes no result
+    int4 s;
+    if (!pack_lanes(mine, make_int4(x, y, z, (int)nt), s)) return;
     const unsigned m = ((unsigned)s.x * L.ny + s.y) * L.nz + s.z;
     sc_band_node<S>(L, E, fin, fout, sout, phi, nrm, own, s.x, s.y, s.z, m, (unsigned)s.w);
 }
@@ -1163,14 +1172,28 @@ static __global__ __launch_bounds__(PB) void k_band_normals(Lat L, Ext E, Part P
 template <class S>
 __global__ __launch_bounds__(PB) void k_band_sc(Lat L, Ext E, Part Pt, const S *__restrict__ fs, float *__restrict__ phi)
 {
-    for_special_nodes<true>(L, E, Pt, [&](int x, int y, int z) {
+    const int p = blockIdx.y;
+    const Box b = box_of(Pt.center + 3 * p, Pt.rad[p], 3.0);
+    const unsigned k = blockIdx.x * PB + threadIdx.x;
+    int X[3] = {0, 0, 0};
+    unsigned n0 = 0, nt0 = 0;
+    bool mine = k < box_size(b);
+    if (mine) {
+        box_node(L, b, k, X, n0);
+        mine = box_in_domain(E, L, X);
+    }
+    if (mine) {
+        nt0 = at(E.ntype, n0);
+        mine = (nt0 & NT_NEAR) && !is_wall_ghost(E, L, wrapn(X[0], L.nx), wrapn(X[1], L.ny), wrapn(X[2], L.nz));
+    }
+    int4 s;
+    if (!pack_lanes(mine, make_int4(wrapn(X[0], L.nx), wrapn(X[1], L.ny), wrapn(X[2], L.nz), (int)nt0), s)) return;
+    {
         using R = float;
-        if (is_wall_ghost(E, L, x, y, z)) return;
         unsigned xo[3], yo[3], zo[3];
-        node_offsets(L, x, y, z, xo, yo, zo);
+        node_offsets(L, s.x, s.y, s.z, xo, yo, zo);
         const unsigned n = xo[1] + yo[1] + zo[1];
-        const unsigned nt = at(E.ntype, n);
-        if (!(nt & NT_NEAR)) return;
+        const unsigned nt = (unsigned)s.w;
         const size_t N = (size_t)L.N;
         const R rho0 = R(L.rho0);
         Moments<R, true> mo;
@@ -1203,7 +1226,7 @@ __global__ __launch_bounds__(PB) void k_band_sc(Lat L, Ext E, Part Pt, const S *
             mo.sb = __builtin_signbitf(qn) ? smin : mo.rho - smin;
         }
         band_macros(L, E, n, nt, mo);
-    });
+    }
 }
 
 } // namespace cgl
