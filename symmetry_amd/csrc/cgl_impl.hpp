@@ -476,8 +476,13 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     // the particles' link lists from the masks in buffer `buf`
     static void build_lists(cgl_handle *h, int buf)
     {
-        k_lists<<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, ext_with(h, h->ntype[buf]), h->P, h->ntype[buf],
-                                                          h->walls_i, h->pin_i, h->pout_i);
+        const Ext E = ext_with(h, h->ntype[buf]);
+        if (h->npar <= LISTS_WIDE_MAX)
+            k_lists<1024><<<pgrid(h, LSLICES), 1024, 0, h->stream>>>(h->L, E, h->P, h->ntype[buf], h->walls_i,
+                                                                     h->pin_i, h->pout_i);
+        else
+            k_lists<PB><<<pgrid(h, LSLICES), PB, 0, h->stream>>>(h->L, E, h->P, h->ntype[buf], h->walls_i, h->pin_i,
+                                                                 h->pout_i);
     }
     // The stages of a step with walls and/or particles (S:216-251).  The
     // masks of the step's start stay in ntype[ntcur] until the step ends;

@@ -28,6 +28,7 @@ namespace cgl {
 constexpr int PB = 256;        // threads per particle-kernel workgroup
 constexpr int PSLICES = 48;    // workgroups per particle for box kernels
 constexpr int LSLICES = 16;    // slices of the link lists per particle (one workgroup each)
+constexpr int LISTS_WIDE_MAX = 16; // particles up to which k_lists takes 1 024-thread workgroups
 constexpr int LAG_BLOCKS = 64; // workgroups per particle for the Lagrange points (one point per thread)
 constexpr int LA_M = 180, LA_N = 90, LA_NUM = LA_M * (LA_N + 1);
 
@@ -311,14 +312,20 @@ static __global__ __launch_bounds__(PB) void k_masks_links(Lat L, Ext E, Part P,
         box_node(L, b, k, X, n);
         if (!box_in_domain(E, L, X) || !interior(E, L, X)) continue;
         if (nt[n] & NT_SOLID) continue;
+        bool link = false;
         for (int i = 1; i < Q; ++i) {
             const unsigned m = nb_index(L, X, CX[i], CY[i], CZ[i]);
             if (walls[m] == p + 1) {
-                atomicMax(pout + n, p + 1);
+                link = true;
                 atomicMax(pin + m, p + 1);
-                atomicOr(nt + n, NT_POUT);
                 atomicOr(nt + m, NT_PIN);
             }
+        }
+        // the node's own marks once, however many links it has (max and or
+        // give the same words in any order and number)
+        if (link) {
+            atomicMax(pout + n, p + 1);
+            atomicOr(nt + n, NT_POUT);
         }
     }
 }
@@ -391,8 +398,14 @@ __global__ __launch_bounds__(PB) void k_ghost_phi(Lat L, Ext E, Part Pt, const u
 // s * per to (s + 1) * per (per = ceil(box / LSLICES)) in box order, each
 // chunk of PB of them taking its place by a block-wide prefix (wave ballots,
 // then the waves in order), so the lists, and every sum over them, are the
-// same from run to run.
-static __global__ __launch_bounds__(PB) void k_lists(Lat L, Ext E, Part P, const unsigned *nt, const int *walls,
+// same from run to run.  LB threads per workgroup: a slice of a box is a few
+// hundred to a thousand nodes, taken in chunks of LB one after another, each
+// a chain of dependent loads; with few particles (LSLICES workgroups each)
+// nothing else overlaps them, so such a handle takes 1 024-thread chunks
+// (33 -> 11 us per step for one sphere of radius 10), while 200 particles
+// keep 256 (1 024 idles most lanes of a slice there: 47 -> 110 us).
+template <int LB>
+__global__ __launch_bounds__(LB) void k_lists(Lat L, Ext E, Part P, const unsigned *nt, const int *walls,
                                                      const int *pin, const int *pout)
 {
     const int p = blockIdx.x;
@@ -400,12 +413,12 @@ static __global__ __launch_bounds__(PB) void k_lists(Lat L, Ext E, Part P, const
     const unsigned nbox = box_size(b), per = (nbox + LSLICES - 1) / LSLICES;
     const unsigned k1 = min(nbox, (blockIdx.y + 1) * per);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __shared__ unsigned wsum[2][PB / 64];
+    __shared__ unsigned wsum[2][LB / 64];
     __shared__ unsigned base[2];
     if (threadIdx.x == 0) base[0] = base[1] = 0;
     __syncthreads();
     const size_t off = ((size_t)p * LSLICES + blockIdx.y) * (size_t)P.lcap;
-    for (unsigned k0 = blockIdx.y * per; k0 < k1; k0 += PB) {
+    for (unsigned k0 = blockIdx.y * per; k0 < k1; k0 += LB) {
         const unsigned k = k0 + threadIdx.x;
         bool isout = false, isin = false;
         unsigned mask = 0;
@@ -443,7 +456,7 @@ static __global__ __launch_bounds__(PB) void k_lists(Lat L, Ext E, Part P, const
         if (isin && ji < (unsigned)P.lcap) P.lin[off + ji] = k;
         __syncthreads();
         if (threadIdx.x == 0)
-            for (int w = 0; w < PB / 64; ++w) {
+            for (int w = 0; w < LB / 64; ++w) {
                 base[0] += wsum[0][w];
                 base[1] += wsum[1][w];
             }
