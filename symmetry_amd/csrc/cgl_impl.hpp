@@ -515,9 +515,9 @@ template <class S, class R, class P, bool SHIFT> struct Impl {
     }
     static void st_move(cgl_handle *h, int phases)
     {
-        if (phases & (MV_MOM | MV_CAP)) k_part_sums<<<h->npar, PB, 0, h->stream>>>(h->P, phases);
-        if (phases & (MV_REP | MV_MOVE))
-            k_move<<<1, PB, 0, h->stream>>>(h->L, ext_with(h, h->ntype[h->ntcur]), h->P, phases);
+        const Ext E = ext_with(h, h->ntype[h->ntcur]);
+        if (phases & (MV_MOM | MV_CAP | MV_REP)) k_part_sums<<<h->npar, PB, 0, h->stream>>>(h->L, E, h->P, phases);
+        if (phases & MV_MOVE) k_move<<<1, PB, 0, h->stream>>>(h->L, E, h->P, phases);
     }
     static void st_set_walls(cgl_handle *h)
     {

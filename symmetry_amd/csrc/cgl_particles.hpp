@@ -1046,11 +1046,69 @@ enum { MV_MOM = 1, MV_CAP = 2, MV_REP = 4, MV_MOVE = 8, MV_ALL = 15 };
 // particle blockIdx.x: its slices' partial results staged in LDS by the whole
 // workgroup, then each component summed over the slices in slice order by
 // one thread (the order a serial sum takes, so every run gives the same
-// bits).
-static __global__ __launch_bounds__(PB) void k_part_sums(Part P, int phases)
+// bits).  MV_REP: the soft wall and pair repulsion (S:1602-1668), the pair
+// terms of every other particle formed by the workgroup's threads at once
+// (JC at a time, in LDS), then added in index order by one thread, as a
+// serial loop over the pairs adds them.
+static __global__ __launch_bounds__(PB) void k_part_sums(Lat L, Ext E, Part P, int phases)
 {
     __shared__ double mom[LSLICES * 6], cap[LAG_BLOCKS * 7];
+    constexpr int JC = 4 * PB;
+    __shared__ double tj[JC][3];
+    __shared__ unsigned char hit[JC];
     const int p = blockIdx.x;
+    if (phases & MV_REP) {
+        const int np = P.npar, n3[3] = {L.nx, L.ny, L.nz};
+        const double range = 3.0, order = 0.01;
+        const double R = P.rad[p];
+        double cp[3], fr[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a) cp[a] = P.center[3 * p + a];
+        if (threadIdx.x == 0)
+            // walls (S:1606-1650): high faces push with +, low faces with -
+            for (int a = 0; a < 3; ++a) {
+                if (P.rep_on[2 * a]) {
+                    const double d = cp[a] - P.rep_plane[2 * a] - R;
+                    if (d < range) fr[a] -= 2.0 * (P.fg / order) * ((d - range) * (d - range)) / (range * range);
+                }
+                if (P.rep_on[2 * a + 1]) {
+                    const double d = P.rep_plane[2 * a + 1] - cp[a] - R;
+                    if (d < range) fr[a] += 2.0 * (P.fg / order) * ((d - range) * (d - range)) / (range * range);
+                }
+            }
+        // pairs (S:1655-1668), minimum image on periodic axes
+        for (int j0 = 0; j0 < np; j0 += JC) {
+            const int jn = min(JC, np - j0);
+            for (int t = threadIdx.x; t < jn; t += PB) {
+                const int j = j0 + t;
+                bool h = false;
+                if (j != p) {
+                    double dc[3];
+                    for (int a = 0; a < 3; ++a) {
+                        dc[a] = cp[a] - P.center[3 * j + a];
+                        if (!(E.wall[a][0] || E.wall[a][1])) {
+                            if (dc[a] > 0.5 * n3[a]) dc[a] -= n3[a];
+                            if (dc[a] < -0.5 * n3[a]) dc[a] += n3[a];
+                        }
+                    }
+                    const double d = sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]) - (R + P.rad[j]);
+                    if (d < range) {
+                        h = true;
+                        for (int a = 0; a < 3; ++a)
+                            tj[t][a] = (P.fg / order) * ((d - range) * (d - range)) / (range * range) * dc[a] / d;
+                    }
+                }
+                hit[t] = h;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int t = 0; t < jn; ++t)
+                    if (hit[t])
+                        for (int a = 0; a < 3; ++a) fr[a] = fr[a] - tj[t][a];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            for (int a = 0; a < 3; ++a) P.frep[3 * p + a] = fr[a];
+    }
     if (phases & MV_MOM)
         for (int t = threadIdx.x; t < LSLICES * 6; t += PB) mom[t] = P.part_mom[(size_t)p * LSLICES * 6 + t];
     if (phases & MV_CAP)
@@ -1077,52 +1135,14 @@ static __global__ __launch_bounds__(PB) void k_part_sums(Part P, int phases)
     }
 }
 
-// ------------------------------------------------- repulsion + Newton-Euler
-// One workgroup, in phases that a step runs together and the staged entry
-// points one at a time: MV_MOM and MV_CAP finish the momentum exchange and
-// the capillary force (k_part_sums, launched before this kernel), MV_REP the
-// soft wall and pair repulsion (S:1602-1668), MV_MOVE the symmetry-plane
-// rules and the explicit update of velocity, angular velocity and centre
-// (S:1272-1346).
+// ------------------------------------------------------------ Newton-Euler
+// One workgroup (MV_MOVE, after k_part_sums has finished the forces): the
+// symmetry-plane rules and the explicit update of velocity, angular velocity
+// and centre (S:1272-1346).
 static __global__ __launch_bounds__(PB) void k_move(Lat L, Ext E, Part P, int phases)
 {
     const int np = P.npar;
-    const double range = 3.0, order = 0.01;
-    const int n3[3] = {L.nx, L.ny, L.nz};
-    for (int p = threadIdx.x; p < np; p += PB) {
-        if (!(phases & MV_REP)) continue;
-        // walls (S:1606-1650): high faces push with +, low faces with -
-        const double R = P.rad[p];
-        double fr[3] = {0, 0, 0};
-        for (int a = 0; a < 3; ++a) {
-            if (P.rep_on[2 * a]) {
-                const double d = P.center[3 * p + a] - P.rep_plane[2 * a] - R;
-                if (d < range) fr[a] -= 2.0 * (P.fg / order) * ((d - range) * (d - range)) / (range * range);
-            }
-            if (P.rep_on[2 * a + 1]) {
-                const double d = P.rep_plane[2 * a + 1] - P.center[3 * p + a] - R;
-                if (d < range) fr[a] += 2.0 * (P.fg / order) * ((d - range) * (d - range)) / (range * range);
-            }
-        }
-        // pairs (S:1655-1668), minimum image on periodic axes
-        for (int j = 0; j < np; ++j) {
-            if (j == p) continue;
-            double dc[3];
-            for (int a = 0; a < 3; ++a) {
-                dc[a] = P.center[3 * p + a] - P.center[3 * j + a];
-                if (!(E.wall[a][0] || E.wall[a][1])) {
-                    if (dc[a] > 0.5 * n3[a]) dc[a] -= n3[a];
-                    if (dc[a] < -0.5 * n3[a]) dc[a] += n3[a];
-                }
-            }
-            const double d = sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]) - (R + P.rad[j]);
-            if (d < range)
-                for (int a = 0; a < 3; ++a) fr[a] = fr[a] - (P.fg / order) * ((d - range) * (d - range)) / (range * range) * dc[a] / d;
-        }
-        for (int a = 0; a < 3; ++a) P.frep[3 * p + a] = fr[a];
-    }
     if (!(phases & MV_MOVE)) return;
-    __syncthreads(); // every repulsion read the old centres
     if (P.sym_rules) {
         // the reference's symmetry-plane rules (S:1272-1335): a particle that
         // started on a mirror plane feels its image, so the in-plane force
