@@ -194,3 +194,34 @@ def test_wall_drop_scatter_close_to_f64(case, prec):
     assert err["scatter"] < (1e-5 if prec == "f32" else 2e-5)
     margin = 2 if prec == "f32" else 4
     assert err["scatter"] < margin * err["gather"] + 1e-7
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_launch_shape_bit_identical_with_particles(prec):
+    """X chunks 8, 2 and 6 (the last leaves a short chunk at the box's end)
+    give bit-identical states and particle centres with z walls and two
+    spheres whose band boxes overlap: every term is an integer in fixed
+    point, so the launch shape changes no sum."""
+    n, rad = 40, 6.0
+    cfg = ic.particle_config2(n=n, rad=rad)
+    c0 = np.array(cfg["center"]) + np.array([0.3, -0.2, 0.4])
+    centers = [c0 - np.array([rad + 2.5, 0.0, 0.0]), c0 + np.array([rad + 2.5, 0.0, 0.0])]
+    parts = ic.sphere_particles(centers, rad, cfg["rho_p"], cfg["cita_deg"], cfg["sigma"])
+    parts["up"] = np.array([[0.02, 0.01, -0.01], [-0.02, 0.0, 0.01]], dtype=np.float64)
+    parts["rp"] = np.zeros((2, 3), dtype=np.float64)
+    ri, bi, _ = ic.flat_interface_state(n, n, n, z0=cfg["z0"], rho0=cfg["rho0"], beta=cfg["beta"])
+    res = []
+    for xc in (8, 2, 6):
+        lat = _lattice(cfg, prec, npar=2, layout="fs", xchunk=xc)
+        lat.upload(ri, bi)
+        lat.set_particles(parts["center"], parts["rad"], parts["mass"], parts["inertia"], parts["cita"],
+                          parts["fg"], up=parts["up"], rp=parts["rp"])
+        assert lat.scatter_form()
+        lat.step(25)
+        tp = lat.tuned_params()
+        assert tp["xchunk"] == xc
+        res.append((lat.download(), lat.get_particles()["center"]))
+        lat.close()
+    for (r, b), c in res[1:]:
+        assert np.array_equal(r, res[0][0][0]) and np.array_equal(b, res[0][0][1])
+        assert np.array_equal(c, res[0][1])
